@@ -1,0 +1,281 @@
+"""MI355X-native constrained-Delaunay quality refinement.
+
+Python bindings (ctypes) over libgcdt.so, the HIP/gfx950 library whose C ABI
+is declared in include/gpu_cdt_refine.h.  The refinement itself always runs
+on the GPU; there is no CPU fallback: if the library or a GPU is missing the
+calls raise.
+
+Main entry points
+  refine(points, triangles, neighbors, segments, theta=25, ...) -> RefineResult
+      the reference's GPU_Refine_Quality (refine.h:385-386) on numpy arrays
+  Session(...)        device-resident input; run() repeatedly (benchmarks)
+  generate_pslg(...)  seeded synthetic PSLG (points + non-crossing segments)
+  build_cdt(...)      constrained Delaunay triangulation with hull segments
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgcdt.so")
+
+Circumcenter, Offcenter = 0, 1
+RUPPERT, CHEW = 1, 0
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("theta_deg", C.c_double),
+        ("offcenter", C.c_int),
+        ("mode", C.c_int),
+        ("cav_cap0", C.c_int),
+        ("cav_cap_max", C.c_int),
+        ("cand_budget", C.c_longlong),
+        ("max_rounds", C.c_int),
+        ("priority", C.c_int),
+        ("pred_mode", C.c_int),
+        ("profile", C.c_int),
+        ("device", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("rounds", C.c_int),
+        ("seg_passes", C.c_int),
+        ("bad_passes", C.c_int),
+        ("inserted", C.c_longlong),
+        ("seg_splits", C.c_longlong),
+        ("rej_encroach", C.c_longlong),
+        ("rej_blocked", C.c_longlong),
+        ("rej_overflow", C.c_longlong),
+        ("lost", C.c_longlong),
+        ("giveups", C.c_longlong),
+        ("final_cav_cap", C.c_int),
+        ("status", C.c_int),
+        ("device_ms", C.c_double),
+        ("pred_fp32", C.c_longlong),
+        ("pred_fp64", C.c_longlong),
+        ("pred_exact", C.c_longlong),
+        ("n_points", C.c_longlong),
+        ("n_triangles", C.c_longlong),
+        ("n_subsegs", C.c_longlong),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_longlong),
+                ("total_ms", C.c_double), ("items", C.c_longlong)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libgcdt.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libgcdt.so is not built; run `python -m gpu_cdt_refine_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+        L.gcdt_default_options.argtypes = [C.POINTER(Options)]
+        L.gcdt_refine_host.argtypes = [vp, C.c_int, vp, vp, C.c_int, vp, C.c_int,
+                                       C.POINTER(Options), C.POINTER(vp), C.POINTER(Stats)]
+        L.gcdt_refine_host.restype = C.c_int
+        L.gcdt_mesh_counts.argtypes = [vp, ip, ip, ip]
+        L.gcdt_mesh_copy.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.gcdt_mesh_free.argtypes = [vp]
+        L.gcdt_session_create.argtypes = [vp, C.c_int, vp, vp, C.c_int, vp, C.c_int,
+                                          C.POINTER(Options)]
+        L.gcdt_session_create.restype = vp
+        L.gcdt_session_run.argtypes = [vp, C.POINTER(Stats)]
+        L.gcdt_session_run.restype = C.c_int
+        L.gcdt_session_download.argtypes = [vp, C.POINTER(vp)]
+        L.gcdt_session_download.restype = C.c_int
+        L.gcdt_session_kernel_times.argtypes = [vp, C.POINTER(KernelTime), C.c_int]
+        L.gcdt_session_kernel_times.restype = C.c_int
+        L.gcdt_session_destroy.argtypes = [vp]
+        L.gcdt_last_error.restype = C.c_char_p
+        L.gcdt_device_predicates.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
+        L.gcdt_device_predicates.restype = C.c_int
+        if hasattr(L, "gcdt_generate_pslg"):
+            L.gcdt_generate_pslg.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double,
+                                             C.c_double, C.c_double, C.c_ulonglong, vp, vp]
+            L.gcdt_generate_pslg.restype = C.c_int
+            L.gcdt_build_cdt.argtypes = [vp, C.c_int, vp, C.c_int, C.POINTER(vp)]
+            L.gcdt_build_cdt.restype = C.c_int
+            L.gcdt_cdt_counts.argtypes = [vp, ip, ip, ip]
+            L.gcdt_cdt_copy.argtypes = [vp, vp, vp, vp]
+            L.gcdt_cdt_free.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _err(rc, what):
+    msg = lib().gcdt_last_error().decode()
+    raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def make_options(theta=25.0, insert=Offcenter, mode=RUPPERT, **kw):
+    o = Options()
+    lib().gcdt_default_options(C.byref(o))
+    o.theta_deg = float(theta)
+    o.offcenter = 1 if insert == Offcenter else 0
+    o.mode = int(mode)
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+class RefineResult:
+    """Refined mesh in the reference's triangulateio convention."""
+
+    def __init__(self, points, triangles, neighbors, segments, pstatus,
+                 tstatus, stats):
+        self.points = points          # (P, 2) float64
+        self.triangles = triangles    # (T, 3) int32, CCW
+        self.neighbors = neighbors    # (T, 3) int32, opposite corner j, -1
+        self.segments = segments      # (S, 2) int32 subsegments
+        self.pstatus = pstatus        # (P,) uint8 PStatus bits
+        self.tstatus = tstatus        # (T,) uint8 TStatus bits
+        self.stats = stats            # dict
+
+
+def _as_inputs(points, triangles, neighbors, segments):
+    P = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    T = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+    N = np.ascontiguousarray(neighbors, dtype=np.int32).reshape(-1, 3)
+    S = np.ascontiguousarray(segments, dtype=np.int32).reshape(-1, 2)
+    return P, T, N, S
+
+
+def _fetch_mesh(m):
+    L = lib()
+    np_, nt, ns = C.c_int(), C.c_int(), C.c_int()
+    L.gcdt_mesh_counts(m, C.byref(np_), C.byref(nt), C.byref(ns))
+    P = np.empty((np_.value, 2), np.float64)
+    T = np.empty((nt.value, 3), np.int32)
+    N = np.empty((nt.value, 3), np.int32)
+    S = np.empty((ns.value, 2), np.int32)
+    ps = np.empty(np_.value, np.uint8)
+    ts = np.empty(nt.value, np.uint8)
+    L.gcdt_mesh_copy(m, _ptr(P), _ptr(T), _ptr(N), _ptr(S), _ptr(ps), _ptr(ts))
+    L.gcdt_mesh_free(m)
+    return P, T, N, S, ps, ts
+
+
+def refine(points, triangles, neighbors, segments, theta=25.0,
+           insert=Offcenter, mode=RUPPERT, **options):
+    """Refine a CDT on the GPU (GPU_Refine_Quality, refine.cu:6500-7139)."""
+    P, T, N, S = _as_inputs(points, triangles, neighbors, segments)
+    o = make_options(theta, insert, mode, **options)
+    st = Stats()
+    m = C.c_void_p()
+    rc = lib().gcdt_refine_host(_ptr(P), len(P), _ptr(T), _ptr(N), len(T),
+                                _ptr(S), len(S), C.byref(o), C.byref(m), C.byref(st))
+    if rc != 0:
+        _err(rc, "gcdt_refine_host")
+    return RefineResult(*_fetch_mesh(m), st.as_dict())
+
+
+class Session:
+    """Device-resident CDT: upload once, refine repeatedly."""
+
+    def __init__(self, points, triangles, neighbors, segments, theta=25.0,
+                 insert=Offcenter, mode=RUPPERT, **options):
+        P, T, N, S = _as_inputs(points, triangles, neighbors, segments)
+        self._keep = (P, T, N, S)
+        o = make_options(theta, insert, mode, **options)
+        self.h = lib().gcdt_session_create(_ptr(P), len(P), _ptr(T), _ptr(N), len(T),
+                                           _ptr(S), len(S), C.byref(o))
+        if not self.h:
+            _err(-1, "gcdt_session_create")
+
+    def run(self):
+        st = Stats()
+        rc = lib().gcdt_session_run(self.h, C.byref(st))
+        if rc != 0:
+            _err(rc, "gcdt_session_run")
+        return st.as_dict()
+
+    def download(self):
+        m = C.c_void_p()
+        rc = lib().gcdt_session_download(self.h, C.byref(m))
+        if rc != 0:
+            _err(rc, "gcdt_session_download")
+        P, T, N, S, ps, ts = _fetch_mesh(m)
+        return RefineResult(P, T, N, S, ps, ts, {})
+
+    def kernel_times(self):
+        rec = (KernelTime * 32)()
+        n = lib().gcdt_session_kernel_times(self.h, rec, 32)
+        return {rec[i].name.decode(): {"launches": rec[i].launches,
+                                       "total_ms": rec[i].total_ms,
+                                       "items": rec[i].items} for i in range(n)}
+
+    def close(self):
+        if self.h:
+            lib().gcdt_session_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_predicates(queries, pred_mode=0):
+    """Evaluate orient2d(a,b,c) and incircle(a,b,c,d) signs on the GPU for
+    an (n, 8) array of (ax, ay, bx, by, cx, cy, dx, dy)."""
+    q = np.ascontiguousarray(queries, dtype=np.float64).reshape(-1, 8)
+    o = np.empty(len(q), np.int32)
+    ic = np.empty(len(q), np.int32)
+    sc = np.zeros(3, np.int64)
+    rc = lib().gcdt_device_predicates(_ptr(q), len(q), pred_mode, _ptr(o), _ptr(ic), _ptr(sc))
+    if rc != 0:
+        _err(rc, "gcdt_device_predicates")
+    return o, ic, sc
+
+
+def generate_pslg(n, nseg, L=1000.0, maxlen=0.0, dist=0, min_angle=5.0, seed=1):
+    """Seeded synthetic PSLG: n points (dist 0 uniform in [0,L]^2, 1 Gaussian
+    clusters) and up to nseg non-crossing segments (host code in libgcdt)."""
+    P = np.empty((n, 2), np.float64)
+    S = np.empty((nseg, 2), np.int32)
+    k = lib().gcdt_generate_pslg(int(dist), int(n), int(nseg), float(L), float(maxlen),
+                                 float(min_angle), int(seed), _ptr(P), _ptr(S))
+    if k < 0:
+        _err(k, "gcdt_generate_pslg")
+    return P, S[:k].copy()
+
+
+def build_cdt(points, segments):
+    """CDT of points + segments with hull segments (Triangle's "pzQnc"
+    contract).  Returns (triangles, neighbors, segments)."""
+    P = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    S = np.ascontiguousarray(segments, dtype=np.int32).reshape(-1, 2)
+    c = C.c_void_p()
+    rc = lib().gcdt_build_cdt(_ptr(P), len(P), _ptr(S), len(S), C.byref(c))
+    if rc != 0:
+        _err(rc, "gcdt_build_cdt")
+    np_, nt, ns = C.c_int(), C.c_int(), C.c_int()
+    lib().gcdt_cdt_counts(c, C.byref(np_), C.byref(nt), C.byref(ns))
+    T = np.empty((nt.value, 3), np.int32)
+    N = np.empty((nt.value, 3), np.int32)
+    SS = np.empty((ns.value, 2), np.int32)
+    lib().gcdt_cdt_copy(c, _ptr(T), _ptr(N), _ptr(SS))
+    lib().gcdt_cdt_free(c)
+    return T, N, SS

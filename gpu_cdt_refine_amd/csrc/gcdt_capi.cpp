@@ -1,0 +1,211 @@
+// C ABI of libgcdt.so and the reference-compatible GPU_Refine_Quality.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "gcdt_internal.h"
+#include "gpu_cdt_refine.h"
+
+using gcdt::MeshHost;
+using gcdt::Refiner;
+
+static thread_local std::string g_err;
+
+struct gcdt_mesh {
+  MeshHost h;
+};
+
+struct gcdt_session {
+  Refiner *r;
+};
+
+extern "C" {
+
+const char *gcdt_last_error(void) { return g_err.c_str(); }
+
+int gcdt_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+void gcdt_default_options(gcdt_options *o) {
+  std::memset(o, 0, sizeof(*o));
+  o->theta_deg = 25.0;
+  o->offcenter = 1;
+  o->mode = 1;
+  o->cav_cap0 = 32;
+  o->cav_cap_max = 16384;
+  o->cand_budget = 1LL << 28;
+  o->max_rounds = 100000;
+  o->priority = 1;
+  o->pred_mode = 0;
+  o->profile = 0;
+  o->device = 0;
+}
+
+static bool valid_options(const gcdt_options *o) {
+  if (!o) { g_err = "options is NULL"; return false; }
+  if (!(o->theta_deg >= 0.0 && o->theta_deg < 60.0)) { g_err = "theta out of range [0, 60)"; return false; }
+  if (o->cav_cap0 < 4 || o->cav_cap_max < o->cav_cap0) { g_err = "bad cavity capacity"; return false; }
+  if (o->cand_budget < 6LL * o->cav_cap_max + 16) { g_err = "cand_budget smaller than one candidate at cav_cap_max"; return false; }
+  if (o->priority < 0 || o->priority > 2) { g_err = "priority must be 0, 1 or 2"; return false; }
+  return true;
+}
+
+int gcdt_refine_host(const double *points, int np, const int *tris,
+                     const int *nbrs, int nt, const int *segs, int ns,
+                     const gcdt_options *opt, gcdt_mesh **out,
+                     gcdt_stats *stats) {
+  g_err.clear();
+  if (!valid_options(opt)) return -1;
+  try {
+    Refiner r(*opt);
+    r.upload(points, np, tris, nbrs, nt, segs, ns);
+    r.run();
+    gcdt_mesh *m = new gcdt_mesh;
+    r.download(m->h);
+    if (stats) *stats = r.stats();
+    *out = m;
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -2;
+  }
+}
+
+void gcdt_mesh_counts(const gcdt_mesh *m, int *np, int *nt, int *ns) {
+  *np = m->h.np;
+  *nt = m->h.nt;
+  *ns = m->h.ns;
+}
+
+void gcdt_mesh_copy(const gcdt_mesh *m, double *points, int *tris, int *nbrs,
+                    int *segs, unsigned char *pstatus, unsigned char *tstatus) {
+  const MeshHost &h = m->h;
+  if (points) std::memcpy(points, h.P.data(), sizeof(double) * h.P.size());
+  if (tris) std::memcpy(tris, h.T.data(), sizeof(int) * h.T.size());
+  if (nbrs) std::memcpy(nbrs, h.N.data(), sizeof(int) * h.N.size());
+  if (segs) std::memcpy(segs, h.S.data(), sizeof(int) * h.S.size());
+  if (pstatus) std::memcpy(pstatus, h.pstat.data(), h.pstat.size());
+  if (tstatus) std::memcpy(tstatus, h.tstat.data(), h.tstat.size());
+}
+
+void gcdt_mesh_free(gcdt_mesh *m) { delete m; }
+
+gcdt_session *gcdt_session_create(const double *points, int np,
+                                  const int *tris, const int *nbrs, int nt,
+                                  const int *segs, int ns,
+                                  const gcdt_options *opt) {
+  g_err.clear();
+  if (!valid_options(opt)) return nullptr;
+  try {
+    gcdt_session *s = new gcdt_session;
+    s->r = new Refiner(*opt);
+    s->r->upload(points, np, tris, nbrs, nt, segs, ns);
+    return s;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+int gcdt_session_run(gcdt_session *s, gcdt_stats *stats) {
+  g_err.clear();
+  try {
+    s->r->run();
+    if (stats) *stats = s->r->stats();
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -2;
+  }
+}
+
+int gcdt_session_download(gcdt_session *s, gcdt_mesh **out) {
+  g_err.clear();
+  try {
+    gcdt_mesh *m = new gcdt_mesh;
+    s->r->download(m->h);
+    *out = m;
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -2;
+  }
+}
+
+int gcdt_session_kernel_times(gcdt_session *s, gcdt_kernel_time *rec,
+                              int max_rec) {
+  return s->r->kernel_times(rec, max_rec);
+}
+
+void gcdt_session_destroy(gcdt_session *s) {
+  if (!s) return;
+  delete s->r;
+  delete s;
+}
+
+int gcdt_device_predicates(const double *pts8, int n, int pred_mode,
+                           int *orient_out, int *incircle_out,
+                           long long *stage_counts) {
+  g_err.clear();
+  try {
+    gcdt::device_predicates(pts8, n, pred_mode, orient_out, incircle_out,
+                            stage_counts);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -2;
+  }
+}
+
+}  // extern "C"
+
+// Drop-in entry point with the reference's signature (refine.h:385-386).
+void GPU_Refine_Quality(struct triangulateio *input,
+                        struct triangulateio *result, double theta,
+                        InsertPolicy insertpolicy, DeletePolicy deletepolicy,
+                        int mode, int debug_iter, PStatus **ps_debug,
+                        TStatus **ts_debug) {
+  (void)deletepolicy;
+  (void)debug_iter;
+  gcdt_options opt;
+  gcdt_default_options(&opt);
+  opt.theta_deg = theta;
+  opt.offcenter = insertpolicy == Offcenter ? 1 : 0;
+  opt.mode = mode;
+  gcdt_mesh *m = nullptr;
+  int rc = gcdt_refine_host(input->pointlist, input->numberofpoints,
+                            input->trianglelist, input->neighborlist,
+                            input->numberoftriangles, input->segmentlist,
+                            input->numberofsegments, &opt, &m, nullptr);
+  if (rc != 0) {
+    std::fprintf(stderr, "GPU_Refine_Quality: %s\n", gcdt_last_error());
+    std::exit(EXIT_FAILURE);
+  }
+  const MeshHost &h = m->h;
+  result->numberofpoints = h.np;
+  result->pointlist = new double[2 * (size_t)h.np];
+  std::memcpy(result->pointlist, h.P.data(), sizeof(double) * 2 * (size_t)h.np);
+  result->numberoftriangles = h.nt;
+  result->numberofcorners = 3;
+  result->trianglelist = new int[3 * (size_t)h.nt];
+  std::memcpy(result->trianglelist, h.T.data(), sizeof(int) * 3 * (size_t)h.nt);
+  result->neighborlist = new int[3 * (size_t)h.nt];
+  std::memcpy(result->neighborlist, h.N.data(), sizeof(int) * 3 * (size_t)h.nt);
+  result->numberofsegments = h.ns;
+  result->segmentlist = new int[2 * (size_t)h.ns];
+  std::memcpy(result->segmentlist, h.S.data(), sizeof(int) * 2 * (size_t)h.ns);
+  if (ps_debug) {
+    *ps_debug = new PStatus[(size_t)h.np];
+    for (int i = 0; i < h.np; i++) (*ps_debug)[i]._status = h.pstat[i];
+  }
+  if (ts_debug) {
+    *ts_debug = new TStatus[(size_t)h.nt];
+    for (int i = 0; i < h.nt; i++) (*ts_debug)[i]._status = h.tstat[i];
+  }
+  gcdt_mesh_free(m);
+}

@@ -1,0 +1,409 @@
+// Device-side mesh model and per-element refinement steps for gfx950.
+//
+// Mesh layout in HBM (structure of arrays, one 16-byte record per triangle
+// per array so a triangle's vertices, neighbours or subsegments are one
+// aligned 128-bit load):
+//   P     double2[capP]   point coordinates
+//   pflag uint8[capP]     PF_* bits;   ppar int[capP]: parent input segment
+//   TV    int4[capT]      (v0, v1, v2, -) counter-clockwise
+//   TN    int4[capT]      neighbour across edge j as otri = tri<<2 | edge, -1
+//   TS    int4[capT]      subsegment on edge j, -1
+//   tflag uint8[capT]     TF_* bits
+//   SV    int2[capS]      subsegment endpoints (org, dest)
+//   SP    int[capS]       parent input segment;  ST int[capS]: an adjacent otri
+//   senc  int[capS]       0 free, 1 encroached, 2 given up
+//   claim uint64[capT]    epoch-tagged conflict claims (atomicMax)
+// Edge j of triangle t is (v[(j+1)%3], v[(j+2)%3]); its apex is v[j]
+// (otri encoding as the reference's encode_tri, refine.cu:95-113).
+//
+// Every step here has a line-for-line counterpart in the CPU oracle
+// (oracle/refine_oracle.c) so that both produce bit-identical meshes; the
+// comments name the reference code each step stands in for.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "predicates.h"
+
+namespace gcdt {
+
+enum : uint8_t { TF_ALIVE = 1, TF_CHECK = 2, TF_BAD = 4, TF_GIVEUP = 8 };
+enum : uint8_t { PF_INPUT = 1, PF_SEGV = 2, PF_FREE = 4 };
+enum : int {
+  ST_VALID = 0,
+  ST_REJ_ENC = 1,
+  ST_REJ_BLOCKED = 2,
+  ST_OVERFLOW = 3,
+  ST_GIVEUP = 4
+};
+enum : int {
+  CNT_MARKS = 0,
+  CNT_GIVEUPS,
+  CNT_OVERFLOW,
+  CNT_REJ_ENC,
+  CNT_REJ_BLOCKED,
+  CNT_LOST,
+  CNT_FP32,
+  CNT_FP64,
+  CNT_EXACT,
+  CNT_ERRORS,
+  CNT_N = 16
+};
+
+#define GCDT_WALK_LIMIT (1 << 22)
+
+struct MeshD {
+  double2 *P;
+  uint8_t *pflag;
+  int *ppar;
+  int4 *TV, *TN, *TS;
+  uint8_t *tflag;
+  int2 *SV;
+  int *SP, *ST, *senc;
+  const int2 *SEG;
+  unsigned long long *claim;
+};
+
+struct ParamsD {
+  double goodangle;  // cos^2(theta)
+  double lens;       // (2 cos^2(theta) - 1)^2
+  double offconst;   // 0.475 sqrt((1+cos)/(1-cos)), kept in double
+  int offcenter, ruppert, use_fp32, priority;
+};
+
+// One conflict-resolution pass over nc candidates.
+struct PassD {
+  const int *ids;
+  int nc;
+  int K;  // cavity capacity (triangles)
+  int R;  // workspace ints per candidate = 6K+16
+  unsigned long long epoch;
+  double2 *cpt;
+  int *cstatus, *cn, *cr, *cprio, *cwin, *crank;
+  int *ws;
+  double *pws;  // exact-predicate arena, GCDT_INCIRCLE_WS doubles per thread
+  unsigned long long *cnt;
+  const unsigned short *learned_prio;  // optional per-candidate priority
+};
+
+__device__ __forceinline__ int comp(const int4 &v, int j) {
+  return j == 0 ? v.x : (j == 1 ? v.y : v.z);
+}
+__device__ __forceinline__ int *compp(int4 *base, int t, int j) {
+  return reinterpret_cast<int *>(base + t) + j;
+}
+
+// checkseg4encroach (refine.cu:115-139, triangle.cpp:7138-7153).
+__device__ __forceinline__ bool encroaches(const MeshD &m, const ParamsD &pr,
+                                           int a, int b, double px,
+                                           double py) {
+  double2 A = m.P[a], B = m.P[b];
+  double dot = (A.x - px) * (B.x - px) + (A.y - py) * (B.y - py);
+  if (dot < 0.0) {
+    if (pr.ruppert) return true;
+    double la = (A.x - px) * (A.x - px) + (A.y - py) * (A.y - py);
+    double lb = (B.x - px) * (B.x - px) + (B.y - py) * (B.y - py);
+    if (dot * dot >= pr.lens * la * lb) return true;
+  }
+  return false;
+}
+
+// testtriangle + Miller-Pav-Walkington (refine.cu:376-497,
+// triangle.cpp:7220-7384); org = v1, dest = v2, apex = v0.
+__device__ __forceinline__ bool tri_is_bad(const MeshD &m, const ParamsD &pr,
+                                           int t, int4 v) {
+  double2 O = m.P[v.y], D = m.P[v.z], A = m.P[v.x];
+  double dxod = O.x - D.x, dyod = O.y - D.y;
+  double dxda = D.x - A.x, dyda = D.y - A.y;
+  double dxao = A.x - O.x, dyao = A.y - O.y;
+  double apexlen = dxod * dxod + dyod * dyod;
+  double orglen = dxda * dxda + dyda * dyda;
+  double destlen = dxao * dxao + dyao * dyao;
+  double angle;
+  int e, b1, b2;
+  if ((apexlen < orglen) && (apexlen < destlen)) {
+    angle = dxda * dxao + dyda * dyao;
+    angle = angle * angle / (orglen * destlen);
+    e = 0; b1 = v.y; b2 = v.z;
+  } else if (orglen < destlen) {
+    angle = dxod * dxao + dyod * dyao;
+    angle = angle * angle / (apexlen * destlen);
+    e = 1; b1 = v.z; b2 = v.x;
+  } else {
+    angle = dxod * dxda + dyod * dyda;
+    angle = angle * angle / (apexlen * orglen);
+    e = 2; b1 = v.x; b2 = v.y;
+  }
+  if (!(angle > pr.goodangle)) return false;
+  if ((m.pflag[b1] & PF_SEGV) && (m.pflag[b2] & PF_SEGV) &&
+      comp(m.TS[t], e) < 0) {
+    int s1 = m.ppar[b1], s2 = m.ppar[b2];
+    int2 g1 = m.SEG[s1], g2 = m.SEG[s2];
+    int common = -1;
+    if (g1.x == g2.x || g1.x == g2.y) common = g1.x;
+    else if (g1.y == g2.x || g1.y == g2.y) common = g1.y;
+    if (common >= 0) {
+      double2 J = m.P[common], B1 = m.P[b1], B2 = m.P[b2];
+      double dist1 = (B1.x - J.x) * (B1.x - J.x) + (B1.y - J.y) * (B1.y - J.y);
+      double dist2 = (B2.x - J.x) * (B2.x - J.x) + (B2.y - J.y) * (B2.y - J.y);
+      if ((dist1 < 1.001 * dist2) && (dist1 > 0.999 * dist2)) return false;
+    }
+  }
+  return true;
+}
+
+// Circumcenter / Ungor off-center (refine.cu:3780-3868,
+// triangle.cpp:6523-6618).  false if numerically degenerate.
+__device__ __forceinline__ bool steiner_point(const MeshD &m,
+                                              const ParamsD &pr, int4 v,
+                                              double &cx, double &cy) {
+  double2 A = m.P[v.x], O = m.P[v.y], D = m.P[v.z];
+  double xdo = D.x - O.x, ydo = D.y - O.y;
+  double xao = A.x - O.x, yao = A.y - O.y;
+  double dodist = xdo * xdo + ydo * ydo;
+  double aodist = xao * xao + yao * yao;
+  double dadist = (D.x - A.x) * (D.x - A.x) + (D.y - A.y) * (D.y - A.y);
+  double det = xdo * yao - xao * ydo;
+  if (det == 0.0) return false;
+  double denominator = 0.5 / det;
+  double dx = (yao * dodist - ydo * aodist) * denominator;
+  double dy = (xdo * aodist - xao * dodist) * denominator;
+  if (pr.offcenter && pr.offconst > 0.0) {
+    double dxoff, dyoff;
+    if ((dodist < aodist) && (dodist < dadist)) {
+      dxoff = 0.5 * xdo - pr.offconst * ydo;
+      dyoff = 0.5 * ydo + pr.offconst * xdo;
+      if (dxoff * dxoff + dyoff * dyoff < dx * dx + dy * dy) { dx = dxoff; dy = dyoff; }
+    } else if (aodist < dadist) {
+      dxoff = 0.5 * xao + pr.offconst * yao;
+      dyoff = 0.5 * yao - pr.offconst * xao;
+      if (dxoff * dxoff + dyoff * dyoff < dx * dx + dy * dy) { dx = dxoff; dy = dyoff; }
+    } else {
+      dxoff = 0.5 * (A.x - D.x) - pr.offconst * (A.y - D.y);
+      dyoff = 0.5 * (A.y - D.y) + pr.offconst * (A.x - D.x);
+      if (dxoff * dxoff + dyoff * dyoff <
+          (dx - xdo) * (dx - xdo) + (dy - ydo) * (dy - ydo)) {
+        dx = xdo + dxoff;
+        dy = ydo + dyoff;
+      }
+    }
+  }
+  cx = O.x + dx;
+  cy = O.y + dy;
+  if ((cx == A.x && cy == A.y) || (cx == O.x && cy == O.y) ||
+      (cx == D.x && cy == D.y))
+    return false;
+  if (!(cx == cx) || !(cy == cy)) return false;
+  return true;
+}
+
+// Half-octave bucket of a squared edge length (Triangle's queue order,
+// triangle.cpp:6932-6968): shorter edges -> smaller bucket -> higher priority.
+__device__ __forceinline__ int edge_bucket(double key) {
+  int e;
+  double f = frexp(key, &e);
+  int b = 2 * e + (f >= 0.70710678118654752440 ? 1 : 0);
+  b += 2200;
+  b = b < 0 ? 0 : (b > 8191 ? 8191 : b);
+  return b;
+}
+
+__device__ __forceinline__ int shortest_edge_bucket(const MeshD &m, int4 v) {
+  double2 A = m.P[v.x], O = m.P[v.y], D = m.P[v.z];
+  double l0 = (O.x - D.x) * (O.x - D.x) + (O.y - D.y) * (O.y - D.y);
+  double l1 = (D.x - A.x) * (D.x - A.x) + (D.y - A.y) * (D.y - A.y);
+  double l2 = (A.x - O.x) * (A.x - O.x) + (A.y - O.y) * (A.y - O.y);
+  double mn = l0 < l1 ? l0 : l1;
+  if (l2 < mn) mn = l2;
+  return edge_bucket(mn);
+}
+
+// Sides of subsegment s: (t1,j1) holds org->dest, (t2,j2) dest->org.
+__device__ __forceinline__ void seg_sides(const MeshD &m, int s, int &t1,
+                                          int &j1, int &t2, int &j2) {
+  int a = m.SV[s].x;
+  int ot = m.ST[s];
+  int t = ot >> 2, j = ot & 3;
+  int4 tv = m.TV[t];
+  int u = comp(tv, (j + 1) % 3);
+  int n = comp(m.TN[t], j);
+  int tb = -1, jb = -1;
+  if (n >= 0) { tb = n >> 2; jb = n & 3; }
+  if (u == a) { t1 = t; j1 = j; t2 = tb; j2 = jb; }
+  else { t1 = tb; j1 = jb; t2 = t; j2 = j; }
+}
+
+// Segment split point with concentric shells + one collinearity step
+// (refine.cu:2686-2725, triangle.cpp:13290-13425).
+__device__ __forceinline__ bool split_point(const MeshD &m, int s, double &px,
+                                            double &py) {
+  int2 ab = m.SV[s];
+  int t1, j1, t2, j2;
+  seg_sides(m, s, t1, j1, t2, j2);
+  bool acuteorg = false, acutedest = false;
+  if (t1 >= 0) {
+    int4 ts = m.TS[t1];
+    if (comp(ts, (j1 + 2) % 3) >= 0) acuteorg = true;
+    if (comp(ts, (j1 + 1) % 3) >= 0) acutedest = true;
+  }
+  if (t2 >= 0) {
+    int4 ts = m.TS[t2];
+    if (comp(ts, (j2 + 1) % 3) >= 0) acuteorg = true;
+    if (comp(ts, (j2 + 2) % 3) >= 0) acutedest = true;
+  }
+  double2 A = m.P[ab.x], B = m.P[ab.y];
+  double split;
+  if (acuteorg || acutedest) {
+    double len = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));
+    double npt = 1.0;
+    while (len > 3.0 * npt) npt *= 2.0;
+    while (len < 1.5 * npt) npt *= 0.5;
+    split = npt / len;
+    if (acutedest) split = 1.0 - split;
+  } else {
+    split = 0.5;
+  }
+  px = A.x + split * (B.x - A.x);
+  py = A.y + split * (B.y - A.y);
+  double mult = (A.x - px) * (B.y - py) - (A.y - py) * (B.x - px);
+  double div = (A.x - B.x) * (A.x - B.x) + (A.y - B.y) * (A.y - B.y);
+  if (mult != 0.0 && div != 0.0) {
+    mult = mult / div;
+    if (mult == mult) {
+      px += mult * (B.y - A.y);
+      py += mult * (A.x - B.x);
+    }
+  }
+  if ((px == A.x && py == A.y) || (px == B.x && py == B.y)) return false;
+  return true;
+}
+
+__device__ __forceinline__ bool in_list(const int *L, int n, int x) {
+  for (int i = 0; i < n; i++)
+    if (L[i] == x) return true;
+  return false;
+}
+
+__device__ __forceinline__ bool is_bnd(const MeshD &m, int t, int j,
+                                       int split_s, const int *L, int n) {
+  int s = comp(m.TS[t], j);
+  int nb = comp(m.TN[t], j);
+  if (s >= 0 && s != split_s) return true;
+  if (nb < 0) return true;
+  return !in_list(L, n, nb >> 2);
+}
+
+// Orientation walk toward (cx,cy), stopping at subsegments
+// (refine.cu:3908-4184).  0 inside, 1 on edge jout, 2 on a vertex,
+// 3 blocked by subsegment seg, 4 failed.
+__device__ __forceinline__ int walk(const MeshD &m, int t, double cx,
+                                    double cy, bool fp32, PredStats *st,
+                                    int &tout, int &jout, int &seg) {
+  for (int steps = 0; steps < GCDT_WALK_LIMIT; steps++) {
+    int4 v = m.TV[t];
+    double2 p0 = m.P[v.x], p1 = m.P[v.y], p2 = m.P[v.z];
+    int o0 = orient2d_sign(p1.x, p1.y, p2.x, p2.y, cx, cy, fp32, st);
+    int o1 = orient2d_sign(p2.x, p2.y, p0.x, p0.y, cx, cy, fp32, st);
+    int o2 = orient2d_sign(p0.x, p0.y, p1.x, p1.y, cx, cy, fp32, st);
+    int jn = o0 < 0 ? 0 : (o1 < 0 ? 1 : (o2 < 0 ? 2 : -1));
+    if (jn < 0) {
+      int zeros = (o0 == 0) + (o1 == 0) + (o2 == 0);
+      tout = t;
+      if (zeros == 0) return 0;
+      if (zeros == 1) {
+        jout = (o0 == 0) ? 0 : (o1 == 0) ? 1 : 2;
+        return 1;
+      }
+      return 2;
+    }
+    int s = comp(m.TS[t], jn);
+    if (s >= 0) { seg = s; tout = t; return 3; }
+    int nb = comp(m.TN[t], jn);
+    if (nb < 0) return 4;
+    t = nb >> 2;
+  }
+  return 4;
+}
+
+// Bowyer-Watson cavity of (cx,cy) grown from the seeds across non-segment
+// edges (in-circle test), then the ring across its boundary and the
+// disk / positive-star checks.  Same visiting order as the oracle.
+__device__ __forceinline__ int build_cavity(const MeshD &m, int *L, double cx,
+                                            double cy, int seed0, int seed1,
+                                            int split_s, int K, double *pws,
+                                            bool fp32, PredStats *st, int &nout,
+                                            int &rout) {
+  int n = 0;
+  L[n++] = seed0;
+  if (seed1 >= 0) L[n++] = seed1;
+  for (int idx = 0; idx < n; idx++) {
+    int t = L[idx];
+    int4 ts = m.TS[t], tn = m.TN[t];
+    for (int j = 0; j < 3; j++) {
+      if (comp(ts, j) >= 0) continue;
+      int nb = comp(tn, j);
+      if (nb < 0) continue;
+      int u = nb >> 2;
+      if (in_list(L, n, u)) continue;
+      int4 w = m.TV[u];
+      double2 a = m.P[w.x], b = m.P[w.y], c = m.P[w.z];
+      if (incircle_sign(a.x, a.y, b.x, b.y, c.x, c.y, cx, cy, pws, fp32, st) > 0) {
+        if (n == K) return ST_OVERFLOW;
+        L[n++] = u;
+      }
+    }
+  }
+  int nbnd = 0;
+  for (int idx = 0; idx < n; idx++) {
+    int t = L[idx];
+    int4 ts = m.TS[t], tn = m.TN[t], tv = m.TV[t];
+    for (int j = 0; j < 3; j++) {
+      int s = comp(ts, j);
+      int nb = comp(tn, j);
+      if (s >= 0 && s != split_s && nb >= 0 && in_list(L, n, nb >> 2))
+        return ST_GIVEUP;
+      if (is_bnd(m, t, j, split_s, L, n)) {
+        nbnd++;
+        if (!(s >= 0 && s == split_s)) {
+          double2 u = m.P[comp(tv, (j + 1) % 3)], v = m.P[comp(tv, (j + 2) % 3)];
+          if (orient2d_sign(cx, cy, u.x, u.y, v.x, v.y, fp32, st) <= 0)
+            return ST_GIVEUP;
+        }
+      }
+    }
+  }
+  if (nbnd != n + 2) return ST_GIVEUP;
+  int r = 0;
+  for (int idx = 0; idx < n; idx++) {
+    int t = L[idx];
+    int4 tn = m.TN[t];
+    for (int j = 0; j < 3; j++) {
+      if (!is_bnd(m, t, j, split_s, L, n)) continue;
+      int nb = comp(tn, j);
+      if (nb < 0) continue;
+      int u = nb >> 2;
+      if (in_list(L + n, r, u)) continue;
+      L[n + r++] = u;
+    }
+  }
+  nout = n;
+  rout = r;
+  return ST_VALID;
+}
+
+__device__ __forceinline__ void next_bnd(const MeshD &m, int t, int j,
+                                         int split_s, const int *L, int n,
+                                         int &tn, int &jn) {
+  int cur = t, k = (j + 1) % 3;
+  for (int guard = 0; guard < 4 * n + 8; guard++) {
+    if (is_bnd(m, cur, k, split_s, L, n)) { tn = cur; jn = k; return; }
+    int nb = comp(m.TN[cur], k);
+    cur = nb >> 2;
+    k = ((nb & 3) + 1) % 3;
+  }
+  tn = t;  // unreachable for a disk-shaped cavity (checked before claims)
+  jn = j;
+}
+
+}  // namespace gcdt

@@ -1,0 +1,126 @@
+// Host-side refinement driver (internal to libgcdt.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <utility>
+#include <vector>
+
+#include "gcdt_device.h"
+#include "gpu_cdt_refine.h"
+
+namespace gcdt {
+
+struct MeshHost {
+  int np = 0, nt = 0, ns = 0;
+  std::vector<double> P;
+  std::vector<int> T, N, S;
+  std::vector<unsigned char> pstat, tstat;
+};
+
+enum KernelId {
+  K_INIT = 0,
+  K_MARKBAD,
+  K_COMPACT,
+  K_SEGCAV,
+  K_BADCAV,
+  K_CLAIM,
+  K_WIN,
+  K_APPLY,
+  K_EXPORT,
+  K_NKERN
+};
+
+class Refiner;
+// Optional hook run before every bad pass: may fill learned_prio for the
+// first nc candidates of ids.
+typedef void (*LearnedHook)(Refiner *, const int *ids, int nc);
+
+class Refiner {
+ public:
+  explicit Refiner(const gcdt_options &o);
+  ~Refiner();
+  void upload(const double *points, int np, const int *tris, const int *nbrs,
+              int nt, const int *segs, int ns);
+  void run();
+  void download(MeshHost &out);
+  int kernel_times(gcdt_kernel_time *rec, int max_rec) const;
+  const gcdt_stats &stats() const { return st; }
+
+  gcdt_options opt;
+  ParamsD pr;
+  MeshD m{};
+  hipStream_t stream = nullptr;
+  int np = 0, nt = 0, ns = 0;
+  // learned-priority extension
+  LearnedHook learned_hook = nullptr;
+  void *learned_ctx = nullptr;
+  unsigned short *learned_prio = nullptr;
+
+ private:
+  template <class Pred>
+  int compact(Pred pred, int n, int *out_list, int *out_rank);
+  int run_pass(const int *ids, int nc, bool is_seg, long long *progress,
+               bool *overflowed);
+  void ensure_capacity(long long needP, long long needT, long long needS);
+  void ensure_cand(int nc);
+  void ensure_tmp(int n);
+  int cands_per_pass(int K) const;
+  void read_counters();
+  void free_mesh();
+  void tbegin(int id);
+  void tend(int id, long long items);
+  void collect_times();
+  static const char *kname(int id);
+
+  gcdt_stats st{};
+  int np0 = 0, nt0 = 0, ns0 = 0;
+  double2 *in_P = nullptr;
+  int *in_T = nullptr, *in_N = nullptr, *in_S = nullptr;
+  int2 *seg_in = nullptr;
+  long long capP = 0, capT = 0, capS = 0;
+  int K = 32;
+  unsigned long long epoch = 0;
+  // candidates
+  int c_cap = 0;
+  double2 *c_pt = nullptr;
+  int *c_status = nullptr, *c_n = nullptr, *c_r = nullptr, *c_prio = nullptr,
+      *c_win = nullptr, *c_rank = nullptr;
+  int *c_ws = nullptr;
+  long long ws_ints = 0;
+  double *c_pws = nullptr;
+  int pws_blocks = 0;
+  int *tmp_list = nullptr;
+  int tmp_list_cap = 0;
+  // compaction scratch
+  int *cb_cnt = nullptr, *cb_off = nullptr;
+  int cb_cap = 0;
+  int *d_total = nullptr;
+  unsigned long long *d_cnt = nullptr;
+  int *h_scalar = nullptr;
+  unsigned long long *h_cnt = nullptr;
+  // segment hash
+  unsigned long long *h_key = nullptr;
+  int *h_val = nullptr;
+  long long h_cap = 0;
+  // export
+  int *newid = nullptr;
+  int newid_cap = 0;
+  int nt_out = 0;
+  int *o_T = nullptr, *o_N = nullptr;
+  uint8_t *o_tstat = nullptr, *o_pstat = nullptr;
+  int o_cap = 0, o_pcap = 0;
+  hipEvent_t ev0, ev1;
+  struct KT {
+    long long launches = 0;
+    double total_ms = 0;
+    long long items = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  };
+  KT ktimes[K_NKERN];
+};
+
+void device_predicates(const double *pts8, int n, int pred_mode, int *o,
+                       int *ic, long long *stage_counts);
+
+}  // namespace gcdt

@@ -1,0 +1,1121 @@
+// MI355X (gfx950) CDT quality refinement: kernels and the host round driver.
+//
+// Round structure (reference GPU_Refine_Quality, refine.cu:6500-7139):
+//   1. segment phase: while subsegments are encroached, one conflict pass
+//      splits an independent set of them (refine.cu:3180-3778 splitEncsegs);
+//   2. bad phase: test new triangles (refine.cu:376-570), then one conflict
+//      pass inserts an independent set of Steiner points
+//      (refine.cu:5107-5547 splitTriangles).
+// A conflict pass is five launches over a dense candidate list:
+//   cavity (walk + Bowyer-Watson cavity + ring)  ->  claim (64-bit epoch-
+//   tagged atomicMax over cavity and ring)  ->  win (owner check)  ->
+//   wave64 ballot/prefix rank of the winners  ->  apply (retriangulate the
+//   cavity as a star, in place).
+// Winners own their cavity and every triangle across its boundary, so an
+// apply never races with another apply and no flip or link-repair phase
+// (refine.cu:2328-2490) is needed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gcdt_device.h"
+#include "gcdt_internal.h"
+
+namespace gcdt {
+
+// ---------------------------------------------------------------------------
+// Wave64 ballot / block-prefix stream compaction (replaces the reference's
+// thrust::copy_if / exclusive_scan calls, refine.cu:149-315, 499-570).
+// Output is in ascending index order, so the result is deterministic.
+// ---------------------------------------------------------------------------
+constexpr int CB = 256;       // threads per block
+constexpr int CI = 16;        // items per thread
+constexpr int CT = CB * CI;   // items per block tile
+
+struct PredSenc {
+  const int *senc;
+  __device__ bool operator()(int i) const { return senc[i] == 1; }
+};
+struct PredBad {
+  const uint8_t *tflag;
+  __device__ bool operator()(int i) const {
+    return (tflag[i] & (TF_ALIVE | TF_BAD | TF_GIVEUP)) == (TF_ALIVE | TF_BAD);
+  }
+};
+struct PredWin {
+  const int *cwin;
+  __device__ bool operator()(int i) const { return cwin[i] != 0; }
+};
+struct PredAlive {
+  const uint8_t *tflag;
+  __device__ bool operator()(int i) const { return (tflag[i] & TF_ALIVE) != 0; }
+};
+
+template <class Pred>
+__global__ __launch_bounds__(CB) void k_count(Pred pred, int n, int *blockcnt) {
+  __shared__ int wsum[CB / 64];
+  int base = blockIdx.x * CT;
+  int c = 0;
+  for (int k = 0; k < CI; k++) {
+    int i = base + k * CB + threadIdx.x;
+    if (i < n && pred(i)) c++;
+  }
+  // wave reduction
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < CB / 64; w++) s += wsum[w];
+    blockcnt[blockIdx.x] = s;
+  }
+}
+
+// Exclusive scan of block counts (single block); total -> *total.
+__global__ __launch_bounds__(1024) void k_scan(const int *blockcnt, int nb,
+                                               int *blockoff, int *total) {
+  __shared__ int sh[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 1024) {
+    int i = base + threadIdx.x;
+    int v = i < nb ? blockcnt[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) blockoff[i] = carry + sh[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += sh[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+template <class Pred>
+__global__ __launch_bounds__(CB) void k_scatter(Pred pred, int n,
+                                                const int *blockoff,
+                                                int *out_list, int *out_rank) {
+  __shared__ int wcnt[CB / 64];
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int base = blockIdx.x * CT;
+  int run = blockoff[blockIdx.x];
+  unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int k = 0; k < CI; k++) {
+    int i = base + k * CB + threadIdx.x;
+    bool f = (i < n) && pred(i);
+    unsigned long long b = __ballot(f);
+    if (lane == 0) wcnt[wave] = __popcll(b);
+    __syncthreads();
+    int pre = 0, tot = 0;
+    for (int w = 0; w < CB / 64; w++) {
+      int c = wcnt[w];
+      if (w < wave) pre += c;
+      tot += c;
+    }
+    if (f) {
+      int pos = run + pre + __popcll(b & lt);
+      if (out_list) out_list[pos] = i;
+      if (out_rank) out_rank[i] = pos;
+    }
+    run += tot;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Initialisation kernels
+// ---------------------------------------------------------------------------
+__global__ void k_init_tris(const int *tris, const int *nbrs, int nt,
+                            int4 *TV, int4 *TN, int4 *TS, uint8_t *tflag,
+                            unsigned long long *claim, unsigned long long *cnt) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  int v0 = tris[3 * t], v1 = tris[3 * t + 1], v2 = tris[3 * t + 2];
+  TV[t] = make_int4(v0, v1, v2, 0);
+  int code[3];
+  int vv[3] = {v0, v1, v2};
+  for (int j = 0; j < 3; j++) {
+    int u = nbrs[3 * t + j];
+    code[j] = -1;
+    if (u >= 0) {
+      int a = vv[(j + 1) % 3], b = vv[(j + 2) % 3];
+      for (int k = 0; k < 3; k++)
+        if (tris[3 * u + (k + 1) % 3] == b && tris[3 * u + (k + 2) % 3] == a)
+          code[j] = (u << 2) | k;
+      if (code[j] < 0) atomicAdd(&cnt[CNT_ERRORS], 1ull);
+    }
+  }
+  TN[t] = make_int4(code[0], code[1], code[2], 0);
+  TS[t] = make_int4(-1, -1, -1, 0);
+  tflag[t] = TF_ALIVE | TF_CHECK;
+  claim[t] = 0ull;
+}
+
+__device__ __forceinline__ unsigned long long edge_key(int a, int b) {
+  unsigned long long lo = (unsigned)(a < b ? a : b), hi = (unsigned)(a < b ? b : a);
+  return (lo << 32) | hi;
+}
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+__global__ void k_init_segs(const int *segs, int ns, int2 *SEG, int2 *SV,
+                            int *SP, int *ST, int *senc,
+                            unsigned long long *hkey, int *hval,
+                            unsigned long long hmask) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  int a = segs[2 * s], b = segs[2 * s + 1];
+  SEG[s] = make_int2(a, b);
+  SV[s] = make_int2(a, b);
+  SP[s] = s;
+  ST[s] = -1;
+  senc[s] = 0;
+  unsigned long long key = edge_key(a, b);
+  unsigned long long h = mix64(key) & hmask;
+  for (;;) {
+    unsigned long long prev = atomicCAS(&hkey[h], ~0ull, key);
+    if (prev == ~0ull || prev == key) {
+      atomicMin(&hval[h], s);
+      break;
+    }
+    h = (h + 1) & hmask;
+  }
+}
+
+__global__ void k_attach_segs(MeshD m, int nt, const unsigned long long *hkey,
+                              const int *hval, unsigned long long hmask) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  int4 tv = m.TV[t];
+  for (int j = 0; j < 3; j++) {
+    int a = comp(tv, (j + 1) % 3), b = comp(tv, (j + 2) % 3);
+    unsigned long long key = edge_key(a, b);
+    unsigned long long h = mix64(key) & hmask;
+    while (hkey[h] != ~0ull) {
+      if (hkey[h] == key) {
+        int s = hval[h];
+        *compp(m.TS, t, j) = s;
+        int code = (t << 2) | j;
+        if (a == m.SEG[s].x) atomicExch(&m.ST[s], code);
+        else atomicCAS(&m.ST[s], -1, code);
+        break;
+      }
+      h = (h + 1) & hmask;
+    }
+  }
+}
+
+__global__ void k_init_enc(MeshD m, ParamsD pr, int nt) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  int4 ts = m.TS[t];
+  int4 tv = m.TV[t];
+  for (int j = 0; j < 3; j++) {
+    int s = comp(ts, j);
+    if (s < 0) continue;
+    double2 p = m.P[comp(tv, j)];
+    int2 sv = m.SV[s];
+    if (encroaches(m, pr, sv.x, sv.y, p.x, p.y)) m.senc[s] = 1;
+  }
+}
+
+__global__ void k_init_points(int np, uint8_t *pflag, int *ppar) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  pflag[i] = PF_INPUT;
+  ppar[i] = -1;
+}
+
+__global__ void k_check_segs(const int *ST, int ns, unsigned long long *cnt) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < ns && ST[s] < 0) atomicAdd(&cnt[CNT_ERRORS], 1ull);
+}
+
+// ---------------------------------------------------------------------------
+// Bad-triangle marking (refine.cu:376-497 kernelMarkBadTriangles)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mark_bad(MeshD m, ParamsD pr, int nt) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  uint8_t f = m.tflag[t];
+  if (!(f & TF_ALIVE) || !(f & TF_CHECK)) return;
+  f &= (uint8_t)~TF_CHECK;
+  if (tri_is_bad(m, pr, t, m.TV[t])) f |= TF_BAD;
+  else f &= (uint8_t)~TF_BAD;
+  m.tflag[t] = f;
+}
+
+// ---------------------------------------------------------------------------
+// Conflict-pass kernels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void flush_stats(unsigned long long *cnt,
+                                            const PredStats &st) {
+  if (st.fp32_hits) atomicAdd(&cnt[CNT_FP32], st.fp32_hits);
+  if (st.fp64_hits) atomicAdd(&cnt[CNT_FP64], st.fp64_hits);
+  if (st.exact_calls) atomicAdd(&cnt[CNT_EXACT], st.exact_calls);
+}
+
+// Segment pass: split point + cavity on both sides of the subsegment
+// (refine.cu:2544-2927 semantics for choosing and placing the split).
+__global__ __launch_bounds__(256) void k_seg_cavity(MeshD m, ParamsD pr,
+                                                    PassD ps, int Kmax) {
+  PredStats st = {0, 0, 0};
+  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
+  for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x) {
+    int s = ps.ids[i];
+    int *L = ps.ws + (size_t)i * ps.R;
+    int status;
+    double cx = 0.0, cy = 0.0;
+    int n = 0, r = 0;
+    if (!split_point(m, s, cx, cy)) {
+      status = ST_GIVEUP;
+    } else {
+      int t1, j1, t2, j2;
+      seg_sides(m, s, t1, j1, t2, j2);
+      int seed0 = t1 >= 0 ? t1 : t2;
+      int seed1 = t1 >= 0 ? t2 : -1;
+      status = build_cavity(m, L, cx, cy, seed0, seed1, s, ps.K, pws,
+                            pr.use_fp32, &st, n, r);
+      if (status == ST_OVERFLOW && ps.K >= Kmax) status = ST_GIVEUP;
+    }
+    ps.cpt[i] = make_double2(cx, cy);
+    ps.cstatus[i] = status;
+    ps.cn[i] = n;
+    ps.cr[i] = r;
+    ps.cprio[i] = 0;
+    if (status == ST_OVERFLOW) atomicAdd(&ps.cnt[CNT_OVERFLOW], 1ull);
+    if (status == ST_GIVEUP) {
+      atomicAdd(&ps.cnt[CNT_GIVEUPS], 1ull);
+      m.senc[s] = 2;
+    }
+  }
+  flush_stats(ps.cnt, st);
+}
+
+__device__ __forceinline__ void mark_enc(const MeshD &m, int s,
+                                         unsigned long long *cnt) {
+  if (atomicCAS(&m.senc[s], 0, 1) == 0) atomicAdd(&cnt[CNT_MARKS], 1ull);
+}
+
+// Bad pass: Steiner point, walk, cavity, encroachment rule
+// (refine.cu:3870-4599, 5061-5089 semantics).
+__global__ __launch_bounds__(256) void k_bad_cavity(MeshD m, ParamsD pr,
+                                                    PassD ps, int Kmax) {
+  PredStats st = {0, 0, 0};
+  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
+  bool fp32 = pr.use_fp32 != 0;
+  for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x) {
+    int t0 = ps.ids[i];
+    int *L = ps.ws + (size_t)i * ps.R;
+    int4 v0 = m.TV[t0];
+    int prio = 0;
+    if (pr.priority == 1) prio = shortest_edge_bucket(m, v0);
+    else if (pr.priority == 2 && ps.learned_prio) prio = ps.learned_prio[i];
+    int status = ST_VALID;
+    double cx = 0.0, cy = 0.0;
+    int n = 0, r = 0;
+    if (!steiner_point(m, pr, v0, cx, cy)) {
+      status = ST_GIVEUP;
+    } else {
+      int tl = -1, jl = -1, sg = -1;
+      int w = walk(m, t0, cx, cy, fp32, &st, tl, jl, sg);
+      int seed1 = -1;
+      if (w == 3) {
+        status = ST_REJ_BLOCKED;
+        mark_enc(m, sg, ps.cnt);
+      } else if (w == 2 || w == 4) {
+        status = ST_GIVEUP;
+      } else if (w == 1) {
+        int s = comp(m.TS[tl], jl);
+        if (s >= 0) {
+          status = ST_REJ_BLOCKED;
+          mark_enc(m, s, ps.cnt);
+        } else {
+          int nb = comp(m.TN[tl], jl);
+          if (nb < 0) status = ST_GIVEUP;
+          else seed1 = nb >> 2;
+        }
+      }
+      if (status == ST_VALID) {
+        status = build_cavity(m, L, cx, cy, tl, seed1, -1, ps.K, pws, fp32,
+                              &st, n, r);
+        if (status == ST_OVERFLOW && ps.K >= Kmax) status = ST_GIVEUP;
+      }
+      if (status == ST_VALID) {
+        bool enc = false;
+        for (int idx = 0; idx < n; idx++) {
+          int t = L[idx];
+          int4 ts = m.TS[t];
+          for (int j = 0; j < 3; j++) {
+            int s = comp(ts, j);
+            if (s < 0) continue;
+            int2 sv = m.SV[s];
+            if (encroaches(m, pr, sv.x, sv.y, cx, cy)) {
+              mark_enc(m, s, ps.cnt);
+              enc = true;
+            }
+          }
+        }
+        if (enc) status = ST_REJ_ENC;
+      }
+    }
+    ps.cpt[i] = make_double2(cx, cy);
+    ps.cstatus[i] = status;
+    ps.cn[i] = n;
+    ps.cr[i] = r;
+    ps.cprio[i] = prio;
+    if (status == ST_OVERFLOW) atomicAdd(&ps.cnt[CNT_OVERFLOW], 1ull);
+    else if (status == ST_REJ_ENC) atomicAdd(&ps.cnt[CNT_REJ_ENC], 1ull);
+    else if (status == ST_REJ_BLOCKED) atomicAdd(&ps.cnt[CNT_REJ_BLOCKED], 1ull);
+    else if (status == ST_GIVEUP) {
+      atomicAdd(&ps.cnt[CNT_GIVEUPS], 1ull);
+      m.tflag[t0] |= TF_GIVEUP;
+    }
+  }
+  flush_stats(ps.cnt, st);
+}
+
+// Claim key: higher wins.  Epoch in the top 19 bits makes older claims lose
+// without a reset pass; then (8191 - priority) and the inverted candidate
+// index, so the smallest (priority, index) wins -- the reference's
+// atomicMin(trimarker, index) rule (refine.cu:4159-4172) with a priority.
+__device__ __forceinline__ unsigned long long claim_key(unsigned long long epoch,
+                                                        int prio, int i) {
+  return (epoch << 45) | ((unsigned long long)(8191 - prio) << 32) |
+         (unsigned long long)(0xffffffffu - (unsigned)i);
+}
+
+__global__ __launch_bounds__(256) void k_claim(MeshD m, PassD ps) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ps.nc;
+       i += gridDim.x * blockDim.x) {
+    if (ps.cstatus[i] != ST_VALID) continue;
+    unsigned long long key = claim_key(ps.epoch, ps.cprio[i], i);
+    const int *L = ps.ws + (size_t)i * ps.R;
+    int k = ps.cn[i] + ps.cr[i];
+    for (int q = 0; q < k; q++) atomicMax(&m.claim[L[q]], key);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_win(MeshD m, PassD ps) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ps.nc;
+       i += gridDim.x * blockDim.x) {
+    int w = 0;
+    if (ps.cstatus[i] == ST_VALID) {
+      unsigned long long key = claim_key(ps.epoch, ps.cprio[i], i);
+      const int *L = ps.ws + (size_t)i * ps.R;
+      int k = ps.cn[i] + ps.cr[i];
+      w = 1;
+      for (int q = 0; q < k && w; q++)
+        if (m.claim[L[q]] != key) w = 0;
+      if (!w) atomicAdd(&ps.cnt[CNT_LOST], 1ull);
+    }
+    ps.cwin[i] = w;
+  }
+}
+
+// Retriangulate a winner's cavity as the star of its new vertex; same star
+// order and slot rule as the oracle.  Also writes the new point, and for a
+// segment split the two subsegment halves.
+__global__ __launch_bounds__(256) void k_apply(MeshD m, ParamsD pr, PassD ps,
+                                               int pbase, int tbase, int sbase,
+                                               int is_seg) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ps.nc;
+       i += gridDim.x * blockDim.x) {
+    if (!ps.cwin[i]) continue;
+    int rank = ps.crank[i];
+    int p = pbase + rank;
+    int tb = tbase + 2 * rank;
+    int id = ps.ids[i];
+    int split_s = is_seg ? id : -1;
+    double2 c = ps.cpt[i];
+    const int *L = ps.ws + (size_t)i * ps.R;
+    int n = ps.cn[i];
+    int k = n + 2;
+    int *Bu = ps.ws + (size_t)i * ps.R + (2 * ps.K + 2);
+    int *Bv = Bu + (ps.K + 2);
+    int *Bo = Bv + (ps.K + 2);
+    int *Bs = Bo + (ps.K + 2);
+    int t0 = -1, j0 = -1;
+    bool open = false;
+    int a = -1, b = -1, par = -1;
+    if (is_seg) {
+      int t1, j1, t2, j2;
+      seg_sides(m, split_s, t1, j1, t2, j2);
+      if (t1 < 0 || t2 < 0) {
+        open = true;
+        int ts = t1 >= 0 ? t1 : t2, js = t1 >= 0 ? j1 : j2;
+        next_bnd(m, ts, js, split_s, L, n, t0, j0);
+      }
+      int2 sv = m.SV[split_s];
+      a = sv.x;
+      b = sv.y;
+      par = m.SP[split_s];
+    }
+    if (!open) {
+      int best = -1;
+      for (int idx = 0; idx < n; idx++) {
+        int t = L[idx];
+        int4 tv = m.TV[t];
+        for (int j = 0; j < 3; j++) {
+          if (!is_bnd(m, t, j, split_s, L, n)) continue;
+          int u = comp(tv, (j + 1) % 3);
+          if (best < 0 || u < best) { best = u; t0 = t; j0 = j; }
+        }
+      }
+    }
+    int nstar = open ? k - 1 : k;
+    {
+      int t = t0, j = j0;
+      for (int q = 0; q < nstar; q++) {
+        int4 tv = m.TV[t];
+        Bu[q] = comp(tv, (j + 1) % 3);
+        Bv[q] = comp(tv, (j + 2) % 3);
+        Bo[q] = comp(m.TN[t], j);
+        Bs[q] = comp(m.TS[t], j);
+        int tn, jn;
+        next_bnd(m, t, j, split_s, L, n, tn, jn);
+        t = tn;
+        j = jn;
+      }
+    }
+    // point
+    m.P[p] = c;
+    m.pflag[p] = is_seg ? PF_SEGV : PF_FREE;
+    m.ppar[p] = is_seg ? par : -1;
+    int sA = split_s, sB = is_seg ? sbase + rank : -1;
+    for (int q = 0; q < nstar; q++) {
+      int st = q < n ? L[q] : tb + (q - n);
+      int nx = (q + 1 < nstar) ? q + 1 : (open ? -1 : 0);
+      int pv = (q > 0) ? q - 1 : (open ? -1 : nstar - 1);
+      int snx = nx < 0 ? -1 : (nx < n ? L[nx] : tb + (nx - n));
+      int spv = pv < 0 ? -1 : (pv < n ? L[pv] : tb + (pv - n));
+      m.TV[st] = make_int4(p, Bu[q], Bv[q], 0);
+      m.TN[st] = make_int4(Bo[q], snx >= 0 ? (snx << 2) | 2 : -1,
+                           spv >= 0 ? (spv << 2) | 1 : -1, 0);
+      m.TS[st] = make_int4(Bs[q], (Bv[q] == a) ? sA : (Bv[q] == b) ? sB : -1,
+                           (Bu[q] == a) ? sA : (Bu[q] == b) ? sB : -1, 0);
+      m.tflag[st] = TF_ALIVE | TF_CHECK;
+    }
+    if (open) {
+      int st = tb + 1;
+      m.TV[st] = make_int4(-1, -1, -1, 0);
+      m.TN[st] = make_int4(-1, -1, -1, 0);
+      m.TS[st] = make_int4(-1, -1, -1, 0);
+      m.tflag[st] = 0;
+    }
+    for (int q = 0; q < nstar; q++) {
+      int st = q < n ? L[q] : tb + (q - n);
+      int o = Bo[q];
+      if (o >= 0) *compp(m.TN, o >> 2, o & 3) = (st << 2) | 0;
+      int s = Bs[q];
+      if (s >= 0) {
+        m.ST[s] = (st << 2) | 0;
+        if (is_seg && m.senc[s] == 0) {
+          int2 sv = m.SV[s];
+          if (encroaches(m, pr, sv.x, sv.y, c.x, c.y)) m.senc[s] = 1;
+        }
+      }
+    }
+    if (is_seg) {
+      m.SV[sA] = make_int2(a, p);
+      m.SV[sB] = make_int2(p, b);
+      m.SP[sB] = par;
+      int stA = -1, stB = -1;
+      bool encA = false, encB = false;
+      for (int q = 0; q < nstar; q++) {
+        int st = q < n ? L[q] : tb + (q - n);
+        int u = Bu[q], v = Bv[q];
+        if (u == a) {
+          if (stA < 0) stA = (st << 2) | 2;
+          double2 pv = m.P[v];
+          encA |= encroaches(m, pr, a, p, pv.x, pv.y);
+        }
+        if (v == a) {
+          if (stA < 0) stA = (st << 2) | 1;
+          double2 pu = m.P[u];
+          encA |= encroaches(m, pr, a, p, pu.x, pu.y);
+        }
+        if (u == b) {
+          if (stB < 0) stB = (st << 2) | 2;
+          double2 pv = m.P[v];
+          encB |= encroaches(m, pr, p, b, pv.x, pv.y);
+        }
+        if (v == b) {
+          if (stB < 0) stB = (st << 2) | 1;
+          double2 pu = m.P[u];
+          encB |= encroaches(m, pr, p, b, pu.x, pu.y);
+        }
+      }
+      m.ST[sA] = stA;
+      m.ST[sB] = stB;
+      m.senc[sA] = encA ? 1 : 0;
+      m.senc[sB] = encB ? 1 : 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Export (refine.cu:5908-6484 compaction + format conversion)
+// ---------------------------------------------------------------------------
+__global__ void k_export(MeshD m, int nt, const int *newid, int *otri,
+                         int *onbr, uint8_t *otstat) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  uint8_t f = m.tflag[t];
+  if (!(f & TF_ALIVE)) return;
+  int q = newid[t];
+  int4 tv = m.TV[t], tn = m.TN[t];
+  otri[3 * q] = tv.x;
+  otri[3 * q + 1] = tv.y;
+  otri[3 * q + 2] = tv.z;
+  onbr[3 * q] = tn.x < 0 ? -1 : newid[tn.x >> 2];
+  onbr[3 * q + 1] = tn.y < 0 ? -1 : newid[tn.y >> 2];
+  onbr[3 * q + 2] = tn.z < 0 ? -1 : newid[tn.z >> 2];
+  // TStatus bits: 0 non-null, 1 bad, 2 check
+  otstat[q] = (uint8_t)(1 | ((f & TF_BAD) ? 2 : 0) | ((f & TF_CHECK) ? 4 : 0));
+}
+
+__global__ void k_pstatus(const uint8_t *pflag, int np, uint8_t *out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  uint8_t f = pflag[i];
+  // PStatus bits: 1 Steiner, 2 segment split
+  out[i] = (uint8_t)(((f & PF_INPUT) ? 0 : 2) | ((f & PF_SEGV) ? 4 : 0));
+}
+
+// Device predicate evaluation for tests.
+__global__ void k_predicates(const double *q, int n, int pred_mode, int *o,
+                             int *ic, double *pws, unsigned long long *cnt) {
+  PredStats st = {0, 0, 0};
+  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  double *ws = pws + (size_t)gtid * GCDT_INCIRCLE_WS;
+  for (int i = gtid; i < n; i += gridDim.x * blockDim.x) {
+    const double *p = q + 8 * (size_t)i;
+    o[i] = orient2d_sign(p[0], p[1], p[2], p[3], p[4], p[5], pred_mode == 1, &st);
+    ic[i] = incircle_sign(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], ws,
+                          pred_mode == 1, &st);
+  }
+  flush_stats(cnt, st);
+}
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+static void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HC(x) hip_check((x), #x)
+
+template <class T>
+static void dalloc(T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  HC(hipMalloc((void **)p, n * sizeof(T)));
+}
+template <class T>
+static void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+template <class T>
+static void dgrow(T **p, size_t oldn, size_t newn, hipStream_t s) {
+  T *q;
+  dalloc(&q, newn);
+  if (*p && oldn) HC(hipMemcpyAsync(q, *p, oldn * sizeof(T), hipMemcpyDeviceToDevice, s));
+  HC(hipStreamSynchronize(s));
+  dfree(*p);
+  *p = q;
+}
+
+static int nblk(long long n, int b = 256) { return (int)((n + b - 1) / b); }
+
+Refiner::Refiner(const gcdt_options &o) : opt(o) {
+  HC(hipSetDevice(opt.device));
+  HC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HC(hipHostMalloc((void **)&h_scalar, 64 * sizeof(int), hipHostMallocDefault));
+  HC(hipHostMalloc((void **)&h_cnt, CNT_N * sizeof(unsigned long long), hipHostMallocDefault));
+  HC(hipEventCreate(&ev0));
+  HC(hipEventCreate(&ev1));
+  dalloc(&d_total, 4);
+  dalloc(&d_cnt, CNT_N);
+  double ga = std::cos(opt.theta_deg * 3.141592653589793238462643383279502884197169399375105820974944592308 / 180.0);
+  pr.offconst = (ga == 1.0) ? 0.0 : 0.475 * std::sqrt((1.0 + ga) / (1.0 - ga));
+  pr.goodangle = ga * ga;
+  pr.lens = (2.0 * pr.goodangle - 1.0) * (2.0 * pr.goodangle - 1.0);
+  pr.offcenter = opt.offcenter;
+  pr.ruppert = opt.mode == 1;
+  pr.use_fp32 = opt.pred_mode == 1;
+  pr.priority = opt.priority;
+}
+
+Refiner::~Refiner() {
+  free_mesh();
+  dfree(in_P); dfree(in_T); dfree(in_N); dfree(in_S);
+  dfree(c_pt); dfree(c_status); dfree(c_n); dfree(c_r);
+  dfree(c_prio); dfree(c_win); dfree(c_rank); dfree(c_ws); dfree(c_pws);
+  dfree(tmp_list);
+  dfree(cb_cnt); dfree(cb_off); dfree(d_total); dfree(d_cnt);
+  dfree(o_T); dfree(o_N); dfree(o_tstat); dfree(o_pstat); dfree(newid);
+  dfree(h_key); dfree(h_val);
+  if (h_scalar) (void)hipHostFree(h_scalar);
+  if (h_cnt) (void)hipHostFree(h_cnt);
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  for (auto &kt : ktimes) {
+    for (auto &e : kt.pending) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+  }
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Refiner::free_mesh() {
+  dfree(m.P); dfree(m.pflag); dfree(m.ppar);
+  dfree(m.TV); dfree(m.TN); dfree(m.TS); dfree(m.tflag); dfree(m.claim);
+  dfree(m.SV); dfree(m.SP); dfree(m.ST); dfree(m.senc);
+  dfree(seg_in);
+  capP = capT = capS = 0;
+}
+
+void Refiner::upload(const double *points, int np, const int *tris,
+                     const int *nbrs, int nt, const int *segs, int ns) {
+  if (np < 3 || nt < 1) throw std::runtime_error("input mesh is empty");
+  if ((long long)nt * 4 >= (1LL << 31) / 2)
+    throw std::runtime_error("too many triangles for 32-bit otri encoding");
+  // Host-side validation: every index the init kernels dereference.
+  for (long long i = 0; i < 3LL * nt; i++) {
+    if (tris[i] < 0 || tris[i] >= np)
+      throw std::runtime_error("triangle vertex index out of range");
+    if (nbrs[i] < -1 || nbrs[i] >= nt)
+      throw std::runtime_error("neighbour index out of range");
+  }
+  for (long long i = 0; i < 2LL * ns; i++)
+    if (segs[i] < 0 || segs[i] >= np)
+      throw std::runtime_error("segment vertex index out of range");
+  np0 = np; nt0 = nt; ns0 = ns;
+  dfree(in_P); dfree(in_T); dfree(in_N); dfree(in_S);
+  dalloc(&in_P, (size_t)np);
+  dalloc(&in_T, 3 * (size_t)nt);
+  dalloc(&in_N, 3 * (size_t)nt);
+  dalloc(&in_S, 2 * (size_t)(ns > 0 ? ns : 1));
+  HC(hipMemcpyAsync(in_P, points, sizeof(double2) * np, hipMemcpyHostToDevice, stream));
+  HC(hipMemcpyAsync(in_T, tris, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, stream));
+  HC(hipMemcpyAsync(in_N, nbrs, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, stream));
+  if (ns) HC(hipMemcpyAsync(in_S, segs, sizeof(int) * 2 * ns, hipMemcpyHostToDevice, stream));
+  HC(hipStreamSynchronize(stream));
+}
+
+void Refiner::ensure_capacity(long long needP, long long needT, long long needS) {
+  if (needP > capP) {
+    long long c = std::max<long long>(needP, capP + capP / 2);
+    dgrow(&m.P, (size_t)np, (size_t)c, stream);
+    dgrow(&m.pflag, (size_t)np, (size_t)c, stream);
+    dgrow(&m.ppar, (size_t)np, (size_t)c, stream);
+    capP = c;
+  }
+  if (needT > capT) {
+    long long c = std::max<long long>(needT, capT + capT / 2);
+    dgrow(&m.TV, (size_t)nt, (size_t)c, stream);
+    dgrow(&m.TN, (size_t)nt, (size_t)c, stream);
+    dgrow(&m.TS, (size_t)nt, (size_t)c, stream);
+    dgrow(&m.tflag, (size_t)nt, (size_t)c, stream);
+    unsigned long long *nc;
+    dalloc(&nc, (size_t)c);
+    HC(hipMemsetAsync(nc, 0, sizeof(unsigned long long) * c, stream));
+    if (nt) HC(hipMemcpyAsync(nc, m.claim, sizeof(unsigned long long) * nt, hipMemcpyDeviceToDevice, stream));
+    HC(hipStreamSynchronize(stream));
+    dfree(m.claim);
+    m.claim = nc;
+    capT = c;
+  }
+  if (needS > capS) {
+    long long c = std::max<long long>(needS, capS + capS / 2);
+    dgrow(&m.SV, (size_t)ns, (size_t)c, stream);
+    dgrow(&m.SP, (size_t)ns, (size_t)c, stream);
+    dgrow(&m.ST, (size_t)ns, (size_t)c, stream);
+    dgrow(&m.senc, (size_t)ns, (size_t)c, stream);
+    capS = c;
+  }
+}
+
+// Kernel-time bookkeeping (options.profile): one event pair per launch.
+void Refiner::tbegin(int id) {
+  if (!opt.profile) return;
+  hipEvent_t a, b;
+  HC(hipEventCreate(&a));
+  HC(hipEventCreate(&b));
+  HC(hipEventRecord(a, stream));
+  ktimes[id].pending.push_back({a, b});
+}
+void Refiner::tend(int id, long long items) {
+  if (!opt.profile) return;
+  HC(hipEventRecord(ktimes[id].pending.back().second, stream));
+  ktimes[id].launches++;
+  ktimes[id].items += items;
+}
+void Refiner::collect_times() {
+  for (auto &kt : ktimes) {
+    for (auto &e : kt.pending) {
+      float ms = 0.f;
+      HC(hipEventSynchronize(e.second));
+      HC(hipEventElapsedTime(&ms, e.first, e.second));
+      kt.total_ms += ms;
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    kt.pending.clear();
+  }
+}
+
+const char *Refiner::kname(int id) {
+  static const char *names[] = {"init",      "mark_bad", "compact",
+                                "seg_cavity", "bad_cavity", "claim",
+                                "win",        "apply",      "export"};
+  return names[id];
+}
+
+template <class Pred>
+int Refiner::compact(Pred pred, int n, int *out_list, int *out_rank) {
+  if (n <= 0) return 0;
+  int nb = (n + CT - 1) / CT;
+  if (nb > cb_cap) {
+    dfree(cb_cnt);
+    dfree(cb_off);
+    cb_cap = std::max(nb, 2 * cb_cap);
+    dalloc(&cb_cnt, (size_t)cb_cap);
+    dalloc(&cb_off, (size_t)cb_cap);
+  }
+  tbegin(K_COMPACT);
+  k_count<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_cnt);
+  k_scan<<<1, 1024, 0, stream>>>(cb_cnt, nb, cb_off, d_total);
+  k_scatter<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_off, out_list, out_rank);
+  tend(K_COMPACT, n);
+  HC(hipGetLastError());
+  HC(hipMemcpyAsync(h_scalar, d_total, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HC(hipStreamSynchronize(stream));
+  return h_scalar[0];
+}
+
+void Refiner::read_counters() {
+  HC(hipMemcpyAsync(h_cnt, d_cnt, CNT_N * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+  HC(hipStreamSynchronize(stream));
+}
+
+void Refiner::ensure_cand(int nc) {
+  if (nc <= c_cap) return;
+  int c = std::max(nc, c_cap * 2);
+  dfree(c_pt); dfree(c_status); dfree(c_n); dfree(c_r);
+  dfree(c_prio); dfree(c_win); dfree(c_rank);
+  dalloc(&c_pt, (size_t)c);
+  dalloc(&c_status, (size_t)c);
+  dalloc(&c_n, (size_t)c);
+  dalloc(&c_r, (size_t)c);
+  dalloc(&c_prio, (size_t)c);
+  dalloc(&c_win, (size_t)c);
+  dalloc(&c_rank, (size_t)c);
+  c_cap = c;
+}
+
+int Refiner::cands_per_pass(int K) const {
+  long long c = opt.cand_budget / (6LL * K + 16);
+  if (c < 1) c = 1;
+  if (c > (1LL << 30)) c = 1LL << 30;
+  return (int)c;
+}
+
+// One conflict pass; returns number of winners.
+int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
+                      bool *overflowed) {
+  PassD ps;
+  ps.ids = ids;
+  ps.nc = nc;
+  ps.K = K;
+  ps.R = 6 * K + 16;
+  if (++epoch >= (1ull << 19)) {  // epoch wrap: clear claims
+    HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
+    epoch = 1;
+  }
+  ps.epoch = epoch;
+  ps.cpt = c_pt;
+  ps.cstatus = c_status;
+  ps.cn = c_n;
+  ps.cr = c_r;
+  ps.cprio = c_prio;
+  ps.cwin = c_win;
+  ps.crank = c_rank;
+  ps.ws = c_ws;
+  ps.pws = c_pws;
+  ps.cnt = d_cnt;
+  ps.learned_prio = learned_prio;
+  HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
+  int gb = std::min(nblk(nc), pws_blocks);
+  if (is_seg) {
+    tbegin(K_SEGCAV);
+    k_seg_cavity<<<gb, 256, 0, stream>>>(m, pr, ps, opt.cav_cap_max);
+    tend(K_SEGCAV, nc);
+  } else {
+    tbegin(K_BADCAV);
+    k_bad_cavity<<<gb, 256, 0, stream>>>(m, pr, ps, opt.cav_cap_max);
+    tend(K_BADCAV, nc);
+  }
+  HC(hipGetLastError());
+  int gc = std::min(nblk(nc), 4096);
+  tbegin(K_CLAIM);
+  k_claim<<<gc, 256, 0, stream>>>(m, ps);
+  tend(K_CLAIM, nc);
+  tbegin(K_WIN);
+  k_win<<<gc, 256, 0, stream>>>(m, ps);
+  tend(K_WIN, nc);
+  HC(hipGetLastError());
+  int nwin = compact(PredWin{c_win}, nc, nullptr, c_rank);
+  ensure_capacity(np + nwin, nt + 2LL * nwin, is_seg ? ns + nwin : ns);
+  if (nwin > 0) {
+    tbegin(K_APPLY);
+    k_apply<<<gc, 256, 0, stream>>>(m, pr, ps, np, nt, ns, is_seg ? 1 : 0);
+    tend(K_APPLY, nwin);
+    HC(hipGetLastError());
+  }
+  read_counters();
+  np += nwin;
+  nt += 2 * nwin;
+  if (is_seg) ns += nwin;
+  st.rej_overflow += h_cnt[CNT_OVERFLOW];
+  st.rej_encroach += h_cnt[CNT_REJ_ENC];
+  st.rej_blocked += h_cnt[CNT_REJ_BLOCKED];
+  st.giveups += h_cnt[CNT_GIVEUPS];
+  st.lost += h_cnt[CNT_LOST];
+  st.pred_fp32 += h_cnt[CNT_FP32];
+  st.pred_fp64 += h_cnt[CNT_FP64];
+  st.pred_exact += h_cnt[CNT_EXACT];
+  if (is_seg) st.seg_splits += nwin;
+  else st.inserted += nwin;
+  *progress = nwin + (long long)h_cnt[CNT_MARKS] + (long long)h_cnt[CNT_GIVEUPS];
+  *overflowed = h_cnt[CNT_OVERFLOW] > 0;
+  return nwin;
+}
+
+void Refiner::run() {
+  memset(&st, 0, sizeof(st));
+  for (auto &kt : ktimes) { kt.launches = 0; kt.total_ms = 0; kt.items = 0; }
+  HC(hipEventRecord(ev0, stream));
+  // --- initialise working mesh from the uploaded input --------------------
+  np = np0; nt = nt0; ns = ns0;
+  long long wantP = std::max<long long>(16, 6LL * np0);
+  long long wantT = std::max<long long>(16, 6LL * nt0);
+  long long wantS = std::max<long long>(16, 8LL * ns0 + 16);
+  if (capP < wantP || capT < wantT || capS < wantS) {
+    free_mesh();
+    capP = wantP; capT = wantT; capS = wantS;
+    dalloc(&m.P, capP); dalloc(&m.pflag, capP); dalloc(&m.ppar, capP);
+    dalloc(&m.TV, capT); dalloc(&m.TN, capT); dalloc(&m.TS, capT);
+    dalloc(&m.tflag, capT); dalloc(&m.claim, capT);
+    dalloc(&m.SV, capS); dalloc(&m.SP, capS); dalloc(&m.ST, capS); dalloc(&m.senc, capS);
+    dalloc(&seg_in, (size_t)std::max(ns0, 1));
+  }
+  m.SEG = seg_in;
+  HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
+  tbegin(K_INIT);
+  HC(hipMemcpyAsync(m.P, in_P, sizeof(double2) * np0, hipMemcpyDeviceToDevice, stream));
+  k_init_points<<<nblk(np0), 256, 0, stream>>>(np0, m.pflag, m.ppar);
+  k_init_tris<<<nblk(nt0), 256, 0, stream>>>(in_T, in_N, nt0, m.TV, m.TN, m.TS, m.tflag, m.claim, d_cnt);
+  // segment hash: capacity power of two >= 2*ns + 16
+  unsigned long long hc = 1;
+  while (hc < 2ull * (unsigned long long)ns0 + 16) hc <<= 1;
+  if ((long long)hc > h_cap) {
+    dfree(h_key); dfree(h_val);
+    dalloc(&h_key, (size_t)hc);
+    dalloc(&h_val, (size_t)hc);
+    h_cap = (long long)hc;
+  }
+  HC(hipMemsetAsync(h_key, 0xff, sizeof(unsigned long long) * hc, stream));
+  HC(hipMemsetAsync(h_val, 0x7f, sizeof(int) * hc, stream));
+  if (ns0 > 0) {
+    k_init_segs<<<nblk(ns0), 256, 0, stream>>>(in_S, ns0, seg_in, m.SV, m.SP, m.ST, m.senc, h_key, h_val, hc - 1);
+    k_attach_segs<<<nblk(nt0), 256, 0, stream>>>(m, nt0, h_key, h_val, hc - 1);
+    k_check_segs<<<nblk(ns0), 256, 0, stream>>>(m.ST, ns0, d_cnt);
+    k_init_enc<<<nblk(nt0), 256, 0, stream>>>(m, pr, nt0);
+  }
+  tend(K_INIT, nt0);
+  HC(hipGetLastError());
+  read_counters();
+  if (h_cnt[CNT_ERRORS])
+    throw std::runtime_error("input is not a valid triangulation: " +
+                             std::to_string(h_cnt[CNT_ERRORS]) +
+                             " neighbour links or segments do not match triangle edges");
+  // --- workspace ----------------------------------------------------------
+  K = opt.cav_cap0;
+  epoch = 0;
+  HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
+  if (!c_ws || ws_ints < opt.cand_budget) {
+    dfree(c_ws);
+    dalloc(&c_ws, (size_t)opt.cand_budget);
+    ws_ints = opt.cand_budget;
+  }
+  if (!c_pws) {
+    pws_blocks = 1024;
+    dalloc(&c_pws, (size_t)pws_blocks * 256 * GCDT_INCIRCLE_WS);
+  }
+  ensure_cand(std::min(cands_per_pass(K), std::max(4 * nt0, 1024)));
+  st.status = 1;
+  for (int round = 0; round < opt.max_rounds; round++) {
+    st.rounds = round + 1;
+    // segment phase
+    for (;;) {
+      ensure_tmp(ns);
+      int ne = compact(PredSenc{m.senc}, ns, tmp_list, nullptr);
+      if (ne == 0) break;
+      int nc = std::min(ne, cands_per_pass(K));
+      ensure_cand(nc);
+      long long prog = 0;
+      bool of = false;
+      run_pass(tmp_list, nc, true, &prog, &of);
+      st.seg_passes++;
+      if (of && K < opt.cav_cap_max) K = std::min(2 * K, opt.cav_cap_max);
+    }
+    // bad phase
+    tbegin(K_MARKBAD);
+    k_mark_bad<<<nblk(nt), 256, 0, stream>>>(m, pr, nt);
+    tend(K_MARKBAD, nt);
+    ensure_tmp(nt);
+    int nb = compact(PredBad{m.tflag}, nt, tmp_list, nullptr);
+    if (nb == 0) { st.status = 0; break; }
+    int nc = std::min(nb, cands_per_pass(K));
+    ensure_cand(nc);
+    if (learned_hook) learned_hook(this, tmp_list, nc);
+    long long prog = 0;
+    bool of = false;
+    run_pass(tmp_list, nc, false, &prog, &of);
+    st.bad_passes++;
+    if (of && K < opt.cav_cap_max) {
+      K = std::min(2 * K, opt.cav_cap_max);
+      prog++;
+    }
+    if (prog == 0) { st.status = 2; break; }
+  }
+  st.final_cav_cap = K;
+  // --- export --------------------------------------------------------------
+  if (nt > newid_cap) {
+    dfree(newid);
+    newid_cap = nt;
+    dalloc(&newid, (size_t)newid_cap);
+  }
+  nt_out = compact(PredAlive{m.tflag}, nt, nullptr, newid);
+  if (nt_out > o_cap) {
+    dfree(o_T); dfree(o_N); dfree(o_tstat);
+    o_cap = nt_out;
+    dalloc(&o_T, 3 * (size_t)o_cap);
+    dalloc(&o_N, 3 * (size_t)o_cap);
+    dalloc(&o_tstat, (size_t)o_cap);
+  }
+  tbegin(K_EXPORT);
+  k_export<<<nblk(nt), 256, 0, stream>>>(m, nt, newid, o_T, o_N, o_tstat);
+  tend(K_EXPORT, nt);
+  HC(hipGetLastError());
+  HC(hipEventRecord(ev1, stream));
+  HC(hipStreamSynchronize(stream));
+  float ms = 0.f;
+  HC(hipEventElapsedTime(&ms, ev0, ev1));
+  st.device_ms = ms;
+  st.n_points = np;
+  st.n_triangles = nt_out;
+  st.n_subsegs = ns;
+  collect_times();
+}
+
+void Refiner::ensure_tmp(int n) {
+  if (n <= tmp_list_cap) return;
+  dfree(tmp_list);
+  tmp_list_cap = std::max(n, 2 * tmp_list_cap);
+  dalloc(&tmp_list, (size_t)tmp_list_cap);
+}
+
+void Refiner::download(MeshHost &out) {
+  out.np = np;
+  out.nt = nt_out;
+  out.ns = ns;
+  out.P.resize(2 * (size_t)np);
+  out.T.resize(3 * (size_t)nt_out);
+  out.N.resize(3 * (size_t)nt_out);
+  out.S.resize(2 * (size_t)ns);
+  out.pstat.resize((size_t)np);
+  out.tstat.resize((size_t)nt_out);
+  if (np > o_pcap) {
+    dfree(o_pstat);
+    o_pcap = np;
+    dalloc(&o_pstat, (size_t)o_pcap);
+  }
+  k_pstatus<<<nblk(np), 256, 0, stream>>>(m.pflag, np, o_pstat);
+  HC(hipGetLastError());
+  HC(hipMemcpyAsync(out.P.data(), m.P, sizeof(double2) * np, hipMemcpyDeviceToHost, stream));
+  if (nt_out) {
+    HC(hipMemcpyAsync(out.T.data(), o_T, sizeof(int) * 3 * nt_out, hipMemcpyDeviceToHost, stream));
+    HC(hipMemcpyAsync(out.N.data(), o_N, sizeof(int) * 3 * nt_out, hipMemcpyDeviceToHost, stream));
+    HC(hipMemcpyAsync(out.tstat.data(), o_tstat, nt_out, hipMemcpyDeviceToHost, stream));
+  }
+  if (ns) HC(hipMemcpyAsync(out.S.data(), m.SV, sizeof(int2) * ns, hipMemcpyDeviceToHost, stream));
+  HC(hipMemcpyAsync(out.pstat.data(), o_pstat, np, hipMemcpyDeviceToHost, stream));
+  HC(hipStreamSynchronize(stream));
+}
+
+int Refiner::kernel_times(gcdt_kernel_time *rec, int max_rec) const {
+  int k = 0;
+  for (int id = 0; id < K_NKERN && k < max_rec; id++) {
+    std::snprintf(rec[k].name, sizeof(rec[k].name), "%s", kname(id));
+    rec[k].launches = ktimes[id].launches;
+    rec[k].total_ms = ktimes[id].total_ms;
+    rec[k].items = ktimes[id].items;
+    k++;
+  }
+  return k;
+}
+
+void device_predicates(const double *pts8, int n, int pred_mode, int *o,
+                       int *ic, long long *stage_counts) {
+  double *dq;
+  int *dout, *din;
+  double *pws;
+  unsigned long long *cnt;
+  int blocks = std::min(nblk(n), 64);
+  dalloc(&dq, 8 * (size_t)n);
+  dalloc(&dout, (size_t)n);
+  dalloc(&din, (size_t)n);
+  dalloc(&pws, (size_t)blocks * 256 * GCDT_INCIRCLE_WS);
+  dalloc(&cnt, CNT_N);
+  HC(hipMemset(cnt, 0, CNT_N * sizeof(unsigned long long)));
+  HC(hipMemcpy(dq, pts8, sizeof(double) * 8 * n, hipMemcpyHostToDevice));
+  k_predicates<<<blocks, 256>>>(dq, n, pred_mode, dout, din, pws, cnt);
+  HC(hipGetLastError());
+  HC(hipDeviceSynchronize());
+  HC(hipMemcpy(o, dout, sizeof(int) * n, hipMemcpyDeviceToHost));
+  HC(hipMemcpy(ic, din, sizeof(int) * n, hipMemcpyDeviceToHost));
+  unsigned long long h[CNT_N];
+  HC(hipMemcpy(h, cnt, sizeof(h), hipMemcpyDeviceToHost));
+  if (stage_counts) {
+    stage_counts[0] = (long long)h[CNT_FP32];
+    stage_counts[1] = (long long)h[CNT_FP64];
+    stage_counts[2] = (long long)h[CNT_EXACT];
+  }
+  dfree(dq); dfree(dout); dfree(din); dfree(pws); dfree(cnt);
+}
+
+}  // namespace gcdt

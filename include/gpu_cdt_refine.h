@@ -1,0 +1,202 @@
+/*
+ * gpu_cdt_refine.h -- MI355X (gfx950) constrained-Delaunay quality refinement.
+ *
+ * Two layers:
+ *
+ * 1. The drop-in entry point, with the reference's exact C++ signature and
+ *    record layouts (reference GPU_Refine/refine.h:385-386, README.md:11-51):
+ *
+ *      void GPU_Refine_Quality(triangulateio *input, triangulateio *result,
+ *          double theta, InsertPolicy insertpolicy, DeletePolicy deletepolicy,
+ *          int mode, int debug_iter, PStatus **ps_debug, TStatus **ts_debug);
+ *
+ *    `triangulateio` is byte-compatible with the reference's modified
+ *    Triangle record, including its trailing `numberofflips`
+ *    (GPU_Refine/triangle.h:253-284).  Define GCDT_HAVE_TRIANGULATEIO before
+ *    including this header when triangle.h is already included.
+ *
+ * 2. A C ABI (extern "C", plain pointers and sizes) that the drop-in is built
+ *    on, used by the Python bindings and by callers that keep their mesh in
+ *    device memory.
+ */
+#ifndef GPU_CDT_REFINE_H
+#define GPU_CDT_REFINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifndef GCDT_HAVE_TRIANGULATEIO
+#define GCDT_HAVE_TRIANGULATEIO
+/* Layout of the reference's triangulateio (GPU_Refine/triangle.h:253-284). */
+struct triangulateio {
+  double *pointlist;
+  double *pointattributelist;
+  int *pointmarkerlist;
+  int numberofpoints;
+  int numberofpointattributes;
+  int *trianglelist;
+  double *triangleattributelist;
+  double *trianglearealist;
+  int *neighborlist;
+  int numberoftriangles;
+  int numberofcorners;
+  int numberoftriangleattributes;
+  int *segmentlist;
+  int *segmentmarkerlist;
+  int numberofsegments;
+  double *holelist;
+  int numberofholes;
+  double *regionlist;
+  int numberofregions;
+  int *edgelist;
+  int *edgemarkerlist;
+  double *normlist;
+  int numberofedges;
+  int numberofflips;
+};
+#endif
+
+/* Enumerations of the reference (GPU_Refine/refine.h:32-33), same values. */
+typedef enum { Circumcenter = 0, Offcenter = 1 } InsertPolicy;
+typedef enum {
+  Strict = 0,
+  Connected = 1,
+  Orthogonal = 2,
+  noDeletion = 3,
+  DeleteFromDT = 4,
+  noPriority = 5
+} DeletePolicy;
+
+/* One-byte debug status records (GPU_Refine/refine.h:36-205).
+ * PStatus bits: 0 deleted, 1 Steiner, 2 segment split, 3 new, 4 redundant.
+ * TStatus bits: 0 non-null, 1 bad, 2 check, 3 flip, 4-7 flip orientation. */
+typedef struct PStatus { unsigned char _status; } PStatus;
+typedef struct TStatus { unsigned char _status; } TStatus;
+
+#ifdef __cplusplus
+/* Drop-in replacement for the reference's GPU_Refine_Quality
+ * (GPU_Refine/refine.h:385-386, refine.cu:6500-7139).  C++ linkage, like the
+ * reference.  `input` is a CDT in Triangle's convention (CCW triangles,
+ * neighbour j opposite corner j, every convex-hull edge in segmentlist).
+ * `result` receives pointlist/trianglelist/neighborlist/segmentlist
+ * allocated with new[] (free with delete[]) and the four counts.
+ * `deletepolicy` and `debug_iter` are accepted and ignored, as in the
+ * reference.  On a device error the function prints it and exits, as the
+ * reference's checkCUDAError does (refine.cu:20-29). */
+void GPU_Refine_Quality(struct triangulateio *input,
+                        struct triangulateio *result, double theta,
+                        InsertPolicy insertpolicy, DeletePolicy deletepolicy,
+                        int mode, int debug_iter, PStatus **ps_debug,
+                        TStatus **ts_debug);
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Options of the refinement.  gcdt_default_options() fills the defaults. */
+typedef struct gcdt_options {
+  double theta_deg;      /* minimum angle in degrees (refine.cu:6500 `theta`) */
+  int offcenter;         /* 1 = off-center (Offcenter), 0 = circumcenter */
+  int mode;              /* 1 = Ruppert (diametral circle), else Chew (lens) */
+  int cav_cap0;          /* initial cavity capacity K (triangles), default 32 */
+  int cav_cap_max;       /* largest cavity capacity, default 16384 */
+  long long cand_budget; /* cavity workspace in ints; candidates per pass =
+                            cand_budget / (6K + 16); default 2^28 */
+  int max_rounds;        /* refinement round cap, default 100000 */
+  int priority;          /* 1 = shortest-edge bucket then index (default);
+                            0 = lowest index only (refine.cu:4159);
+                            2 = learned priority (gcdt_set_priority_model) */
+  int pred_mode;         /* 0 = FP64 filter + exact; 1 = FP32 semi-static,
+                            then FP64, then exact */
+  int profile;           /* 1 = time every kernel launch with HIP events */
+  int device;            /* HIP device ordinal */
+} gcdt_options;
+
+typedef struct gcdt_stats {
+  int rounds, seg_passes, bad_passes;
+  long long inserted, seg_splits, rej_encroach, rej_blocked, rej_overflow,
+      lost, giveups;
+  int final_cav_cap;
+  int status;            /* 0 converged, 1 round cap, 2 stalled */
+  double device_ms;      /* HIP-event time of the refinement on the device */
+  long long pred_fp32, pred_fp64, pred_exact; /* predicate stage hits */
+  long long n_points, n_triangles, n_subsegs;
+} gcdt_stats;
+
+typedef struct gcdt_mesh gcdt_mesh;       /* host-side result */
+typedef struct gcdt_session gcdt_session; /* device-resident input + state */
+
+/* Per-kernel timing record (filled when options.profile = 1). */
+typedef struct gcdt_kernel_time {
+  char name[32];
+  long long launches;
+  double total_ms;
+  long long items; /* work items processed by those launches */
+} gcdt_kernel_time;
+
+void gcdt_default_options(gcdt_options *opt);
+
+/* Refine a CDT given in host memory; returns 0 on success.  On success
+ * *out holds the refined mesh (free with gcdt_mesh_free). */
+int gcdt_refine_host(const double *points, int np, const int *tris,
+                     const int *nbrs, int nt, const int *segs, int ns,
+                     const gcdt_options *opt, gcdt_mesh **out,
+                     gcdt_stats *stats);
+
+void gcdt_mesh_counts(const gcdt_mesh *m, int *np, int *nt, int *ns);
+/* Any output pointer may be NULL.  pstatus/tstatus use the PStatus /
+ * TStatus bit layouts. */
+void gcdt_mesh_copy(const gcdt_mesh *m, double *points, int *tris, int *nbrs,
+                    int *segs, unsigned char *pstatus, unsigned char *tstatus);
+void gcdt_mesh_free(gcdt_mesh *m);
+
+/* Device-resident sessions: upload once, refine many times (each run
+ * starts again from the uploaded CDT), download the last result.  The
+ * timed work of a run starts with the input already in HBM. */
+gcdt_session *gcdt_session_create(const double *points, int np,
+                                  const int *tris, const int *nbrs, int nt,
+                                  const int *segs, int ns,
+                                  const gcdt_options *opt);
+int gcdt_session_run(gcdt_session *s, gcdt_stats *stats);
+int gcdt_session_download(gcdt_session *s, gcdt_mesh **out);
+int gcdt_session_kernel_times(gcdt_session *s, gcdt_kernel_time *rec,
+                              int max_rec);
+void gcdt_session_destroy(gcdt_session *s);
+
+/* Last error message of this thread ("" if none). */
+const char *gcdt_last_error(void);
+
+/* Number of visible HIP devices (0 if none). */
+int gcdt_device_count(void);
+
+/* Host-side input preparation (no device use).
+ * gcdt_generate_pslg: seeded synthetic PSLG -- `dist` 0 uniform in
+ * [0,L]^2, 1 Gaussian clusters (sigma = 0.05 L); nseg random non-crossing
+ * segments between point pairs, each no longer than maxlen (<= 0: no cap),
+ * whose angle with any segment sharing an endpoint is at least
+ * min_angle_deg.  Returns the number of segments produced.
+ * gcdt_build_cdt: constrained Delaunay triangulation of points + segments
+ * with the convex hull enclosed by segments (Triangle's "pzQnc" contract),
+ * written as triangles (CCW), neighbours (opposite corner j, -1 on the
+ * boundary) and the final segment list (input segments, split where they
+ * pass through points, plus hull edges).  Returns 0 on success. */
+int gcdt_generate_pslg(int dist, int n, int nseg, double L, double maxlen,
+                       double min_angle_deg, unsigned long long seed,
+                       double *points, int *segs);
+typedef struct gcdt_cdt gcdt_cdt;
+int gcdt_build_cdt(const double *points, int np, const int *segs, int ns,
+                   gcdt_cdt **out);
+void gcdt_cdt_counts(const gcdt_cdt *c, int *np, int *nt, int *ns);
+void gcdt_cdt_copy(const gcdt_cdt *c, int *tris, int *nbrs, int *segs);
+void gcdt_cdt_free(gcdt_cdt *c);
+
+/* Exact predicates on the device, for tests: evaluates n queries. */
+int gcdt_device_predicates(const double *pts8, int n, int pred_mode,
+                           int *orient_out, int *incircle_out,
+                           long long *stage_counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,82 @@
+/*
+ * CPU oracle for the MI355X CDT quality-refinement path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (gpu_cdt_refine_amd/,
+ * include/) includes, links or calls this code; only tests/, the smoke check
+ * and bench.py's cpu_baseline leg load oracle/liboracle.so, and only as the
+ * checker.
+ *
+ * Contents:
+ *   predicates_oracle.c  exact orient2d / incircle by big-integer arithmetic
+ *   refine_oracle.c      sequential restatement of the bulk-synchronous
+ *                        refinement rounds (reference refine.cu:6500-7139 and
+ *                        Triangle's quality rules, triangle.cpp:6523-6618,
+ *                        7106-7384, 13256-13632); the GPU path must reproduce
+ *                        its output bit for bit
+ *   check_oracle.c       mesh invariant checks (reference main.cpp:79-380,
+ *                        with exact predicates and an O(T) segment lookup)
+ */
+#ifndef GCDT_ORACLE_H
+#define GCDT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int orc_orient2d(const double *a, const double *b, const double *c);
+int orc_incircle(const double *a, const double *b, const double *c,
+                 const double *d);
+int orc_orient2d_exact_sign(const double *a, const double *b, const double *c);
+int orc_incircle_exact_sign(const double *a, const double *b, const double *c,
+                            const double *d);
+
+/* Parameters that determine the refined mesh.  The GPU path takes the same
+ * values; with equal parameters both produce identical output. */
+typedef struct {
+  double theta_deg;   /* minimum angle */
+  int offcenter;      /* 1 = off-center (Ungor), 0 = circumcenter */
+  int mode;           /* 1 = Ruppert (diametral circle), else Chew (lens) */
+  int cav_cap0;       /* initial cavity capacity K (triangles) */
+  int cav_cap_max;    /* largest K */
+  long long cand_budget; /* workspace ints; candidates per pass = budget / (6K+16) */
+  int max_rounds;
+  int priority;       /* 0 = lowest candidate index wins (refine.cu:4159);
+                         1 = shortest-edge half-octave bucket first, then
+                         index (Triangle's queue order, triangle.cpp:6932-6968,
+                         and the reference's commented-out rule) */
+} orc_params;
+
+typedef struct {
+  int rounds, seg_passes, bad_passes;
+  long long inserted, seg_splits, rej_encroach, rej_blocked, rej_overflow,
+      lost, giveups;
+  int final_cav_cap;
+  int status; /* 0 ok, 1 round cap hit, 2 stalled */
+} orc_stats;
+
+typedef struct orc_result orc_result;
+
+orc_result *orc_refine(int np, const double *points, int nt, const int *tris,
+                       const int *nbrs, int ns, const int *segs,
+                       const orc_params *prm, orc_stats *st);
+void orc_result_counts(const orc_result *r, int *np, int *nt, int *ns);
+void orc_result_copy(const orc_result *r, double *points, int *tris, int *nbrs,
+                     int *segs, uint8_t *pflag);
+void orc_result_free(orc_result *r);
+
+/* Invariant checker.  Returns 0 when every check passes; otherwise the
+ * number of violations, with per-check counts in viol[] (see check_oracle.c
+ * for the index meaning). */
+long long orc_check_mesh(int np, const double *points, int nt, const int *tris,
+                         const int *nbrs, int ns, const int *segs, int np_in,
+                         const double *points_in, int ns_in,
+                         const int *segs_in, double theta_deg, double seg_tol,
+                         long long *viol, double *min_angle_deg,
+                         long long *n_exempt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

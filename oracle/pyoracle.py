@@ -1,0 +1,253 @@
+"""ctypes bindings for the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loaded only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, and only as the checker / CPU baseline:
+  oracle/liboracle.so       plain-C restatement of the refinement rounds,
+                            exact big-integer predicates, invariant checker
+  oracle/_ref/libtriref.so  the reference's Triangle 1.6 (GPU_Refine/triangle.cpp)
+                            compiled from its own source: the CPU "-q" path
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+TRIREF_SO = os.path.join(HERE, "_ref", "libtriref.so")
+
+
+def build():
+    """Compile liboracle.so and, when the reference tree is present,
+    _ref/libtriref.so (oracle/Makefile)."""
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+class TriangulateIO(C.Structure):
+    """Layout of the reference's triangulateio (GPU_Refine/triangle.h:253-284)."""
+    _fields_ = [
+        ("pointlist", C.POINTER(C.c_double)), ("pointattributelist", C.POINTER(C.c_double)),
+        ("pointmarkerlist", C.POINTER(C.c_int)), ("numberofpoints", C.c_int),
+        ("numberofpointattributes", C.c_int), ("trianglelist", C.POINTER(C.c_int)),
+        ("triangleattributelist", C.POINTER(C.c_double)),
+        ("trianglearealist", C.POINTER(C.c_double)), ("neighborlist", C.POINTER(C.c_int)),
+        ("numberoftriangles", C.c_int), ("numberofcorners", C.c_int),
+        ("numberoftriangleattributes", C.c_int), ("segmentlist", C.POINTER(C.c_int)),
+        ("segmentmarkerlist", C.POINTER(C.c_int)), ("numberofsegments", C.c_int),
+        ("holelist", C.POINTER(C.c_double)), ("numberofholes", C.c_int),
+        ("regionlist", C.POINTER(C.c_double)), ("numberofregions", C.c_int),
+        ("edgelist", C.POINTER(C.c_int)), ("edgemarkerlist", C.POINTER(C.c_int)),
+        ("normlist", C.POINTER(C.c_double)), ("numberofedges", C.c_int),
+        ("numberofflips", C.c_int)]
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("theta_deg", C.c_double), ("offcenter", C.c_int), ("mode", C.c_int),
+                ("cav_cap0", C.c_int), ("cav_cap_max", C.c_int),
+                ("cand_budget", C.c_longlong), ("max_rounds", C.c_int),
+                ("priority", C.c_int)]
+
+
+class OrcStats(C.Structure):
+    _fields_ = [("rounds", C.c_int), ("seg_passes", C.c_int), ("bad_passes", C.c_int),
+                ("inserted", C.c_longlong), ("seg_splits", C.c_longlong),
+                ("rej_encroach", C.c_longlong), ("rej_blocked", C.c_longlong),
+                ("rej_overflow", C.c_longlong), ("lost", C.c_longlong),
+                ("giveups", C.c_longlong), ("final_cav_cap", C.c_int), ("status", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_orc = None
+_tri = None
+
+
+def orc():
+    global _orc
+    if _orc is None:
+        L = C.CDLL(ORACLE_SO)
+        vp = C.c_void_p
+        L.orc_refine.restype = vp
+        L.orc_refine.argtypes = [C.c_int, vp, C.c_int, vp, vp, C.c_int, vp,
+                                 C.POINTER(OrcParams), C.POINTER(OrcStats)]
+        L.orc_result_counts.argtypes = [vp, vp, vp, vp]
+        L.orc_result_copy.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.orc_result_free.argtypes = [vp]
+        L.orc_check_mesh.restype = C.c_longlong
+        L.orc_check_mesh.argtypes = [C.c_int, vp, C.c_int, vp, vp, C.c_int, vp, C.c_int, vp,
+                                     C.c_int, vp, C.c_double, C.c_double, vp, vp, vp]
+        for f in ("orc_orient2d", "orc_incircle", "orc_orient2d_exact_sign",
+                  "orc_incircle_exact_sign"):
+            getattr(L, f).restype = C.c_int
+        _orc = L
+    return _orc
+
+
+def triref():
+    global _tri
+    if _tri is None:
+        if not os.path.exists(TRIREF_SO):
+            raise RuntimeError("oracle/_ref/libtriref.so not built (needs the reference tree)")
+        L = C.CDLL(TRIREF_SO)
+        L.triref_orient2d.restype = C.c_double
+        L.triref_incircle.restype = C.c_double
+        _tri = L
+    return _tri
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _take(o):
+    P = np.ctypeslib.as_array(o.pointlist, (o.numberofpoints * 2,)).copy().reshape(-1, 2)
+    T = np.ctypeslib.as_array(o.trianglelist, (o.numberoftriangles * 3,)).copy().reshape(-1, 3)
+    N = (np.ctypeslib.as_array(o.neighborlist, (o.numberoftriangles * 3,)).copy().reshape(-1, 3)
+         if o.neighborlist else None)
+    S = (np.ctypeslib.as_array(o.segmentlist, (o.numberofsegments * 2,)).copy().reshape(-1, 2)
+         if o.numberofsegments else np.zeros((0, 2), np.int32))
+    L = triref()
+    for f in ("pointlist", "trianglelist", "neighborlist", "segmentlist", "pointmarkerlist",
+              "segmentmarkerlist", "triangleattributelist", "pointattributelist"):
+        ptr = getattr(o, f)
+        if ptr:
+            L.triref_free(C.cast(ptr, C.c_void_p))
+    return P, T, N, S
+
+
+def triangle_cdt(points, segments):
+    """Triangle "pzQnc": CDT with hull segments (mesh.cpp:397)."""
+    P = np.ascontiguousarray(points, np.float64).reshape(-1)
+    S = np.ascontiguousarray(segments, np.int32).reshape(-1)
+    i, o = TriangulateIO(), TriangulateIO()
+    i.pointlist = _dp(P)
+    i.numberofpoints = len(P) // 2
+    i.segmentlist = _ip(S)
+    i.numberofsegments = len(S) // 2
+    triref().triref_triangulate(b"pzQnc", C.byref(i), C.byref(o))
+    return _take(o)
+
+
+def triangle_refine(points, triangles, segments, theta, mode=1):
+    """The reference's CPU_Triangle_Quality (main.cpp:389-407):
+    "pzDQnrq<theta>" (Ruppert) or "pzQnrq<theta>" (Chew) on a CDT."""
+    P = np.ascontiguousarray(points, np.float64).reshape(-1)
+    T = np.ascontiguousarray(triangles, np.int32).reshape(-1)
+    S = np.ascontiguousarray(segments, np.int32).reshape(-1)
+    i, o = TriangulateIO(), TriangulateIO()
+    i.pointlist = _dp(P)
+    i.numberofpoints = len(P) // 2
+    i.trianglelist = _ip(T)
+    i.numberoftriangles = len(T) // 3
+    i.numberofcorners = 3
+    i.segmentlist = _ip(S)
+    i.numberofsegments = len(S) // 2
+    sw = ("pzDQnrq" if mode == 1 else "pzQnrq") + ("%g" % theta)
+    triref().triref_triangulate(sw.encode(), C.byref(i), C.byref(o))
+    return _take(o)
+
+
+def oracle_refine(points, triangles, neighbors, segments, theta=25.0, offcenter=1,
+                  mode=1, cav_cap0=32, cav_cap_max=16384, cand_budget=1 << 28,
+                  max_rounds=100000, priority=1):
+    """Run the CPU restatement; returns (P, T, N, S, pflag, stats)."""
+    P = np.ascontiguousarray(points, np.float64).reshape(-1, 2)
+    T = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
+    N = np.ascontiguousarray(neighbors, np.int32).reshape(-1, 3)
+    S = np.ascontiguousarray(segments, np.int32).reshape(-1, 2)
+    prm = OrcParams(theta, offcenter, mode, cav_cap0, cav_cap_max, cand_budget,
+                    max_rounds, priority)
+    st = OrcStats()
+    L = orc()
+    r = L.orc_refine(len(P), _p(P), len(T), _p(T), _p(N), len(S), _p(S),
+                     C.byref(prm), C.byref(st))
+    np_, nt, ns = C.c_int(), C.c_int(), C.c_int()
+    L.orc_result_counts(r, C.byref(np_), C.byref(nt), C.byref(ns))
+    Po = np.empty((np_.value, 2))
+    To = np.empty((nt.value, 3), np.int32)
+    No = np.empty((nt.value, 3), np.int32)
+    So = np.empty((ns.value, 2), np.int32)
+    pf = np.empty(np_.value, np.uint8)
+    L.orc_result_copy(r, _p(Po), _p(To), _p(No), _p(So), _p(pf))
+    L.orc_result_free(r)
+    return Po, To, No, So, pf, st.as_dict()
+
+
+VIOLATION_NAMES = ["range", "neighbors", "orientation", "unsegmented_boundary",
+                   "delaunay", "subseg_not_edge", "segment_not_covered",
+                   "input_point_moved", "quality", "nonmanifold"]
+
+
+def check_mesh(points, triangles, neighbors, segments, points_in, segments_in,
+               theta, seg_tol=1e-9):
+    """Invariant checker; returns (total, {name: count}, min_angle, n_exempt)."""
+    P = np.ascontiguousarray(points, np.float64).reshape(-1, 2)
+    T = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
+    N = np.ascontiguousarray(neighbors, np.int32).reshape(-1, 3)
+    S = np.ascontiguousarray(segments, np.int32).reshape(-1, 2)
+    Pi = np.ascontiguousarray(points_in, np.float64).reshape(-1, 2)
+    Si = np.ascontiguousarray(segments_in, np.int32).reshape(-1, 2)
+    viol = np.zeros(10, np.int64)
+    mina = C.c_double()
+    ex = C.c_longlong()
+    tot = orc().orc_check_mesh(len(P), _p(P), len(T), _p(T), _p(N), len(S), _p(S), len(Pi),
+                               _p(Pi), len(Si), _p(Si), float(theta), float(seg_tol),
+                               _p(viol), C.byref(mina), C.byref(ex))
+    return tot, dict(zip(VIOLATION_NAMES, viol.tolist())), mina.value, ex.value
+
+
+def orient2d(a, b, c):
+    A, B, Cc = (np.ascontiguousarray(x, np.float64) for x in (a, b, c))
+    return orc().orc_orient2d(_p(A), _p(B), _p(Cc))
+
+
+def incircle(a, b, c, d):
+    A, B, Cc, D = (np.ascontiguousarray(x, np.float64) for x in (a, b, c, d))
+    return orc().orc_incircle(_p(A), _p(B), _p(Cc), _p(D))
+
+
+def random_pslg(n, nseg, L=1000.0, maxlen=None, seed=1, dist="uniform"):
+    """Small seeded PSLG for tests (numpy; exact-sign crossing test through
+    the oracle's orient2d).  Shared endpoints allowed."""
+    rng = np.random.default_rng(seed)
+    if dist == "uniform":
+        pts = rng.uniform(0, L, size=(n, 2))
+    else:
+        k = max(1, n // 200)
+        centers = rng.uniform(0.1 * L, 0.9 * L, size=(k, 2))
+        pts = centers[rng.integers(0, k, n)] + rng.normal(0, 0.05 * L, size=(n, 2))
+        pts = np.clip(pts, 0, L)
+    segs = []
+    tries = 0
+    while len(segs) < nseg and tries < nseg * 100:
+        tries += 1
+        i, j = (int(x) for x in rng.integers(0, n, 2))
+        if i == j:
+            continue
+        if maxlen and np.hypot(*(pts[i] - pts[j])) > maxlen:
+            continue
+        ok = True
+        for a, b in segs:
+            if {a, b} & {i, j}:
+                if {a, b} == {i, j}:
+                    ok = False
+                    break
+                continue
+            o1 = orient2d(pts[i], pts[j], pts[a]) * orient2d(pts[i], pts[j], pts[b])
+            o2 = orient2d(pts[a], pts[b], pts[i]) * orient2d(pts[a], pts[b], pts[j])
+            if o1 <= 0 and o2 <= 0:
+                ok = False
+                break
+        if ok:
+            segs.append((i, j))
+    return pts, np.array(segs, dtype=np.int32).reshape(-1, 2)

@@ -1,0 +1,898 @@
+/*
+ * Sequential restatement of the bulk-synchronous CDT quality refinement
+ * (TEST INFRASTRUCTURE ONLY; see oracle.h).
+ *
+ * What it restates, and from where:
+ *   - round structure: encroached subsegments are split first, then bad
+ *     triangles are marked and split, repeat until none remain
+ *     (reference refine.cu:6831-7024 GPU_Refine_Quality; Triangle
+ *     enforcequality triangle.cpp:13646-13738)
+ *   - bad-triangle test incl. the Miller-Pav-Walkington exemption
+ *     (refine.cu:376-497 kernelMarkBadTriangles; triangle.cpp:7220-7384)
+ *   - circumcenter / off-center (refine.cu:3780-3868; triangle.cpp:6523-6618)
+ *     with the off-center constant kept in double (the reference truncates it
+ *     to int through splitTriangles' parameter, refine.cu:5132)
+ *   - encroachment test, diametral circle (Ruppert) or lens (Chew)
+ *     (refine.cu:115-139; triangle.cpp:7106-7203)
+ *   - segment split point with concentric power-of-two shells and one
+ *     collinearity correction step (refine.cu:2686-2725; triangle.cpp:13368-13415)
+ *   - point location by orientation walk that stops at subsegments, and
+ *     rejection of Steiner points that land on / behind a subsegment or
+ *     encroach one (refine.cu:3908-4184, 5061-5089; triangle.cpp:8290-8325)
+ *   - conflict resolution: a candidate claims its Bowyer-Watson cavity and the
+ *     triangles across the cavity boundary; the lowest candidate index wins
+ *     every contested triangle (the reference's atomicMin rule,
+ *     refine.cu:4159-4172, 4296)
+ *
+ * Insertion itself is cavity retriangulation (a star from the new vertex over
+ * the cavity boundary), which leaves the mesh constrained-Delaunay in one step
+ * (the reference reaches the same mesh by 1->3 / 2->4 splits followed by
+ * rounds of edge flips, refine.cu:2328-2490).  The exact tie rules below
+ * (candidate order, cavity order, star ordering, slot assignment) are the
+ * contract the GPU path reproduces.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+#define TF_ALIVE 1
+#define TF_CHECK 2
+#define TF_BAD 4
+#define TF_GIVEUP 8
+#define PF_INPUT 1
+#define PF_SEGV 2
+#define PF_FREE 4
+
+#define ST_VALID 0
+#define ST_REJ_ENC 1
+#define ST_REJ_BLOCKED 2
+#define ST_OVERFLOW 3
+#define ST_GIVEUP 4
+
+#define WALK_LIMIT (1 << 22)
+
+typedef struct {
+  int np, capp;
+  double *P;
+  uint8_t *pflag;
+  int *ppar;
+  int nt, capt;
+  int *TV, *TN, *TS;
+  uint8_t *tflag;
+  int ns, caps;
+  int *SV, *SP, *ST;
+  uint8_t *senc;
+  int nseg_in;
+  int *SEG;
+  double goodangle, lens, offconst;
+  int offcenter, ruppert;
+} Mesh;
+
+struct orc_result {
+  int np, nt, ns;
+  double *P;
+  int *T, *N, *S;
+  uint8_t *pflag;
+};
+
+static void *xrealloc(void *p, size_t n) {
+  void *q = realloc(p, n);
+  if (!q && n) abort();
+  return q;
+}
+
+static void grow_points(Mesh *m, int need) {
+  if (need <= m->capp) return;
+  int c = m->capp ? m->capp : 16;
+  while (c < need) c *= 2;
+  m->P = xrealloc(m->P, sizeof(double) * 2 * (size_t)c);
+  m->pflag = xrealloc(m->pflag, (size_t)c);
+  m->ppar = xrealloc(m->ppar, sizeof(int) * (size_t)c);
+  m->capp = c;
+}
+
+static void grow_tris(Mesh *m, int need) {
+  if (need <= m->capt) return;
+  int c = m->capt ? m->capt : 16;
+  while (c < need) c *= 2;
+  m->TV = xrealloc(m->TV, sizeof(int) * 3 * (size_t)c);
+  m->TN = xrealloc(m->TN, sizeof(int) * 3 * (size_t)c);
+  m->TS = xrealloc(m->TS, sizeof(int) * 3 * (size_t)c);
+  m->tflag = xrealloc(m->tflag, (size_t)c);
+  m->capt = c;
+}
+
+static void grow_segs(Mesh *m, int need) {
+  if (need <= m->caps) return;
+  int c = m->caps ? m->caps : 16;
+  while (c < need) c *= 2;
+  m->SV = xrealloc(m->SV, sizeof(int) * 2 * (size_t)c);
+  m->SP = xrealloc(m->SP, sizeof(int) * (size_t)c);
+  m->ST = xrealloc(m->ST, sizeof(int) * (size_t)c);
+  m->senc = xrealloc(m->senc, (size_t)c);
+  m->caps = c;
+}
+
+static const double *PT(const Mesh *m, int v) { return m->P + 2 * (size_t)v; }
+
+/* checkseg4encroach: does apex p encroach subsegment (a,b)?
+ * refine.cu:115-139, triangle.cpp:7138-7153 */
+static int encroaches(const Mesh *m, int a, int b, const double *p) {
+  const double *A = PT(m, a), *B = PT(m, b);
+  double dot = (A[0] - p[0]) * (B[0] - p[0]) + (A[1] - p[1]) * (B[1] - p[1]);
+  if (dot < 0.0) {
+    if (m->ruppert) return 1;
+    double la = (A[0] - p[0]) * (A[0] - p[0]) + (A[1] - p[1]) * (A[1] - p[1]);
+    double lb = (B[0] - p[0]) * (B[0] - p[0]) + (B[1] - p[1]) * (B[1] - p[1]);
+    if (dot * dot >= m->lens * la * lb) return 1;
+  }
+  return 0;
+}
+
+/* testtriangle with Miller-Pav-Walkington exemption; triangle.cpp:7220-7384,
+ * refine.cu:376-497.  org = v1, dest = v2, apex = v0 (orientation 0). */
+static int tri_is_bad(const Mesh *m, int t) {
+  const int *v = m->TV + 3 * (size_t)t;
+  const double *O = PT(m, v[1]), *D = PT(m, v[2]), *A = PT(m, v[0]);
+  double dxod = O[0] - D[0], dyod = O[1] - D[1];
+  double dxda = D[0] - A[0], dyda = D[1] - A[1];
+  double dxao = A[0] - O[0], dyao = A[1] - O[1];
+  double apexlen = dxod * dxod + dyod * dyod;
+  double orglen = dxda * dxda + dyda * dyda;
+  double destlen = dxao * dxao + dyao * dyao;
+  double angle;
+  int e, b1, b2;
+  if ((apexlen < orglen) && (apexlen < destlen)) {
+    angle = dxda * dxao + dyda * dyao;
+    angle = angle * angle / (orglen * destlen);
+    e = 0; b1 = v[1]; b2 = v[2];
+  } else if (orglen < destlen) {
+    angle = dxod * dxao + dyod * dyao;
+    angle = angle * angle / (apexlen * destlen);
+    e = 1; b1 = v[2]; b2 = v[0];
+  } else {
+    angle = dxod * dxda + dyod * dyda;
+    angle = angle * angle / (apexlen * orglen);
+    e = 2; b1 = v[0]; b2 = v[1];
+  }
+  if (!(angle > m->goodangle)) return 0;
+  if ((m->pflag[b1] & PF_SEGV) && (m->pflag[b2] & PF_SEGV) &&
+      m->TS[3 * (size_t)t + e] < 0) {
+    int s1 = m->ppar[b1], s2 = m->ppar[b2];
+    int o1 = m->SEG[2 * s1], d1 = m->SEG[2 * s1 + 1];
+    int o2 = m->SEG[2 * s2], d2 = m->SEG[2 * s2 + 1];
+    int common = -1;
+    if (o1 == o2 || o1 == d2) common = o1;
+    else if (d1 == o2 || d1 == d2) common = d1;
+    if (common >= 0) {
+      const double *J = PT(m, common), *B1 = PT(m, b1), *B2 = PT(m, b2);
+      double dist1 = (B1[0] - J[0]) * (B1[0] - J[0]) + (B1[1] - J[1]) * (B1[1] - J[1]);
+      double dist2 = (B2[0] - J[0]) * (B2[0] - J[0]) + (B2[1] - J[1]) * (B2[1] - J[1]);
+      if ((dist1 < 1.001 * dist2) && (dist1 > 0.999 * dist2)) return 0;
+    }
+  }
+  return 1;
+}
+
+/* findcircumcenter / computeCircumcenter: triangle.cpp:6523-6618,
+ * refine.cu:3780-3868.  Returns 0 if the triangle is numerically degenerate. */
+static int steiner_point(const Mesh *m, int t, double *c) {
+  const int *v = m->TV + 3 * (size_t)t;
+  const double *A = PT(m, v[0]), *O = PT(m, v[1]), *D = PT(m, v[2]);
+  double xdo = D[0] - O[0], ydo = D[1] - O[1];
+  double xao = A[0] - O[0], yao = A[1] - O[1];
+  double dodist = xdo * xdo + ydo * ydo;
+  double aodist = xao * xao + yao * yao;
+  double dadist = (D[0] - A[0]) * (D[0] - A[0]) + (D[1] - A[1]) * (D[1] - A[1]);
+  double det = xdo * yao - xao * ydo;
+  if (det == 0.0) return 0;
+  double denominator = 0.5 / det;
+  double dx = (yao * dodist - ydo * aodist) * denominator;
+  double dy = (xdo * aodist - xao * dodist) * denominator;
+  if (m->offcenter && m->offconst > 0.0) {
+    double dxoff, dyoff;
+    if ((dodist < aodist) && (dodist < dadist)) {
+      dxoff = 0.5 * xdo - m->offconst * ydo;
+      dyoff = 0.5 * ydo + m->offconst * xdo;
+      if (dxoff * dxoff + dyoff * dyoff < dx * dx + dy * dy) { dx = dxoff; dy = dyoff; }
+    } else if (aodist < dadist) {
+      dxoff = 0.5 * xao + m->offconst * yao;
+      dyoff = 0.5 * yao - m->offconst * xao;
+      if (dxoff * dxoff + dyoff * dyoff < dx * dx + dy * dy) { dx = dxoff; dy = dyoff; }
+    } else {
+      dxoff = 0.5 * (A[0] - D[0]) - m->offconst * (A[1] - D[1]);
+      dyoff = 0.5 * (A[1] - D[1]) + m->offconst * (A[0] - D[0]);
+      if (dxoff * dxoff + dyoff * dyoff <
+          (dx - xdo) * (dx - xdo) + (dy - ydo) * (dy - ydo)) {
+        dx = xdo + dxoff;
+        dy = ydo + dyoff;
+      }
+    }
+  }
+  c[0] = O[0] + dx;
+  c[1] = O[1] + dy;
+  if ((c[0] == A[0] && c[1] == A[1]) || (c[0] == O[0] && c[1] == O[1]) ||
+      (c[0] == D[0] && c[1] == D[1]))
+    return 0;
+  if (!(c[0] == c[0]) || !(c[1] == c[1])) return 0;
+  return 1;
+}
+
+/* Locate the adjacent triangle(s) of subsegment s: t1 holds the directed
+ * edge org->dest of the subsegment if it exists, t2 the reverse. */
+static void seg_sides(const Mesh *m, int s, int *t1, int *j1, int *t2, int *j2) {
+  int a = m->SV[2 * s];
+  int ot = m->ST[s];
+  int t = ot >> 2, j = ot & 3;
+  int u = m->TV[3 * t + (j + 1) % 3];
+  int n = m->TN[3 * t + j];
+  int ta = t, ja = j, tb = -1, jb = -1;
+  if (n >= 0) { tb = n >> 2; jb = n & 3; }
+  if (u == a) { *t1 = ta; *j1 = ja; *t2 = tb; *j2 = jb; }
+  else { *t1 = tb; *j1 = jb; *t2 = ta; *j2 = ja; }
+}
+
+/* splitencsegs split point: triangle.cpp:13290-13425, refine.cu:2686-2725.
+ * Returns 0 if the point coincides with an endpoint (precision exhausted). */
+static int split_point(const Mesh *m, int s, double *p) {
+  int a = m->SV[2 * s], b = m->SV[2 * s + 1];
+  int t1, j1, t2, j2;
+  seg_sides(m, s, &t1, &j1, &t2, &j2);
+  int acuteorg = 0, acutedest = 0;
+  /* side where the edge runs a->b: the other edge at a is (j+2)%3, at b (j+1)%3 */
+  if (t1 >= 0) {
+    if (m->TS[3 * t1 + (j1 + 2) % 3] >= 0) acuteorg = 1;
+    if (m->TS[3 * t1 + (j1 + 1) % 3] >= 0) acutedest = 1;
+  }
+  /* side where the edge runs b->a: the other edge at a is (j+1)%3, at b (j+2)%3 */
+  if (t2 >= 0) {
+    if (m->TS[3 * t2 + (j2 + 1) % 3] >= 0) acuteorg = 1;
+    if (m->TS[3 * t2 + (j2 + 2) % 3] >= 0) acutedest = 1;
+  }
+  const double *A = PT(m, a), *B = PT(m, b);
+  double split;
+  if (acuteorg || acutedest) {
+    double len = sqrt((B[0] - A[0]) * (B[0] - A[0]) + (B[1] - A[1]) * (B[1] - A[1]));
+    double npt = 1.0;
+    while (len > 3.0 * npt) npt *= 2.0;
+    while (len < 1.5 * npt) npt *= 0.5;
+    split = npt / len;
+    if (acutedest) split = 1.0 - split;
+  } else {
+    split = 0.5;
+  }
+  p[0] = A[0] + split * (B[0] - A[0]);
+  p[1] = A[1] + split * (B[1] - A[1]);
+  double mult = (A[0] - p[0]) * (B[1] - p[1]) - (A[1] - p[1]) * (B[0] - p[0]);
+  double div = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]);
+  if (mult != 0.0 && div != 0.0) {
+    mult = mult / div;
+    if (mult == mult) {
+      p[0] += mult * (B[1] - A[1]);
+      p[1] += mult * (A[0] - B[0]);
+    }
+  }
+  if ((p[0] == A[0] && p[1] == A[1]) || (p[0] == B[0] && p[1] == B[1])) return 0;
+  return 1;
+}
+
+/* Half-octave bucket of the squared shortest edge: 2*e + (f >= sqrt(1/2))
+ * with key = f * 2^e, 0.5 <= f < 1 (exact via frexp).  Shorter edges give
+ * smaller buckets, i.e. higher priority, as in Triangle's queue. */
+static int edge_bucket(double key) {
+  int e;
+  double f = frexp(key, &e);
+  int b = 2 * e + (f >= 0.70710678118654752440 ? 1 : 0);
+  b += 2200;
+  if (b < 0) b = 0;
+  if (b > 8191) b = 8191;
+  return b;
+}
+
+static int shortest_edge_bucket(const Mesh *m, int t) {
+  const int *v = m->TV + 3 * (size_t)t;
+  const double *A = PT(m, v[0]), *O = PT(m, v[1]), *D = PT(m, v[2]);
+  double l0 = (O[0] - D[0]) * (O[0] - D[0]) + (O[1] - D[1]) * (O[1] - D[1]);
+  double l1 = (D[0] - A[0]) * (D[0] - A[0]) + (D[1] - A[1]) * (D[1] - A[1]);
+  double l2 = (A[0] - O[0]) * (A[0] - O[0]) + (A[1] - O[1]) * (A[1] - O[1]);
+  double mn = l0 < l1 ? l0 : l1;
+  if (l2 < mn) mn = l2;
+  return edge_bucket(mn);
+}
+
+static int in_list(const int *L, int n, int x) {
+  for (int i = 0; i < n; i++)
+    if (L[i] == x) return 1;
+  return 0;
+}
+
+/* Is edge j of cavity triangle t on the cavity boundary? */
+static int is_bnd(const Mesh *m, int t, int j, int split_s, const int *L, int n) {
+  int s = m->TS[3 * t + j];
+  int nb = m->TN[3 * t + j];
+  if (s >= 0 && s != split_s) return 1;
+  if (nb < 0) return 1;
+  return !in_list(L, n, nb >> 2);
+}
+
+/* Walk toward c.  Result codes: 0 inside t, 1 on edge *j, 2 on a vertex,
+ * 3 blocked by subsegment *seg, 4 failed.  refine.cu:3908-4184 semantics. */
+static int walk(const Mesh *m, int t, const double *c, int *tout, int *jout,
+                int *seg) {
+  for (long steps = 0; steps < WALK_LIMIT; steps++) {
+    const int *v = m->TV + 3 * (size_t)t;
+    int o[3];
+    for (int j = 0; j < 3; j++)
+      o[j] = orc_orient2d(PT(m, v[(j + 1) % 3]), PT(m, v[(j + 2) % 3]), c);
+    int jn = -1;
+    for (int j = 0; j < 3; j++)
+      if (o[j] < 0) { jn = j; break; }
+    if (jn < 0) {
+      int zeros = (o[0] == 0) + (o[1] == 0) + (o[2] == 0);
+      *tout = t;
+      if (zeros == 0) return 0;
+      if (zeros == 1) {
+        *jout = (o[0] == 0) ? 0 : (o[1] == 0) ? 1 : 2;
+        return 1;
+      }
+      return 2;
+    }
+    int s = m->TS[3 * t + jn];
+    if (s >= 0) { *seg = s; *tout = t; return 3; }
+    int nb = m->TN[3 * t + jn];
+    if (nb < 0) return 4;
+    t = nb >> 2;
+  }
+  return 4;
+}
+
+typedef struct {
+  int id;       /* triangle (bad pass) or subsegment (segment pass) */
+  int prio;     /* smaller wins; ties by candidate index */
+  int status;
+  double c[2];
+  int n, r;     /* cavity size, ring size */
+  int *L;       /* cavity then ring */
+} Cand;
+
+/* Grow the Bowyer-Watson cavity of c from the seeds; then collect the ring
+ * of triangles across the boundary, and check that the cavity is a disk
+ * whose star from c is positively oriented. */
+static int build_cavity(const Mesh *m, Cand *cd, int seed0, int seed1,
+                        int split_s, int K) {
+  int *L = cd->L;
+  int n = 0;
+  L[n++] = seed0;
+  if (seed1 >= 0) L[n++] = seed1;
+  for (int idx = 0; idx < n; idx++) {
+    int t = L[idx];
+    for (int j = 0; j < 3; j++) {
+      int s = m->TS[3 * t + j];
+      if (s >= 0) continue; /* subsegments (and the split one) are never crossed */
+      int nb = m->TN[3 * t + j];
+      if (nb < 0) continue;
+      int u = nb >> 2;
+      if (in_list(L, n, u)) continue;
+      const int *w = m->TV + 3 * (size_t)u;
+      if (orc_incircle(PT(m, w[0]), PT(m, w[1]), PT(m, w[2]), cd->c) > 0) {
+        if (n == K) return ST_OVERFLOW;
+        L[n++] = u;
+      }
+    }
+  }
+  /* disk check: boundary edges == n + 2; no subsegment strictly inside */
+  int nbnd = 0;
+  for (int idx = 0; idx < n; idx++) {
+    int t = L[idx];
+    for (int j = 0; j < 3; j++) {
+      int s = m->TS[3 * t + j];
+      int nb = m->TN[3 * t + j];
+      if (s >= 0 && s != split_s && nb >= 0 && in_list(L, n, nb >> 2))
+        return ST_GIVEUP;
+      if (is_bnd(m, t, j, split_s, L, n)) {
+        nbnd++;
+        /* the star triangle (c, u, v) must be positively oriented, except
+         * for the split subsegment's own edge on a one-sided split */
+        if (!(s >= 0 && s == split_s)) {
+          const int *v = m->TV + 3 * (size_t)t;
+          if (orc_orient2d(cd->c, PT(m, v[(j + 1) % 3]), PT(m, v[(j + 2) % 3])) <= 0)
+            return ST_GIVEUP;
+        }
+      }
+    }
+  }
+  if (nbnd != n + 2) return ST_GIVEUP;
+  /* ring: distinct triangles across boundary edges */
+  int r = 0;
+  for (int idx = 0; idx < n; idx++) {
+    int t = L[idx];
+    for (int j = 0; j < 3; j++) {
+      if (!is_bnd(m, t, j, split_s, L, n)) continue;
+      int nb = m->TN[3 * t + j];
+      if (nb < 0) continue;
+      int u = nb >> 2;
+      if (in_list(L + n, r, u)) continue;
+      L[n + r++] = u;
+    }
+  }
+  cd->n = n;
+  cd->r = r;
+  return ST_VALID;
+}
+
+/* Next boundary edge (counter-clockwise around the cavity) after boundary
+ * edge (t,j): pivot around its destination through cavity triangles. */
+static void next_bnd(const Mesh *m, int t, int j, int split_s, const int *L,
+                     int n, int *tn, int *jn) {
+  int cur = t, k = (j + 1) % 3;
+  for (;;) {
+    if (is_bnd(m, cur, k, split_s, L, n)) { *tn = cur; *jn = k; return; }
+    int nb = m->TN[3 * cur + k];
+    cur = nb >> 2;
+    k = ((nb & 3) + 1) % 3;
+  }
+}
+
+typedef struct {
+  int u, v, outer, seg;
+} Bnd;
+
+/* Retriangulate a winner's cavity as the star of its new vertex.
+ * Star ordering: closed fans start at the boundary edge whose origin has the
+ * smallest vertex id; the open fan of a one-sided segment split starts at the
+ * boundary edge after the split subsegment's edge.  Star triangle i is
+ * (p, u_i, v_i) and takes slot L[i] for i < n, else new slot tbase + i - n. */
+static void apply_star(Mesh *m, const Cand *cd, int split_s, int p, int tbase,
+                       int snew, int is_seg) {
+  const int *L = cd->L;
+  int n = cd->n;
+  int k = n + 2;
+  Bnd *B = malloc(sizeof(Bnd) * (size_t)k);
+  int t0 = -1, j0 = -1;
+  int open = 0;
+  if (is_seg) {
+    int t1, j1, t2, j2;
+    seg_sides(m, split_s, &t1, &j1, &t2, &j2);
+    if (t1 < 0 || t2 < 0) {
+      open = 1;
+      int ts = t1 >= 0 ? t1 : t2, js = t1 >= 0 ? j1 : j2;
+      next_bnd(m, ts, js, split_s, L, n, &t0, &j0);
+    }
+  }
+  if (!open) {
+    int best = -1;
+    for (int idx = 0; idx < n; idx++) {
+      int t = L[idx];
+      for (int j = 0; j < 3; j++) {
+        if (!is_bnd(m, t, j, split_s, L, n)) continue;
+        int u = m->TV[3 * t + (j + 1) % 3];
+        if (best < 0 || u < best) { best = u; t0 = t; j0 = j; }
+      }
+    }
+  }
+  int nstar = open ? k - 1 : k;
+  int t = t0, j = j0;
+  for (int i = 0; i < nstar; i++) {
+    B[i].u = m->TV[3 * t + (j + 1) % 3];
+    B[i].v = m->TV[3 * t + (j + 2) % 3];
+    B[i].outer = m->TN[3 * t + j];
+    B[i].seg = m->TS[3 * t + j];
+    int tn, jn;
+    next_bnd(m, t, j, split_s, L, n, &tn, &jn);
+    t = tn;
+    j = jn;
+  }
+  int a = -1, b = -1, sA = -1, sB = -1;
+  if (is_seg) {
+    a = m->SV[2 * split_s];
+    b = m->SV[2 * split_s + 1];
+    sA = split_s;
+    sB = snew;
+  }
+  int *slot = malloc(sizeof(int) * (size_t)nstar);
+  for (int i = 0; i < nstar; i++) slot[i] = i < n ? L[i] : tbase + (i - n);
+  for (int i = 0; i < nstar; i++) {
+    int st = slot[i];
+    int nx = (i + 1 < nstar) ? i + 1 : (open ? -1 : 0);
+    int pv = (i > 0) ? i - 1 : (open ? -1 : nstar - 1);
+    m->TV[3 * st + 0] = p;
+    m->TV[3 * st + 1] = B[i].u;
+    m->TV[3 * st + 2] = B[i].v;
+    m->TN[3 * st + 0] = B[i].outer;
+    m->TN[3 * st + 1] = nx >= 0 ? (slot[nx] << 2) | 2 : -1;
+    m->TN[3 * st + 2] = pv >= 0 ? (slot[pv] << 2) | 1 : -1;
+    m->TS[3 * st + 0] = B[i].seg;
+    m->TS[3 * st + 1] = (B[i].v == a) ? sA : (B[i].v == b) ? sB : -1;
+    m->TS[3 * st + 2] = (B[i].u == a) ? sA : (B[i].u == b) ? sB : -1;
+    m->tflag[st] = TF_ALIVE | TF_CHECK;
+  }
+  if (open) { /* the second reserved slot stays unused */
+    int st = tbase + 1;
+    m->TV[3 * st] = m->TV[3 * st + 1] = m->TV[3 * st + 2] = -1;
+    m->TN[3 * st] = m->TN[3 * st + 1] = m->TN[3 * st + 2] = -1;
+    m->TS[3 * st] = m->TS[3 * st + 1] = m->TS[3 * st + 2] = -1;
+    m->tflag[st] = 0;
+  }
+  for (int i = 0; i < nstar; i++) {
+    int st = slot[i];
+    if (B[i].outer >= 0) m->TN[3 * (B[i].outer >> 2) + (B[i].outer & 3)] = (st << 2) | 0;
+    if (B[i].seg >= 0) {
+      m->ST[B[i].seg] = (st << 2) | 0;
+      if (is_seg && m->senc[B[i].seg] == 0 &&
+          encroaches(m, m->SV[2 * B[i].seg], m->SV[2 * B[i].seg + 1], PT(m, p)))
+        m->senc[B[i].seg] = 1;
+    }
+  }
+  if (is_seg) {
+    int par = m->SP[split_s];
+    m->SV[2 * sA + 0] = a;
+    m->SV[2 * sA + 1] = p;
+    m->SV[2 * sB + 0] = p;
+    m->SV[2 * sB + 1] = b;
+    m->SP[sB] = par;
+    int stA = -1, stB = -1, encA = 0, encB = 0;
+    for (int i = 0; i < nstar; i++) {
+      int st = slot[i];
+      if (B[i].u == a) { if (stA < 0) stA = (st << 2) | 2; encA |= encroaches(m, a, p, PT(m, B[i].v)); }
+      if (B[i].v == a) { if (stA < 0) stA = (st << 2) | 1; encA |= encroaches(m, a, p, PT(m, B[i].u)); }
+      if (B[i].u == b) { if (stB < 0) stB = (st << 2) | 2; encB |= encroaches(m, p, b, PT(m, B[i].v)); }
+      if (B[i].v == b) { if (stB < 0) stB = (st << 2) | 1; encB |= encroaches(m, p, b, PT(m, B[i].u)); }
+    }
+    m->ST[sA] = stA;
+    m->ST[sB] = stB;
+    m->senc[sA] = (uint8_t)encA;
+    m->senc[sB] = (uint8_t)encB;
+  }
+  free(slot);
+  free(B);
+}
+
+/* Mark subsegment s encroached unless it was given up on (senc == 2). */
+static void mark_enc(Mesh *m, int s, long long *marks) {
+  if (m->senc[s] == 0) {
+    m->senc[s] = 1;
+    (*marks)++;
+  }
+}
+
+static int cands_per_pass(long long budget, int K) {
+  long long c = budget / (6LL * K + 16);
+  if (c < 1) c = 1;
+  if (c > (1LL << 30)) c = 1LL << 30;
+  return (int)c;
+}
+
+/* One conflict-resolution pass over candidates ids[0..nc). is_seg selects
+ * the segment-split or the bad-triangle flavour.  Returns the number of
+ * winners; sets *overflowed if any cavity exceeded K. */
+static int run_pass(Mesh *m, const int *ids, int nc, int is_seg, int K,
+                    int Kmax, int priority, orc_stats *st, int *overflowed,
+                    long long *progress) {
+  long long marks = 0, gave = st->giveups;
+  Cand *cd = calloc((size_t)nc, sizeof(Cand));
+  int *pool = malloc(sizeof(int) * (size_t)(2 * K + 2));
+  *overflowed = 0;
+  for (int i = 0; i < nc; i++) {
+    Cand *c = &cd[i];
+    c->id = ids[i];
+    c->prio = (priority == 1 && !is_seg) ? shortest_edge_bucket(m, ids[i]) : 0;
+    c->L = pool; /* temporary; copied out below if valid */
+    int seed0 = -1, seed1 = -1, split_s = -1;
+    if (is_seg) {
+      int s = ids[i];
+      split_s = s;
+      if (!split_point(m, s, c->c)) { c->status = ST_GIVEUP; goto done; }
+      int t1, j1, t2, j2;
+      seg_sides(m, s, &t1, &j1, &t2, &j2);
+      seed0 = t1 >= 0 ? t1 : t2;
+      seed1 = t1 >= 0 ? t2 : -1;
+    } else {
+      int t0 = ids[i];
+      if (!steiner_point(m, t0, c->c)) { c->status = ST_GIVEUP; goto done; }
+      int tl = -1, jl = -1, sg = -1;
+      int w = walk(m, t0, c->c, &tl, &jl, &sg);
+      if (w == 3) { c->status = ST_REJ_BLOCKED; mark_enc(m, sg, &marks); goto done; }
+      if (w == 2 || w == 4) { c->status = ST_GIVEUP; goto done; }
+      seed0 = tl;
+      if (w == 1) {
+        int s = m->TS[3 * tl + jl];
+        if (s >= 0) { c->status = ST_REJ_BLOCKED; mark_enc(m, s, &marks); goto done; }
+        int nb = m->TN[3 * tl + jl];
+        if (nb < 0) { c->status = ST_GIVEUP; goto done; }
+        seed1 = nb >> 2;
+      }
+    }
+    c->status = build_cavity(m, c, seed0, seed1, split_s, K);
+    if (c->status == ST_OVERFLOW && K >= Kmax) c->status = ST_GIVEUP;
+    if (c->status == ST_VALID && !is_seg) {
+      /* Ruppert/Chew rule: a Steiner point that encroaches a subsegment on
+       * its cavity boundary is not inserted; every such subsegment is marked. */
+      int enc = 0;
+      for (int idx = 0; idx < c->n; idx++) {
+        int t = c->L[idx];
+        for (int j = 0; j < 3; j++) {
+          int s = m->TS[3 * t + j];
+          if (s < 0) continue;
+          if (encroaches(m, m->SV[2 * s], m->SV[2 * s + 1], c->c)) {
+            mark_enc(m, s, &marks);
+            enc = 1;
+          }
+        }
+      }
+      if (enc) c->status = ST_REJ_ENC;
+    }
+  done:
+    if (c->status == ST_VALID) {
+      int *Lc = malloc(sizeof(int) * (size_t)(c->n + c->r));
+      memcpy(Lc, c->L, sizeof(int) * (size_t)(c->n + c->r));
+      c->L = Lc;
+    } else {
+      c->L = NULL;
+      if (c->status == ST_OVERFLOW) { *overflowed = 1; st->rej_overflow++; }
+      else if (c->status == ST_REJ_ENC) st->rej_encroach++;
+      else if (c->status == ST_REJ_BLOCKED) st->rej_blocked++;
+      else if (c->status == ST_GIVEUP) {
+        st->giveups++;
+        if (is_seg) m->senc[c->id] = 2;
+        else m->tflag[c->id] |= TF_GIVEUP;
+      }
+    }
+  }
+  /* claims: the smallest (priority, candidate index) wins each triangle */
+  long long *claim = malloc(sizeof(long long) * (size_t)m->nt);
+  for (int t = 0; t < m->nt; t++) claim[t] = 0x7fffffffffffffffLL;
+  for (int i = 0; i < nc; i++) {
+    if (cd[i].status != ST_VALID) continue;
+    long long key = ((long long)cd[i].prio << 32) | i;
+    for (int q = 0; q < cd[i].n + cd[i].r; q++) {
+      int t = cd[i].L[q];
+      if (claim[t] > key) claim[t] = key;
+    }
+  }
+  int nwin = 0;
+  int *win = calloc((size_t)nc, sizeof(int));
+  for (int i = 0; i < nc; i++) {
+    if (cd[i].status != ST_VALID) continue;
+    int ok = 1;
+    long long key = ((long long)cd[i].prio << 32) | i;
+    for (int q = 0; q < cd[i].n + cd[i].r && ok; q++)
+      if (claim[cd[i].L[q]] != key) ok = 0;
+    win[i] = ok;
+    if (ok) nwin++;
+    else st->lost++;
+  }
+  /* slot bases (winners take points/triangles/subsegments in candidate order) */
+  int pbase = m->np, tbase = m->nt, sbase = m->ns;
+  grow_points(m, pbase + nwin);
+  grow_tris(m, tbase + 2 * nwin);
+  if (is_seg) grow_segs(m, sbase + nwin);
+  int rank = 0;
+  for (int i = 0; i < nc; i++) {
+    if (!win[i]) continue;
+    int p = pbase + rank;
+    m->P[2 * p] = cd[i].c[0];
+    m->P[2 * p + 1] = cd[i].c[1];
+    m->pflag[p] = is_seg ? PF_SEGV : PF_FREE;
+    m->ppar[p] = is_seg ? m->SP[cd[i].id] : -1;
+    apply_star(m, &cd[i], is_seg ? cd[i].id : -1, p, tbase + 2 * rank,
+               is_seg ? sbase + rank : -1, is_seg);
+    rank++;
+  }
+  m->np = pbase + nwin;
+  m->nt = tbase + 2 * nwin;
+  if (is_seg) m->ns = sbase + nwin;
+  if (is_seg) st->seg_splits += nwin;
+  else st->inserted += nwin;
+  *progress = nwin + marks + (st->giveups - gave);
+  for (int i = 0; i < nc; i++) free(cd[i].L);
+  free(cd);
+  free(pool);
+  free(claim);
+  free(win);
+  return nwin;
+}
+
+static void mark_bad(Mesh *m) {
+  for (int t = 0; t < m->nt; t++) {
+    uint8_t f = m->tflag[t];
+    if (!(f & TF_ALIVE) || !(f & TF_CHECK)) continue;
+    f &= (uint8_t)~TF_CHECK;
+    if (tri_is_bad(m, t)) f |= TF_BAD;
+    else f &= (uint8_t)~TF_BAD;
+    m->tflag[t] = f;
+  }
+}
+
+orc_result *orc_refine(int np, const double *points, int nt, const int *tris,
+                       const int *nbrs, int ns, const int *segs,
+                       const orc_params *prm, orc_stats *st) {
+  Mesh M;
+  memset(&M, 0, sizeof(M));
+  memset(st, 0, sizeof(*st));
+  Mesh *m = &M;
+  double ga = cos(prm->theta_deg * 3.141592653589793238462643383279502884197169399375105820974944592308 / 180.0);
+  m->offconst = (ga == 1.0) ? 0.0 : 0.475 * sqrt((1.0 + ga) / (1.0 - ga));
+  m->goodangle = ga * ga;
+  m->lens = (2.0 * m->goodangle - 1.0) * (2.0 * m->goodangle - 1.0);
+  m->offcenter = prm->offcenter;
+  m->ruppert = prm->mode == 1;
+  grow_points(m, np);
+  memcpy(m->P, points, sizeof(double) * 2 * (size_t)np);
+  for (int i = 0; i < np; i++) { m->pflag[i] = PF_INPUT; m->ppar[i] = -1; }
+  m->np = np;
+  grow_tris(m, nt);
+  memcpy(m->TV, tris, sizeof(int) * 3 * (size_t)nt);
+  for (int t = 0; t < nt; t++) {
+    for (int j = 0; j < 3; j++) {
+      int u = nbrs[3 * t + j];
+      int code = -1;
+      if (u >= 0) { /* find the neighbour's edge that matches (reversed) */
+        int a = tris[3 * t + (j + 1) % 3], b = tris[3 * t + (j + 2) % 3];
+        for (int k = 0; k < 3; k++)
+          if (tris[3 * u + (k + 1) % 3] == b && tris[3 * u + (k + 2) % 3] == a) code = (u << 2) | k;
+      }
+      m->TN[3 * t + j] = code;
+      m->TS[3 * t + j] = -1;
+    }
+    m->tflag[t] = TF_ALIVE | TF_CHECK;
+  }
+  m->nt = nt;
+  /* input segments -> subsegments on triangle edges (the reference matches
+   * segments to edges by brute force, refine.cu:5618-5747; a vertex-keyed
+   * scan gives the same incidence) */
+  m->nseg_in = ns;
+  m->SEG = malloc(sizeof(int) * 2 * (size_t)(ns > 0 ? ns : 1));
+  memcpy(m->SEG, segs, sizeof(int) * 2 * (size_t)ns);
+  grow_segs(m, ns);
+  for (int s = 0; s < ns; s++) {
+    m->SV[2 * s] = segs[2 * s];
+    m->SV[2 * s + 1] = segs[2 * s + 1];
+    m->SP[s] = s;
+    m->ST[s] = -1;
+    m->senc[s] = 0;
+  }
+  m->ns = ns;
+  {
+    /* hash edges: for each segment find an adjacent triangle edge */
+    size_t hc = 1;
+    while (hc < (size_t)ns * 4 + 16) hc <<= 1;
+    long long *hk = malloc(sizeof(long long) * hc);
+    int *hv = malloc(sizeof(int) * hc);
+    for (size_t i = 0; i < hc; i++) hk[i] = -1;
+    for (int s = 0; s < ns; s++) {
+      long long a = segs[2 * s], b = segs[2 * s + 1];
+      long long key = a < b ? (a << 32) | b : (b << 32) | a;
+      size_t h = (size_t)((unsigned long long)key * 0x9E3779B97F4A7C15ULL) & (hc - 1);
+      while (hk[h] != -1) h = (h + 1) & (hc - 1);
+      hk[h] = key;
+      hv[h] = s;
+    }
+    for (int t = 0; t < nt; t++) {
+      for (int j = 0; j < 3; j++) {
+        long long a = tris[3 * t + (j + 1) % 3], b = tris[3 * t + (j + 2) % 3];
+        long long key = a < b ? (a << 32) | b : (b << 32) | a;
+        size_t h = (size_t)((unsigned long long)key * 0x9E3779B97F4A7C15ULL) & (hc - 1);
+        while (hk[h] != -1) {
+          if (hk[h] == key) {
+            int s = hv[h];
+            m->TS[3 * t + j] = s;
+            /* ST: prefer the side where the edge runs org->dest */
+            if (m->ST[s] < 0 || tris[3 * t + (j + 1) % 3] == segs[2 * s]) m->ST[s] = (t << 2) | j;
+            break;
+          }
+          h = (h + 1) & (hc - 1);
+        }
+      }
+    }
+    free(hk);
+    free(hv);
+  }
+  /* initial encroachment: every subsegment against both apexes */
+  for (int t = 0; t < nt; t++)
+    for (int j = 0; j < 3; j++) {
+      int s = m->TS[3 * t + j];
+      if (s >= 0 && encroaches(m, m->SV[2 * s], m->SV[2 * s + 1], PT(m, m->TV[3 * t + j])))
+        m->senc[s] = 1;
+    }
+
+  int K = prm->cav_cap0;
+  int *ids = NULL;
+  int idcap = 0;
+  st->status = 1;
+  for (int round = 0; round < prm->max_rounds; round++) {
+    st->rounds = round + 1;
+    /* segment phase */
+    for (;;) {
+      int ne = 0;
+      for (int s = 0; s < m->ns; s++) if (m->senc[s] == 1) ne++;
+      if (ne == 0) break;
+      if (ne > idcap) { idcap = ne; ids = xrealloc(ids, sizeof(int) * (size_t)idcap); }
+      ne = 0;
+      for (int s = 0; s < m->ns; s++) if (m->senc[s] == 1) ids[ne++] = s;
+      int cap = cands_per_pass(prm->cand_budget, K);
+      if (ne > cap) ne = cap;
+      int of = 0;
+      long long prog = 0;
+      run_pass(m, ids, ne, 1, K, prm->cav_cap_max, prm->priority, st, &of, &prog);
+      st->seg_passes++;
+      if (of && K < prm->cav_cap_max) K = K * 2 > prm->cav_cap_max ? prm->cav_cap_max : K * 2;
+    }
+    /* bad-triangle phase */
+    mark_bad(m);
+    int nb = 0;
+    for (int t = 0; t < m->nt; t++)
+      if ((m->tflag[t] & (TF_ALIVE | TF_BAD | TF_GIVEUP)) == (TF_ALIVE | TF_BAD)) nb++;
+    if (nb == 0) { st->status = 0; break; }
+    if (nb > idcap) { idcap = nb; ids = xrealloc(ids, sizeof(int) * (size_t)idcap); }
+    nb = 0;
+    for (int t = 0; t < m->nt; t++)
+      if ((m->tflag[t] & (TF_ALIVE | TF_BAD | TF_GIVEUP)) == (TF_ALIVE | TF_BAD)) ids[nb++] = t;
+    int cap = cands_per_pass(prm->cand_budget, K);
+    if (nb > cap) nb = cap;
+    int of = 0;
+    long long prog = 0;
+    run_pass(m, ids, nb, 0, K, prm->cav_cap_max, prm->priority, st, &of, &prog);
+    st->bad_passes++;
+    if (of && K < prm->cav_cap_max) {
+      K = K * 2 > prm->cav_cap_max ? prm->cav_cap_max : K * 2;
+      prog++;
+    }
+    if (prog == 0) { st->status = 2; break; } /* nothing can change any more */
+  }
+  st->final_cav_cap = K;
+  free(ids);
+
+  /* export: live triangles in slot order */
+  orc_result *r = calloc(1, sizeof(orc_result));
+  int *newid = malloc(sizeof(int) * (size_t)(m->nt > 0 ? m->nt : 1));
+  int nalive = 0;
+  for (int t = 0; t < m->nt; t++) newid[t] = (m->tflag[t] & TF_ALIVE) ? nalive++ : -1;
+  r->np = m->np;
+  r->nt = nalive;
+  r->ns = m->ns;
+  r->P = malloc(sizeof(double) * 2 * (size_t)(m->np > 0 ? m->np : 1));
+  memcpy(r->P, m->P, sizeof(double) * 2 * (size_t)m->np);
+  r->pflag = malloc((size_t)(m->np > 0 ? m->np : 1));
+  memcpy(r->pflag, m->pflag, (size_t)m->np);
+  r->T = malloc(sizeof(int) * 3 * (size_t)(nalive > 0 ? nalive : 1));
+  r->N = malloc(sizeof(int) * 3 * (size_t)(nalive > 0 ? nalive : 1));
+  for (int t = 0; t < m->nt; t++) {
+    int q = newid[t];
+    if (q < 0) continue;
+    for (int j = 0; j < 3; j++) {
+      r->T[3 * q + j] = m->TV[3 * t + j];
+      int nb = m->TN[3 * t + j];
+      r->N[3 * q + j] = nb < 0 ? -1 : newid[nb >> 2];
+    }
+  }
+  r->S = malloc(sizeof(int) * 2 * (size_t)(m->ns > 0 ? m->ns : 1));
+  memcpy(r->S, m->SV, sizeof(int) * 2 * (size_t)m->ns);
+  free(newid);
+  free(m->P); free(m->pflag); free(m->ppar);
+  free(m->TV); free(m->TN); free(m->TS); free(m->tflag);
+  free(m->SV); free(m->SP); free(m->ST); free(m->senc); free(m->SEG);
+  return r;
+}
+
+void orc_result_counts(const orc_result *r, int *np, int *nt, int *ns) {
+  *np = r->np;
+  *nt = r->nt;
+  *ns = r->ns;
+}
+
+void orc_result_copy(const orc_result *r, double *points, int *tris, int *nbrs,
+                     int *segs, uint8_t *pflag) {
+  if (points) memcpy(points, r->P, sizeof(double) * 2 * (size_t)r->np);
+  if (tris) memcpy(tris, r->T, sizeof(int) * 3 * (size_t)r->nt);
+  if (nbrs) memcpy(nbrs, r->N, sizeof(int) * 3 * (size_t)r->nt);
+  if (segs) memcpy(segs, r->S, sizeof(int) * 2 * (size_t)r->ns);
+  if (pflag) memcpy(pflag, r->pflag, (size_t)r->np);
+}
+
+void orc_result_free(orc_result *r) {
+  if (!r) return;
+  free(r->P); free(r->T); free(r->N); free(r->S); free(r->pflag);
+  free(r);
+}
