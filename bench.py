@@ -76,20 +76,30 @@ def make_input(args):
     return P, T, N, SS, time.time() - t0
 
 
-def cpu_baseline(P, T, SS, theta, budget_s):
-    """Reference Triangle -q on the same CDT, one core (bounded sample)."""
+def cpu_baseline(args, n_sample):
+    """Reference Triangle -q, one host core, on a bounded sample of the same
+    workload: an n_sample-point instance of the same PSLG family (same point
+    density, segment density and length cap, same seed), refined with the
+    reference's CPU_Triangle_Quality switches (main.cpp:389-407)."""
     try:
         from oracle import pyoracle
+        import gpu_cdt_refine_amd as g
         if not os.path.exists(pyoracle.TRIREF_SO):
             return None
+        L = args.domain * (n_sample / args.points) ** 0.5
+        P, S = g.generate_pslg(n_sample, int(args.segments * n_sample / args.points), L=L,
+                               maxlen=0.02 * args.domain, dist=0, min_angle=5.0, seed=args.seed)
+        T, N, SS = g.build_cdt(P, S)
         t0 = time.time()
-        Po, To, No, So = pyoracle.triangle_refine(P, T, SS, theta, mode=1)
+        Po, To, No, So = pyoracle.triangle_refine(P, T, SS, args.theta, mode=1)
         dt = time.time() - t0
-        return {"value": len(To) / dt, "unit": "output triangles/s", "cores": 1,
+        return {"value": len(To) / dt, "unit": "triangles/s", "cores": 1,
                 "kind": "reference",
-                "sample": f"full workload once: Triangle 1.6 'pzDQnrq{theta:g}' on the same "
-                          f"{len(P)}-point CDT ({dt:.1f} s)",
-                "seconds": dt, "triangles_out": int(len(To)), "points_out": int(len(Po))}
+                "sample": f"Triangle 1.6 'pzDQnrq{args.theta:g}' (reference GPU_Refine/triangle.cpp) "
+                          f"on a {n_sample}-point instance of the same PSLG family "
+                          f"({len(SS)} segments, same density); {dt:.1f} s on 1 core",
+                "seconds": dt, "triangles_out": int(len(To)), "points_out": int(len(Po)),
+                "points_in": int(len(P))}
     except Exception as e:  # baseline is informative only
         return {"error": str(e)}
 
@@ -107,6 +117,8 @@ def main():
     ap.add_argument("--pred-mode", type=int, default=0)
     ap.add_argument("--priority", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-points", type=int, default=200_000)
+    ap.add_argument("--claim-iters", type=int, default=1)
     ap.add_argument("--no-profile-step", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -119,7 +131,8 @@ def main():
     print(f"# input: {len(P)} points, {len(T)} triangles, {len(SS)} segments "
           f"(prep {prep_s:.1f}s)", file=sys.stderr, flush=True)
     sess = g.Session(P, T, N, SS, theta=args.theta, insert=g.Offcenter, mode=g.RUPPERT,
-                     pred_mode=args.pred_mode, priority=args.priority)
+                     pred_mode=args.pred_mode, priority=args.priority,
+                     claim_iters=args.claim_iters)
     for i in range(args.warmup):
         st = sess.run()
         print(f"# warmup {i}: {st['device_ms']:.1f} ms device, {st['n_triangles']} tris, "
@@ -147,7 +160,8 @@ def main():
     kt = None
     if not args.no_profile_step:
         prof = g.Session(P, T, N, SS, theta=args.theta, insert=g.Offcenter, mode=g.RUPPERT,
-                         pred_mode=args.pred_mode, priority=args.priority, profile=1)
+                         pred_mode=args.pred_mode, priority=args.priority,
+                         claim_iters=args.claim_iters, profile=1)
         prof.run()
         kt = prof.kernel_times()
         prof.close()
@@ -163,7 +177,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(P, T, SS, args.theta, 30.0)
+        cpu = cpu_baseline(args, args.cpu_sample_points)
 
     out = {
         "metric": "output triangles/sec (refine, theta=25 deg, off-center, Ruppert)",
@@ -182,14 +196,16 @@ def main():
         "config": workload_config(args),
         "output": {"points": int(st["n_points"]), "triangles": tris,
                    "subsegments": int(st["n_subsegs"]), "rounds": st["rounds"],
-                   "status": st["status"], "giveups": st["giveups"]},
+                   "seg_passes": st["seg_passes"], "bad_passes": st["bad_passes"],
+                   "status": st["status"], "giveups": st["giveups"],
+                   "cand_evals": st["cand_evals"], "lost": st["lost"],
+                   "big_cavities": st["big_cavities"], "inserted": st["inserted"],
+                   "seg_splits": st["seg_splits"]},
         "roofline": roof,
         "whole_job": whole,
         "cpu_baseline": cpu,
         "kernel_times_ms": {k: round(v["total_ms"], 3) for k, v in kt.items()} if kt else None,
     }
-    if cpu and "points_out" in cpu:
-        out["output"]["points_vs_triangle_q"] = st["n_points"] / cpu["points_out"]
     print(json.dumps(out))
 
 

@@ -37,6 +37,9 @@ class Options(C.Structure):
         ("pred_mode", C.c_int),
         ("profile", C.c_int),
         ("device", C.c_int),
+        ("cand_factor", C.c_int),
+        ("cand_min", C.c_int),
+        ("claim_iters", C.c_int),
     ]
 
 
@@ -61,6 +64,8 @@ class Stats(C.Structure):
         ("n_points", C.c_longlong),
         ("n_triangles", C.c_longlong),
         ("n_subsegs", C.c_longlong),
+        ("cand_evals", C.c_longlong),
+        ("big_cavities", C.c_longlong),
     ]
 
     def as_dict(self):

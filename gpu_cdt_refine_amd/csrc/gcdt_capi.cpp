@@ -38,12 +38,15 @@ void gcdt_default_options(gcdt_options *o) {
   o->mode = 1;
   o->cav_cap0 = 32;
   o->cav_cap_max = 16384;
-  o->cand_budget = 1LL << 28;
+  o->cand_budget = 1LL << 30;
   o->max_rounds = 100000;
   o->priority = 1;
   o->pred_mode = 0;
   o->profile = 0;
   o->device = 0;
+  o->cand_factor = 4;
+  o->cand_min = 16384;
+  o->claim_iters = 1;
 }
 
 static bool valid_options(const gcdt_options *o) {
@@ -52,6 +55,7 @@ static bool valid_options(const gcdt_options *o) {
   if (o->cav_cap0 < 4 || o->cav_cap_max < o->cav_cap0) { g_err = "bad cavity capacity"; return false; }
   if (o->cand_budget < 6LL * o->cav_cap_max + 16) { g_err = "cand_budget smaller than one candidate at cav_cap_max"; return false; }
   if (o->priority < 0 || o->priority > 2) { g_err = "priority must be 0, 1 or 2"; return false; }
+  if (o->claim_iters < 1 || o->claim_iters > 64) { g_err = "claim_iters must be in [1, 64]"; return false; }
   return true;
 }
 

@@ -35,7 +35,8 @@ enum : int {
   ST_REJ_ENC = 1,
   ST_REJ_BLOCKED = 2,
   ST_OVERFLOW = 3,
-  ST_GIVEUP = 4
+  ST_GIVEUP = 4,
+  ST_BIG = 5  // cavity larger than the LDS list; recomputed in global memory
 };
 enum : int {
   CNT_MARKS = 0,
@@ -48,6 +49,8 @@ enum : int {
   CNT_FP64,
   CNT_EXACT,
   CNT_ERRORS,
+  CNT_VALID,
+  CNT_BIG,
   CNT_N = 16
 };
 
@@ -85,7 +88,14 @@ struct PassD {
   double *pws;  // exact-predicate arena, GCDT_INCIRCLE_WS doubles per thread
   unsigned long long *cnt;
   const unsigned short *learned_prio;  // optional per-candidate priority
+  unsigned long long epoch0;  // first claim epoch of this pass
+  int iter;                   // claim iteration
 };
+
+// Per-thread cavity list kept in LDS: cavity (<= LDS_CAV) + ring (<= LDS_CAV+2).
+constexpr int LDS_CAV = 32;
+constexpr int LDS_LIST = 2 * LDS_CAV + 2;
+constexpr int CAV_BLOCK = 128;
 
 __device__ __forceinline__ int comp(const int4 &v, int j) {
   return j == 0 ? v.x : (j == 1 ? v.y : v.z);

@@ -59,9 +59,10 @@ class Refiner {
 
  private:
   template <class Pred>
-  int compact(Pred pred, int n, int *out_list, int *out_rank);
+  int compact(Pred pred, int n, int *out_list, int *out_rank, const int *src);
   int run_pass(const int *ids, int nc, bool is_seg, long long *progress,
                bool *overflowed);
+  int select_by_priority(int *ids, int nb, int cap, int **out_ids);
   void ensure_capacity(long long needP, long long needT, long long needS);
   void ensure_cand(int nc);
   void ensure_tmp(int n);
@@ -85,13 +86,16 @@ class Refiner {
   int c_cap = 0;
   double2 *c_pt = nullptr;
   int *c_status = nullptr, *c_n = nullptr, *c_r = nullptr, *c_prio = nullptr,
-      *c_win = nullptr, *c_rank = nullptr;
+      *c_win = nullptr, *c_rank = nullptr, *c_act = nullptr;
   int *c_ws = nullptr;
   long long ws_ints = 0;
   double *c_pws = nullptr;
   int pws_blocks = 0;
   int *tmp_list = nullptr;
   int tmp_list_cap = 0;
+  int *sel_prio = nullptr, *sel_tmp = nullptr, *sel_list = nullptr;
+  int sel_cap = 0;
+  unsigned *d_hist = nullptr;
   // compaction scratch
   int *cb_cnt = nullptr, *cb_off = nullptr;
   int cb_cap = 0;

@@ -57,6 +57,19 @@ struct PredAlive {
   const uint8_t *tflag;
   __device__ bool operator()(int i) const { return (tflag[i] & TF_ALIVE) != 0; }
 };
+struct PredPrioEq {
+  const int *prio;
+  int b;
+  __device__ bool operator()(int i) const { return prio[i] == b; }
+};
+struct PredPrioSel {  // (prio, index) <= (b, icut)
+  const int *prio;
+  int b, icut;
+  __device__ bool operator()(int i) const {
+    int p = prio[i];
+    return p < b || (p == b && i <= icut);
+  }
+};
 
 template <class Pred>
 __global__ __launch_bounds__(CB) void k_count(Pred pred, int n, int *blockcnt) {
@@ -107,7 +120,8 @@ __global__ __launch_bounds__(1024) void k_scan(const int *blockcnt, int nb,
 template <class Pred>
 __global__ __launch_bounds__(CB) void k_scatter(Pred pred, int n,
                                                 const int *blockoff,
-                                                int *out_list, int *out_rank) {
+                                                int *out_list, int *out_rank,
+                                                const int *src) {
   __shared__ int wcnt[CB / 64];
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int base = blockIdx.x * CT;
@@ -127,7 +141,7 @@ __global__ __launch_bounds__(CB) void k_scatter(Pred pred, int n,
     }
     if (f) {
       int pos = run + pre + __popcll(b & lt);
-      if (out_list) out_list[pos] = i;
+      if (out_list) out_list[pos] = src ? src[i] : i;
       if (out_rank) out_rank[i] = pos;
     }
     run += tot;
@@ -274,125 +288,202 @@ __device__ __forceinline__ void flush_stats(unsigned long long *cnt,
   if (st.exact_calls) atomicAdd(&cnt[CNT_EXACT], st.exact_calls);
 }
 
-// Segment pass: split point + cavity on both sides of the subsegment
-// (refine.cu:2544-2927 semantics for choosing and placing the split).
-__global__ __launch_bounds__(256) void k_seg_cavity(MeshD m, ParamsD pr,
-                                                    PassD ps, int Kmax) {
-  PredStats st = {0, 0, 0};
-  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
-  for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x) {
-    int s = ps.ids[i];
-    int *L = ps.ws + (size_t)i * ps.R;
-    int status;
-    double cx = 0.0, cy = 0.0;
-    int n = 0, r = 0;
-    if (!split_point(m, s, cx, cy)) {
-      status = ST_GIVEUP;
-    } else {
-      int t1, j1, t2, j2;
-      seg_sides(m, s, t1, j1, t2, j2);
-      int seed0 = t1 >= 0 ? t1 : t2;
-      int seed1 = t1 >= 0 ? t2 : -1;
-      status = build_cavity(m, L, cx, cy, seed0, seed1, s, ps.K, pws,
-                            pr.use_fp32, &st, n, r);
-      if (status == ST_OVERFLOW && ps.K >= Kmax) status = ST_GIVEUP;
-    }
-    ps.cpt[i] = make_double2(cx, cy);
-    ps.cstatus[i] = status;
-    ps.cn[i] = n;
-    ps.cr[i] = r;
-    ps.cprio[i] = 0;
-    if (status == ST_OVERFLOW) atomicAdd(&ps.cnt[CNT_OVERFLOW], 1ull);
-    if (status == ST_GIVEUP) {
-      atomicAdd(&ps.cnt[CNT_GIVEUPS], 1ull);
-      m.senc[s] = 2;
-    }
-  }
-  flush_stats(ps.cnt, st);
-}
-
 __device__ __forceinline__ void mark_enc(const MeshD &m, int s,
                                          unsigned long long *cnt) {
   if (atomicCAS(&m.senc[s], 0, 1) == 0) atomicAdd(&cnt[CNT_MARKS], 1ull);
 }
 
-// Bad pass: Steiner point, walk, cavity, encroachment rule
-// (refine.cu:3870-4599, 5061-5089 semantics).
-__global__ __launch_bounds__(256) void k_bad_cavity(MeshD m, ParamsD pr,
-                                                    PassD ps, int Kmax) {
+// Candidate evaluation.  L is the cavity list storage with room for `cap`
+// cavity triangles plus the ring: a per-thread LDS slice in the fast kernels
+// (cap = min(K, LDS_CAV)) or the candidate's global workspace in the big
+// kernels (cap = K).  A fast-path overflow with K > LDS_CAV becomes ST_BIG and
+// is recomputed by the big kernel; no side effect has happened by then, so
+// both paths give exactly the result of one evaluation with capacity K.
+template <bool FAST>
+__device__ __forceinline__ void eval_seg(const MeshD &m, const ParamsD &pr,
+                                         const PassD &ps, int i, int *L,
+                                         int cap, int Kmax, double *pws,
+                                         PredStats *st) {
+  int s = ps.ids[i];
+  int status;
+  double cx = 0.0, cy = 0.0;
+  int n = 0, r = 0;
+  if (!split_point(m, s, cx, cy)) {
+    status = ST_GIVEUP;
+  } else {
+    int t1, j1, t2, j2;
+    seg_sides(m, s, t1, j1, t2, j2);
+    int seed0 = t1 >= 0 ? t1 : t2;
+    int seed1 = t1 >= 0 ? t2 : -1;
+    status = build_cavity(m, L, cx, cy, seed0, seed1, s, cap, pws,
+                          pr.use_fp32, st, n, r);
+    if (status == ST_OVERFLOW) {
+      if (FAST && ps.K > cap) status = ST_BIG;
+      else if (ps.K >= Kmax) status = ST_GIVEUP;
+    }
+  }
+  ps.cstatus[i] = status;
+  if (status == ST_BIG) {
+    atomicAdd(&ps.cnt[CNT_BIG], 1ull);
+    return;
+  }
+  ps.cpt[i] = make_double2(cx, cy);
+  ps.cn[i] = n;
+  ps.cr[i] = r;
+  ps.cprio[i] = 0;
+  if (status == ST_VALID) {
+    atomicAdd(&ps.cnt[CNT_VALID], 1ull);
+    if (FAST) {
+      int *G = ps.ws + (size_t)i * ps.R;
+      for (int q = 0; q < n + r; q++) G[q] = L[q];
+    }
+  } else if (status == ST_OVERFLOW) {
+    atomicAdd(&ps.cnt[CNT_OVERFLOW], 1ull);
+  } else if (status == ST_GIVEUP) {
+    atomicAdd(&ps.cnt[CNT_GIVEUPS], 1ull);
+    m.senc[s] = 2;
+  }
+}
+
+template <bool FAST>
+__device__ __forceinline__ void eval_bad(const MeshD &m, const ParamsD &pr,
+                                         const PassD &ps, int i, int *L,
+                                         int cap, int Kmax, double *pws,
+                                         PredStats *st) {
+  bool fp32 = pr.use_fp32 != 0;
+  int t0 = ps.ids[i];
+  int4 v0 = m.TV[t0];
+  int prio = 0;
+  if (pr.priority == 1) prio = shortest_edge_bucket(m, v0);
+  else if (pr.priority == 2 && ps.learned_prio) prio = ps.learned_prio[i];
+  int status = ST_VALID;
+  double cx = 0.0, cy = 0.0;
+  int n = 0, r = 0;
+  if (!steiner_point(m, pr, v0, cx, cy)) {
+    status = ST_GIVEUP;
+  } else {
+    int tl = -1, jl = -1, sg = -1;
+    int w = walk(m, t0, cx, cy, fp32, st, tl, jl, sg);
+    int seed1 = -1;
+    if (w == 3) {
+      status = ST_REJ_BLOCKED;
+      mark_enc(m, sg, ps.cnt);
+    } else if (w == 2 || w == 4) {
+      status = ST_GIVEUP;
+    } else if (w == 1) {
+      int s = comp(m.TS[tl], jl);
+      if (s >= 0) {
+        status = ST_REJ_BLOCKED;
+        mark_enc(m, s, ps.cnt);
+      } else {
+        int nb = comp(m.TN[tl], jl);
+        if (nb < 0) status = ST_GIVEUP;
+        else seed1 = nb >> 2;
+      }
+    }
+    if (status == ST_VALID) {
+      status = build_cavity(m, L, cx, cy, tl, seed1, -1, cap, pws, fp32, st,
+                            n, r);
+      if (status == ST_OVERFLOW) {
+        if (FAST && ps.K > cap) status = ST_BIG;
+        else if (ps.K >= Kmax) status = ST_GIVEUP;
+      }
+    }
+    if (status == ST_VALID) {
+      // Ruppert/Chew rule: a Steiner point that encroaches a subsegment on its
+      // cavity boundary is not inserted; every such subsegment is marked.
+      bool enc = false;
+      for (int idx = 0; idx < n; idx++) {
+        int t = L[idx];
+        int4 ts = m.TS[t];
+        for (int j = 0; j < 3; j++) {
+          int s = comp(ts, j);
+          if (s < 0) continue;
+          int2 sv = m.SV[s];
+          if (encroaches(m, pr, sv.x, sv.y, cx, cy)) {
+            mark_enc(m, s, ps.cnt);
+            enc = true;
+          }
+        }
+      }
+      if (enc) status = ST_REJ_ENC;
+    }
+  }
+  ps.cstatus[i] = status;
+  if (status == ST_BIG) {
+    atomicAdd(&ps.cnt[CNT_BIG], 1ull);
+    return;
+  }
+  ps.cpt[i] = make_double2(cx, cy);
+  ps.cn[i] = n;
+  ps.cr[i] = r;
+  ps.cprio[i] = prio;
+  if (status == ST_VALID) {
+    atomicAdd(&ps.cnt[CNT_VALID], 1ull);
+    if (FAST) {
+      int *G = ps.ws + (size_t)i * ps.R;
+      for (int q = 0; q < n + r; q++) G[q] = L[q];
+    }
+  } else if (status == ST_OVERFLOW) {
+    atomicAdd(&ps.cnt[CNT_OVERFLOW], 1ull);
+  } else if (status == ST_REJ_ENC) {
+    atomicAdd(&ps.cnt[CNT_REJ_ENC], 1ull);
+  } else if (status == ST_REJ_BLOCKED) {
+    atomicAdd(&ps.cnt[CNT_REJ_BLOCKED], 1ull);
+  } else if (status == ST_GIVEUP) {
+    atomicAdd(&ps.cnt[CNT_GIVEUPS], 1ull);
+    m.tflag[t0] |= TF_GIVEUP;
+  }
+}
+
+// Fast kernels: per-thread cavity list in LDS.
+__global__ __launch_bounds__(CAV_BLOCK) void k_seg_cavity(MeshD m, ParamsD pr,
+                                                          PassD ps, int Kmax) {
+  __shared__ int lds[CAV_BLOCK * LDS_LIST];
   PredStats st = {0, 0, 0};
   int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
-  bool fp32 = pr.use_fp32 != 0;
-  for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x) {
-    int t0 = ps.ids[i];
-    int *L = ps.ws + (size_t)i * ps.R;
-    int4 v0 = m.TV[t0];
-    int prio = 0;
-    if (pr.priority == 1) prio = shortest_edge_bucket(m, v0);
-    else if (pr.priority == 2 && ps.learned_prio) prio = ps.learned_prio[i];
-    int status = ST_VALID;
-    double cx = 0.0, cy = 0.0;
-    int n = 0, r = 0;
-    if (!steiner_point(m, pr, v0, cx, cy)) {
-      status = ST_GIVEUP;
-    } else {
-      int tl = -1, jl = -1, sg = -1;
-      int w = walk(m, t0, cx, cy, fp32, &st, tl, jl, sg);
-      int seed1 = -1;
-      if (w == 3) {
-        status = ST_REJ_BLOCKED;
-        mark_enc(m, sg, ps.cnt);
-      } else if (w == 2 || w == 4) {
-        status = ST_GIVEUP;
-      } else if (w == 1) {
-        int s = comp(m.TS[tl], jl);
-        if (s >= 0) {
-          status = ST_REJ_BLOCKED;
-          mark_enc(m, s, ps.cnt);
-        } else {
-          int nb = comp(m.TN[tl], jl);
-          if (nb < 0) status = ST_GIVEUP;
-          else seed1 = nb >> 2;
-        }
-      }
-      if (status == ST_VALID) {
-        status = build_cavity(m, L, cx, cy, tl, seed1, -1, ps.K, pws, fp32,
-                              &st, n, r);
-        if (status == ST_OVERFLOW && ps.K >= Kmax) status = ST_GIVEUP;
-      }
-      if (status == ST_VALID) {
-        bool enc = false;
-        for (int idx = 0; idx < n; idx++) {
-          int t = L[idx];
-          int4 ts = m.TS[t];
-          for (int j = 0; j < 3; j++) {
-            int s = comp(ts, j);
-            if (s < 0) continue;
-            int2 sv = m.SV[s];
-            if (encroaches(m, pr, sv.x, sv.y, cx, cy)) {
-              mark_enc(m, s, ps.cnt);
-              enc = true;
-            }
-          }
-        }
-        if (enc) status = ST_REJ_ENC;
-      }
-    }
-    ps.cpt[i] = make_double2(cx, cy);
-    ps.cstatus[i] = status;
-    ps.cn[i] = n;
-    ps.cr[i] = r;
-    ps.cprio[i] = prio;
-    if (status == ST_OVERFLOW) atomicAdd(&ps.cnt[CNT_OVERFLOW], 1ull);
-    else if (status == ST_REJ_ENC) atomicAdd(&ps.cnt[CNT_REJ_ENC], 1ull);
-    else if (status == ST_REJ_BLOCKED) atomicAdd(&ps.cnt[CNT_REJ_BLOCKED], 1ull);
-    else if (status == ST_GIVEUP) {
-      atomicAdd(&ps.cnt[CNT_GIVEUPS], 1ull);
-      m.tflag[t0] |= TF_GIVEUP;
-    }
-  }
+  int *L = lds + threadIdx.x * LDS_LIST;
+  int cap = ps.K < LDS_CAV ? ps.K : LDS_CAV;
+  for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x)
+    eval_seg<true>(m, pr, ps, i, L, cap, Kmax, pws, &st);
+  flush_stats(ps.cnt, st);
+}
+
+__global__ __launch_bounds__(CAV_BLOCK) void k_bad_cavity(MeshD m, ParamsD pr,
+                                                          PassD ps, int Kmax) {
+  __shared__ int lds[CAV_BLOCK * LDS_LIST];
+  PredStats st = {0, 0, 0};
+  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
+  int *L = lds + threadIdx.x * LDS_LIST;
+  int cap = ps.K < LDS_CAV ? ps.K : LDS_CAV;
+  for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x)
+    eval_bad<true>(m, pr, ps, i, L, cap, Kmax, pws, &st);
+  flush_stats(ps.cnt, st);
+}
+
+// Big kernels: candidates whose cavity exceeded the LDS list, recomputed
+// with the global workspace and the full capacity K.
+__global__ __launch_bounds__(256) void k_seg_cavity_big(MeshD m, ParamsD pr,
+                                                        PassD ps, int Kmax) {
+  PredStats st = {0, 0, 0};
+  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
+  for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x)
+    if (ps.cstatus[i] == ST_BIG)
+      eval_seg<false>(m, pr, ps, i, ps.ws + (size_t)i * ps.R, ps.K, Kmax, pws, &st);
+  flush_stats(ps.cnt, st);
+}
+
+__global__ __launch_bounds__(256) void k_bad_cavity_big(MeshD m, ParamsD pr,
+                                                        PassD ps, int Kmax) {
+  PredStats st = {0, 0, 0};
+  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
+  for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x)
+    if (ps.cstatus[i] == ST_BIG)
+      eval_bad<false>(m, pr, ps, i, ps.ws + (size_t)i * ps.R, ps.K, Kmax, pws, &st);
   flush_stats(ps.cnt, st);
 }
 
@@ -406,10 +497,16 @@ __device__ __forceinline__ unsigned long long claim_key(unsigned long long epoch
          (unsigned long long)(0xffffffffu - (unsigned)i);
 }
 
-__global__ __launch_bounds__(256) void k_claim(MeshD m, PassD ps) {
+// Candidate i takes part in claim iteration ps.iter when it is valid, has not
+// won and touches no winner's region (cact != 0).
+__global__ __launch_bounds__(256) void k_claim(MeshD m, PassD ps, int *cact) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ps.nc;
        i += gridDim.x * blockDim.x) {
-    if (ps.cstatus[i] != ST_VALID) continue;
+    if (ps.iter == 0) {
+      cact[i] = ps.cstatus[i] == ST_VALID;
+      ps.cwin[i] = 0;
+    }
+    if (!cact[i]) continue;
     unsigned long long key = claim_key(ps.epoch, ps.cprio[i], i);
     const int *L = ps.ws + (size_t)i * ps.R;
     int k = ps.cn[i] + ps.cr[i];
@@ -417,29 +514,47 @@ __global__ __launch_bounds__(256) void k_claim(MeshD m, PassD ps) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_win(MeshD m, PassD ps) {
+__global__ __launch_bounds__(256) void k_win(MeshD m, PassD ps, const int *cact) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ps.nc;
        i += gridDim.x * blockDim.x) {
-    int w = 0;
-    if (ps.cstatus[i] == ST_VALID) {
-      unsigned long long key = claim_key(ps.epoch, ps.cprio[i], i);
-      const int *L = ps.ws + (size_t)i * ps.R;
-      int k = ps.cn[i] + ps.cr[i];
-      w = 1;
-      for (int q = 0; q < k && w; q++)
-        if (m.claim[L[q]] != key) w = 0;
-      if (!w) atomicAdd(&ps.cnt[CNT_LOST], 1ull);
+    if (!cact[i]) continue;
+    unsigned long long key = claim_key(ps.epoch, ps.cprio[i], i);
+    const int *L = ps.ws + (size_t)i * ps.R;
+    int k = ps.cn[i] + ps.cr[i];
+    int w = 1;
+    for (int q = 0; q < k && w; q++)
+      if (m.claim[L[q]] != key) w = 0;
+    if (w) ps.cwin[i] = 1;
+  }
+}
+
+// After a claim iteration: winners and candidates touching a winner's region
+// (a triangle whose claim comes from this pass and from a winner) drop out.
+__global__ __launch_bounds__(256) void k_block(MeshD m, PassD ps, int *cact) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ps.nc;
+       i += gridDim.x * blockDim.x) {
+    if (!cact[i]) continue;
+    if (ps.cwin[i]) { cact[i] = 0; continue; }
+    const int *L = ps.ws + (size_t)i * ps.R;
+    int k = ps.cn[i] + ps.cr[i];
+    for (int q = 0; q < k; q++) {
+      unsigned long long c = m.claim[L[q]];
+      if ((c >> 45) >= ps.epoch0) {
+        int j = (int)(0xffffffffu - (unsigned)(c & 0xffffffffull));
+        if (ps.cwin[j]) { cact[i] = 0; break; }
+      }
     }
-    ps.cwin[i] = w;
   }
 }
 
 // Retriangulate a winner's cavity as the star of its new vertex; same star
 // order and slot rule as the oracle.  Also writes the new point, and for a
 // segment split the two subsegment halves.
-__global__ __launch_bounds__(256) void k_apply(MeshD m, ParamsD pr, PassD ps,
-                                               int pbase, int tbase, int sbase,
-                                               int is_seg) {
+__global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
+                                                     PassD ps, int pbase,
+                                                     int tbase, int sbase,
+                                                     int is_seg) {
+  __shared__ int lds[CAV_BLOCK * LDS_CAV];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ps.nc;
        i += gridDim.x * blockDim.x) {
     if (!ps.cwin[i]) continue;
@@ -449,8 +564,13 @@ __global__ __launch_bounds__(256) void k_apply(MeshD m, ParamsD pr, PassD ps,
     int id = ps.ids[i];
     int split_s = is_seg ? id : -1;
     double2 c = ps.cpt[i];
-    const int *L = ps.ws + (size_t)i * ps.R;
     int n = ps.cn[i];
+    const int *L = ps.ws + (size_t)i * ps.R;
+    if (n <= LDS_CAV) {  // membership scans below run on an LDS copy
+      int *Ls = lds + threadIdx.x * LDS_CAV;
+      for (int q = 0; q < n; q++) Ls[q] = L[q];
+      L = Ls;
+    }
     int k = n + 2;
     int *Bu = ps.ws + (size_t)i * ps.R + (2 * ps.K + 2);
     int *Bv = Bu + (ps.K + 2);
@@ -675,7 +795,8 @@ Refiner::~Refiner() {
   dfree(in_P); dfree(in_T); dfree(in_N); dfree(in_S);
   dfree(c_pt); dfree(c_status); dfree(c_n); dfree(c_r);
   dfree(c_prio); dfree(c_win); dfree(c_rank); dfree(c_ws); dfree(c_pws);
-  dfree(tmp_list);
+  dfree(c_act); dfree(tmp_list);
+  dfree(sel_prio); dfree(sel_tmp); dfree(sel_list); dfree(d_hist);
   dfree(cb_cnt); dfree(cb_off); dfree(d_total); dfree(d_cnt);
   dfree(o_T); dfree(o_N); dfree(o_tstat); dfree(o_pstat); dfree(newid);
   dfree(h_key); dfree(h_val);
@@ -795,7 +916,8 @@ const char *Refiner::kname(int id) {
 }
 
 template <class Pred>
-int Refiner::compact(Pred pred, int n, int *out_list, int *out_rank) {
+int Refiner::compact(Pred pred, int n, int *out_list, int *out_rank,
+                     const int *src) {
   if (n <= 0) return 0;
   int nb = (n + CT - 1) / CT;
   if (nb > cb_cap) {
@@ -808,7 +930,7 @@ int Refiner::compact(Pred pred, int n, int *out_list, int *out_rank) {
   tbegin(K_COMPACT);
   k_count<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_cnt);
   k_scan<<<1, 1024, 0, stream>>>(cb_cnt, nb, cb_off, d_total);
-  k_scatter<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_off, out_list, out_rank);
+  k_scatter<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_off, out_list, out_rank, src);
   tend(K_COMPACT, n);
   HC(hipGetLastError());
   HC(hipMemcpyAsync(h_scalar, d_total, sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -825,7 +947,8 @@ void Refiner::ensure_cand(int nc) {
   if (nc <= c_cap) return;
   int c = std::max(nc, c_cap * 2);
   dfree(c_pt); dfree(c_status); dfree(c_n); dfree(c_r);
-  dfree(c_prio); dfree(c_win); dfree(c_rank);
+  dfree(c_prio); dfree(c_win); dfree(c_rank); dfree(c_act);
+  dalloc(&c_act, (size_t)c);
   dalloc(&c_pt, (size_t)c);
   dalloc(&c_status, (size_t)c);
   dalloc(&c_n, (size_t)c);
@@ -843,6 +966,64 @@ int Refiner::cands_per_pass(int K) const {
   return (int)c;
 }
 
+// Bad-pass candidate selection: when more triangles are bad than a pass
+// takes, keep the `cap` smallest (priority, index) candidates, in index order
+// -- the order Triangle's bad-triangle queue serves them
+// (triangle.cpp:6932-6968).  Histogram of priority buckets -> cut bucket ->
+// ordered compactions; no host-side sort.
+constexpr int NBUCKET = 8192;
+
+__global__ __launch_bounds__(256) void k_prio_hist(MeshD m, const int *ids,
+                                                   int n, int *prio,
+                                                   unsigned *hist) {
+  __shared__ unsigned h[NBUCKET];
+  for (int b = threadIdx.x; b < NBUCKET; b += blockDim.x) h[b] = 0;
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    int p = shortest_edge_bucket(m, m.TV[ids[i]]);
+    prio[i] = p;
+    atomicAdd(&h[p], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < NBUCKET; b += blockDim.x)
+    if (h[b]) atomicAdd(&hist[b], h[b]);
+}
+
+int Refiner::select_by_priority(int *ids, int nb, int cap, int **out_ids) {
+  *out_ids = ids;
+  if (nb <= cap) return nb;
+  if (opt.priority != 1) return cap;  // index order
+  if (nb > sel_cap) {
+    dfree(sel_prio); dfree(sel_tmp); dfree(sel_list);
+    sel_cap = std::max(nb, 2 * sel_cap);
+    dalloc(&sel_prio, (size_t)sel_cap);
+    dalloc(&sel_tmp, (size_t)sel_cap);
+    dalloc(&sel_list, (size_t)sel_cap);
+  }
+  if (!d_hist) dalloc(&d_hist, NBUCKET);
+  HC(hipMemsetAsync(d_hist, 0, sizeof(unsigned) * NBUCKET, stream));
+  k_prio_hist<<<std::min(nblk(nb), 1024), 256, 0, stream>>>(m, ids, nb, sel_prio, d_hist);
+  HC(hipGetLastError());
+  std::vector<unsigned> h(NBUCKET);
+  HC(hipMemcpyAsync(h.data(), d_hist, sizeof(unsigned) * NBUCKET, hipMemcpyDeviceToHost, stream));
+  HC(hipStreamSynchronize(stream));
+  long long acc = 0;
+  int bcut = 0;
+  for (; bcut < NBUCKET; bcut++) {
+    if (acc + h[bcut] >= (unsigned long long)cap) break;
+    acc += h[bcut];
+  }
+  int take = (int)(cap - acc);  // 1 <= take <= h[bcut]
+  compact(PredPrioEq{sel_prio, bcut}, nb, sel_tmp, nullptr, nullptr);
+  HC(hipMemcpyAsync(h_scalar + 1, sel_tmp + (take - 1), sizeof(int), hipMemcpyDeviceToHost, stream));
+  HC(hipStreamSynchronize(stream));
+  int icut = h_scalar[1];
+  int got = compact(PredPrioSel{sel_prio, bcut, icut}, nb, sel_list, nullptr, ids);
+  *out_ids = sel_list;
+  return got;
+}
+
 // One conflict pass; returns number of winners.
 int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
                       bool *overflowed) {
@@ -851,11 +1032,14 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   ps.nc = nc;
   ps.K = K;
   ps.R = 6 * K + 16;
-  if (++epoch >= (1ull << 19)) {  // epoch wrap: clear claims
+  int iters = std::max(1, opt.claim_iters);
+  if (epoch + iters + 1 >= (1ull << 19)) {  // epoch wrap: clear claims
     HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
-    epoch = 1;
+    epoch = 0;
   }
-  ps.epoch = epoch;
+  ps.epoch0 = epoch + 1;
+  ps.epoch = epoch + 1;
+  ps.iter = 0;
   ps.cpt = c_pt;
   ps.cstatus = c_status;
   ps.cn = c_n;
@@ -868,30 +1052,39 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   ps.cnt = d_cnt;
   ps.learned_prio = learned_prio;
   HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
+  int gf = std::min(nblk(nc, CAV_BLOCK), 2 * pws_blocks);
   int gb = std::min(nblk(nc), pws_blocks);
   if (is_seg) {
     tbegin(K_SEGCAV);
-    k_seg_cavity<<<gb, 256, 0, stream>>>(m, pr, ps, opt.cav_cap_max);
+    k_seg_cavity<<<gf, CAV_BLOCK, 0, stream>>>(m, pr, ps, opt.cav_cap_max);
+    if (K > LDS_CAV) k_seg_cavity_big<<<gb, 256, 0, stream>>>(m, pr, ps, opt.cav_cap_max);
     tend(K_SEGCAV, nc);
   } else {
     tbegin(K_BADCAV);
-    k_bad_cavity<<<gb, 256, 0, stream>>>(m, pr, ps, opt.cav_cap_max);
+    k_bad_cavity<<<gf, CAV_BLOCK, 0, stream>>>(m, pr, ps, opt.cav_cap_max);
+    if (K > LDS_CAV) k_bad_cavity_big<<<gb, 256, 0, stream>>>(m, pr, ps, opt.cav_cap_max);
     tend(K_BADCAV, nc);
   }
   HC(hipGetLastError());
   int gc = std::min(nblk(nc), 4096);
-  tbegin(K_CLAIM);
-  k_claim<<<gc, 256, 0, stream>>>(m, ps);
-  tend(K_CLAIM, nc);
-  tbegin(K_WIN);
-  k_win<<<gc, 256, 0, stream>>>(m, ps);
-  tend(K_WIN, nc);
-  HC(hipGetLastError());
-  int nwin = compact(PredWin{c_win}, nc, nullptr, c_rank);
+  for (int it = 0; it < iters; it++) {
+    ps.epoch = ++epoch;
+    ps.iter = it;
+    tbegin(K_CLAIM);
+    k_claim<<<gc, 256, 0, stream>>>(m, ps, c_act);
+    tend(K_CLAIM, nc);
+    tbegin(K_WIN);
+    k_win<<<gc, 256, 0, stream>>>(m, ps, c_act);
+    if (it + 1 < iters) k_block<<<gc, 256, 0, stream>>>(m, ps, c_act);
+    tend(K_WIN, nc);
+    HC(hipGetLastError());
+  }
+  int nwin = compact(PredWin{c_win}, nc, nullptr, c_rank, nullptr);
   ensure_capacity(np + nwin, nt + 2LL * nwin, is_seg ? ns + nwin : ns);
   if (nwin > 0) {
     tbegin(K_APPLY);
-    k_apply<<<gc, 256, 0, stream>>>(m, pr, ps, np, nt, ns, is_seg ? 1 : 0);
+    k_apply<<<std::min(nblk(nc, CAV_BLOCK), 8192), CAV_BLOCK, 0, stream>>>(
+        m, pr, ps, np, nt, ns, is_seg ? 1 : 0);
     tend(K_APPLY, nwin);
     HC(hipGetLastError());
   }
@@ -903,10 +1096,11 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   st.rej_encroach += h_cnt[CNT_REJ_ENC];
   st.rej_blocked += h_cnt[CNT_REJ_BLOCKED];
   st.giveups += h_cnt[CNT_GIVEUPS];
-  st.lost += h_cnt[CNT_LOST];
+  st.lost += (long long)h_cnt[CNT_VALID] - nwin;
   st.pred_fp32 += h_cnt[CNT_FP32];
   st.pred_fp64 += h_cnt[CNT_FP64];
   st.pred_exact += h_cnt[CNT_EXACT];
+  st.big_cavities += h_cnt[CNT_BIG];
   if (is_seg) st.seg_splits += nwin;
   else st.inserted += nwin;
   *progress = nwin + (long long)h_cnt[CNT_MARKS] + (long long)h_cnt[CNT_GIVEUPS];
@@ -965,6 +1159,7 @@ void Refiner::run() {
   // --- workspace ----------------------------------------------------------
   K = opt.cav_cap0;
   epoch = 0;
+  int prev_win = -1;
   HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
   if (!c_ws || ws_ints < opt.cand_budget) {
     dfree(c_ws);
@@ -982,7 +1177,7 @@ void Refiner::run() {
     // segment phase
     for (;;) {
       ensure_tmp(ns);
-      int ne = compact(PredSenc{m.senc}, ns, tmp_list, nullptr);
+      int ne = compact(PredSenc{m.senc}, ns, tmp_list, nullptr, nullptr);
       if (ne == 0) break;
       int nc = std::min(ne, cands_per_pass(K));
       ensure_cand(nc);
@@ -997,15 +1192,20 @@ void Refiner::run() {
     k_mark_bad<<<nblk(nt), 256, 0, stream>>>(m, pr, nt);
     tend(K_MARKBAD, nt);
     ensure_tmp(nt);
-    int nb = compact(PredBad{m.tflag}, nt, tmp_list, nullptr);
+    int nb = compact(PredBad{m.tflag}, nt, tmp_list, nullptr, nullptr);
     if (nb == 0) { st.status = 0; break; }
-    int nc = std::min(nb, cands_per_pass(K));
+    int cap = cands_per_pass(K);
+    if (opt.cand_factor > 0 && prev_win >= 0)
+      cap = (int)std::min<long long>(cap, std::max<long long>(opt.cand_min, (long long)opt.cand_factor * prev_win));
+    int *ids = tmp_list;
+    int nc = select_by_priority(tmp_list, nb, cap, &ids);
     ensure_cand(nc);
-    if (learned_hook) learned_hook(this, tmp_list, nc);
+    if (learned_hook) learned_hook(this, ids, nc);
     long long prog = 0;
     bool of = false;
-    run_pass(tmp_list, nc, false, &prog, &of);
+    prev_win = run_pass(ids, nc, false, &prog, &of);
     st.bad_passes++;
+    st.cand_evals += nc;
     if (of && K < opt.cav_cap_max) {
       K = std::min(2 * K, opt.cav_cap_max);
       prog++;
@@ -1019,7 +1219,7 @@ void Refiner::run() {
     newid_cap = nt;
     dalloc(&newid, (size_t)newid_cap);
   }
-  nt_out = compact(PredAlive{m.tflag}, nt, nullptr, newid);
+  nt_out = compact(PredAlive{m.tflag}, nt, nullptr, newid, nullptr);
   if (nt_out > o_cap) {
     dfree(o_T); dfree(o_N); dfree(o_tstat);
     o_cap = nt_out;

@@ -102,7 +102,9 @@ typedef struct gcdt_options {
   int cav_cap0;          /* initial cavity capacity K (triangles), default 32 */
   int cav_cap_max;       /* largest cavity capacity, default 16384 */
   long long cand_budget; /* cavity workspace in ints; candidates per pass =
-                            cand_budget / (6K + 16); default 2^28 */
+                            cand_budget / (6K + 16); default 2^30.  When more
+                            triangles are bad, the smallest (priority, index)
+                            ones are taken */
   int max_rounds;        /* refinement round cap, default 100000 */
   int priority;          /* 1 = shortest-edge bucket then index (default);
                             0 = lowest index only (refine.cu:4159);
@@ -111,6 +113,15 @@ typedef struct gcdt_options {
                             then FP64, then exact */
   int profile;           /* 1 = time every kernel launch with HIP events */
   int device;            /* HIP device ordinal */
+  int cand_factor;       /* a bad pass evaluates at most max(cand_min,
+                            cand_factor * winners of the previous bad pass)
+                            candidates, the smallest (priority, index) ones;
+                            0 = every bad triangle (default 4) */
+  int cand_min;          /* default 16384 */
+  int claim_iters;       /* conflict-resolution iterations per pass (default 1,
+                            the reference's single atomicMin claim); more
+                            iterations let candidates that touch no winner
+                            compete again, enlarging each independent set */
 } gcdt_options;
 
 typedef struct gcdt_stats {
@@ -122,6 +133,8 @@ typedef struct gcdt_stats {
   double device_ms;      /* HIP-event time of the refinement on the device */
   long long pred_fp32, pred_fp64, pred_exact; /* predicate stage hits */
   long long n_points, n_triangles, n_subsegs;
+  long long cand_evals;  /* bad-pass candidates evaluated */
+  long long big_cavities; /* cavity evaluations that left the LDS fast path */
 } gcdt_stats;
 
 typedef struct gcdt_mesh gcdt_mesh;       /* host-side result */

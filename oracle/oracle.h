@@ -46,6 +46,13 @@ typedef struct {
                          1 = shortest-edge half-octave bucket first, then
                          index (Triangle's queue order, triangle.cpp:6932-6968,
                          and the reference's commented-out rule) */
+  int cand_factor;    /* bad passes evaluate at most max(cand_min,
+                         cand_factor * winners of the previous bad pass)
+                         candidates, chosen by priority (0 = no limit) */
+  int cand_min;
+  int claim_iters;    /* conflict-resolution rounds per pass (>= 1): after the
+                         first, valid candidates whose region touches no winner
+                         claim again among themselves */
 } orc_params;
 
 typedef struct {
@@ -54,6 +61,7 @@ typedef struct {
       lost, giveups;
   int final_cav_cap;
   int status; /* 0 ok, 1 round cap hit, 2 stalled */
+  long long cand_evals; /* bad-pass candidates evaluated */
 } orc_stats;
 
 typedef struct orc_result orc_result;

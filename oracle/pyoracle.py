@@ -46,7 +46,8 @@ class OrcParams(C.Structure):
     _fields_ = [("theta_deg", C.c_double), ("offcenter", C.c_int), ("mode", C.c_int),
                 ("cav_cap0", C.c_int), ("cav_cap_max", C.c_int),
                 ("cand_budget", C.c_longlong), ("max_rounds", C.c_int),
-                ("priority", C.c_int)]
+                ("priority", C.c_int), ("cand_factor", C.c_int), ("cand_min", C.c_int),
+                ("claim_iters", C.c_int)]
 
 
 class OrcStats(C.Structure):
@@ -54,7 +55,8 @@ class OrcStats(C.Structure):
                 ("inserted", C.c_longlong), ("seg_splits", C.c_longlong),
                 ("rej_encroach", C.c_longlong), ("rej_blocked", C.c_longlong),
                 ("rej_overflow", C.c_longlong), ("lost", C.c_longlong),
-                ("giveups", C.c_longlong), ("final_cav_cap", C.c_int), ("status", C.c_int)]
+                ("giveups", C.c_longlong), ("final_cav_cap", C.c_int), ("status", C.c_int),
+                ("cand_evals", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -158,15 +160,16 @@ def triangle_refine(points, triangles, segments, theta, mode=1):
 
 
 def oracle_refine(points, triangles, neighbors, segments, theta=25.0, offcenter=1,
-                  mode=1, cav_cap0=32, cav_cap_max=16384, cand_budget=1 << 28,
-                  max_rounds=100000, priority=1):
+                  mode=1, cav_cap0=32, cav_cap_max=16384, cand_budget=1 << 30,
+                  max_rounds=100000, priority=1, cand_factor=4, cand_min=16384,
+                  claim_iters=1):
     """Run the CPU restatement; returns (P, T, N, S, pflag, stats)."""
     P = np.ascontiguousarray(points, np.float64).reshape(-1, 2)
     T = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
     N = np.ascontiguousarray(neighbors, np.int32).reshape(-1, 3)
     S = np.ascontiguousarray(segments, np.int32).reshape(-1, 2)
     prm = OrcParams(theta, offcenter, mode, cav_cap0, cav_cap_max, cand_budget,
-                    max_rounds, priority)
+                    max_rounds, priority, cand_factor, cand_min, claim_iters)
     st = OrcStats()
     L = orc()
     r = L.orc_refine(len(P), _p(P), len(T), _p(T), _p(N), len(S), _p(S),

@@ -34,6 +34,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <stdio.h>
 
 #include "oracle.h"
 
@@ -557,6 +558,16 @@ static void mark_enc(Mesh *m, int s, long long *marks) {
   }
 }
 
+static int cmp_ll(const void *a, const void *b) {
+  long long x = *(const long long *)a, y = *(const long long *)b;
+  return x < y ? -1 : (x > y);
+}
+
+static int cmp_int(const void *a, const void *b) {
+  int x = *(const int *)a, y = *(const int *)b;
+  return x < y ? -1 : (x > y);
+}
+
 static int cands_per_pass(long long budget, int K) {
   long long c = budget / (6LL * K + 16);
   if (c < 1) c = 1;
@@ -568,8 +579,8 @@ static int cands_per_pass(long long budget, int K) {
  * the segment-split or the bad-triangle flavour.  Returns the number of
  * winners; sets *overflowed if any cavity exceeded K. */
 static int run_pass(Mesh *m, const int *ids, int nc, int is_seg, int K,
-                    int Kmax, int priority, orc_stats *st, int *overflowed,
-                    long long *progress) {
+                    int Kmax, int priority, int claim_iters, orc_stats *st,
+                    int *overflowed, long long *progress) {
   long long marks = 0, gave = st->giveups;
   Cand *cd = calloc((size_t)nc, sizeof(Cand));
   int *pool = malloc(sizeof(int) * (size_t)(2 * K + 2));
@@ -640,29 +651,51 @@ static int run_pass(Mesh *m, const int *ids, int nc, int is_seg, int K,
       }
     }
   }
-  /* claims: the smallest (priority, candidate index) wins each triangle */
+  /* claims: the smallest (priority, candidate index) wins each triangle.
+   * Repeated claim_iters times: candidates that neither won nor touch a
+   * winner's region compete again among themselves. */
   long long *claim = malloc(sizeof(long long) * (size_t)m->nt);
-  for (int t = 0; t < m->nt; t++) claim[t] = 0x7fffffffffffffffLL;
-  for (int i = 0; i < nc; i++) {
-    if (cd[i].status != ST_VALID) continue;
-    long long key = ((long long)cd[i].prio << 32) | i;
-    for (int q = 0; q < cd[i].n + cd[i].r; q++) {
-      int t = cd[i].L[q];
-      if (claim[t] > key) claim[t] = key;
-    }
-  }
+  int *owner = malloc(sizeof(int) * (size_t)m->nt);
+  for (int t = 0; t < m->nt; t++) owner[t] = -1;
   int nwin = 0;
   int *win = calloc((size_t)nc, sizeof(int));
-  for (int i = 0; i < nc; i++) {
-    if (cd[i].status != ST_VALID) continue;
-    int ok = 1;
-    long long key = ((long long)cd[i].prio << 32) | i;
-    for (int q = 0; q < cd[i].n + cd[i].r && ok; q++)
-      if (claim[cd[i].L[q]] != key) ok = 0;
-    win[i] = ok;
-    if (ok) nwin++;
-    else st->lost++;
+  int *active = malloc(sizeof(int) * (size_t)(nc > 0 ? nc : 1));
+  for (int i = 0; i < nc; i++) active[i] = cd[i].status == ST_VALID;
+  for (int it = 0; it < claim_iters; it++) {
+    for (int t = 0; t < m->nt; t++) claim[t] = 0x7fffffffffffffffLL;
+    for (int i = 0; i < nc; i++) {
+      if (!active[i]) continue;
+      long long key = ((long long)cd[i].prio << 32) | i;
+      for (int q = 0; q < cd[i].n + cd[i].r; q++) {
+        int t = cd[i].L[q];
+        if (claim[t] > key) claim[t] = key;
+      }
+    }
+    int newwin = 0;
+    for (int i = 0; i < nc; i++) {
+      if (!active[i]) continue;
+      long long key = ((long long)cd[i].prio << 32) | i;
+      int ok = 1;
+      for (int q = 0; q < cd[i].n + cd[i].r && ok; q++)
+        if (claim[cd[i].L[q]] != key) ok = 0;
+      if (ok) { win[i] = 1; newwin++; }
+    }
+    for (int i = 0; i < nc; i++)
+      if (win[i] && active[i])
+        for (int q = 0; q < cd[i].n + cd[i].r; q++) owner[cd[i].L[q]] = i;
+    for (int i = 0; i < nc; i++) {
+      if (!active[i]) continue;
+      if (win[i]) { active[i] = 0; continue; }
+      for (int q = 0; q < cd[i].n + cd[i].r; q++)
+        if (owner[cd[i].L[q]] >= 0) { active[i] = 0; break; }
+    }
+    nwin += newwin;
+    if (newwin == 0) break;
   }
+  for (int i = 0; i < nc; i++)
+    if (cd[i].status == ST_VALID && !win[i]) st->lost++;
+  free(owner);
+  free(active);
   /* slot bases (winners take points/triangles/subsegments in candidate order) */
   int pbase = m->np, tbase = m->nt, sbase = m->ns;
   grow_points(m, pbase + nwin);
@@ -686,6 +719,17 @@ static int run_pass(Mesh *m, const int *ids, int nc, int is_seg, int K,
   if (is_seg) st->seg_splits += nwin;
   else st->inserted += nwin;
   *progress = nwin + marks + (st->giveups - gave);
+  if (getenv("ORC_TRACE")) {
+    long long nval = 0, csum = 0, cmax = 0;
+    for (int i = 0; i < nc; i++)
+      if (cd[i].status == ST_VALID) {
+        nval++;
+        csum += cd[i].n;
+        if (cd[i].n > cmax) cmax = cd[i].n;
+      }
+    fprintf(stderr, "pass %s nc=%d valid=%lld win=%d marks=%lld avgcav=%.2f maxcav=%lld K=%d\n",
+            is_seg ? "seg" : "bad", nc, nval, nwin, marks, nval ? (double)csum / nval : 0.0, cmax, K);
+  }
   for (int i = 0; i < nc; i++) free(cd[i].L);
   free(cd);
   free(pool);
@@ -798,6 +842,7 @@ orc_result *orc_refine(int np, const double *points, int nt, const int *tris,
     }
 
   int K = prm->cav_cap0;
+  int prev_win = -1;
   int *ids = NULL;
   int idcap = 0;
   st->status = 1;
@@ -815,7 +860,7 @@ orc_result *orc_refine(int np, const double *points, int nt, const int *tris,
       if (ne > cap) ne = cap;
       int of = 0;
       long long prog = 0;
-      run_pass(m, ids, ne, 1, K, prm->cav_cap_max, prm->priority, st, &of, &prog);
+      run_pass(m, ids, ne, 1, K, prm->cav_cap_max, prm->priority, prm->claim_iters, st, &of, &prog);
       st->seg_passes++;
       if (of && K < prm->cav_cap_max) K = K * 2 > prm->cav_cap_max ? prm->cav_cap_max : K * 2;
     }
@@ -830,11 +875,30 @@ orc_result *orc_refine(int np, const double *points, int nt, const int *tris,
     for (int t = 0; t < m->nt; t++)
       if ((m->tflag[t] & (TF_ALIVE | TF_BAD | TF_GIVEUP)) == (TF_ALIVE | TF_BAD)) ids[nb++] = t;
     int cap = cands_per_pass(prm->cand_budget, K);
-    if (nb > cap) nb = cap;
+    if (prm->cand_factor > 0 && prev_win >= 0) {
+      long long lim = (long long)prm->cand_factor * prev_win;
+      if (lim < prm->cand_min) lim = prm->cand_min;
+      if (lim < cap) cap = (int)lim;
+    }
+    if (nb > cap) {
+      if (prm->priority == 1) {
+        /* keep the cap smallest (priority, index) candidates, in index order */
+        long long *key = malloc(sizeof(long long) * (size_t)nb);
+        for (int i = 0; i < nb; i++)
+          key[i] = ((long long)shortest_edge_bucket(m, ids[i]) << 32) | ids[i];
+        qsort(key, (size_t)nb, sizeof(long long), cmp_ll);
+        for (int i = 0; i < cap; i++) ids[i] = (int)(key[i] & 0xffffffffLL);
+        qsort(ids, (size_t)cap, sizeof(int), cmp_int);
+        free(key);
+      }
+      nb = cap;
+    }
     int of = 0;
     long long prog = 0;
-    run_pass(m, ids, nb, 0, K, prm->cav_cap_max, prm->priority, st, &of, &prog);
+    int nw = run_pass(m, ids, nb, 0, K, prm->cav_cap_max, prm->priority, prm->claim_iters, st, &of, &prog);
     st->bad_passes++;
+    st->cand_evals += nb;
+    prev_win = nw;
     if (of && K < prm->cav_cap_max) {
       K = K * 2 > prm->cav_cap_max ? prm->cav_cap_max : K * 2;
       prog++;

@@ -14,6 +14,8 @@ CASES = [
     (2000, 100, 3, 25.0, 0, 1),
     (2000, 100, 4, 30.0, 0, 0),
     (2000, 100, 5, 20.7, 1, 1),
+    (20000, 1000, 6, 25.0, 1, 1),
+    (20000, 1000, 7, 30.0, 1, 0),
 ]
 
 
@@ -49,3 +51,58 @@ def test_gpu_deterministic(gpu, oracle):
     assert np.array_equal(a.triangles, b.triangles)
     assert np.array_equal(a.neighbors, b.neighbors)
     assert np.array_equal(a.segments, b.segments)
+
+
+@pytest.mark.parametrize("iters", [2, 3])
+def test_gpu_equals_oracle_claim_iters(gpu, oracle, iters):
+    P, T, N, S = _input(oracle, 5000, 250, 21)
+    r = gpu.refine(P, T, N, S, theta=28.0, claim_iters=iters)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, theta=28.0, claim_iters=iters)
+    assert np.array_equal(r.points.view(np.uint64), Po.view(np.uint64))
+    assert np.array_equal(r.triangles, To)
+    assert np.array_equal(r.neighbors, No)
+
+
+def circle_pslg(oracle, k=72, n_out=1500, seed=3):
+    """k points on a circle of radius 100 around (500, 500), empty inside,
+    plus random points outside: circumcenters of the skinny fan triangles
+    inside sit near the centre, so their cavities hold most of the fan."""
+    rng = np.random.default_rng(seed)
+    a = 2 * np.pi * np.arange(k) / k
+    circ = np.stack([500 + 100 * np.cos(a), 500 + 100 * np.sin(a)], 1)
+    out = rng.uniform(0, 1000, size=(4 * n_out, 2))
+    out = out[np.hypot(out[:, 0] - 500, out[:, 1] - 500) > 130][:n_out]
+    pts = np.concatenate([circ, out])
+    P, T, N, S = oracle.triangle_cdt(pts, np.zeros((0, 2), np.int32))
+    return P, T, N, S
+
+
+@pytest.mark.parametrize("cap0", [4, 64])
+def test_gpu_cavity_capacity_paths(gpu, oracle, cap0):
+    """Small K forces overflow and K growth; K > 32 with large cavities
+    exercises the global-memory cavity path."""
+    P, T, N, S = circle_pslg(oracle)
+    kw = dict(cav_cap0=cap0, cav_cap_max=256)
+    r = gpu.refine(P, T, N, S, theta=25.0, **kw)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, theta=25.0, **kw)
+    if cap0 == 4:
+        assert st["rej_overflow"] > 0
+    else:
+        assert r.stats["big_cavities"] > 0
+    assert np.array_equal(r.points.view(np.uint64), Po.view(np.uint64))
+    assert np.array_equal(r.triangles, To)
+    assert np.array_equal(r.neighbors, No)
+    assert np.array_equal(r.segments, So)
+    tot, viol, mina, ex = oracle.check_mesh(r.points, r.triangles, r.neighbors, r.segments,
+                                            P, S, 25.0)
+    assert tot == 0, viol
+
+
+def test_gpu_candidate_cap_by_priority(gpu, oracle):
+    """A small workspace forces priority-ordered truncation every pass."""
+    P, T, N, S = _input(oracle, 4000, 200, 41)
+    kw = dict(cand_min=64, cand_factor=2, cand_budget=(6 * 32 + 16) * 500)
+    r = gpu.refine(P, T, N, S, theta=25.0, **kw)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, theta=25.0, **kw)
+    assert np.array_equal(r.points.view(np.uint64), Po.view(np.uint64))
+    assert np.array_equal(r.triangles, To)
