@@ -255,6 +255,17 @@ def device_predicates(queries, pred_mode=0):
     return o, ic, sc
 
 
+def host_predicates(queries, pred_mode=0):
+    """The kernels' predicate code evaluated on the host (no device)."""
+    q = np.ascontiguousarray(queries, dtype=np.float64).reshape(-1, 8)
+    o = np.empty(len(q), np.int32)
+    ic = np.empty(len(q), np.int32)
+    L = lib()
+    L.gcdt_host_predicates.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    L.gcdt_host_predicates(_ptr(q), len(q), pred_mode, _ptr(o), _ptr(ic))
+    return o, ic
+
+
 def generate_pslg(n, nseg, L=1000.0, maxlen=0.0, dist=0, min_angle=5.0, seed=1):
     """Seeded synthetic PSLG: n points (dist 0 uniform in [0,L]^2, 1 Gaussian
     clusters) and up to nseg non-crossing segments (host code in libgcdt)."""

@@ -47,5 +47,24 @@ def build(verbose=True):
     return LIB
 
 
+DROPIN_SRC = os.path.join(ROOT, "examples", "dropin_main.cpp")
+DROPIN_BIN = os.path.join(ROOT, "examples", "dropin_main")
+
+
+def build_dropin_example(verbose=True):
+    """Compile the reference-style caller against include/ and libgcdt.so
+    (plain g++: the drop-in needs no HIP headers)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), DROPIN_SRC,
+           "-L" + HERE, "-lgcdt", "-Wl,-rpath," + HERE, "-o", DROPIN_BIN]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building the drop-in example")
+    return DROPIN_BIN
+
+
 if __name__ == "__main__":
     build()
+    build_dropin_example()

@@ -204,6 +204,12 @@ void gcdt_cdt_counts(const gcdt_cdt *c, int *np, int *nt, int *ns);
 void gcdt_cdt_copy(const gcdt_cdt *c, int *tris, int *nbrs, int *segs);
 void gcdt_cdt_free(gcdt_cdt *c);
 
+/* The refinement's predicates evaluated on the host (same source as the
+ * kernels): signs of orient2d(a,b,c) and incircle(a,b,c,d) for n queries
+ * (ax, ay, bx, by, cx, cy, dx, dy).  No device use. */
+int gcdt_host_predicates(const double *pts8, int n, int pred_mode,
+                         int *orient_out, int *incircle_out);
+
 /* Exact predicates on the device, for tests: evaluates n queries. */
 int gcdt_device_predicates(const double *pts8, int n, int pred_mode,
                            int *orient_out, int *incircle_out,

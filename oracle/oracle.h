@@ -74,6 +74,11 @@ void orc_result_copy(const orc_result *r, double *points, int *tris, int *nbrs,
                      int *segs, uint8_t *pflag);
 void orc_result_free(orc_result *r);
 
+/* Circumcenter / off-center of triangle (apex, org, dest) as the refinement
+ * computes it; returns 0 if degenerate. */
+int orc_steiner_point(const double *apex, const double *org, const double *dest,
+                      double theta_deg, int offcenter, double *out);
+
 /* Invariant checker.  Returns 0 when every check passes; otherwise the
  * number of violations, with per-check counts in viol[] (see check_oracle.c
  * for the index meaning). */

@@ -83,6 +83,9 @@ def orc():
         for f in ("orc_orient2d", "orc_incircle", "orc_orient2d_exact_sign",
                   "orc_incircle_exact_sign"):
             getattr(L, f).restype = C.c_int
+        L.orc_orient2d.argtypes = [vp, vp, vp]
+        L.orc_incircle.argtypes = [vp, vp, vp, vp]
+        L.orc_steiner_point.argtypes = [vp, vp, vp, C.c_double, C.c_int, vp]
         _orc = L
     return _orc
 
@@ -93,8 +96,14 @@ def triref():
         if not os.path.exists(TRIREF_SO):
             raise RuntimeError("oracle/_ref/libtriref.so not built (needs the reference tree)")
         L = C.CDLL(TRIREF_SO)
+        vp = C.c_void_p
         L.triref_orient2d.restype = C.c_double
+        L.triref_orient2d.argtypes = [vp, vp, vp]
         L.triref_incircle.restype = C.c_double
+        L.triref_incircle.argtypes = [vp, vp, vp, vp]
+        L.triref_findcircumcenter.argtypes = [vp, vp, vp, C.c_double, C.c_int, vp]
+        L.triref_triangulate.argtypes = [C.c_char_p, vp, vp]
+        L.triref_free.argtypes = [vp]
         _tri = L
     return _tri
 

@@ -960,3 +960,18 @@ void orc_result_free(orc_result *r) {
   free(r->P); free(r->T); free(r->N); free(r->S); free(r->pflag);
   free(r);
 }
+
+/* Closed-form constructions exposed for tests. */
+int orc_steiner_point(const double *apex, const double *org, const double *dest,
+                      double theta_deg, int offcenter, double *out) {
+  Mesh m;
+  memset(&m, 0, sizeof(m));
+  double P[6] = {apex[0], apex[1], org[0], org[1], dest[0], dest[1]};
+  int TV[3] = {0, 1, 2};
+  double ga = cos(theta_deg * 3.141592653589793238462643383279502884197169399375105820974944592308 / 180.0);
+  m.offconst = (ga == 1.0) ? 0.0 : 0.475 * sqrt((1.0 + ga) / (1.0 - ga));
+  m.offcenter = offcenter;
+  m.P = P;
+  m.TV = TV;
+  return steiner_point(&m, 0, out);
+}

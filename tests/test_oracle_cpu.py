@@ -1,0 +1,129 @@
+"""The CPU oracle of the refinement rounds, pinned by what the mathematics
+and the reference's own code fix: exact mesh invariants (constrained
+Delaunay, segment conformity, minimum angle with the Miller-Pav-Walkington
+exemption), the vertex count of the reference's Triangle -q on the same
+PSLG, Triangle's own circumcenter/off-center routine (findcircumcenter,
+triangle.cpp:6523), and run-to-run determinism."""
+import os
+
+import numpy as np
+import pytest
+
+
+def _cdt(oracle, n, ns, seed, L=1000.0, maxlen=200.0):
+    pts, segs = oracle.random_pslg(n, ns, L=L, maxlen=maxlen, seed=seed)
+    return oracle.triangle_cdt(pts, segs)
+
+
+@pytest.mark.parametrize("theta,off,mode", [(25, 1, 1), (25, 1, 0), (30, 1, 1), (25, 0, 1),
+                                            (20.7, 0, 0), (30, 0, 0)])
+def test_oracle_invariants(oracle, theta, off, mode):
+    P, T, N, S = _cdt(oracle, 1500, 75, 3)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, theta=theta, offcenter=off, mode=mode)
+    assert st["status"] == 0 and st["giveups"] == 0
+    tot, viol, mina, ex = oracle.check_mesh(Po, To, No, So, P, S, theta)
+    assert tot == 0, viol
+    assert mina >= theta - 1e-7
+    assert np.array_equal(Po[:len(P)].view(np.uint64), P.view(np.uint64))
+
+
+def test_checker_catches_planted_faults(oracle):
+    """A mistake planted in the mesh (on the CPU) must be reported."""
+    P, T, N, S = _cdt(oracle, 800, 40, 5)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, theta=25.0)
+    # flip one interior non-segment edge: breaks Delaunay (and nothing else)
+    segset = {tuple(sorted(s)) for s in So.tolist()}
+    for t in range(len(To)):
+        for j in range(3):
+            u = No[t, j]
+            a, b = To[t, (j + 1) % 3], To[t, (j + 2) % 3]
+            if u > t and tuple(sorted((a, b))) not in segset:
+                k = [kk for kk in range(3) if No[u, kk] == t][0]
+                c, d = To[t, j], To[u, k]
+                T2 = To.copy()
+                T2[t] = [c, a, d]
+                T2[u] = [d, b, c]
+                N2 = No.copy()
+                tot, viol, _, _ = oracle.check_mesh(Po, T2, N2, So, P, S, 25.0)
+                assert tot > 0
+                break
+        else:
+            continue
+        break
+    # move an input point
+    P2 = Po.copy()
+    P2[0, 0] = np.nextafter(P2[0, 0], 1e9)
+    tot, viol, _, _ = oracle.check_mesh(P2, To, No, So, P, S, 25.0)
+    assert viol["input_point_moved"] == 1
+    # drop a subsegment: its input segment is no longer covered
+    tot, viol, _, _ = oracle.check_mesh(Po, To, No, So[1:], P, S, 25.0)
+    assert viol["segment_not_covered"] > 0
+
+
+@pytest.mark.parametrize("theta,mode,band", [(25, 1, 0.03), (30, 1, 0.04), (25, 0, 0.03)])
+def test_vertex_count_band_vs_triangle(oracle, theta, mode, band):
+    """Steiner count within a stated band of the reference's Triangle -q."""
+    if not os.path.exists(oracle.TRIREF_SO):
+        pytest.skip("reference Triangle not built")
+    P, T, N, S = _cdt(oracle, 4000, 200, 11)
+    Pt, Tt, Nt, St = oracle.triangle_refine(P, T, S, theta, mode)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, theta=theta, mode=mode)
+    ratio = len(Po) / len(Pt)
+    assert abs(ratio - 1.0) < band, ratio
+
+
+def test_steiner_point_matches_triangle_findcircumcenter(oracle):
+    """Bit-exact against the reference's Triangle findcircumcenter for
+    well-shaped and skinny triangles (org, dest, apex = v1, v2, v0)."""
+    if not os.path.exists(oracle.TRIREF_SO):
+        pytest.skip("reference Triangle not built")
+    import ctypes as C
+    L = oracle.orc()
+    R = oracle.triref()
+    rng = np.random.default_rng(3)
+    n_off = 0
+    for k in range(2000):
+        p = rng.uniform(0, 1000, size=(3, 2))
+        if k % 2:
+            p[2] = p[0] + (p[1] - p[0]) * rng.uniform(0.2, 0.8) + rng.normal(0, 1e-3, 2)
+        # CCW order (apex, org, dest)
+        if oracle.orient2d(p[0], p[1], p[2]) < 0:
+            p[[1, 2]] = p[[2, 1]]
+        for theta, off in ((25.0, 1), (30.0, 1), (25.0, 0)):
+            out = np.zeros(2)
+            apex, org, dest = (np.ascontiguousarray(x) for x in p)
+            ok = L.orc_steiner_point(apex.ctypes.data_as(C.c_void_p), org.ctypes.data_as(C.c_void_p),
+                                     dest.ctypes.data_as(C.c_void_p), C.c_double(theta),
+                                     off, out.ctypes.data_as(C.c_void_p))
+            if not ok:
+                continue
+            ref = np.zeros(2)
+            R.triref_findcircumcenter(org.ctypes.data_as(C.c_void_p), dest.ctypes.data_as(C.c_void_p),
+                                      apex.ctypes.data_as(C.c_void_p), C.c_double(theta), off,
+                                      ref.ctypes.data_as(C.c_void_p))
+            assert np.array_equal(out.view(np.uint64), ref.view(np.uint64)), (p, theta, off)
+            n_off += off
+    assert n_off > 0
+
+
+def test_small_input_angles_terminate(oracle):
+    """Two segments meeting at 10 degrees: concentric shells + MPW keep the
+    refinement finite; the only sub-theta triangles are MPW-exempt."""
+    a = np.deg2rad(10.0)
+    pts = np.array([[0, 0], [400, 0], [400 * np.cos(a), 400 * np.sin(a)],
+                    [-300, -300], [700, -300], [700, 500], [-300, 500]], float)
+    segs = np.array([[0, 1], [0, 2]], np.int32)
+    P, T, N, S = oracle.triangle_cdt(pts, segs)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, theta=25.0)
+    assert st["status"] == 0
+    tot, viol, mina, ex = oracle.check_mesh(Po, To, No, So, P, S, 25.0)
+    assert tot == 0, viol
+    assert ex > 0
+
+
+def test_oracle_deterministic(oracle):
+    P, T, N, S = _cdt(oracle, 1200, 60, 9)
+    r1 = oracle.oracle_refine(P, T, N, S, theta=28.0)
+    r2 = oracle.oracle_refine(P, T, N, S, theta=28.0)
+    for a, b in zip(r1[:4], r2[:4]):
+        assert np.array_equal(a, b)
