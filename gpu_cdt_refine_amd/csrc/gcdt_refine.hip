@@ -463,27 +463,51 @@ __global__ __launch_bounds__(CAV_BLOCK) void k_bad_cavity(MeshD m, ParamsD pr,
   flush_stats(ps.cnt, st);
 }
 
-// Big kernels: candidates whose cavity exceeded the LDS list, recomputed
-// with the global workspace and the full capacity K.
-__global__ __launch_bounds__(256) void k_seg_cavity_big(MeshD m, ParamsD pr,
-                                                        PassD ps, int Kmax) {
+// Big kernels: candidates whose cavity exceeded the per-thread LDS list,
+// recomputed with the full capacity K.  Each thread's list lives in dynamic
+// LDS (2K+2 ints; blockDim threads per block sized on the host to fit), so
+// membership scans never touch HBM; only when even one list cannot fit in
+// LDS (K > BIG_LDS_K) does the list stay in the global workspace.
+constexpr int BIG_LDS_BYTES = 128 * 1024;
+constexpr int BIG_LDS_K = (BIG_LDS_BYTES / 4 - 2) / 2;
+
+__global__ __launch_bounds__(64) void k_seg_cavity_big(MeshD m, ParamsD pr,
+                                                       PassD ps, int Kmax,
+                                                       int use_lds) {
+  extern __shared__ int dyn[];
   PredStats st = {0, 0, 0};
   int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
   for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x)
-    if (ps.cstatus[i] == ST_BIG)
-      eval_seg<false>(m, pr, ps, i, ps.ws + (size_t)i * ps.R, ps.K, Kmax, pws, &st);
+    if (ps.cstatus[i] == ST_BIG) {
+      int *G = ps.ws + (size_t)i * ps.R;
+      if (use_lds) {
+        int *L = dyn + threadIdx.x * (2 * ps.K + 2);
+        eval_seg<true>(m, pr, ps, i, L, ps.K, Kmax, pws, &st);
+      } else {
+        eval_seg<false>(m, pr, ps, i, G, ps.K, Kmax, pws, &st);
+      }
+    }
   flush_stats(ps.cnt, st);
 }
 
-__global__ __launch_bounds__(256) void k_bad_cavity_big(MeshD m, ParamsD pr,
-                                                        PassD ps, int Kmax) {
+__global__ __launch_bounds__(64) void k_bad_cavity_big(MeshD m, ParamsD pr,
+                                                       PassD ps, int Kmax,
+                                                       int use_lds) {
+  extern __shared__ int dyn[];
   PredStats st = {0, 0, 0};
   int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
   for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x)
-    if (ps.cstatus[i] == ST_BIG)
-      eval_bad<false>(m, pr, ps, i, ps.ws + (size_t)i * ps.R, ps.K, Kmax, pws, &st);
+    if (ps.cstatus[i] == ST_BIG) {
+      int *G = ps.ws + (size_t)i * ps.R;
+      if (use_lds) {
+        int *L = dyn + threadIdx.x * (2 * ps.K + 2);
+        eval_bad<true>(m, pr, ps, i, L, ps.K, Kmax, pws, &st);
+      } else {
+        eval_bad<false>(m, pr, ps, i, G, ps.K, Kmax, pws, &st);
+      }
+    }
   flush_stats(ps.cnt, st);
 }
 
@@ -553,8 +577,8 @@ __global__ __launch_bounds__(256) void k_block(MeshD m, PassD ps, int *cact) {
 __global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
                                                      PassD ps, int pbase,
                                                      int tbase, int sbase,
-                                                     int is_seg) {
-  __shared__ int lds[CAV_BLOCK * LDS_CAV];
+                                                     int is_seg, int lds_k) {
+  extern __shared__ int lds[];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ps.nc;
        i += gridDim.x * blockDim.x) {
     if (!ps.cwin[i]) continue;
@@ -566,8 +590,8 @@ __global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
     double2 c = ps.cpt[i];
     int n = ps.cn[i];
     const int *L = ps.ws + (size_t)i * ps.R;
-    if (n <= LDS_CAV) {  // membership scans below run on an LDS copy
-      int *Ls = lds + threadIdx.x * LDS_CAV;
+    if (n <= lds_k) {  // membership scans below run on an LDS copy
+      int *Ls = lds + threadIdx.x * lds_k;
       for (int q = 0; q < n; q++) Ls[q] = L[q];
       L = Ls;
     }
@@ -778,6 +802,13 @@ Refiner::Refiner(const gcdt_options &o) : opt(o) {
   HC(hipHostMalloc((void **)&h_cnt, CNT_N * sizeof(unsigned long long), hipHostMallocDefault));
   HC(hipEventCreate(&ev0));
   HC(hipEventCreate(&ev1));
+  // allow the big-cavity and apply kernels up to BIG_LDS_BYTES of dynamic LDS
+  HC(hipFuncSetAttribute((const void *)k_bad_cavity_big,
+                         hipFuncAttributeMaxDynamicSharedMemorySize, BIG_LDS_BYTES));
+  HC(hipFuncSetAttribute((const void *)k_seg_cavity_big,
+                         hipFuncAttributeMaxDynamicSharedMemorySize, BIG_LDS_BYTES));
+  HC(hipFuncSetAttribute((const void *)k_apply,
+                         hipFuncAttributeMaxDynamicSharedMemorySize, BIG_LDS_BYTES));
   dalloc(&d_total, 4);
   dalloc(&d_cnt, CNT_N);
   double ga = std::cos(opt.theta_deg * 3.141592653589793238462643383279502884197169399375105820974944592308 / 180.0);
@@ -1053,16 +1084,27 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   ps.learned_prio = learned_prio;
   HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
   int gf = std::min(nblk(nc, CAV_BLOCK), 2 * pws_blocks);
-  int gb = std::min(nblk(nc), pws_blocks);
+  // big-cavity launch shape: as many lists of 2K+2 ints per block as fit
+  struct { int blocks, threads, use_lds; size_t lds; } bg;
+  {
+    size_t per = sizeof(int) * (2 * (size_t)K + 2);
+    int th = (int)std::min<size_t>(64, BIG_LDS_BYTES / per);
+    bg.use_lds = th >= 1 ? 1 : 0;
+    bg.threads = th >= 1 ? th : 64;
+    bg.lds = th >= 1 ? per * th : 0;
+    bg.blocks = std::min(nblk(nc, bg.threads), pws_blocks * 256 / bg.threads);
+  }
   if (is_seg) {
     tbegin(K_SEGCAV);
     k_seg_cavity<<<gf, CAV_BLOCK, 0, stream>>>(m, pr, ps, opt.cav_cap_max);
-    if (K > LDS_CAV) k_seg_cavity_big<<<gb, 256, 0, stream>>>(m, pr, ps, opt.cav_cap_max);
+    if (K > LDS_CAV)
+      k_seg_cavity_big<<<bg.blocks, bg.threads, bg.lds, stream>>>(m, pr, ps, opt.cav_cap_max, bg.use_lds);
     tend(K_SEGCAV, nc);
   } else {
     tbegin(K_BADCAV);
     k_bad_cavity<<<gf, CAV_BLOCK, 0, stream>>>(m, pr, ps, opt.cav_cap_max);
-    if (K > LDS_CAV) k_bad_cavity_big<<<gb, 256, 0, stream>>>(m, pr, ps, opt.cav_cap_max);
+    if (K > LDS_CAV)
+      k_bad_cavity_big<<<bg.blocks, bg.threads, bg.lds, stream>>>(m, pr, ps, opt.cav_cap_max, bg.use_lds);
     tend(K_BADCAV, nc);
   }
   HC(hipGetLastError());
@@ -1083,8 +1125,10 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   ensure_capacity(np + nwin, nt + 2LL * nwin, is_seg ? ns + nwin : ns);
   if (nwin > 0) {
     tbegin(K_APPLY);
-    k_apply<<<std::min(nblk(nc, CAV_BLOCK), 8192), CAV_BLOCK, 0, stream>>>(
-        m, pr, ps, np, nt, ns, is_seg ? 1 : 0);
+    // apply keeps each winner's cavity in LDS: lds_k ints per thread
+    int lds_k = std::min(K, (int)(BIG_LDS_BYTES / (sizeof(int) * CAV_BLOCK)));
+    k_apply<<<std::min(nblk(nc, CAV_BLOCK), 8192), CAV_BLOCK, sizeof(int) * lds_k * CAV_BLOCK,
+              stream>>>(m, pr, ps, np, nt, ns, is_seg ? 1 : 0, lds_k);
     tend(K_APPLY, nwin);
     HC(hipGetLastError());
   }
