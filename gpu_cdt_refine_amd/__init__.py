@@ -66,6 +66,7 @@ class Stats(C.Structure):
         ("n_subsegs", C.c_longlong),
         ("cand_evals", C.c_longlong),
         ("big_cavities", C.c_longlong),
+        ("pred_dd", C.c_longlong),
     ]
 
     def as_dict(self):
@@ -248,7 +249,7 @@ def device_predicates(queries, pred_mode=0):
     q = np.ascontiguousarray(queries, dtype=np.float64).reshape(-1, 8)
     o = np.empty(len(q), np.int32)
     ic = np.empty(len(q), np.int32)
-    sc = np.zeros(3, np.int64)
+    sc = np.zeros(4, np.int64)
     rc = lib().gcdt_device_predicates(_ptr(q), len(q), pred_mode, _ptr(o), _ptr(ic), _ptr(sc))
     if rc != 0:
         _err(rc, "gcdt_device_predicates")

@@ -607,7 +607,7 @@ extern "C" int gcdt_generate_pslg(int dist, int n, int nseg, double L,
 // Host evaluation of the same predicates the kernels use (for tests).
 extern "C" int gcdt_host_predicates(const double *q, int n, int pred_mode,
                                     int *orient_out, int *incircle_out) {
-  gcdt::PredStats st = {0, 0, 0};
+  gcdt::PredStats st = {0, 0, 0, 0};
   for (int i = 0; i < n; i++) {
     const double *p = q + 8 * (size_t)i;
     orient_out[i] = orient2d_sign(p[0], p[1], p[2], p[3], p[4], p[5], pred_mode == 1, &st);

@@ -51,6 +51,7 @@ enum : int {
   CNT_ERRORS,
   CNT_VALID,
   CNT_BIG,
+  CNT_DD,
   CNT_N = 16
 };
 

@@ -286,6 +286,7 @@ __device__ __forceinline__ void flush_stats(unsigned long long *cnt,
   if (st.fp32_hits) atomicAdd(&cnt[CNT_FP32], st.fp32_hits);
   if (st.fp64_hits) atomicAdd(&cnt[CNT_FP64], st.fp64_hits);
   if (st.exact_calls) atomicAdd(&cnt[CNT_EXACT], st.exact_calls);
+  if (st.dd_hits) atomicAdd(&cnt[CNT_DD], st.dd_hits);
 }
 
 __device__ __forceinline__ void mark_enc(const MeshD &m, int s,
@@ -440,7 +441,7 @@ __device__ __forceinline__ void eval_bad(const MeshD &m, const ParamsD &pr,
 __global__ __launch_bounds__(CAV_BLOCK) void k_seg_cavity(MeshD m, ParamsD pr,
                                                           PassD ps, int Kmax) {
   __shared__ int lds[CAV_BLOCK * LDS_LIST];
-  PredStats st = {0, 0, 0};
+  PredStats st = {0, 0, 0, 0};
   int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
   int *L = lds + threadIdx.x * LDS_LIST;
@@ -453,7 +454,7 @@ __global__ __launch_bounds__(CAV_BLOCK) void k_seg_cavity(MeshD m, ParamsD pr,
 __global__ __launch_bounds__(CAV_BLOCK) void k_bad_cavity(MeshD m, ParamsD pr,
                                                           PassD ps, int Kmax) {
   __shared__ int lds[CAV_BLOCK * LDS_LIST];
-  PredStats st = {0, 0, 0};
+  PredStats st = {0, 0, 0, 0};
   int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
   int *L = lds + threadIdx.x * LDS_LIST;
@@ -475,7 +476,7 @@ __global__ __launch_bounds__(64) void k_seg_cavity_big(MeshD m, ParamsD pr,
                                                        PassD ps, int Kmax,
                                                        int use_lds) {
   extern __shared__ int dyn[];
-  PredStats st = {0, 0, 0};
+  PredStats st = {0, 0, 0, 0};
   int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
   for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x)
@@ -495,7 +496,7 @@ __global__ __launch_bounds__(64) void k_bad_cavity_big(MeshD m, ParamsD pr,
                                                        PassD ps, int Kmax,
                                                        int use_lds) {
   extern __shared__ int dyn[];
-  PredStats st = {0, 0, 0};
+  PredStats st = {0, 0, 0, 0};
   int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
   for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x)
@@ -751,7 +752,7 @@ __global__ void k_pstatus(const uint8_t *pflag, int np, uint8_t *out) {
 // Device predicate evaluation for tests.
 __global__ void k_predicates(const double *q, int n, int pred_mode, int *o,
                              int *ic, double *pws, unsigned long long *cnt) {
-  PredStats st = {0, 0, 0};
+  PredStats st = {0, 0, 0, 0};
   int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   double *ws = pws + (size_t)gtid * GCDT_INCIRCLE_WS;
   for (int i = gtid; i < n; i += gridDim.x * blockDim.x) {
@@ -1144,6 +1145,7 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   st.pred_fp32 += h_cnt[CNT_FP32];
   st.pred_fp64 += h_cnt[CNT_FP64];
   st.pred_exact += h_cnt[CNT_EXACT];
+  st.pred_dd += h_cnt[CNT_DD];
   st.big_cavities += h_cnt[CNT_BIG];
   if (is_seg) st.seg_splits += nwin;
   else st.inserted += nwin;
@@ -1358,6 +1360,7 @@ void device_predicates(const double *pts8, int n, int pred_mode, int *o,
     stage_counts[0] = (long long)h[CNT_FP32];
     stage_counts[1] = (long long)h[CNT_FP64];
     stage_counts[2] = (long long)h[CNT_EXACT];
+    stage_counts[3] = (long long)h[CNT_DD];
   }
   dfree(dq); dfree(dout); dfree(din); dfree(pws); dfree(cnt);
 }

@@ -9,7 +9,10 @@
 // exact error term (two_product = a*b + fma(a,b,-a*b)), which is exact on both
 // the host and gfx950.
 //
-// Optional first stage (predicate mode GCDT_PRED_FP32): a semi-static FP32
+// Between the FP64 filter and the exact stage sits a double-double stage
+// evaluated entirely in registers (see incircle_dd_sign).
+//
+// Optional first stage (predicate mode 1): a semi-static FP32
 // filter.  Coordinates differences are rounded to FP32 and the determinant is
 // evaluated in FP32; its sign is accepted only when it exceeds an error bound
 // that includes the FP64->FP32 rounding of every difference (relative 2^-24
@@ -49,7 +52,7 @@ namespace gcdt {
 
 // Per-thread counters for the predicate stages (filter hit rates).
 struct PredStats {
-  unsigned long long fp32_hits, fp64_hits, exact_calls;
+  unsigned long long fp32_hits, fp64_hits, exact_calls, dd_hits;
 };
 
 GCDT_HD void two_sum(double a, double b, double &x, double &y) {
@@ -407,6 +410,93 @@ GCDT_HD int incircle_fp32_sign(double ax, double ay, double bx, double by,
   return 0;
 }
 
+// --- double-double stage (registers only) ---------------------------------
+// Between the FP64 filter and the exact expansions: the coordinate
+// differences are formed exactly as (hi, lo) pairs and the determinant is
+// evaluated in double-double arithmetic (about 106-bit significands).  With
+// u = 2^-53, |lo| <= u|hi| and FMA-exact products, a double-double product has
+// absolute error <= 8u^2 |a||b| and a sum <= 2u^2 (|a| + |b|); carried through
+// lift_k * minor_k and the three-term sum this bounds the error by
+// 48 u^2 * permanent < 2^-100 * permanent.  The sign is accepted when |det|
+// exceeds 2^-94 * permanent (a 2^6 margin), with the permanent held in the
+// normal FP64 range so that nothing overflows and underflow stays far below
+// the bound.  Only (near-)exact degeneracies reach the expansion stage.
+struct DD {
+  double hi, lo;
+};
+
+GCDT_HD DD dd_from_diff(double a, double b) {
+  DD r;
+  two_diff(a, b, r.hi, r.lo);
+  return r;
+}
+
+GCDT_HD DD dd_fast(double a, double b) {  // exact renormalisation (two_sum)
+  DD r;
+  two_sum(a, b, r.hi, r.lo);
+  return r;
+}
+
+GCDT_HD DD dd_add(DD a, DD b) {
+  double s, e, t, f;
+  two_sum(a.hi, b.hi, s, e);
+  two_sum(a.lo, b.lo, t, f);
+  e += t;
+  DD r = dd_fast(s, e);
+  r.lo += f;
+  return dd_fast(r.hi, r.lo);
+}
+
+GCDT_HD DD dd_neg(DD a) {
+  a.hi = -a.hi;
+  a.lo = -a.lo;
+  return a;
+}
+
+GCDT_HD DD dd_mul(DD a, DD b) {
+  double p, e;
+  two_product(a.hi, b.hi, p, e);
+  e += a.hi * b.lo + a.lo * b.hi;
+  return dd_fast(p, e);
+}
+
+// +1 / -1 certified, 0 undecided.
+GCDT_HD int incircle_dd_sign(double ax, double ay, double bx, double by,
+                             double cx, double cy, double dx, double dy) {
+  DD adx = dd_from_diff(ax, dx), ady = dd_from_diff(ay, dy);
+  DD bdx = dd_from_diff(bx, dx), bdy = dd_from_diff(by, dy);
+  DD cdx = dd_from_diff(cx, dx), cdy = dd_from_diff(cy, dy);
+  DD alift = dd_add(dd_mul(adx, adx), dd_mul(ady, ady));
+  DD blift = dd_add(dd_mul(bdx, bdx), dd_mul(bdy, bdy));
+  DD clift = dd_add(dd_mul(cdx, cdx), dd_mul(cdy, cdy));
+  DD ma = dd_add(dd_mul(bdx, cdy), dd_neg(dd_mul(cdx, bdy)));
+  DD mb = dd_add(dd_mul(cdx, ady), dd_neg(dd_mul(adx, cdy)));
+  DD mc = dd_add(dd_mul(adx, bdy), dd_neg(dd_mul(bdx, ady)));
+  DD det = dd_add(dd_add(dd_mul(alift, ma), dd_mul(blift, mb)), dd_mul(clift, mc));
+  double permanent = (fabs(bdx.hi * cdy.hi) + fabs(cdx.hi * bdy.hi)) * alift.hi +
+                     (fabs(cdx.hi * ady.hi) + fabs(adx.hi * cdy.hi)) * blift.hi +
+                     (fabs(adx.hi * bdy.hi) + fabs(bdx.hi * ady.hi)) * clift.hi;
+  // keep every intermediate far from underflow/overflow, else defer
+  if (!(permanent > 1e-250 && permanent < 1e250)) return 0;
+  double bound = permanent * 5.048709793414476e-29;  // 2^-94
+  if (det.hi > bound) return 1;
+  if (det.hi < -bound) return -1;
+  return 0;
+}
+
+GCDT_HD int orient2d_dd_sign(double ax, double ay, double bx, double by,
+                             double cx, double cy) {
+  DD acx = dd_from_diff(ax, cx), bcy = dd_from_diff(by, cy);
+  DD acy = dd_from_diff(ay, cy), bcx = dd_from_diff(bx, cx);
+  DD det = dd_add(dd_mul(acx, bcy), dd_neg(dd_mul(acy, bcx)));
+  double permanent = fabs(acx.hi * bcy.hi) + fabs(acy.hi * bcx.hi);
+  if (!(permanent > 1e-280 && permanent < 1e280)) return 0;
+  double bound = permanent * 5.048709793414476e-29;  // 2^-94
+  if (det.hi > bound) return 1;
+  if (det.hi < -bound) return -1;
+  return 0;
+}
+
 // Sign-exact incircle: >0 if d strictly inside circle(a,b,c), a,b,c CCW.
 // use_fp32 enables the FP32 semi-static stage.  stats may be null.
 GCDT_HD int incircle_sign(double ax, double ay, double bx, double by,
@@ -424,6 +514,11 @@ GCDT_HD int incircle_sign(double ax, double ay, double bx, double by,
   if (ok) {
     if (stats) stats->fp64_hits++;
     return det > 0.0 ? 1 : -1;
+  }
+  int sd = incircle_dd_sign(ax, ay, bx, by, cx, cy, dx, dy);
+  if (sd != 0) {
+    if (stats) stats->dd_hits++;
+    return sd;
   }
   if (stats) stats->exact_calls++;
   double s = incircle_exact(ax, ay, bx, by, cx, cy, dx, dy, ws);
@@ -447,6 +542,11 @@ GCDT_HD int orient2d_sign(double ax, double ay, double bx, double by,
   if ((det > GCDT_CCW_BOUND_A * detsum) || (-det > GCDT_CCW_BOUND_A * detsum)) {
     if (stats) stats->fp64_hits++;
     return det > 0.0 ? 1 : -1;
+  }
+  int sd = orient2d_dd_sign(ax, ay, bx, by, cx, cy);
+  if (sd != 0) {
+    if (stats) stats->dd_hits++;
+    return sd;
   }
   if (stats) stats->exact_calls++;
   double s = orient2d_exact(ax, ay, bx, by, cx, cy);

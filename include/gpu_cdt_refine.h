@@ -135,6 +135,7 @@ typedef struct gcdt_stats {
   long long n_points, n_triangles, n_subsegs;
   long long cand_evals;  /* bad-pass candidates evaluated */
   long long big_cavities; /* cavity evaluations that left the LDS fast path */
+  long long pred_dd;     /* signs decided by the double-double stage */
 } gcdt_stats;
 
 typedef struct gcdt_mesh gcdt_mesh;       /* host-side result */
@@ -210,7 +211,9 @@ void gcdt_cdt_free(gcdt_cdt *c);
 int gcdt_host_predicates(const double *pts8, int n, int pred_mode,
                          int *orient_out, int *incircle_out);
 
-/* Exact predicates on the device, for tests: evaluates n queries. */
+/* Exact predicates on the device, for tests: evaluates n queries;
+ * stage_counts (4 entries, may be NULL): signs decided by the FP32, FP64,
+ * exact-expansion and double-double stages. */
 int gcdt_device_predicates(const double *pts8, int n, int pred_mode,
                            int *orient_out, int *incircle_out,
                            long long *stage_counts);
