@@ -37,7 +37,7 @@ void gcdt_default_options(gcdt_options *o) {
   o->offcenter = 1;
   o->mode = 1;
   o->cav_cap0 = 32;
-  o->cav_cap_max = 16384;
+  o->cav_cap_max = 2048;
   o->cand_budget = 1LL << 30;
   o->max_rounds = 100000;
   o->priority = 1;
@@ -52,7 +52,10 @@ void gcdt_default_options(gcdt_options *o) {
 static bool valid_options(const gcdt_options *o) {
   if (!o) { g_err = "options is NULL"; return false; }
   if (!(o->theta_deg >= 0.0 && o->theta_deg < 60.0)) { g_err = "theta out of range [0, 60)"; return false; }
-  if (o->cav_cap0 < 4 || o->cav_cap_max < o->cav_cap0) { g_err = "bad cavity capacity"; return false; }
+  if (o->cav_cap0 < 4 || o->cav_cap_max < o->cav_cap0 || o->cav_cap_max > 2048) {
+    g_err = "cavity capacity must satisfy 4 <= cav_cap0 <= cav_cap_max <= 2048";
+    return false;
+  }
   if (o->cand_budget < 6LL * o->cav_cap_max + 16) { g_err = "cand_budget smaller than one candidate at cav_cap_max"; return false; }
   if (o->priority < 0 || o->priority > 2) { g_err = "priority must be 0, 1 or 2"; return false; }
   if (o->claim_iters < 1 || o->claim_iters > 64) { g_err = "claim_iters must be in [1, 64]"; return false; }

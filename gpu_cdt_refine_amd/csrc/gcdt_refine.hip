@@ -294,221 +294,249 @@ __device__ __forceinline__ void mark_enc(const MeshD &m, int s,
   if (atomicCAS(&m.senc[s], 0, 1) == 0) atomicAdd(&cnt[CNT_MARKS], 1ull);
 }
 
-// Candidate evaluation.  L is the cavity list storage with room for `cap`
-// cavity triangles plus the ring: a per-thread LDS slice in the fast kernels
-// (cap = min(K, LDS_CAV)) or the candidate's global workspace in the big
-// kernels (cap = K).  A fast-path overflow with K > LDS_CAV becomes ST_BIG and
-// is recomputed by the big kernel; no side effect has happened by then, so
-// both paths give exactly the result of one evaluation with capacity K.
-template <bool FAST>
-__device__ __forceinline__ void eval_seg(const MeshD &m, const ParamsD &pr,
-                                         const PassD &ps, int i, int *L,
-                                         int cap, int Kmax, double *pws,
-                                         PredStats *st) {
-  int s = ps.ids[i];
-  int status;
-  double cx = 0.0, cy = 0.0;
-  int n = 0, r = 0;
-  if (!split_point(m, s, cx, cy)) {
-    status = ST_GIVEUP;
-  } else {
-    int t1, j1, t2, j2;
-    seg_sides(m, s, t1, j1, t2, j2);
-    int seed0 = t1 >= 0 ? t1 : t2;
-    int seed1 = t1 >= 0 ? t2 : -1;
-    status = build_cavity(m, L, cx, cy, seed0, seed1, s, cap, pws,
-                          pr.use_fp32, st, n, r);
-    if (status == ST_OVERFLOW) {
-      if (FAST && ps.K > cap) status = ST_BIG;
-      else if (ps.K >= Kmax) status = ST_GIVEUP;
-    }
+constexpr int BIG_LDS_BYTES = 160 * 1024;
+
+// ---------------------------------------------------------------------------
+// Candidate evaluation: one candidate per wave64 (block of 64 lanes).
+//
+// Every lane computes the Steiner / split point and the walk (same loads,
+// broadcast); the Bowyer-Watson cavity is then grown level by level, the 64
+// lanes testing the three edges of every frontier triangle in parallel.
+// Membership is an LDS hash set (atomicCAS insert), the cavity list an LDS
+// array appended with atomicAdd and finally sorted, so the result is the
+// cavity *set* in ascending triangle order -- independent of timing.  The
+// boundary / disk / orientation checks, the ring and the encroachment test
+// run lane-parallel over the cavity's edges.
+// LDS per block: list (2K+2) | cavity hash (HC) | ring hash (HR) | 8 scalars.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int pow2_at_least(int x) {
+  int p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+__host__ __device__ inline int cav_hash_cap(int K) { return pow2_at_least(2 * (K + 64)); }
+__host__ __device__ inline int ring_hash_cap(int K) { return pow2_at_least(4 * K + 8); }
+__host__ __device__ inline int wave_lds_ints(int K) {
+  return (2 * K + 2) + cav_hash_cap(K) + ring_hash_cap(K) + 8;
+}
+
+__device__ __forceinline__ unsigned hmix(int x) {
+  unsigned h = (unsigned)x * 0x9E3779B1u;
+  return h ^ (h >> 15);
+}
+// true if x was not yet present (and is now)
+__device__ __forceinline__ bool hset_insert(int *h, int H, int x) {
+  unsigned k = hmix(x) & (unsigned)(H - 1);
+  for (;;) {
+    int prev = atomicCAS(&h[k], -1, x);
+    if (prev == -1) return true;
+    if (prev == x) return false;
+    k = (k + 1) & (unsigned)(H - 1);
   }
-  ps.cstatus[i] = status;
-  if (status == ST_BIG) {
-    atomicAdd(&ps.cnt[CNT_BIG], 1ull);
-    return;
-  }
-  ps.cpt[i] = make_double2(cx, cy);
-  ps.cn[i] = n;
-  ps.cr[i] = r;
-  ps.cprio[i] = 0;
-  if (status == ST_VALID) {
-    atomicAdd(&ps.cnt[CNT_VALID], 1ull);
-    if (FAST) {
-      int *G = ps.ws + (size_t)i * ps.R;
-      for (int q = 0; q < n + r; q++) G[q] = L[q];
-    }
-  } else if (status == ST_OVERFLOW) {
-    atomicAdd(&ps.cnt[CNT_OVERFLOW], 1ull);
-  } else if (status == ST_GIVEUP) {
-    atomicAdd(&ps.cnt[CNT_GIVEUPS], 1ull);
-    m.senc[s] = 2;
+}
+__device__ __forceinline__ bool hset_has(const int *h, int H, int x) {
+  unsigned k = hmix(x) & (unsigned)(H - 1);
+  for (;;) {
+    int v = h[k];
+    if (v == x) return true;
+    if (v == -1) return false;
+    k = (k + 1) & (unsigned)(H - 1);
   }
 }
 
-template <bool FAST>
-__device__ __forceinline__ void eval_bad(const MeshD &m, const ParamsD &pr,
-                                         const PassD &ps, int i, int *L,
-                                         int cap, int Kmax, double *pws,
-                                         PredStats *st) {
-  bool fp32 = pr.use_fp32 != 0;
-  int t0 = ps.ids[i];
-  int4 v0 = m.TV[t0];
-  int prio = 0;
-  if (pr.priority == 1) prio = shortest_edge_bucket(m, v0);
-  else if (pr.priority == 2 && ps.learned_prio) prio = ps.learned_prio[i];
-  int status = ST_VALID;
-  double cx = 0.0, cy = 0.0;
-  int n = 0, r = 0;
-  if (!steiner_point(m, pr, v0, cx, cy)) {
-    status = ST_GIVEUP;
-  } else {
-    int tl = -1, jl = -1, sg = -1;
-    int w = walk(m, t0, cx, cy, fp32, st, tl, jl, sg);
-    int seed1 = -1;
-    if (w == 3) {
-      status = ST_REJ_BLOCKED;
-      mark_enc(m, sg, ps.cnt);
-    } else if (w == 2 || w == 4) {
-      status = ST_GIVEUP;
-    } else if (w == 1) {
-      int s = comp(m.TS[tl], jl);
-      if (s >= 0) {
-        status = ST_REJ_BLOCKED;
-        mark_enc(m, s, ps.cnt);
+enum { SC_N = 0, SC_R, SC_OVF, SC_GIVEUP, SC_NBND, SC_ENC };
+
+template <bool IS_SEG>
+__global__ __launch_bounds__(64) void k_cavity(MeshD m, ParamsD pr, PassD ps,
+                                               int Kmax) {
+  extern __shared__ int sm[];
+  const int K = ps.K;
+  const int HC = cav_hash_cap(K), HR = ring_hash_cap(K);
+  int *L = sm;
+  int *hc = L + (2 * K + 2);
+  int *hr = hc + HC;
+  int *sc = hr + HR;
+  const int lane = threadIdx.x;
+  const bool fp32 = pr.use_fp32 != 0;
+  PredStats st = {0, 0, 0, 0}, junk = {0, 0, 0, 0};
+  double *pws = ps.pws + ((size_t)blockIdx.x * 64 + lane) * GCDT_INCIRCLE_WS;
+  for (int i = blockIdx.x; i < ps.nc; i += gridDim.x) {
+    // ---- point + seeds (uniform across the wave) ----
+    int status = ST_VALID, prio = 0, seed0 = -1, seed1 = -1, split_s = -1, t0 = -1;
+    double cx = 0.0, cy = 0.0;
+    if (IS_SEG) {
+      split_s = ps.ids[i];
+      if (!split_point(m, split_s, cx, cy)) {
+        status = ST_GIVEUP;
       } else {
-        int nb = comp(m.TN[tl], jl);
-        if (nb < 0) status = ST_GIVEUP;
-        else seed1 = nb >> 2;
+        int t1, j1, t2, j2;
+        seg_sides(m, split_s, t1, j1, t2, j2);
+        seed0 = t1 >= 0 ? t1 : t2;
+        seed1 = t1 >= 0 ? t2 : -1;
       }
-    }
-    if (status == ST_VALID) {
-      status = build_cavity(m, L, cx, cy, tl, seed1, -1, cap, pws, fp32, st,
-                            n, r);
-      if (status == ST_OVERFLOW) {
-        if (FAST && ps.K > cap) status = ST_BIG;
-        else if (ps.K >= Kmax) status = ST_GIVEUP;
-      }
-    }
-    if (status == ST_VALID) {
-      // Ruppert/Chew rule: a Steiner point that encroaches a subsegment on its
-      // cavity boundary is not inserted; every such subsegment is marked.
-      bool enc = false;
-      for (int idx = 0; idx < n; idx++) {
-        int t = L[idx];
-        int4 ts = m.TS[t];
-        for (int j = 0; j < 3; j++) {
-          int s = comp(ts, j);
-          if (s < 0) continue;
-          int2 sv = m.SV[s];
-          if (encroaches(m, pr, sv.x, sv.y, cx, cy)) {
-            mark_enc(m, s, ps.cnt);
-            enc = true;
+    } else {
+      t0 = ps.ids[i];
+      int4 v0 = m.TV[t0];
+      if (pr.priority == 1) prio = shortest_edge_bucket(m, v0);
+      else if (pr.priority == 2 && ps.learned_prio) prio = ps.learned_prio[i];
+      if (!steiner_point(m, pr, v0, cx, cy)) {
+        status = ST_GIVEUP;
+      } else {
+        int tl = -1, jl = -1, sg = -1;
+        int w = walk(m, t0, cx, cy, fp32, lane == 0 ? &st : &junk, tl, jl, sg);
+        if (w == 3) {
+          status = ST_REJ_BLOCKED;
+          if (lane == 0) mark_enc(m, sg, ps.cnt);
+        } else if (w == 2 || w == 4) {
+          status = ST_GIVEUP;
+        } else {
+          seed0 = tl;
+          if (w == 1) {
+            int s = comp(m.TS[tl], jl);
+            if (s >= 0) {
+              status = ST_REJ_BLOCKED;
+              if (lane == 0) mark_enc(m, s, ps.cnt);
+            } else {
+              int nb = comp(m.TN[tl], jl);
+              if (nb < 0) status = ST_GIVEUP;
+              else seed1 = nb >> 2;
+            }
           }
         }
       }
-      if (enc) status = ST_REJ_ENC;
     }
-  }
-  ps.cstatus[i] = status;
-  if (status == ST_BIG) {
-    atomicAdd(&ps.cnt[CNT_BIG], 1ull);
-    return;
-  }
-  ps.cpt[i] = make_double2(cx, cy);
-  ps.cn[i] = n;
-  ps.cr[i] = r;
-  ps.cprio[i] = prio;
-  if (status == ST_VALID) {
-    atomicAdd(&ps.cnt[CNT_VALID], 1ull);
-    if (FAST) {
-      int *G = ps.ws + (size_t)i * ps.R;
-      for (int q = 0; q < n + r; q++) G[q] = L[q];
-    }
-  } else if (status == ST_OVERFLOW) {
-    atomicAdd(&ps.cnt[CNT_OVERFLOW], 1ull);
-  } else if (status == ST_REJ_ENC) {
-    atomicAdd(&ps.cnt[CNT_REJ_ENC], 1ull);
-  } else if (status == ST_REJ_BLOCKED) {
-    atomicAdd(&ps.cnt[CNT_REJ_BLOCKED], 1ull);
-  } else if (status == ST_GIVEUP) {
-    atomicAdd(&ps.cnt[CNT_GIVEUPS], 1ull);
-    m.tflag[t0] |= TF_GIVEUP;
-  }
-}
-
-// Fast kernels: per-thread cavity list in LDS.
-__global__ __launch_bounds__(CAV_BLOCK) void k_seg_cavity(MeshD m, ParamsD pr,
-                                                          PassD ps, int Kmax) {
-  __shared__ int lds[CAV_BLOCK * LDS_LIST];
-  PredStats st = {0, 0, 0, 0};
-  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
-  int *L = lds + threadIdx.x * LDS_LIST;
-  int cap = ps.K < LDS_CAV ? ps.K : LDS_CAV;
-  for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x)
-    eval_seg<true>(m, pr, ps, i, L, cap, Kmax, pws, &st);
-  flush_stats(ps.cnt, st);
-}
-
-__global__ __launch_bounds__(CAV_BLOCK) void k_bad_cavity(MeshD m, ParamsD pr,
-                                                          PassD ps, int Kmax) {
-  __shared__ int lds[CAV_BLOCK * LDS_LIST];
-  PredStats st = {0, 0, 0, 0};
-  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
-  int *L = lds + threadIdx.x * LDS_LIST;
-  int cap = ps.K < LDS_CAV ? ps.K : LDS_CAV;
-  for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x)
-    eval_bad<true>(m, pr, ps, i, L, cap, Kmax, pws, &st);
-  flush_stats(ps.cnt, st);
-}
-
-// Big kernels: candidates whose cavity exceeded the per-thread LDS list,
-// recomputed with the full capacity K.  Each thread's list lives in dynamic
-// LDS (2K+2 ints; blockDim threads per block sized on the host to fit), so
-// membership scans never touch HBM; only when even one list cannot fit in
-// LDS (K > BIG_LDS_K) does the list stay in the global workspace.
-constexpr int BIG_LDS_BYTES = 128 * 1024;
-constexpr int BIG_LDS_K = (BIG_LDS_BYTES / 4 - 2) / 2;
-
-__global__ __launch_bounds__(64) void k_seg_cavity_big(MeshD m, ParamsD pr,
-                                                       PassD ps, int Kmax,
-                                                       int use_lds) {
-  extern __shared__ int dyn[];
-  PredStats st = {0, 0, 0, 0};
-  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
-  for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x)
-    if (ps.cstatus[i] == ST_BIG) {
-      int *G = ps.ws + (size_t)i * ps.R;
-      if (use_lds) {
-        int *L = dyn + threadIdx.x * (2 * ps.K + 2);
-        eval_seg<true>(m, pr, ps, i, L, ps.K, Kmax, pws, &st);
+    int n = 0, r = 0;
+    if (status == ST_VALID) {
+      // ---- breadth-first growth of the cavity ----
+      for (int k = lane; k < HC; k += 64) hc[k] = -1;
+      for (int k = lane; k < HR; k += 64) hr[k] = -1;
+      if (lane < 8) sc[lane] = 0;
+      __syncthreads();
+      if (lane == 0) {
+        L[0] = seed0;
+        hset_insert(hc, HC, seed0);
+        int c0 = 1;
+        if (seed1 >= 0) {
+          L[1] = seed1;
+          hset_insert(hc, HC, seed1);
+          c0 = 2;
+        }
+        sc[SC_N] = c0;
+      }
+      __syncthreads();
+      int fb = 0;
+      for (;;) {
+        int fe = sc[SC_N];
+        int ovf = sc[SC_OVF];
+        __syncthreads();
+        if (ovf || fb >= fe) break;
+        int items = (fe - fb) * 3;
+        for (int it = lane; it < items; it += 64) {
+          int t = L[fb + it / 3], j = it % 3;
+          if (comp(m.TS[t], j) >= 0) continue;  // subsegments are never crossed
+          int nb = comp(m.TN[t], j);
+          if (nb < 0) continue;
+          int u = nb >> 2;
+          if (hset_has(hc, HC, u)) continue;
+          int4 w = m.TV[u];
+          double2 a = m.P[w.x], b = m.P[w.y], c = m.P[w.z];
+          if (incircle_sign(a.x, a.y, b.x, b.y, c.x, c.y, cx, cy, pws, fp32, &st) > 0) {
+            // u is in the cavity.  Once K triangles are in, a triangle that is
+            // still absent proves the cavity larger than K; inserts stay
+            // below K + 64, so the hash set never fills.
+            if (hset_has(hc, HC, u)) continue;
+            if (sc[SC_N] >= K) {
+              sc[SC_OVF] = 1;
+            } else if (hset_insert(hc, HC, u)) {
+              int pos = atomicAdd(&sc[SC_N], 1);
+              if (pos < K) L[pos] = u;
+              else sc[SC_OVF] = 1;
+            }
+          }
+        }
+        __syncthreads();
+        fb = fe;
+      }
+      n = sc[SC_N];
+      if (sc[SC_OVF] || n > K) {
+        status = (K >= Kmax) ? ST_GIVEUP : ST_OVERFLOW;
       } else {
-        eval_seg<false>(m, pr, ps, i, G, ps.K, Kmax, pws, &st);
+        // ---- sort the cavity (rank sort into the free half of L) ----
+        int *tmp = L + K + 2;
+        for (int e = lane; e < n; e += 64) {
+          int v = L[e], rank = 0;
+          for (int x = 0; x < n; x++) rank += L[x] < v;
+          tmp[rank] = v;
+        }
+        __syncthreads();
+        for (int e = lane; e < n; e += 64) L[e] = tmp[e];
+        __syncthreads();
+        // ---- disk and star checks over every cavity edge ----
+        for (int it = lane; it < 3 * n; it += 64) {
+          int t = L[it / 3], j = it % 3;
+          int s = comp(m.TS[t], j);
+          int nb = comp(m.TN[t], j);
+          bool inside = nb >= 0 && hset_has(hc, HC, nb >> 2);
+          if (s >= 0 && s != split_s && inside) sc[SC_GIVEUP] = 1;
+          bool bnd = (s >= 0 && s != split_s) || nb < 0 || !inside;
+          if (bnd) {
+            atomicAdd(&sc[SC_NBND], 1);
+            if (!(s >= 0 && s == split_s)) {
+              int4 tv = m.TV[t];
+              double2 u = m.P[comp(tv, (j + 1) % 3)], v = m.P[comp(tv, (j + 2) % 3)];
+              if (orient2d_sign(cx, cy, u.x, u.y, v.x, v.y, fp32, &st) <= 0) sc[SC_GIVEUP] = 1;
+            }
+            if (nb >= 0 && !inside && hset_insert(hr, HR, nb >> 2)) {
+              int pos = atomicAdd(&sc[SC_R], 1);
+              if (pos < K + 2) L[n + pos] = nb >> 2;
+            }
+          }
+          if (!IS_SEG && s >= 0) {
+            int2 sv = m.SV[s];
+            if (encroaches(m, pr, sv.x, sv.y, cx, cy)) sc[SC_ENC] = 1;
+          }
+        }
+        __syncthreads();
+        r = sc[SC_R];
+        if (sc[SC_GIVEUP] || sc[SC_NBND] != n + 2 || r > n + 2) {
+          status = ST_GIVEUP;
+        } else if (!IS_SEG && sc[SC_ENC]) {
+          // Ruppert/Chew rule: a Steiner point that encroaches a subsegment on
+          // its cavity boundary is not inserted; every such subsegment is marked.
+          for (int it = lane; it < 3 * n; it += 64) {
+            int t = L[it / 3], j = it % 3;
+            int s = comp(m.TS[t], j);
+            if (s < 0) continue;
+            int2 sv = m.SV[s];
+            if (encroaches(m, pr, sv.x, sv.y, cx, cy)) mark_enc(m, s, ps.cnt);
+          }
+          status = ST_REJ_ENC;
+        }
       }
     }
-  flush_stats(ps.cnt, st);
-}
-
-__global__ __launch_bounds__(64) void k_bad_cavity_big(MeshD m, ParamsD pr,
-                                                       PassD ps, int Kmax,
-                                                       int use_lds) {
-  extern __shared__ int dyn[];
-  PredStats st = {0, 0, 0, 0};
-  int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  double *pws = ps.pws + (size_t)gtid * GCDT_INCIRCLE_WS;
-  for (int i = gtid; i < ps.nc; i += gridDim.x * blockDim.x)
-    if (ps.cstatus[i] == ST_BIG) {
+    // ---- outputs ----
+    if (status == ST_VALID) {
       int *G = ps.ws + (size_t)i * ps.R;
-      if (use_lds) {
-        int *L = dyn + threadIdx.x * (2 * ps.K + 2);
-        eval_bad<true>(m, pr, ps, i, L, ps.K, Kmax, pws, &st);
-      } else {
-        eval_bad<false>(m, pr, ps, i, G, ps.K, Kmax, pws, &st);
+      for (int q = lane; q < n + r; q += 64) G[q] = L[q];
+    }
+    if (lane == 0) {
+      ps.cstatus[i] = status;
+      ps.cpt[i] = make_double2(cx, cy);
+      ps.cn[i] = status == ST_VALID ? n : 0;
+      ps.cr[i] = status == ST_VALID ? r : 0;
+      ps.cprio[i] = prio;
+      if (status == ST_VALID) atomicAdd(&ps.cnt[CNT_VALID], 1ull);
+      else if (status == ST_OVERFLOW) atomicAdd(&ps.cnt[CNT_OVERFLOW], 1ull);
+      else if (status == ST_REJ_ENC) atomicAdd(&ps.cnt[CNT_REJ_ENC], 1ull);
+      else if (status == ST_REJ_BLOCKED) atomicAdd(&ps.cnt[CNT_REJ_BLOCKED], 1ull);
+      else if (status == ST_GIVEUP) {
+        atomicAdd(&ps.cnt[CNT_GIVEUPS], 1ull);
+        if (IS_SEG) m.senc[split_s] = 2;
+        else m.tflag[t0] |= TF_GIVEUP;
       }
     }
+    __syncthreads();  // LDS is reused by the next candidate
+  }
   flush_stats(ps.cnt, st);
 }
 
@@ -803,10 +831,10 @@ Refiner::Refiner(const gcdt_options &o) : opt(o) {
   HC(hipHostMalloc((void **)&h_cnt, CNT_N * sizeof(unsigned long long), hipHostMallocDefault));
   HC(hipEventCreate(&ev0));
   HC(hipEventCreate(&ev1));
-  // allow the big-cavity and apply kernels up to BIG_LDS_BYTES of dynamic LDS
-  HC(hipFuncSetAttribute((const void *)k_bad_cavity_big,
+  // cavity and apply kernels take up to BIG_LDS_BYTES of dynamic LDS
+  HC(hipFuncSetAttribute((const void *)k_cavity<true>,
                          hipFuncAttributeMaxDynamicSharedMemorySize, BIG_LDS_BYTES));
-  HC(hipFuncSetAttribute((const void *)k_seg_cavity_big,
+  HC(hipFuncSetAttribute((const void *)k_cavity<false>,
                          hipFuncAttributeMaxDynamicSharedMemorySize, BIG_LDS_BYTES));
   HC(hipFuncSetAttribute((const void *)k_apply,
                          hipFuncAttributeMaxDynamicSharedMemorySize, BIG_LDS_BYTES));
@@ -1084,28 +1112,17 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   ps.cnt = d_cnt;
   ps.learned_prio = learned_prio;
   HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
-  int gf = std::min(nblk(nc, CAV_BLOCK), 2 * pws_blocks);
-  // big-cavity launch shape: as many lists of 2K+2 ints per block as fit
-  struct { int blocks, threads, use_lds; size_t lds; } bg;
-  {
-    size_t per = sizeof(int) * (2 * (size_t)K + 2);
-    int th = (int)std::min<size_t>(64, BIG_LDS_BYTES / per);
-    bg.use_lds = th >= 1 ? 1 : 0;
-    bg.threads = th >= 1 ? th : 64;
-    bg.lds = th >= 1 ? per * th : 0;
-    bg.blocks = std::min(nblk(nc, bg.threads), pws_blocks * 256 / bg.threads);
-  }
+  // one candidate per wave64 block; the exact-predicate arena holds
+  // pws_blocks*256 lanes, i.e. pws_blocks*4 blocks of 64
+  int gw = std::min(nc, pws_blocks * 4);
+  size_t lds = sizeof(int) * (size_t)wave_lds_ints(K);
   if (is_seg) {
     tbegin(K_SEGCAV);
-    k_seg_cavity<<<gf, CAV_BLOCK, 0, stream>>>(m, pr, ps, opt.cav_cap_max);
-    if (K > LDS_CAV)
-      k_seg_cavity_big<<<bg.blocks, bg.threads, bg.lds, stream>>>(m, pr, ps, opt.cav_cap_max, bg.use_lds);
+    k_cavity<true><<<gw, 64, lds, stream>>>(m, pr, ps, opt.cav_cap_max);
     tend(K_SEGCAV, nc);
   } else {
     tbegin(K_BADCAV);
-    k_bad_cavity<<<gf, CAV_BLOCK, 0, stream>>>(m, pr, ps, opt.cav_cap_max);
-    if (K > LDS_CAV)
-      k_bad_cavity_big<<<bg.blocks, bg.threads, bg.lds, stream>>>(m, pr, ps, opt.cav_cap_max, bg.use_lds);
+    k_cavity<false><<<gw, 64, lds, stream>>>(m, pr, ps, opt.cav_cap_max);
     tend(K_BADCAV, nc);
   }
   HC(hipGetLastError());
@@ -1213,7 +1230,7 @@ void Refiner::run() {
     ws_ints = opt.cand_budget;
   }
   if (!c_pws) {
-    pws_blocks = 1024;
+    pws_blocks = 2048;
     dalloc(&c_pws, (size_t)pws_blocks * 256 * GCDT_INCIRCLE_WS);
   }
   ensure_cand(std::min(cands_per_pass(K), std::max(4 * nt0, 1024)));

@@ -100,7 +100,8 @@ typedef struct gcdt_options {
   int offcenter;         /* 1 = off-center (Offcenter), 0 = circumcenter */
   int mode;              /* 1 = Ruppert (diametral circle), else Chew (lens) */
   int cav_cap0;          /* initial cavity capacity K (triangles), default 32 */
-  int cav_cap_max;       /* largest cavity capacity, default 16384 */
+  int cav_cap_max;       /* largest cavity capacity, default and limit 2048
+                            (one wave's LDS); a larger cavity gives up */
   long long cand_budget; /* cavity workspace in ints; candidates per pass =
                             cand_budget / (6K + 16); default 2^30.  When more
                             triangles are bad, the smallest (priority, index)

@@ -169,7 +169,7 @@ def triangle_refine(points, triangles, segments, theta, mode=1):
 
 
 def oracle_refine(points, triangles, neighbors, segments, theta=25.0, offcenter=1,
-                  mode=1, cav_cap0=32, cav_cap_max=16384, cand_budget=1 << 30,
+                  mode=1, cav_cap0=32, cav_cap_max=2048, cand_budget=1 << 30,
                   max_rounds=100000, priority=1, cand_factor=4, cand_min=16384,
                   claim_iters=1):
     """Run the CPU restatement; returns (P, T, N, S, pflag, stats)."""

@@ -303,6 +303,11 @@ static int shortest_edge_bucket(const Mesh *m, int t) {
   return edge_bucket(mn);
 }
 
+static int cmp_int_c(const void *a, const void *b) {
+  int x = *(const int *)a, y = *(const int *)b;
+  return x < y ? -1 : (x > y);
+}
+
 static int in_list(const int *L, int n, int x) {
   for (int i = 0; i < n; i++)
     if (L[i] == x) return 1;
@@ -383,6 +388,8 @@ static int build_cavity(const Mesh *m, Cand *cd, int seed0, int seed1,
       }
     }
   }
+  /* the cavity is a set: keep it sorted by triangle index (slot order) */
+  qsort(L, (size_t)n, sizeof(int), cmp_int_c);
   /* disk check: boundary edges == n + 2; no subsegment strictly inside */
   int nbnd = 0;
   for (int idx = 0; idx < n; idx++) {
@@ -444,7 +451,8 @@ typedef struct {
  * Star ordering: closed fans start at the boundary edge whose origin has the
  * smallest vertex id; the open fan of a one-sided segment split starts at the
  * boundary edge after the split subsegment's edge.  Star triangle i is
- * (p, u_i, v_i) and takes slot L[i] for i < n, else new slot tbase + i - n. */
+ * (p, u_i, v_i) and takes slot L[i] (cavity triangles in ascending index
+ * order) for i < n, else new slot tbase + i - n. */
 static void apply_star(Mesh *m, const Cand *cd, int split_s, int p, int tbase,
                        int snew, int is_seg) {
   const int *L = cd->L;

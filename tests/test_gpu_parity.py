@@ -79,16 +79,15 @@ def circle_pslg(oracle, k=72, n_out=1500, seed=3):
 
 @pytest.mark.parametrize("cap0", [4, 64])
 def test_gpu_cavity_capacity_paths(gpu, oracle, cap0):
-    """Small K forces overflow and K growth; K > 32 with large cavities
-    exercises the global-memory cavity path."""
+    """Small K forces overflow and K growth; large cavities (most of a
+    72-point circle's fan) exercise multi-level wave-parallel growth."""
     P, T, N, S = circle_pslg(oracle)
     kw = dict(cav_cap0=cap0, cav_cap_max=256)
     r = gpu.refine(P, T, N, S, theta=25.0, **kw)
     Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, theta=25.0, **kw)
     if cap0 == 4:
         assert st["rej_overflow"] > 0
-    else:
-        assert r.stats["big_cavities"] > 0
+    assert st["final_cav_cap"] >= 64
     assert np.array_equal(r.points.view(np.uint64), Po.view(np.uint64))
     assert np.array_equal(r.triangles, To)
     assert np.array_equal(r.neighbors, No)
