@@ -39,6 +39,7 @@ class Options(C.Structure):
         ("device", C.c_int),
         ("cand_factor", C.c_int),
         ("cand_min", C.c_int),
+        ("log_training", C.c_int),
         ("claim_iters", C.c_int),
     ]
 
@@ -110,6 +111,21 @@ def lib():
         L.gcdt_last_error.restype = C.c_char_p
         L.gcdt_device_predicates.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
         L.gcdt_device_predicates.restype = C.c_int
+        L.gcdt_session_set_priority_model.argtypes = [vp, vp]
+        L.gcdt_session_set_priority_model.restype = C.c_int
+        L.gcdt_session_logs.argtypes = [vp, vp, vp, C.c_longlong]
+        L.gcdt_session_logs.restype = C.c_longlong
+        L.gcdt_mlp_nparam.restype = C.c_int
+        L.gcdt_mlp_forward.argtypes = [vp, C.c_int, vp, vp]
+        L.gcdt_mlp_forward.restype = C.c_int
+        L.gcdt_mlp_grad.argtypes = [vp, vp, C.c_int, vp, vp, vp]
+        L.gcdt_mlp_grad.restype = C.c_int
+        L.gcdt_mlp_train.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_float,
+                                     C.c_ulonglong, vp, vp]
+        L.gcdt_mlp_train.restype = C.c_int
+        L.gcdt_mfma_probe.argtypes = [vp, vp, vp]
+        L.gcdt_mfma_probe.restype = C.c_int
+        L.gcdt_mlp_last_error.restype = C.c_char_p
         if hasattr(L, "gcdt_generate_pslg"):
             L.gcdt_generate_pslg.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double,
                                              C.c_double, C.c_double, C.c_ulonglong, vp, vp]
@@ -224,6 +240,22 @@ class Session:
         P, T, N, S, ps, ts = _fetch_mesh(m)
         return RefineResult(P, T, N, S, ps, ts, {})
 
+    def set_priority_model(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == lib().gcdt_mlp_nparam()
+        if lib().gcdt_session_set_priority_model(self.h, _ptr(p)) != 0:
+            _err(-2, "gcdt_session_set_priority_model")
+
+    def logs(self):
+        """(X[n,16], t[n]) recorded during the last run (log_training=1)."""
+        L = lib()
+        n = L.gcdt_session_logs(self.h, None, None, 0)
+        X = np.empty((n, 16), np.float32)
+        t = np.empty(n, np.float32)
+        if n:
+            L.gcdt_session_logs(self.h, _ptr(X), _ptr(t), n)
+        return X, t
+
     def kernel_times(self):
         rec = (KernelTime * 32)()
         n = lib().gcdt_session_kernel_times(self.h, rec, 32)
@@ -296,3 +328,72 @@ def build_cdt(points, segments):
     lib().gcdt_cdt_copy(c, _ptr(T), _ptr(N), _ptr(SS))
     lib().gcdt_cdt_free(c)
     return T, N, SS
+
+
+# --------------------------------------------------------------- learned MLP
+MLP_IN, MLP_H = 16, 64
+
+
+def mlp_nparam():
+    return lib().gcdt_mlp_nparam()
+
+
+def mlp_init(seed=0):
+    """He-initialised parameters (FP32 master copy)."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(mlp_nparam(), np.float32)
+    o = 0
+    p[o:o + MLP_IN * MLP_H] = rng.normal(0, np.sqrt(2 / MLP_IN), MLP_IN * MLP_H); o += MLP_IN * MLP_H
+    o += MLP_H
+    p[o:o + MLP_H * MLP_H] = rng.normal(0, np.sqrt(2 / MLP_H), MLP_H * MLP_H); o += MLP_H * MLP_H
+    o += MLP_H
+    p[o:o + MLP_H] = rng.normal(0, np.sqrt(1 / MLP_H), MLP_H)
+    return p
+
+
+def _mlp_err(rc, what):
+    raise RuntimeError(f"{what} failed ({rc}): {lib().gcdt_mlp_last_error().decode()}")
+
+
+def mlp_forward(X, params):
+    X = np.ascontiguousarray(X, np.float32).reshape(-1, MLP_IN)
+    p = np.ascontiguousarray(params, np.float32)
+    y = np.empty(len(X), np.float32)
+    rc = lib().gcdt_mlp_forward(_ptr(X), len(X), _ptr(p), _ptr(y))
+    if rc:
+        _mlp_err(rc, "gcdt_mlp_forward")
+    return y
+
+
+def mlp_grad(X, t, params):
+    X = np.ascontiguousarray(X, np.float32).reshape(-1, MLP_IN)
+    t = np.ascontiguousarray(t, np.float32)
+    p = np.ascontiguousarray(params, np.float32)
+    g = np.empty(mlp_nparam(), np.float32)
+    loss = np.zeros(1, np.float32)
+    rc = lib().gcdt_mlp_grad(_ptr(X), _ptr(t), len(X), _ptr(p), _ptr(g), _ptr(loss))
+    if rc:
+        _mlp_err(rc, "gcdt_mlp_grad")
+    return g, float(loss[0])
+
+
+def mlp_train(X, t, params, epochs=10, batch=4096, lr=1e-3, seed=0):
+    X = np.ascontiguousarray(X, np.float32).reshape(-1, MLP_IN)
+    t = np.ascontiguousarray(t, np.float32)
+    p = np.ascontiguousarray(params, np.float32).copy()
+    losses = np.zeros(epochs, np.float32)
+    rc = lib().gcdt_mlp_train(_ptr(X), _ptr(t), len(X), epochs, batch, lr, seed, _ptr(p),
+                              _ptr(losses))
+    if rc:
+        _mlp_err(rc, "gcdt_mlp_train")
+    return p, losses
+
+
+def mfma_probe(A, B):
+    A = np.ascontiguousarray(A, np.float32).reshape(16, 16)
+    B = np.ascontiguousarray(B, np.float32).reshape(16, 16)
+    D = np.empty((16, 16), np.float32)
+    rc = lib().gcdt_mfma_probe(_ptr(A), _ptr(B), _ptr(D))
+    if rc:
+        _mlp_err(rc, "gcdt_mfma_probe")
+    return D

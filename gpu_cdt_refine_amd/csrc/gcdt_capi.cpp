@@ -4,6 +4,7 @@
 #include <cstring>
 #include <exception>
 #include <string>
+#include <algorithm>
 
 #include "gcdt_internal.h"
 #include "gpu_cdt_refine.h"
@@ -47,6 +48,7 @@ void gcdt_default_options(gcdt_options *o) {
   o->cand_factor = 4;
   o->cand_min = 16384;
   o->claim_iters = 1;
+  o->log_training = 0;
 }
 
 static bool valid_options(const gcdt_options *o) {
@@ -142,6 +144,25 @@ int gcdt_session_download(gcdt_session *s, gcdt_mesh **out) {
     g_err = e.what();
     return -2;
   }
+}
+
+int gcdt_session_set_priority_model(gcdt_session *s, const float *params) {
+  g_err.clear();
+  try {
+    s->r->set_priority_model(params, gcdt_mlp_nparam());
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -2;
+  }
+}
+
+long long gcdt_session_logs(gcdt_session *s, float *X, float *t, long long max_n) {
+  long long n = (long long)s->r->log_t.size();
+  long long k = std::min(n, max_n);
+  if (X && k > 0) std::memcpy(X, s->r->log_X.data(), sizeof(float) * 16 * (size_t)k);
+  if (t && k > 0) std::memcpy(t, s->r->log_t.data(), sizeof(float) * (size_t)k);
+  return n;
 }
 
 int gcdt_session_kernel_times(gcdt_session *s, gcdt_kernel_time *rec,

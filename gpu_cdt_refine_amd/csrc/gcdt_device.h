@@ -91,6 +91,8 @@ struct PassD {
   const unsigned short *learned_prio;  // optional per-candidate priority
   unsigned long long epoch0;  // first claim epoch of this pass
   int iter;                   // claim iteration
+  float *feat;                // optional: FEAT features per candidate
+  int pass_index;             // bad-pass counter (feature)
 };
 
 // Per-thread cavity list kept in LDS: cavity (<= LDS_CAV) + ring (<= LDS_CAV+2).

@@ -28,6 +28,7 @@ enum KernelId {
   K_WIN,
   K_APPLY,
   K_EXPORT,
+  K_MLP,
   K_NKERN
 };
 
@@ -56,6 +57,10 @@ class Refiner {
   LearnedHook learned_hook = nullptr;
   void *learned_ctx = nullptr;
   unsigned short *learned_prio = nullptr;
+  float *mlp_params = nullptr;      // device FP32 master weights (priority 2)
+  void set_priority_model(const float *host_params, int nparam);
+  // training logs (options.log_training): features and conflict-degree labels
+  std::vector<float> log_X, log_t;
 
  private:
   template <class Pred>
@@ -96,6 +101,11 @@ class Refiner {
   int *sel_prio = nullptr, *sel_tmp = nullptr, *sel_list = nullptr;
   int sel_cap = 0;
   unsigned *d_hist = nullptr;
+  float *c_feat = nullptr, *c_score = nullptr;
+  int *c_deg = nullptr, *t_cnt = nullptr;
+  long long t_cnt_cap = 0;
+  int feat_cap = 0;
+  int pass_index = 0;
   // compaction scratch
   int *cb_cnt = nullptr, *cb_off = nullptr;
   int cb_cap = 0;
@@ -126,5 +136,8 @@ class Refiner {
 
 void device_predicates(const double *pts8, int n, int pred_mode, int *o,
                        int *ic, long long *stage_counts);
+void mlp_forward_device(const float *dX, int n, const float *dP, float *dy,
+                        hipStream_t stream);
+constexpr int FEAT = 16;  // features per candidate (learned priority)
 
 }  // namespace gcdt

@@ -344,7 +344,7 @@ __device__ __forceinline__ bool hset_has(const int *h, int H, int x) {
   }
 }
 
-enum { SC_N = 0, SC_R, SC_OVF, SC_GIVEUP, SC_NBND, SC_ENC };
+enum { SC_N = 0, SC_R, SC_OVF, SC_GIVEUP, SC_NBND, SC_ENC, SC_DSEG };
 
 template <bool IS_SEG>
 __global__ __launch_bounds__(64) void k_cavity(MeshD m, ParamsD pr, PassD ps,
@@ -377,8 +377,7 @@ __global__ __launch_bounds__(64) void k_cavity(MeshD m, ParamsD pr, PassD ps,
     } else {
       t0 = ps.ids[i];
       int4 v0 = m.TV[t0];
-      if (pr.priority == 1) prio = shortest_edge_bucket(m, v0);
-      else if (pr.priority == 2 && ps.learned_prio) prio = ps.learned_prio[i];
+      if (pr.priority == 1 || pr.priority == 2) prio = shortest_edge_bucket(m, v0);
       if (!steiner_point(m, pr, v0, cx, cy)) {
         status = ST_GIVEUP;
       } else {
@@ -411,6 +410,7 @@ __global__ __launch_bounds__(64) void k_cavity(MeshD m, ParamsD pr, PassD ps,
       for (int k = lane; k < HC; k += 64) hc[k] = -1;
       for (int k = lane; k < HR; k += 64) hr[k] = -1;
       if (lane < 8) sc[lane] = 0;
+      if (lane == 0) sc[SC_DSEG] = 0x7f7fffff;  // +FLT_MAX bits: no subsegment yet
       __syncthreads();
       if (lane == 0) {
         L[0] = seed0;
@@ -494,6 +494,15 @@ __global__ __launch_bounds__(64) void k_cavity(MeshD m, ParamsD pr, PassD ps,
           if (!IS_SEG && s >= 0) {
             int2 sv = m.SV[s];
             if (encroaches(m, pr, sv.x, sv.y, cx, cy)) sc[SC_ENC] = 1;
+            if (ps.feat) {  // distance of the Steiner point to this subsegment
+              double2 A = m.P[sv.x], B = m.P[sv.y];
+              float ex = (float)(B.x - A.x), ey = (float)(B.y - A.y);
+              float px = (float)(cx - A.x), py = (float)(cy - A.y);
+              float len2 = ex * ex + ey * ey;
+              float tt = len2 > 0.f ? fminf(fmaxf((px * ex + py * ey) / len2, 0.f), 1.f) : 0.f;
+              float ddx = px - tt * ex, ddy = py - tt * ey;
+              atomicMin(&sc[SC_DSEG], __float_as_int(sqrtf(ddx * ddx + ddy * ddy)));
+            }
           }
         }
         __syncthreads();
@@ -519,6 +528,32 @@ __global__ __launch_bounds__(64) void k_cavity(MeshD m, ParamsD pr, PassD ps,
       int *G = ps.ws + (size_t)i * ps.R;
       for (int q = lane; q < n + r; q += 64) G[q] = L[q];
     }
+    if (!IS_SEG && ps.feat && lane < FEAT) {
+      // per-candidate features of the learned priority
+      float fv = 0.f;
+      if (status == ST_VALID) {
+        int4 v0 = m.TV[t0];
+        double2 A = m.P[v0.x], O = m.P[v0.y], D = m.P[v0.z];
+        float l0 = (float)((O.x - D.x) * (O.x - D.x) + (O.y - D.y) * (O.y - D.y));
+        float l1 = (float)((D.x - A.x) * (D.x - A.x) + (D.y - A.y) * (D.y - A.y));
+        float l2 = (float)((A.x - O.x) * (A.x - O.x) + (A.y - O.y) * (A.y - O.y));
+        float sh2 = fminf(l0, fminf(l1, l2));
+        float area2 = fabsf((float)((O.x - A.x) * (D.y - A.y) - (O.y - A.y) * (D.x - A.x)));
+        float R = sqrtf(l0 * l1 * l2) / fmaxf(2.f * area2, 1e-30f);
+        float sh = sqrtf(sh2);
+        // smallest angle from the law of sines opposite the shortest edge
+        float sina = fminf(sh / fmaxf(2.f * R, 1e-30f), 1.f);
+        float dseg = __int_as_float(sc[SC_DSEG]);
+        if (lane == 0) fv = asinf(sina) * (180.f / 3.14159265f) / 60.f;
+        else if (lane == 1) fv = log2f(fmaxf(R / fmaxf(sh, 1e-30f), 1e-6f)) / 4.f;
+        else if (lane == 2) fv = (float)n / 16.f;
+        else if (lane == 3) fv = (float)r / 16.f;
+        else if (lane == 4) fv = dseg > 1e30f ? 1.f : fminf(dseg / fmaxf(sh, 1e-30f), 4.f) / 4.f;
+        else if (lane == 5) fv = (float)ps.pass_index / 256.f;
+        else if (lane == 7) fv = 1.f;
+      }
+      ps.feat[(size_t)i * FEAT + lane] = fv;
+    }
     if (lane == 0) {
       ps.cstatus[i] = status;
       ps.cpt[i] = make_double2(cx, cy);
@@ -540,13 +575,13 @@ __global__ __launch_bounds__(64) void k_cavity(MeshD m, ParamsD pr, PassD ps,
   flush_stats(ps.cnt, st);
 }
 
-// Claim key: higher wins.  Epoch in the top 19 bits makes older claims lose
-// without a reset pass; then (8191 - priority) and the inverted candidate
+// Claim key: higher wins.  Epoch in the top 11 bits makes older claims lose
+// without a reset pass; then (2^21-1 - priority) and the inverted candidate
 // index, so the smallest (priority, index) wins -- the reference's
 // atomicMin(trimarker, index) rule (refine.cu:4159-4172) with a priority.
 __device__ __forceinline__ unsigned long long claim_key(unsigned long long epoch,
                                                         int prio, int i) {
-  return (epoch << 45) | ((unsigned long long)(8191 - prio) << 32) |
+  return (epoch << 53) | ((unsigned long long)(0x1fffff - prio) << 32) |
          (unsigned long long)(0xffffffffu - (unsigned)i);
 }
 
@@ -592,7 +627,7 @@ __global__ __launch_bounds__(256) void k_block(MeshD m, PassD ps, int *cact) {
     int k = ps.cn[i] + ps.cr[i];
     for (int q = 0; q < k; q++) {
       unsigned long long c = m.claim[L[q]];
-      if ((c >> 45) >= ps.epoch0) {
+      if ((c >> 53) >= ps.epoch0) {
         int j = (int)(0xffffffffu - (unsigned)(c & 0xffffffffull));
         if (ps.cwin[j]) { cact[i] = 0; break; }
       }
@@ -792,6 +827,39 @@ __global__ void k_predicates(const double *q, int n, int pred_mode, int *o,
   flush_stats(cnt, st);
 }
 
+// Learned priority: cprio = bucket * 256 + quantised predicted conflict
+// degree (smaller predicted degree wins ties inside a bucket).
+__global__ void k_learned_prio(PassD ps, const float *score) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ps.nc || ps.cstatus[i] != ST_VALID) return;
+  float q = score[i] * 32.f;
+  int qi = q != q ? 255 : (int)fminf(fmaxf(q, 0.f), 255.f);
+  ps.cprio[i] = (ps.cprio[i] << 8) | qi;
+}
+
+// Conflict logs: how many candidates claim each triangle, then per candidate
+// the number of other claims on its cavity and ring.
+__global__ void k_claim_count(PassD ps, int *tcnt) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ps.nc; i += gridDim.x * blockDim.x) {
+    if (ps.cstatus[i] != ST_VALID) continue;
+    const int *L = ps.ws + (size_t)i * ps.R;
+    int k = ps.cn[i] + ps.cr[i];
+    for (int q = 0; q < k; q++) atomicAdd(&tcnt[L[q]], 1);
+  }
+}
+__global__ void k_conflict_degree(PassD ps, const int *tcnt, int *deg) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ps.nc; i += gridDim.x * blockDim.x) {
+    int d = -1;
+    if (ps.cstatus[i] == ST_VALID) {
+      const int *L = ps.ws + (size_t)i * ps.R;
+      int k = ps.cn[i] + ps.cr[i];
+      d = 0;
+      for (int q = 0; q < k; q++) d += tcnt[L[q]] - 1;
+    }
+    deg[i] = d;
+  }
+}
+
 // ===========================================================================
 // Host side
 // ===========================================================================
@@ -857,6 +925,7 @@ Refiner::~Refiner() {
   dfree(c_prio); dfree(c_win); dfree(c_rank); dfree(c_ws); dfree(c_pws);
   dfree(c_act); dfree(tmp_list);
   dfree(sel_prio); dfree(sel_tmp); dfree(sel_list); dfree(d_hist);
+  dfree(c_feat); dfree(c_score); dfree(c_deg); dfree(t_cnt); dfree(mlp_params);
   dfree(cb_cnt); dfree(cb_off); dfree(d_total); dfree(d_cnt);
   dfree(o_T); dfree(o_N); dfree(o_tstat); dfree(o_pstat); dfree(newid);
   dfree(h_key); dfree(h_val);
@@ -969,9 +1038,10 @@ void Refiner::collect_times() {
 }
 
 const char *Refiner::kname(int id) {
-  static const char *names[] = {"init",      "mark_bad", "compact",
+  static const char *names[] = {"init",       "mark_bad", "compact",
                                 "seg_cavity", "bad_cavity", "claim",
-                                "win",        "apply",      "export"};
+                                "win",        "apply",      "export",
+                                "mlp"};
   return names[id];
 }
 
@@ -1053,7 +1123,7 @@ __global__ __launch_bounds__(256) void k_prio_hist(MeshD m, const int *ids,
 int Refiner::select_by_priority(int *ids, int nb, int cap, int **out_ids) {
   *out_ids = ids;
   if (nb <= cap) return nb;
-  if (opt.priority != 1) return cap;  // index order
+  if (opt.priority != 1 && opt.priority != 2) return cap;  // index order
   if (nb > sel_cap) {
     dfree(sel_prio); dfree(sel_tmp); dfree(sel_list);
     sel_cap = std::max(nb, 2 * sel_cap);
@@ -1093,7 +1163,7 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   ps.K = K;
   ps.R = 6 * K + 16;
   int iters = std::max(1, opt.claim_iters);
-  if (epoch + iters + 1 >= (1ull << 19)) {  // epoch wrap: clear claims
+  if (epoch + iters + 1 >= (1ull << 11)) {  // epoch wrap: clear claims
     HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
     epoch = 0;
   }
@@ -1111,6 +1181,16 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   ps.pws = c_pws;
   ps.cnt = d_cnt;
   ps.learned_prio = learned_prio;
+  bool want_feat = !is_seg && (opt.priority == 2 || opt.log_training);
+  if (want_feat && nc > feat_cap) {
+    dfree(c_feat); dfree(c_score); dfree(c_deg);
+    feat_cap = std::max(nc, 2 * feat_cap);
+    dalloc(&c_feat, (size_t)feat_cap * FEAT);
+    dalloc(&c_score, (size_t)feat_cap);
+    dalloc(&c_deg, (size_t)feat_cap);
+  }
+  ps.feat = want_feat ? c_feat : nullptr;
+  ps.pass_index = pass_index;
   HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
   // one candidate per wave64 block; the exact-predicate arena holds
   // pws_blocks*256 lanes, i.e. pws_blocks*4 blocks of 64
@@ -1127,6 +1207,34 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   }
   HC(hipGetLastError());
   int gc = std::min(nblk(nc), 4096);
+  if (!is_seg && opt.priority == 2) {
+    if (!mlp_params) throw std::runtime_error("priority 2 needs a model (set_priority_model)");
+    tbegin(K_MLP);
+    mlp_forward_device(c_feat, nc, mlp_params, c_score, stream);
+    k_learned_prio<<<nblk(nc), 256, 0, stream>>>(ps, c_score);
+    tend(K_MLP, nc);
+    HC(hipGetLastError());
+  }
+  if (!is_seg && opt.log_training) {  // conflict logs for training
+    if (nt > t_cnt_cap) {
+      dfree(t_cnt);
+      t_cnt_cap = std::max<long long>(nt, capT);
+      dalloc(&t_cnt, (size_t)t_cnt_cap);
+    }
+    HC(hipMemsetAsync(t_cnt, 0, sizeof(int) * nt, stream));
+    k_claim_count<<<gc, 256, 0, stream>>>(ps, t_cnt);
+    k_conflict_degree<<<gc, 256, 0, stream>>>(ps, t_cnt, c_deg);
+    std::vector<float> hf((size_t)nc * FEAT);
+    std::vector<int> hd(nc);
+    HC(hipMemcpyAsync(hf.data(), c_feat, sizeof(float) * hf.size(), hipMemcpyDeviceToHost, stream));
+    HC(hipMemcpyAsync(hd.data(), c_deg, sizeof(int) * nc, hipMemcpyDeviceToHost, stream));
+    HC(hipStreamSynchronize(stream));
+    for (int i = 0; i < nc; i++) {
+      if (hd[i] < 0) continue;
+      log_X.insert(log_X.end(), hf.begin() + (size_t)i * FEAT, hf.begin() + (size_t)(i + 1) * FEAT);
+      log_t.push_back(std::log2(1.0f + (float)hd[i]));
+    }
+  }
   for (int it = 0; it < iters; it++) {
     ps.epoch = ++epoch;
     ps.iter = it;
@@ -1222,6 +1330,9 @@ void Refiner::run() {
   // --- workspace ----------------------------------------------------------
   K = opt.cav_cap0;
   epoch = 0;
+  pass_index = 0;
+  log_X.clear();
+  log_t.clear();
   int prev_win = -1;
   HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
   if (!c_ws || ws_ints < opt.cand_budget) {
@@ -1267,6 +1378,7 @@ void Refiner::run() {
     long long prog = 0;
     bool of = false;
     prev_win = run_pass(ids, nc, false, &prog, &of);
+    pass_index++;
     st.bad_passes++;
     st.cand_evals += nc;
     if (of && K < opt.cav_cap_max) {
@@ -1338,6 +1450,11 @@ void Refiner::download(MeshHost &out) {
   if (ns) HC(hipMemcpyAsync(out.S.data(), m.SV, sizeof(int2) * ns, hipMemcpyDeviceToHost, stream));
   HC(hipMemcpyAsync(out.pstat.data(), o_pstat, np, hipMemcpyDeviceToHost, stream));
   HC(hipStreamSynchronize(stream));
+}
+
+void Refiner::set_priority_model(const float *host_params, int nparam) {
+  if (!mlp_params) dalloc(&mlp_params, (size_t)nparam);
+  HC(hipMemcpy(mlp_params, host_params, sizeof(float) * nparam, hipMemcpyHostToDevice));
 }
 
 int Refiner::kernel_times(gcdt_kernel_time *rec, int max_rec) const {

@@ -119,6 +119,8 @@ typedef struct gcdt_options {
                             candidates, the smallest (priority, index) ones;
                             0 = every bad triangle (default 4) */
   int cand_min;          /* default 16384 */
+  int log_training;      /* 1 = record per-candidate features and conflict
+                            degrees of every bad pass (gcdt_session_logs) */
   int claim_iters;       /* conflict-resolution iterations per pass (default 1,
                             the reference's single atomicMin claim); more
                             iterations let candidates that touch no winner
@@ -178,6 +180,26 @@ int gcdt_session_download(gcdt_session *s, gcdt_mesh **out);
 int gcdt_session_kernel_times(gcdt_session *s, gcdt_kernel_time *rec,
                               int max_rec);
 void gcdt_session_destroy(gcdt_session *s);
+
+/* Learned priority (options.priority = 2): load the MLP parameters
+ * (gcdt_mlp_nparam() floats: W1[16][64] b1[64] W2[64][64] b2[64] W3[64] b3).
+ * The model predicts log2(1 + conflict degree) from 16 candidate features and
+ * orders candidates inside a shortest-edge bucket. */
+int gcdt_session_set_priority_model(gcdt_session *s, const float *params);
+/* After a run with options.log_training = 1: copy up to max_n logged samples
+ * (features X[max_n][16], targets t[max_n]); returns the number available. */
+long long gcdt_session_logs(gcdt_session *s, float *X, float *t, long long max_n);
+
+/* MLP kernels (BF16 MFMA, FP32 accumulate) on the device. */
+int gcdt_mlp_nparam(void);
+int gcdt_mlp_forward(const float *X, int n, const float *params, float *y);
+int gcdt_mlp_grad(const float *X, const float *t, int n, const float *params,
+                  float *grad, float *loss);
+int gcdt_mlp_train(const float *X, const float *t, int n, int epochs, int batch,
+                   float lr, unsigned long long seed, float *params,
+                   float *losses);
+int gcdt_mfma_probe(const float *A, const float *B, float *D);
+const char *gcdt_mlp_last_error(void);
 
 /* Last error message of this thread ("" if none). */
 const char *gcdt_last_error(void);
