@@ -5,7 +5,8 @@
 // generating random points and non-crossing segments (mesh.cpp:49-392,
 // GenerateRandomInput).  This file provides the same contract from its own
 // code:
-//   - gcdt_generate_pslg: points uniform in [0,L]^2 or Gaussian clusters;
+//   - gcdt_generate_pslg: points uniform in [0,L]^2, Gaussian clusters, or a
+//     lattice + cocircular + uniform precision-stress mix;
 //     random segments accepted only if they cross no accepted segment (exact
 //     orient2d), are not longer than maxlen, and make at least
 //     min_angle_deg with every accepted segment sharing an endpoint
@@ -510,7 +511,59 @@ extern "C" int gcdt_generate_pslg(int dist, int n, int nseg, double L,
       centers.push_back(L * (0.1 + 0.8 * rng.uniform()));
     }
   }
-  for (int i = 0; i < n;) {
+  auto accept = [&](int i, double x, double y) {
+    uint64_t hx, hy;
+    std::memcpy(&hx, &x, 8);
+    std::memcpy(&hy, &y, 8);
+    if (!seen.insert(hx * 0x9E3779B97F4A7C15ull ^ hy).second) return false;
+    points[2 * i] = x;
+    points[2 * i + 1] = y;
+    return true;
+  };
+  int i = 0;
+  // lattice bookkeeping for the precision-stress mix (dist == 2)
+  int side = 0;
+  std::vector<int> lat;  // lattice cell -> point index or -1
+  if (dist == 2) {
+    // Precision stress: thirds of (a) a random third of the cells of a
+    // side x side integer lattice (side = 1000 for 10^6 points) scaled to
+    // [0,L]^2, with 1e-12 relative jitter, (b) points on up to 2000 random
+    // circles, cocircular in exact rational arithmetic before rounding to
+    // FP64 (x = cx + r(1-u^2)/(1+u^2), y = cy +- 2ru/(1+u^2), u = k/1000),
+    // and (c) uniform background points.
+    int nl = n / 3, nc = n / 3;
+    side = std::max(2, std::min(1000, (int)std::ceil(std::sqrt(3.0 * nl))));
+    nl = std::min(nl, side * side);
+    double h = L / side;
+    std::vector<int> cells((size_t)side * side);
+    for (size_t c = 0; c < cells.size(); c++) cells[c] = (int)c;
+    lat.assign(cells.size(), -1);
+    for (int t = 0; t < nl; t++) {  // partial Fisher-Yates
+      size_t r = t + rng.below(cells.size() - t);
+      std::swap(cells[t], cells[r]);
+      int c = cells[t];
+      double x = (c % side) * h, y = (c / side) * h;
+      x += x * 1e-12 * (2 * rng.uniform() - 1);
+      y += y * 1e-12 * (2 * rng.uniform() - 1);
+      if (accept(i, x, y)) lat[c] = i++;
+    }
+    int ncirc = std::max(1, std::min(2000, nc / 8));
+    int per = std::max(3, nc / ncirc);
+    for (int c = 0; c < ncirc && i < nl + nc; c++) {
+      double r = L * (0.002 + 0.02 * rng.uniform());
+      double cx = r + (L - 2 * r) * rng.uniform(), cy = r + (L - 2 * r) * rng.uniform();
+      for (int k = 0; k < per && i < nl + nc; k++) {
+        double num = (double)rng.below(2001) - 1000.0, den = 1000.0;
+        double u2n = num * num, u2d = den * den;
+        double x = cx + r * ((u2d - u2n) / (u2d + u2n));
+        double y = cy + r * ((2.0 * num * den) / (u2d + u2n));
+        if (rng.below(2)) y = 2 * cy - y;
+        if (x < 0 || x > L || y < 0 || y > L) continue;
+        if (accept(i, x, y)) i++;
+      }
+    }
+  }
+  while (i < n) {
     double x, y;
     if (dist == 1) {
       int k = (int)rng.below((uint64_t)ncl);
@@ -521,13 +574,7 @@ extern "C" int gcdt_generate_pslg(int dist, int n, int nseg, double L,
       x = L * rng.uniform();
       y = L * rng.uniform();
     }
-    uint64_t hx, hy;
-    std::memcpy(&hx, &x, 8);
-    std::memcpy(&hy, &y, 8);
-    if (!seen.insert(hx * 0x9E3779B97F4A7C15ull ^ hy).second) continue;
-    points[2 * i] = x;
-    points[2 * i + 1] = y;
-    i++;
+    if (accept(i, x, y)) i++;
   }
   if (nseg <= 0) return 0;
   // uniform grid over points and accepted segments
@@ -543,9 +590,20 @@ extern "C" int gcdt_generate_pslg(int dist, int n, int nseg, double L,
   int k = 0;
   long attempts = 0, max_attempts = 200L * nseg + 1000;
   std::vector<int> cand;
+  int nlat_seg = dist == 2 ? nseg / 2 : 0;  // segments along lattice lines
   while (k < nseg && attempts++ < max_attempts) {
-    int i = (int)rng.below((uint64_t)n), j;
-    if (maxlen > 0) {
+    int i = (int)rng.below((uint64_t)n), j = -1;
+    if (k < nlat_seg) {
+      // next lattice point to the right or above a random lattice point
+      int c = (int)rng.below((uint64_t)side * side);
+      if (lat[c] < 0) continue;
+      i = lat[c];
+      int step = rng.below(2) ? 1 : side;
+      int lim = step == 1 ? side - 1 - c % side : side - 1 - c / side;
+      for (int t = 1; t <= lim; t++)
+        if (lat[c + t * step] >= 0) { j = lat[c + t * step]; break; }
+      if (j < 0) continue;
+    } else if (maxlen > 0) {
       int cx = cidx(points[2 * i]), cy = cidx(points[2 * i + 1]);
       cand.clear();
       for (int dy = -1; dy <= 1; dy++)
