@@ -41,6 +41,12 @@ class Options(C.Structure):
         ("cand_min", C.c_int),
         ("log_training", C.c_int),
         ("claim_iters", C.c_int),
+        ("graph", C.c_int),
+        ("graph_seg_passes", C.c_int),
+        ("graph_rounds", C.c_int),
+        ("grid_cavity", C.c_int),
+        ("grid_claim", C.c_int),
+        ("grid_apply", C.c_int),
     ]
 
 
@@ -126,6 +132,23 @@ def lib():
         L.gcdt_mfma_probe.argtypes = [vp, vp, vp]
         L.gcdt_mfma_probe.restype = C.c_int
         L.gcdt_mlp_last_error.restype = C.c_char_p
+        L.gcdt_batch_create.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                        C.POINTER(Options), C.c_longlong, C.POINTER(vp)]
+        L.gcdt_batch_create.restype = C.c_int
+        L.gcdt_batch_waves.argtypes = [vp]
+        L.gcdt_batch_waves.restype = C.c_int
+        L.gcdt_batch_run.argtypes = [vp, C.POINTER(Stats)]
+        L.gcdt_batch_run.restype = C.c_int
+        L.gcdt_batch_run_ms.argtypes = [vp]
+        L.gcdt_batch_run_ms.restype = C.c_double
+        L.gcdt_batch_set_priority_model.argtypes = [vp, vp]
+        L.gcdt_batch_set_priority_model.restype = C.c_int
+        L.gcdt_batch_counts.argtypes = [vp, vp, vp, vp]
+        L.gcdt_batch_counts.restype = C.c_int
+        L.gcdt_batch_copy.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.gcdt_batch_copy.restype = C.c_int
+        L.gcdt_batch_destroy.argtypes = [vp]
+        L.gcdt_batch_last_error.restype = C.c_char_p
         if hasattr(L, "gcdt_generate_pslg"):
             L.gcdt_generate_pslg.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double,
                                              C.c_double, C.c_double, C.c_ulonglong, vp, vp]
@@ -210,6 +233,93 @@ def refine(points, triangles, neighbors, segments, theta=25.0,
     if rc != 0:
         _err(rc, "gcdt_refine_host")
     return RefineResult(*_fetch_mesh(m), st.as_dict())
+
+
+class Batch:
+    """Many independent CDTs refined together (gcdt_batch_*): `instances` is
+    a list of (points, triangles, neighbors, segments) with instance-local
+    indices; theta / insert / mode are per instance (scalars broadcast)."""
+
+    def __init__(self, instances, theta=25.0, insert=Offcenter, mode=RUPPERT,
+                 wave_triangles=0, **options):
+        k = len(instances)
+        ins = [_as_inputs(*x) for x in instances]
+        self.n_inst = k
+        np_ = np.array([len(x[0]) for x in ins], np.int32)
+        nt_ = np.array([len(x[1]) for x in ins], np.int32)
+        ns_ = np.array([len(x[3]) for x in ins], np.int32)
+        P = np.ascontiguousarray(np.concatenate([x[0] for x in ins]))
+        T = np.ascontiguousarray(np.concatenate([x[1] for x in ins]))
+        N = np.ascontiguousarray(np.concatenate([x[2] for x in ins]))
+        S = np.ascontiguousarray(np.concatenate([x[3].reshape(-1, 2) for x in ins]).astype(np.int32))
+        th = np.ascontiguousarray(np.broadcast_to(np.asarray(theta, np.float64), (k,)))
+        off = np.ascontiguousarray(np.broadcast_to(
+            np.asarray([1 if v == Offcenter else 0 for v in np.atleast_1d(insert)], np.int32), (k,)))
+        md = np.ascontiguousarray(np.broadcast_to(np.asarray(mode, np.int32), (k,)))
+        self._keep = (np_, nt_, ns_, P, T, N, S, th, off, md)
+        o = make_options(25.0, Offcenter, RUPPERT, **options)
+        h = C.c_void_p()
+        rc = lib().gcdt_batch_create(k, _ptr(np_), _ptr(nt_), _ptr(ns_), _ptr(P), _ptr(T),
+                                     _ptr(N), _ptr(S), _ptr(th), _ptr(off), _ptr(md),
+                                     C.byref(o), int(wave_triangles), C.byref(h))
+        if rc != 0:
+            raise RuntimeError("gcdt_batch_create: " + lib().gcdt_batch_last_error().decode())
+        self.h = h
+
+    def waves(self):
+        return lib().gcdt_batch_waves(self.h)
+
+    def run(self):
+        st = Stats()
+        rc = lib().gcdt_batch_run(self.h, C.byref(st))
+        if rc != 0:
+            raise RuntimeError("gcdt_batch_run: " + lib().gcdt_batch_last_error().decode())
+        d = st.as_dict()
+        d["run_ms"] = lib().gcdt_batch_run_ms(self.h)
+        return d
+
+    def set_priority_model(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != mlp_nparam():
+            raise ValueError("wrong parameter count")
+        rc = lib().gcdt_batch_set_priority_model(self.h, _ptr(p))
+        if rc != 0:
+            raise RuntimeError("gcdt_batch_set_priority_model: " + lib().gcdt_batch_last_error().decode())
+
+    def results(self):
+        """List of RefineResult, one per instance."""
+        k = self.n_inst
+        cp = np.zeros(k, np.int32); ct = np.zeros(k, np.int32); cs = np.zeros(k, np.int32)
+        rc = lib().gcdt_batch_counts(self.h, _ptr(cp), _ptr(ct), _ptr(cs))
+        if rc != 0:
+            raise RuntimeError("gcdt_batch_counts: " + lib().gcdt_batch_last_error().decode())
+        P = np.zeros((int(cp.sum()), 2), np.float64)
+        T = np.zeros((int(ct.sum()), 3), np.int32)
+        N = np.zeros((int(ct.sum()), 3), np.int32)
+        S = np.zeros((int(cs.sum()), 2), np.int32)
+        ps = np.zeros(int(cp.sum()), np.uint8)
+        ts = np.zeros(int(ct.sum()), np.uint8)
+        rc = lib().gcdt_batch_copy(self.h, _ptr(P), _ptr(T), _ptr(N), _ptr(S), _ptr(ps), _ptr(ts))
+        if rc != 0:
+            raise RuntimeError("gcdt_batch_copy: " + lib().gcdt_batch_last_error().decode())
+        out = []
+        op = np.concatenate([[0], np.cumsum(cp)]); ot = np.concatenate([[0], np.cumsum(ct)])
+        os_ = np.concatenate([[0], np.cumsum(cs)])
+        for i in range(k):
+            out.append(RefineResult(P[op[i]:op[i + 1]], T[ot[i]:ot[i + 1]], N[ot[i]:ot[i + 1]],
+                                    S[os_[i]:os_[i + 1]], ps[op[i]:op[i + 1]], ts[ot[i]:ot[i + 1]], {}))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().gcdt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Session:

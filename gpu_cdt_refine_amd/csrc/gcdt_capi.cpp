@@ -49,9 +49,18 @@ void gcdt_default_options(gcdt_options *o) {
   o->cand_min = 16384;
   o->claim_iters = 1;
   o->log_training = 0;
+  o->graph = 0;
+  o->graph_seg_passes = 2;
+  o->graph_rounds = 4;
+  o->grid_cavity = 8192;
+  o->grid_claim = 2048;
+  o->grid_apply = 4096;
 }
 
-static bool valid_options(const gcdt_options *o) {
+}  // extern "C"
+
+namespace gcdt {
+bool options_ok(const gcdt_options *o, std::string &g_err) {
   if (!o) { g_err = "options is NULL"; return false; }
   if (!(o->theta_deg >= 0.0 && o->theta_deg < 60.0)) { g_err = "theta out of range [0, 60)"; return false; }
   if (o->cav_cap0 < 4 || o->cav_cap_max < o->cav_cap0 || o->cav_cap_max > 2048) {
@@ -61,8 +70,23 @@ static bool valid_options(const gcdt_options *o) {
   if (o->cand_budget < 6LL * o->cav_cap_max + 16) { g_err = "cand_budget smaller than one candidate at cav_cap_max"; return false; }
   if (o->priority < 0 || o->priority > 2) { g_err = "priority must be 0, 1 or 2"; return false; }
   if (o->claim_iters < 1 || o->claim_iters > 64) { g_err = "claim_iters must be in [1, 64]"; return false; }
+  if (o->graph_seg_passes < 1 || o->graph_seg_passes > 16 || o->graph_rounds < 1 || o->graph_rounds > 64) {
+    g_err = "graph_seg_passes must be in [1, 16] and graph_rounds in [1, 64]";
+    return false;
+  }
+  if (o->grid_cavity < 1 || o->grid_cavity > 16384 || o->grid_claim < 1 || o->grid_claim > 65536 ||
+      o->grid_apply < 1 || o->grid_apply > 65536) {
+    g_err = "grid sizes must be in [1, 16384] (cavity) and [1, 65536] (claim, apply)";
+    return false;
+  }
+  if (o->graph && o->log_training) { g_err = "log_training needs the host-driven loop (graph = 0)"; return false; }
   return true;
 }
+}  // namespace gcdt
+
+static bool valid_options(const gcdt_options *o) { return gcdt::options_ok(o, g_err); }
+
+extern "C" {
 
 int gcdt_refine_host(const double *points, int np, const int *tris,
                      const int *nbrs, int nt, const int *segs, int ns,

@@ -13,6 +13,7 @@
 //   SP    int[capS]       parent input segment;  ST int[capS]: an adjacent otri
 //   senc  int[capS]       0 free, 1 encroached, 2 given up
 //   claim uint64[capT]    epoch-tagged conflict claims (atomicMax)
+//   tinst int[capT]       instance of each triangle (batched refinement only)
 // Edge j of triangle t is (v[(j+1)%3], v[(j+2)%3]); its apex is v[j]
 // (otri encoding as the reference's encode_tri, refine.cu:95-113).
 //
@@ -67,6 +68,16 @@ struct MeshD {
   int *SP, *ST, *senc;
   const int2 *SEG;
   unsigned long long *claim;
+  int *tinst;  // nullptr: one instance, parameters from ParamsD
+};
+
+// Quality parameters of one PSLG instance (the reference's per-call theta,
+// insertpolicy and mode; refine.cu:6557-6567 derives them from theta).
+struct InstPar {
+  double goodangle;  // cos^2(theta)
+  double lens;       // (2 cos^2(theta) - 1)^2
+  double offconst;   // 0.475 sqrt((1+cos)/(1-cos)), kept in double
+  int offcenter, ruppert;
 };
 
 struct ParamsD {
@@ -74,6 +85,55 @@ struct ParamsD {
   double lens;       // (2 cos^2(theta) - 1)^2
   double offconst;   // 0.475 sqrt((1+cos)/(1-cos)), kept in double
   int offcenter, ruppert, use_fp32, priority;
+  const InstPar *ipar;  // per-instance parameters when MeshD::tinst is set
+};
+
+// Parameters that apply to triangle t (its instance's, in a batch).
+__device__ __forceinline__ ParamsD params_at(const ParamsD &pr, const MeshD &m,
+                                             int t) {
+  if (!m.tinst) return pr;
+  InstPar q = pr.ipar[m.tinst[t]];
+  ParamsD r = pr;
+  r.goodangle = q.goodangle;
+  r.lens = q.lens;
+  r.offconst = q.offconst;
+  r.offcenter = q.offcenter;
+  r.ruppert = q.ruppert;
+  return r;
+}
+
+// Device-resident control block of the graph-driven round loop: live
+// counts, loop state and statistics, read and written only by kernels
+// between two graph launches (the host reads it once per launch).
+enum : int {
+  C_GO = 0,      // the current pass runs (1) or its kernels return at once (0)
+  C_DONE,        // refinement finished
+  C_SUSPEND,     // 1: the next pass needs more mesh capacity (host grows)
+  C_STATUS,      // gcdt_stats.status
+  C_NP, C_NT, C_NS,  // live slot counts
+  C_K,           // cavity capacity wanted by the loop
+  C_KCAP,        // cavity capacity the captured graph was built for
+  C_NE, C_NB, C_NC, C_NWIN, C_PREVWIN,
+  C_ROUNDS, C_ROUND_OPEN, C_SEG_DONE,
+  C_SEL,         // bad pass: priority selection needed
+  C_USESEL,      // candidates are read from the selected list
+  C_PASS_INDEX, C_SEG_PASSES, C_BAD_PASSES,
+  C_EPOCH,       // last claim epoch used
+  C_BCUT, C_TAKE, C_ICUT,
+  C_CAPP, C_CAPT, C_CAPS,
+  C_MAXROUNDS, C_KMAX, C_CANDMIN, C_CANDFACTOR, C_ITERS, C_PRIORITY,
+  C_NTOUT,
+  C_NI = 64
+};
+enum : int {
+  A_REJ_OVF = 0, A_REJ_ENC, A_REJ_BLK, A_GIVEUPS, A_LOST, A_FP32, A_FP64,
+  A_EXACT, A_DD, A_BIG, A_SEGSPLITS, A_INSERTED, A_CANDEVALS, A_BUDGET,
+  A_NA = 16
+};
+struct Ctl {
+  int i[C_NI];
+  unsigned long long cnt[CNT_N];  // per-pass counters (CNT_*)
+  unsigned long long acc[A_NA];   // run totals
 };
 
 // One conflict-resolution pass over nc candidates.
@@ -88,17 +148,33 @@ struct PassD {
   int *ws;
   double *pws;  // exact-predicate arena, GCDT_INCIRCLE_WS doubles per thread
   unsigned long long *cnt;
-  const unsigned short *learned_prio;  // optional per-candidate priority
   unsigned long long epoch0;  // first claim epoch of this pass
   int iter;                   // claim iteration
   float *feat;                // optional: FEAT features per candidate
   int pass_index;             // bad-pass counter (feature)
+  // graph-driven loop: counts, epochs and the candidate list come from ctl
+  Ctl *ctl;
+  const int *ids_sel;  // selected candidates when ctl->i[C_USESEL]
 };
 
-// Per-thread cavity list kept in LDS: cavity (<= LDS_CAV) + ring (<= LDS_CAV+2).
-constexpr int LDS_CAV = 32;
-constexpr int LDS_LIST = 2 * LDS_CAV + 2;
-constexpr int CAV_BLOCK = 128;
+__device__ __forceinline__ int pass_nc(const PassD &ps) {
+  if (!ps.ctl) return ps.nc;
+  return ps.ctl->i[C_GO] ? ps.ctl->i[C_NC] : 0;
+}
+__device__ __forceinline__ const int *pass_ids(const PassD &ps) {
+  return (ps.ctl && ps.ctl->i[C_USESEL]) ? ps.ids_sel : ps.ids;
+}
+__device__ __forceinline__ unsigned long long pass_epoch0(const PassD &ps) {
+  return ps.ctl ? (unsigned long long)ps.ctl->i[C_EPOCH] + 1ull : ps.epoch0;
+}
+__device__ __forceinline__ unsigned long long pass_epoch(const PassD &ps) {
+  return ps.ctl ? pass_epoch0(ps) + (unsigned long long)ps.iter : ps.epoch;
+}
+__device__ __forceinline__ int pass_index_of(const PassD &ps) {
+  return ps.ctl ? ps.ctl->i[C_PASS_INDEX] : ps.pass_index;
+}
+
+constexpr int CAV_BLOCK = 128;  // threads per block of the apply kernel
 
 __device__ __forceinline__ int comp(const int4 &v, int j) {
   return j == 0 ? v.x : (j == 1 ? v.y : v.z);
@@ -337,72 +413,6 @@ __device__ __forceinline__ int walk(const MeshD &m, int t, double cx,
     t = nb >> 2;
   }
   return 4;
-}
-
-// Bowyer-Watson cavity of (cx,cy) grown from the seeds across non-segment
-// edges (in-circle test), then the ring across its boundary and the
-// disk / positive-star checks.  Same visiting order as the oracle.
-__device__ __forceinline__ int build_cavity(const MeshD &m, int *L, double cx,
-                                            double cy, int seed0, int seed1,
-                                            int split_s, int K, double *pws,
-                                            bool fp32, PredStats *st, int &nout,
-                                            int &rout) {
-  int n = 0;
-  L[n++] = seed0;
-  if (seed1 >= 0) L[n++] = seed1;
-  for (int idx = 0; idx < n; idx++) {
-    int t = L[idx];
-    int4 ts = m.TS[t], tn = m.TN[t];
-    for (int j = 0; j < 3; j++) {
-      if (comp(ts, j) >= 0) continue;
-      int nb = comp(tn, j);
-      if (nb < 0) continue;
-      int u = nb >> 2;
-      if (in_list(L, n, u)) continue;
-      int4 w = m.TV[u];
-      double2 a = m.P[w.x], b = m.P[w.y], c = m.P[w.z];
-      if (incircle_sign(a.x, a.y, b.x, b.y, c.x, c.y, cx, cy, pws, fp32, st) > 0) {
-        if (n == K) return ST_OVERFLOW;
-        L[n++] = u;
-      }
-    }
-  }
-  int nbnd = 0;
-  for (int idx = 0; idx < n; idx++) {
-    int t = L[idx];
-    int4 ts = m.TS[t], tn = m.TN[t], tv = m.TV[t];
-    for (int j = 0; j < 3; j++) {
-      int s = comp(ts, j);
-      int nb = comp(tn, j);
-      if (s >= 0 && s != split_s && nb >= 0 && in_list(L, n, nb >> 2))
-        return ST_GIVEUP;
-      if (is_bnd(m, t, j, split_s, L, n)) {
-        nbnd++;
-        if (!(s >= 0 && s == split_s)) {
-          double2 u = m.P[comp(tv, (j + 1) % 3)], v = m.P[comp(tv, (j + 2) % 3)];
-          if (orient2d_sign(cx, cy, u.x, u.y, v.x, v.y, fp32, st) <= 0)
-            return ST_GIVEUP;
-        }
-      }
-    }
-  }
-  if (nbnd != n + 2) return ST_GIVEUP;
-  int r = 0;
-  for (int idx = 0; idx < n; idx++) {
-    int t = L[idx];
-    int4 tn = m.TN[t];
-    for (int j = 0; j < 3; j++) {
-      if (!is_bnd(m, t, j, split_s, L, n)) continue;
-      int nb = comp(tn, j);
-      if (nb < 0) continue;
-      int u = nb >> 2;
-      if (in_list(L + n, r, u)) continue;
-      L[n + r++] = u;
-    }
-  }
-  nout = n;
-  rout = r;
-  return ST_VALID;
 }
 
 __device__ __forceinline__ void next_bnd(const MeshD &m, int t, int j,

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -16,6 +17,16 @@ struct MeshHost {
   std::vector<double> P;
   std::vector<int> T, N, S;
   std::vector<unsigned char> pstat, tstat;
+  std::vector<int> tinst;  // batched refinement: instance of every triangle
+};
+
+// One device-resident input CDT (a single mesh, or the union of a batch).
+struct InputSet {
+  int np = 0, nt = 0, ns = 0, n_inst = 0;
+  double2 *P = nullptr;
+  int *T = nullptr, *N = nullptr, *S = nullptr;
+  int *tinst = nullptr;     // instance of every input triangle (batches)
+  InstPar *ipar = nullptr;  // per-instance parameters (batches)
 };
 
 enum KernelId {
@@ -32,17 +43,23 @@ enum KernelId {
   K_NKERN
 };
 
-class Refiner;
-// Optional hook run before every bad pass: may fill learned_prio for the
-// first nc candidates of ids.
-typedef void (*LearnedHook)(Refiner *, const int *ids, int nc);
-
 class Refiner {
  public:
   explicit Refiner(const gcdt_options &o);
   ~Refiner();
   void upload(const double *points, int np, const int *tris, const int *nbrs,
               int nt, const int *segs, int ns);
+  // Resident inputs: add_input uploads one (for a batch, the union of
+  // independent meshes with tri_inst[nt] the instance of every triangle and
+  // ipar[n_inst] their parameters); run() refines the selected one.
+  int add_input(const double *points, int np, const int *tris, const int *nbrs,
+                int nt, const int *segs, int ns, const int *tri_inst,
+                const InstPar *ipar, int n_inst);
+  void select_input(int k);
+  int input_count() const { return (int)inputs.size(); }
+  // device copies of the last run's output (for batches kept across waves)
+  const double2 *out_points() const { return m.P; }
+  int out_np() const { return np; }
   void run();
   void download(MeshHost &out);
   int kernel_times(gcdt_kernel_time *rec, int max_rec) const;
@@ -54,9 +71,6 @@ class Refiner {
   hipStream_t stream = nullptr;
   int np = 0, nt = 0, ns = 0;
   // learned-priority extension
-  LearnedHook learned_hook = nullptr;
-  void *learned_ctx = nullptr;
-  unsigned short *learned_prio = nullptr;
   float *mlp_params = nullptr;      // device FP32 master weights (priority 2)
   void set_priority_model(const float *host_params, int nparam);
   // training logs (options.log_training): features and conflict-degree labels
@@ -67,6 +81,15 @@ class Refiner {
   int compact(Pred pred, int n, int *out_list, int *out_rank, const int *src);
   int run_pass(const int *ids, int nc, bool is_seg, long long *progress,
                bool *overflowed);
+  PassD pass_base(int Kc) const;
+  void enqueue_claims(PassD ps, int gc);
+  void enqueue_seg_pass();
+  void enqueue_bad_pass();
+  void capture_graph();
+  void loop_graph();
+  void loop_host();
+  void size_pass_buffers();
+  unsigned long long *ctl_cnt_ptr() const;
   int select_by_priority(int *ids, int nb, int cap, int **out_ids);
   void ensure_capacity(long long needP, long long needT, long long needS);
   void ensure_cand(int nc);
@@ -80,9 +103,11 @@ class Refiner {
   static const char *kname(int id);
 
   gcdt_stats st{};
+  std::vector<InputSet> inputs;
   int np0 = 0, nt0 = 0, ns0 = 0;
   double2 *in_P = nullptr;
   int *in_T = nullptr, *in_N = nullptr, *in_S = nullptr;
+  int seg_in_cap = 0;
   int2 *seg_in = nullptr;
   long long capP = 0, capT = 0, capS = 0;
   int K = 32;
@@ -124,6 +149,18 @@ class Refiner {
   int *o_T = nullptr, *o_N = nullptr;
   uint8_t *o_tstat = nullptr, *o_pstat = nullptr;
   int o_cap = 0, o_pcap = 0;
+  // graph-driven loop
+  Ctl *d_ctl = nullptr, *h_ctl = nullptr;
+  int *g_cnt = nullptr, *g_off = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int g_K = 0;
+  bool capturing = false;
+  // batched refinement
+  int *in_tinst = nullptr;
+  InstPar *d_ipar = nullptr;
+  int n_inst = 0;
+  int *o_inst = nullptr;
   hipEvent_t ev0, ev1;
   struct KT {
     long long launches = 0;
@@ -138,6 +175,11 @@ void device_predicates(const double *pts8, int n, int pred_mode, int *o,
                        int *ic, long long *stage_counts);
 void mlp_forward_device(const float *dX, int n, const float *dP, float *dy,
                         hipStream_t stream);
+void mlp_forward_device_n(const float *dX, const int *dn, const int *dgo,
+                          const float *dP, float *dy, int blocks,
+                          hipStream_t stream);
 constexpr int FEAT = 16;  // features per candidate (learned priority)
+// Validates options; on failure err holds the reason.
+bool options_ok(const gcdt_options *o, std::string &err);
 
 }  // namespace gcdt

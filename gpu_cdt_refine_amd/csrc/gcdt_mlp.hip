@@ -108,9 +108,13 @@ __device__ __forceinline__ void mlp_forward16(const float *__restrict__ X,
   }
 }
 
+// dn (optional): the row count is read from device memory, and the kernel
+// does nothing unless *dgo is set (graph-driven refinement loop).
 __global__ __launch_bounds__(64) void k_mlp_forward(const float *X, int n,
-                                                    const float *P, float *y) {
+                                                    const float *P, float *y,
+                                                    const int *dn, const int *dgo) {
   __shared__ short lds[16 * MLP_H];
+  if (dn) n = *dgo ? *dn : 0;
   int l = threadIdx.x;
   for (int row0 = blockIdx.x * 16; row0 < n; row0 += gridDim.x * 16) {
     Fwd f;
@@ -258,7 +262,14 @@ static void hc(hipError_t e, const char *w) {
 void mlp_forward_device(const float *dX, int n, const float *dP, float *dy,
                         hipStream_t stream) {
   int blocks = std::min((n + 15) / 16, 8192);
-  if (blocks > 0) k_mlp_forward<<<blocks, 64, 0, stream>>>(dX, n, dP, dy);
+  if (blocks > 0) k_mlp_forward<<<blocks, 64, 0, stream>>>(dX, n, dP, dy, nullptr, nullptr);
+  hc(hipGetLastError(), "mlp forward launch");
+}
+
+void mlp_forward_device_n(const float *dX, const int *dn, const int *dgo,
+                          const float *dP, float *dy, int blocks,
+                          hipStream_t stream) {
+  k_mlp_forward<<<blocks, 64, 0, stream>>>(dX, 0, dP, dy, dn, dgo);
   hc(hipGetLastError(), "mlp forward launch");
 }
 
@@ -282,7 +293,7 @@ int gcdt_mlp_forward(const float *X, int n, const float *params, float *y) {
     hc(hipMemcpy(dX, X, sizeof(float) * (size_t)n * MLP_IN, hipMemcpyHostToDevice), "copy");
     hc(hipMemcpy(dP, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice), "copy");
     int blocks = std::min((n + 15) / 16, 4096);
-    if (blocks > 0) k_mlp_forward<<<blocks, 64>>>(dX, n, dP, dy);
+    if (blocks > 0) k_mlp_forward<<<blocks, 64>>>(dX, n, dP, dy, nullptr, nullptr);
     hc(hipGetLastError(), "launch");
     hc(hipDeviceSynchronize(), "sync");
     hc(hipMemcpy(y, dy, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost), "copy");

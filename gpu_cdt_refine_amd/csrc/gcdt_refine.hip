@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -38,6 +39,7 @@ namespace gcdt {
 constexpr int CB = 256;       // threads per block
 constexpr int CI = 16;        // items per thread
 constexpr int CT = CB * CI;   // items per block tile
+constexpr int NBUCKET = 8192;  // priority buckets (edge_bucket range)
 
 struct PredSenc {
   const int *senc;
@@ -248,7 +250,7 @@ __global__ void k_init_enc(MeshD m, ParamsD pr, int nt) {
     if (s < 0) continue;
     double2 p = m.P[comp(tv, j)];
     int2 sv = m.SV[s];
-    if (encroaches(m, pr, sv.x, sv.y, p.x, p.y)) m.senc[s] = 1;
+    if (encroaches(m, params_at(pr, m, t), sv.x, sv.y, p.x, p.y)) m.senc[s] = 1;
   }
 }
 
@@ -267,15 +269,19 @@ __global__ void k_check_segs(const int *ST, int ns, unsigned long long *cnt) {
 // ---------------------------------------------------------------------------
 // Bad-triangle marking (refine.cu:376-497 kernelMarkBadTriangles)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_mark_bad(MeshD m, ParamsD pr, int nt) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nt) return;
-  uint8_t f = m.tflag[t];
-  if (!(f & TF_ALIVE) || !(f & TF_CHECK)) return;
-  f &= (uint8_t)~TF_CHECK;
-  if (tri_is_bad(m, pr, t, m.TV[t])) f |= TF_BAD;
-  else f &= (uint8_t)~TF_BAD;
-  m.tflag[t] = f;
+// nt < 0: the live count and the go flag come from ctl (graph-driven loop).
+__global__ __launch_bounds__(256) void k_mark_bad(MeshD m, ParamsD pr, int nt,
+                                                  const Ctl *ctl) {
+  if (ctl) nt = ctl->i[C_GO] ? ctl->i[C_NT] : 0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt;
+       t += gridDim.x * blockDim.x) {
+    uint8_t f = m.tflag[t];
+    if (!(f & TF_ALIVE) || !(f & TF_CHECK)) continue;
+    f &= (uint8_t)~TF_CHECK;
+    if (tri_is_bad(m, params_at(pr, m, t), t, m.TV[t])) f |= TF_BAD;
+    else f &= (uint8_t)~TF_BAD;
+    m.tflag[t] = f;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -360,25 +366,30 @@ __global__ __launch_bounds__(64) void k_cavity(MeshD m, ParamsD pr, PassD ps,
   const bool fp32 = pr.use_fp32 != 0;
   PredStats st = {0, 0, 0, 0}, junk = {0, 0, 0, 0};
   double *pws = ps.pws + ((size_t)blockIdx.x * 64 + lane) * GCDT_INCIRCLE_WS;
-  for (int i = blockIdx.x; i < ps.nc; i += gridDim.x) {
+  const int nc = pass_nc(ps);
+  const int *ids = pass_ids(ps);
+  for (int i = blockIdx.x; i < nc; i += gridDim.x) {
     // ---- point + seeds (uniform across the wave) ----
     int status = ST_VALID, prio = 0, seed0 = -1, seed1 = -1, split_s = -1, t0 = -1;
     double cx = 0.0, cy = 0.0;
+    ParamsD lp;
     if (IS_SEG) {
-      split_s = ps.ids[i];
+      split_s = ids[i];
+      int t1, j1, t2, j2;
+      seg_sides(m, split_s, t1, j1, t2, j2);
+      lp = params_at(pr, m, t1 >= 0 ? t1 : t2);
       if (!split_point(m, split_s, cx, cy)) {
         status = ST_GIVEUP;
       } else {
-        int t1, j1, t2, j2;
-        seg_sides(m, split_s, t1, j1, t2, j2);
         seed0 = t1 >= 0 ? t1 : t2;
         seed1 = t1 >= 0 ? t2 : -1;
       }
     } else {
-      t0 = ps.ids[i];
+      t0 = ids[i];
+      lp = params_at(pr, m, t0);
       int4 v0 = m.TV[t0];
       if (pr.priority == 1 || pr.priority == 2) prio = shortest_edge_bucket(m, v0);
-      if (!steiner_point(m, pr, v0, cx, cy)) {
+      if (!steiner_point(m, lp, v0, cx, cy)) {
         status = ST_GIVEUP;
       } else {
         int tl = -1, jl = -1, sg = -1;
@@ -493,7 +504,7 @@ __global__ __launch_bounds__(64) void k_cavity(MeshD m, ParamsD pr, PassD ps,
           }
           if (!IS_SEG && s >= 0) {
             int2 sv = m.SV[s];
-            if (encroaches(m, pr, sv.x, sv.y, cx, cy)) sc[SC_ENC] = 1;
+            if (encroaches(m, lp, sv.x, sv.y, cx, cy)) sc[SC_ENC] = 1;
             if (ps.feat) {  // distance of the Steiner point to this subsegment
               double2 A = m.P[sv.x], B = m.P[sv.y];
               float ex = (float)(B.x - A.x), ey = (float)(B.y - A.y);
@@ -517,7 +528,7 @@ __global__ __launch_bounds__(64) void k_cavity(MeshD m, ParamsD pr, PassD ps,
             int s = comp(m.TS[t], j);
             if (s < 0) continue;
             int2 sv = m.SV[s];
-            if (encroaches(m, pr, sv.x, sv.y, cx, cy)) mark_enc(m, s, ps.cnt);
+            if (encroaches(m, lp, sv.x, sv.y, cx, cy)) mark_enc(m, s, ps.cnt);
           }
           status = ST_REJ_ENC;
         }
@@ -549,7 +560,7 @@ __global__ __launch_bounds__(64) void k_cavity(MeshD m, ParamsD pr, PassD ps,
         else if (lane == 2) fv = (float)n / 16.f;
         else if (lane == 3) fv = (float)r / 16.f;
         else if (lane == 4) fv = dseg > 1e30f ? 1.f : fminf(dseg / fmaxf(sh, 1e-30f), 4.f) / 4.f;
-        else if (lane == 5) fv = (float)ps.pass_index / 256.f;
+        else if (lane == 5) fv = (float)pass_index_of(ps) / 256.f;
         else if (lane == 7) fv = 1.f;
       }
       ps.feat[(size_t)i * FEAT + lane] = fv;
@@ -588,14 +599,16 @@ __device__ __forceinline__ unsigned long long claim_key(unsigned long long epoch
 // Candidate i takes part in claim iteration ps.iter when it is valid, has not
 // won and touches no winner's region (cact != 0).
 __global__ __launch_bounds__(256) void k_claim(MeshD m, PassD ps, int *cact) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ps.nc;
+  const int nc = pass_nc(ps);
+  const unsigned long long ep = pass_epoch(ps);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc;
        i += gridDim.x * blockDim.x) {
     if (ps.iter == 0) {
       cact[i] = ps.cstatus[i] == ST_VALID;
       ps.cwin[i] = 0;
     }
     if (!cact[i]) continue;
-    unsigned long long key = claim_key(ps.epoch, ps.cprio[i], i);
+    unsigned long long key = claim_key(ep, ps.cprio[i], i);
     const int *L = ps.ws + (size_t)i * ps.R;
     int k = ps.cn[i] + ps.cr[i];
     for (int q = 0; q < k; q++) atomicMax(&m.claim[L[q]], key);
@@ -603,10 +616,12 @@ __global__ __launch_bounds__(256) void k_claim(MeshD m, PassD ps, int *cact) {
 }
 
 __global__ __launch_bounds__(256) void k_win(MeshD m, PassD ps, const int *cact) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ps.nc;
+  const int nc = pass_nc(ps);
+  const unsigned long long ep = pass_epoch(ps);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc;
        i += gridDim.x * blockDim.x) {
     if (!cact[i]) continue;
-    unsigned long long key = claim_key(ps.epoch, ps.cprio[i], i);
+    unsigned long long key = claim_key(ep, ps.cprio[i], i);
     const int *L = ps.ws + (size_t)i * ps.R;
     int k = ps.cn[i] + ps.cr[i];
     int w = 1;
@@ -619,7 +634,9 @@ __global__ __launch_bounds__(256) void k_win(MeshD m, PassD ps, const int *cact)
 // After a claim iteration: winners and candidates touching a winner's region
 // (a triangle whose claim comes from this pass and from a winner) drop out.
 __global__ __launch_bounds__(256) void k_block(MeshD m, PassD ps, int *cact) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ps.nc;
+  const int nc = pass_nc(ps);
+  const unsigned long long ep0 = pass_epoch0(ps);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc;
        i += gridDim.x * blockDim.x) {
     if (!cact[i]) continue;
     if (ps.cwin[i]) { cact[i] = 0; continue; }
@@ -627,7 +644,7 @@ __global__ __launch_bounds__(256) void k_block(MeshD m, PassD ps, int *cact) {
     int k = ps.cn[i] + ps.cr[i];
     for (int q = 0; q < k; q++) {
       unsigned long long c = m.claim[L[q]];
-      if ((c >> 53) >= ps.epoch0) {
+      if ((c >> 53) >= ep0) {
         int j = (int)(0xffffffffu - (unsigned)(c & 0xffffffffull));
         if (ps.cwin[j]) { cact[i] = 0; break; }
       }
@@ -643,13 +660,20 @@ __global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
                                                      int tbase, int sbase,
                                                      int is_seg, int lds_k) {
   extern __shared__ int lds[];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ps.nc;
+  const int nc = pass_nc(ps);
+  const int *ids = pass_ids(ps);
+  if (ps.ctl) {
+    pbase = ps.ctl->i[C_NP];
+    tbase = ps.ctl->i[C_NT];
+    sbase = ps.ctl->i[C_NS];
+  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc;
        i += gridDim.x * blockDim.x) {
     if (!ps.cwin[i]) continue;
     int rank = ps.crank[i];
     int p = pbase + rank;
     int tb = tbase + 2 * rank;
-    int id = ps.ids[i];
+    int id = ids[i];
     int split_s = is_seg ? id : -1;
     double2 c = ps.cpt[i];
     int n = ps.cn[i];
@@ -660,6 +684,8 @@ __global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
       L = Ls;
     }
     int k = n + 2;
+    const int inst = m.tinst ? m.tinst[L[0]] : 0;
+    const ParamsD lp = params_at(pr, m, L[0]);
     int *Bu = ps.ws + (size_t)i * ps.R + (2 * ps.K + 2);
     int *Bv = Bu + (ps.K + 2);
     int *Bo = Bv + (ps.K + 2);
@@ -724,6 +750,7 @@ __global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
       m.TS[st] = make_int4(Bs[q], (Bv[q] == a) ? sA : (Bv[q] == b) ? sB : -1,
                            (Bu[q] == a) ? sA : (Bu[q] == b) ? sB : -1, 0);
       m.tflag[st] = TF_ALIVE | TF_CHECK;
+      if (m.tinst) m.tinst[st] = inst;
     }
     if (open) {
       int st = tb + 1;
@@ -741,7 +768,7 @@ __global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
         m.ST[s] = (st << 2) | 0;
         if (is_seg && m.senc[s] == 0) {
           int2 sv = m.SV[s];
-          if (encroaches(m, pr, sv.x, sv.y, c.x, c.y)) m.senc[s] = 1;
+          if (encroaches(m, lp, sv.x, sv.y, c.x, c.y)) m.senc[s] = 1;
         }
       }
     }
@@ -757,22 +784,22 @@ __global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
         if (u == a) {
           if (stA < 0) stA = (st << 2) | 2;
           double2 pv = m.P[v];
-          encA |= encroaches(m, pr, a, p, pv.x, pv.y);
+          encA |= encroaches(m, lp, a, p, pv.x, pv.y);
         }
         if (v == a) {
           if (stA < 0) stA = (st << 2) | 1;
           double2 pu = m.P[u];
-          encA |= encroaches(m, pr, a, p, pu.x, pu.y);
+          encA |= encroaches(m, lp, a, p, pu.x, pu.y);
         }
         if (u == b) {
           if (stB < 0) stB = (st << 2) | 2;
           double2 pv = m.P[v];
-          encB |= encroaches(m, pr, p, b, pv.x, pv.y);
+          encB |= encroaches(m, lp, p, b, pv.x, pv.y);
         }
         if (v == b) {
           if (stB < 0) stB = (st << 2) | 1;
           double2 pu = m.P[u];
-          encB |= encroaches(m, pr, p, b, pu.x, pu.y);
+          encB |= encroaches(m, lp, p, b, pu.x, pu.y);
         }
       }
       m.ST[sA] = stA;
@@ -787,7 +814,7 @@ __global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
 // Export (refine.cu:5908-6484 compaction + format conversion)
 // ---------------------------------------------------------------------------
 __global__ void k_export(MeshD m, int nt, const int *newid, int *otri,
-                         int *onbr, uint8_t *otstat) {
+                         int *onbr, uint8_t *otstat, int *oinst) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
   uint8_t f = m.tflag[t];
@@ -802,6 +829,7 @@ __global__ void k_export(MeshD m, int nt, const int *newid, int *otri,
   onbr[3 * q + 2] = tn.z < 0 ? -1 : newid[tn.z >> 2];
   // TStatus bits: 0 non-null, 1 bad, 2 check
   otstat[q] = (uint8_t)(1 | ((f & TF_BAD) ? 2 : 0) | ((f & TF_CHECK) ? 4 : 0));
+  if (oinst) oinst[q] = m.tinst[t];
 }
 
 __global__ void k_pstatus(const uint8_t *pflag, int np, uint8_t *out) {
@@ -830,11 +858,14 @@ __global__ void k_predicates(const double *q, int n, int pred_mode, int *o,
 // Learned priority: cprio = bucket * 256 + quantised predicted conflict
 // degree (smaller predicted degree wins ties inside a bucket).
 __global__ void k_learned_prio(PassD ps, const float *score) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ps.nc || ps.cstatus[i] != ST_VALID) return;
-  float q = score[i] * 32.f;
-  int qi = q != q ? 255 : (int)fminf(fmaxf(q, 0.f), 255.f);
-  ps.cprio[i] = (ps.cprio[i] << 8) | qi;
+  const int nc = pass_nc(ps);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+       i += gridDim.x * blockDim.x) {
+    if (ps.cstatus[i] != ST_VALID) continue;
+    float q = score[i] * 32.f;
+    int qi = q != q ? 255 : (int)fminf(fmaxf(q, 0.f), 255.f);
+    ps.cprio[i] = (ps.cprio[i] << 8) | qi;
+  }
 }
 
 // Conflict logs: how many candidates claim each triangle, then per candidate
@@ -858,6 +889,340 @@ __global__ void k_conflict_degree(PassD ps, const int *tcnt, int *deg) {
     }
     deg[i] = d;
   }
+}
+
+// ===========================================================================
+// Graph-driven round loop: fixed-grid ("persistent") kernels whose work
+// counts, epochs and loop decisions live in the device control block Ctl, so
+// a block of rounds is captured once as a hipGraph and replayed with a single
+// host synchronisation per launch.  Every decision is the host loop's
+// (Refiner::run), made on the device by single-lane gate / finish kernels.
+// ===========================================================================
+constexpr int GC = 1024;  // blocks of the fixed-grid compaction
+
+__device__ __forceinline__ bool ctl_on(const Ctl *c, int gidx) {
+  return c->i[C_GO] && (gidx < 0 || c->i[gidx]);
+}
+__device__ __forceinline__ int ctl_cands_per_pass(const Ctl *c, int K) {
+  long long v = (long long)(c->acc[A_BUDGET] / (6ull * (unsigned long long)K + 16ull));
+  if (v < 1) v = 1;
+  if (v > (1LL << 30)) v = 1LL << 30;
+  return (int)v;
+}
+
+// Tiles of CT items, split into contiguous per-block ranges: block b scans
+// tiles [b*tpb, (b+1)*tpb), so the scatter keeps ascending index order.
+template <class Pred>
+__global__ __launch_bounds__(CB) void k_dcount(Pred pred, const Ctl *c, int nidx,
+                                               int gidx, int *blockcnt) {
+  __shared__ int wsum[CB / 64];
+  int n = ctl_on(c, gidx) ? c->i[nidx] : 0;
+  int ntiles = (n + CT - 1) / CT;
+  int tpb = (ntiles + gridDim.x - 1) / gridDim.x;
+  int tb0 = blockIdx.x * tpb, tb1 = min(tb0 + tpb, ntiles);
+  int cnt = 0;
+  for (int tile = tb0; tile < tb1; tile++)
+    for (int k = 0; k < CI; k++) {
+      int i = tile * CT + k * CB + threadIdx.x;
+      if (i < n && pred(i)) cnt++;
+    }
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int sum = 0;
+    for (int w = 0; w < CB / 64; w++) sum += wsum[w];
+    blockcnt[blockIdx.x] = sum;
+  }
+}
+
+// Exclusive scan of the GC block counts; the finisher sees the total.
+template <class Fin>
+__global__ __launch_bounds__(1024) void k_dscan(const int *blockcnt, int *blockoff,
+                                                Ctl *c, int gidx, Fin fin) {
+  __shared__ int sh[GC];
+  if (!ctl_on(c, gidx)) return;
+  int v = blockcnt[threadIdx.x];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < GC; off <<= 1) {
+    int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  blockoff[threadIdx.x] = sh[threadIdx.x] - v;
+  int total = sh[GC - 1];
+  __syncthreads();
+  fin(c, total, (int)threadIdx.x);
+}
+
+template <class Pred>
+__global__ __launch_bounds__(CB) void k_dscatter(Pred pred, const Ctl *c, int nidx,
+                                                 int gidx, const int *blockoff,
+                                                 int *out_list, int *out_rank,
+                                                 const int *src) {
+  __shared__ int wcnt[CB / 64];
+  int n = ctl_on(c, gidx) ? c->i[nidx] : 0;
+  int ntiles = (n + CT - 1) / CT;
+  int tpb = (ntiles + gridDim.x - 1) / gridDim.x;
+  int tb0 = blockIdx.x * tpb, tb1 = min(tb0 + tpb, ntiles);
+  if (tb0 >= tb1) return;
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int run = blockoff[blockIdx.x];
+  unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int tile = tb0; tile < tb1; tile++)
+    for (int k = 0; k < CI; k++) {
+      int i = tile * CT + k * CB + threadIdx.x;
+      bool f = (i < n) && pred(i);
+      unsigned long long b = __ballot(f);
+      if (lane == 0) wcnt[wave] = __popcll(b);
+      __syncthreads();
+      int pre = 0, tot = 0;
+      for (int w = 0; w < CB / 64; w++) {
+        int cw = wcnt[w];
+        if (w < wave) pre += cw;
+        tot += cw;
+      }
+      if (f) {
+        int pos = run + pre + __popcll(b & lt);
+        if (out_list) out_list[pos] = src ? src[i] : i;
+        if (out_rank) out_rank[i] = pos;
+      }
+      run += tot;
+      __syncthreads();
+    }
+}
+
+// Priority predicates whose cut comes from the control block.
+struct PredPrioEqC {
+  const int *prio;
+  const Ctl *c;
+  __device__ bool operator()(int i) const { return prio[i] == c->i[C_BCUT]; }
+};
+struct PredPrioSelC {
+  const int *prio;
+  const Ctl *c;
+  __device__ bool operator()(int i) const {
+    int p = prio[i], b = c->i[C_BCUT];
+    return p < b || (p == b && i <= c->i[C_ICUT]);
+  }
+};
+
+__device__ __forceinline__ bool ctl_fits(const Ctl *c, int nc, bool seg) {
+  return (long long)c->i[C_NP] + nc <= c->i[C_CAPP] &&
+         (long long)c->i[C_NT] + 2LL * nc <= c->i[C_CAPT] &&
+         (!seg || (long long)c->i[C_NS] + nc <= c->i[C_CAPS]);
+}
+
+// Segment phase: encroached subsegments counted (refine.cu:3180-3778 loop).
+struct FinSeg {
+  __device__ void operator()(Ctl *c, int total, int tid) const {
+    if (tid != 0) return;
+    c->i[C_NE] = total;
+    c->i[C_USESEL] = 0;
+    if (total == 0) {
+      c->i[C_SEG_DONE] = 1;
+      c->i[C_GO] = 0;
+      return;
+    }
+    int nc = min(total, ctl_cands_per_pass(c, c->i[C_K]));
+    c->i[C_NC] = nc;
+    if (!ctl_fits(c, nc, true)) {
+      c->i[C_SUSPEND] = 1;
+      c->i[C_GO] = 0;
+    }
+  }
+};
+
+// Bad phase: bad triangles counted; candidate cap and selection decided.
+struct FinBad {
+  unsigned *hist;
+  __device__ void operator()(Ctl *c, int total, int tid) const {
+    int nb = total;
+    long long cap = ctl_cands_per_pass(c, c->i[C_K]);
+    int prev = c->i[C_PREVWIN];
+    if (c->i[C_CANDFACTOR] > 0 && prev >= 0) {
+      long long f = (long long)c->i[C_CANDFACTOR] * prev;
+      long long lo = c->i[C_CANDMIN];
+      cap = min(cap, f > lo ? f : lo);
+    }
+    int prio = c->i[C_PRIORITY];
+    bool sel = nb > cap && (prio == 1 || prio == 2);
+    int nc = nb > cap ? (int)cap : nb;
+    bool fits = ctl_fits(c, nc, false);
+    if (nb > 0 && fits && sel)
+      for (int b = tid; b < NBUCKET; b += 1024) hist[b] = 0u;
+    if (tid != 0) return;
+    c->i[C_NB] = nb;
+    c->i[C_USESEL] = 0;
+    c->i[C_SEL] = 0;
+    if (nb == 0) {
+      c->i[C_DONE] = 1;
+      c->i[C_STATUS] = 0;
+      c->i[C_GO] = 0;
+      return;
+    }
+    c->i[C_NC] = nc;
+    if (!fits) {
+      c->i[C_SUSPEND] = 1;
+      c->i[C_GO] = 0;
+      return;
+    }
+    c->i[C_SEL] = sel ? 1 : 0;
+  }
+};
+
+struct FinNone {
+  __device__ void operator()(Ctl *, int, int) const {}
+};
+struct FinSel {
+  __device__ void operator()(Ctl *c, int total, int tid) const {
+    if (tid != 0) return;
+    c->i[C_NC] = total;
+    c->i[C_USESEL] = 1;
+  }
+};
+struct FinWin {
+  __device__ void operator()(Ctl *c, int total, int tid) const {
+    if (tid == 0) c->i[C_NWIN] = total;
+  }
+};
+__global__ void k_gate_seg(Ctl *c) {
+  int go = !c->i[C_DONE] && !c->i[C_SUSPEND] && !c->i[C_SEG_DONE] &&
+           c->i[C_K] == c->i[C_KCAP];
+  if (go && !c->i[C_ROUND_OPEN]) {
+    if (c->i[C_ROUNDS] >= c->i[C_MAXROUNDS]) {
+      c->i[C_DONE] = 1;  // round cap: status stays 1
+      go = 0;
+    } else {
+      c->i[C_ROUND_OPEN] = 1;
+      c->i[C_ROUNDS]++;
+    }
+  }
+  c->i[C_GO] = go;
+  if (go)
+    for (int k = 0; k < CNT_N; k++) c->cnt[k] = 0ull;
+}
+
+__global__ void k_gate_bad(Ctl *c) {
+  int go = !c->i[C_DONE] && !c->i[C_SUSPEND] && c->i[C_SEG_DONE] &&
+           c->i[C_K] == c->i[C_KCAP];
+  c->i[C_GO] = go;
+  c->i[C_SEL] = 0;
+  if (go)
+    for (int k = 0; k < CNT_N; k++) c->cnt[k] = 0ull;
+}
+
+__device__ void ctl_accumulate(Ctl *c, int nwin) {
+  const unsigned long long *n = c->cnt;
+  c->acc[A_REJ_OVF] += n[CNT_OVERFLOW];
+  c->acc[A_REJ_ENC] += n[CNT_REJ_ENC];
+  c->acc[A_REJ_BLK] += n[CNT_REJ_BLOCKED];
+  c->acc[A_GIVEUPS] += n[CNT_GIVEUPS];
+  c->acc[A_LOST] += n[CNT_VALID] - (unsigned long long)nwin;
+  c->acc[A_FP32] += n[CNT_FP32];
+  c->acc[A_FP64] += n[CNT_FP64];
+  c->acc[A_EXACT] += n[CNT_EXACT];
+  c->acc[A_DD] += n[CNT_DD];
+  c->acc[A_BIG] += n[CNT_BIG];
+}
+
+__global__ void k_end_seg(Ctl *c, int iters) {
+  if (!c->i[C_GO]) return;
+  int nwin = c->i[C_NWIN];
+  ctl_accumulate(c, nwin);
+  c->acc[A_SEGSPLITS] += nwin;
+  c->i[C_NP] += nwin;
+  c->i[C_NT] += 2 * nwin;
+  c->i[C_NS] += nwin;
+  c->i[C_SEG_PASSES]++;
+  c->i[C_EPOCH] += iters;
+  int K = c->i[C_K];
+  if (c->cnt[CNT_OVERFLOW] > 0 && K < c->i[C_KMAX]) c->i[C_K] = min(2 * K, c->i[C_KMAX]);
+}
+
+__global__ void k_end_bad(Ctl *c, int iters) {
+  if (!c->i[C_GO]) return;
+  int nwin = c->i[C_NWIN];
+  ctl_accumulate(c, nwin);
+  c->acc[A_INSERTED] += nwin;
+  c->acc[A_CANDEVALS] += c->i[C_NC];
+  c->i[C_NP] += nwin;
+  c->i[C_NT] += 2 * nwin;
+  c->i[C_PREVWIN] = nwin;
+  c->i[C_PASS_INDEX]++;
+  c->i[C_BAD_PASSES]++;
+  c->i[C_EPOCH] += iters;
+  long long prog = nwin + (long long)c->cnt[CNT_MARKS] + (long long)c->cnt[CNT_GIVEUPS];
+  int K = c->i[C_K];
+  if (c->cnt[CNT_OVERFLOW] > 0 && K < c->i[C_KMAX]) {
+    c->i[C_K] = min(2 * K, c->i[C_KMAX]);
+    prog++;
+  }
+  if (prog == 0) {
+    c->i[C_DONE] = 1;
+    c->i[C_STATUS] = 2;
+  }
+  c->i[C_ROUND_OPEN] = 0;
+  c->i[C_SEG_DONE] = 0;
+}
+
+// Priority histogram of the nb bad triangles (selection pass only).
+__global__ __launch_bounds__(256) void k_dprio_hist(MeshD m, const int *ids,
+                                                    const Ctl *c, int *prio,
+                                                    unsigned *hist) {
+  __shared__ unsigned h[NBUCKET];
+  if (!ctl_on(c, C_SEL)) return;
+  int n = c->i[C_NB];
+  for (int b = threadIdx.x; b < NBUCKET; b += blockDim.x) h[b] = 0;
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    int p = shortest_edge_bucket(m, m.TV[ids[i]]);
+    prio[i] = p;
+    atomicAdd(&h[p], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < NBUCKET; b += blockDim.x)
+    if (h[b]) atomicAdd(&hist[b], h[b]);
+}
+
+// Cut bucket: the bucket holding the cap-th smallest priority, and how many
+// of its members are taken (one block of 1024 lanes, 8 buckets each).
+__global__ __launch_bounds__(1024) void k_dcut(const unsigned *hist, Ctl *c) {
+  __shared__ long long sh[1024];
+  if (!ctl_on(c, C_SEL)) return;
+  const int per = NBUCKET / 1024;
+  long long cap = c->i[C_NC];
+  long long mine = 0;
+  for (int k = 0; k < per; k++) mine += hist[threadIdx.x * per + k];
+  sh[threadIdx.x] = mine;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    long long t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  long long before = sh[threadIdx.x] - mine;
+  if (before < cap && sh[threadIdx.x] >= cap) {
+    long long acc = before;
+    for (int k = 0; k < per; k++) {
+      int b = threadIdx.x * per + k;
+      if (acc + hist[b] >= cap) {
+        c->i[C_BCUT] = b;
+        c->i[C_TAKE] = (int)(cap - acc);
+        break;
+      }
+      acc += hist[b];
+    }
+  }
+}
+
+__global__ void k_dicut(const int *sel_tmp, Ctl *c) {
+  if (!ctl_on(c, C_SEL)) return;
+  c->i[C_ICUT] = sel_tmp[c->i[C_TAKE] - 1];
 }
 
 // ===========================================================================
@@ -908,6 +1273,10 @@ Refiner::Refiner(const gcdt_options &o) : opt(o) {
                          hipFuncAttributeMaxDynamicSharedMemorySize, BIG_LDS_BYTES));
   dalloc(&d_total, 4);
   dalloc(&d_cnt, CNT_N);
+  dalloc(&d_ctl, 1);
+  dalloc(&g_cnt, GC);
+  dalloc(&g_off, GC);
+  HC(hipHostMalloc((void **)&h_ctl, sizeof(Ctl), hipHostMallocDefault));
   double ga = std::cos(opt.theta_deg * 3.141592653589793238462643383279502884197169399375105820974944592308 / 180.0);
   pr.offconst = (ga == 1.0) ? 0.0 : 0.475 * std::sqrt((1.0 + ga) / (1.0 - ga));
   pr.goodangle = ga * ga;
@@ -916,11 +1285,15 @@ Refiner::Refiner(const gcdt_options &o) : opt(o) {
   pr.ruppert = opt.mode == 1;
   pr.use_fp32 = opt.pred_mode == 1;
   pr.priority = opt.priority;
+  pr.ipar = nullptr;
 }
 
 Refiner::~Refiner() {
   free_mesh();
-  dfree(in_P); dfree(in_T); dfree(in_N); dfree(in_S);
+  for (auto &in : inputs) {
+    dfree(in.P); dfree(in.T); dfree(in.N); dfree(in.S); dfree(in.tinst); dfree(in.ipar);
+  }
+  dfree(seg_in);
   dfree(c_pt); dfree(c_status); dfree(c_n); dfree(c_r);
   dfree(c_prio); dfree(c_win); dfree(c_rank); dfree(c_ws); dfree(c_pws);
   dfree(c_act); dfree(tmp_list);
@@ -929,6 +1302,11 @@ Refiner::~Refiner() {
   dfree(cb_cnt); dfree(cb_off); dfree(d_total); dfree(d_cnt);
   dfree(o_T); dfree(o_N); dfree(o_tstat); dfree(o_pstat); dfree(newid);
   dfree(h_key); dfree(h_val);
+  dfree(d_ctl); dfree(g_cnt); dfree(g_off);
+  dfree(o_inst);
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (h_ctl) (void)hipHostFree(h_ctl);
   if (h_scalar) (void)hipHostFree(h_scalar);
   if (h_cnt) (void)hipHostFree(h_cnt);
   (void)hipEventDestroy(ev0);
@@ -943,12 +1321,23 @@ void Refiner::free_mesh() {
   dfree(m.P); dfree(m.pflag); dfree(m.ppar);
   dfree(m.TV); dfree(m.TN); dfree(m.TS); dfree(m.tflag); dfree(m.claim);
   dfree(m.SV); dfree(m.SP); dfree(m.ST); dfree(m.senc);
-  dfree(seg_in);
+  dfree(m.tinst);
   capP = capT = capS = 0;
 }
 
 void Refiner::upload(const double *points, int np, const int *tris,
                      const int *nbrs, int nt, const int *segs, int ns) {
+  for (auto &in : inputs) {
+    dfree(in.P); dfree(in.T); dfree(in.N); dfree(in.S); dfree(in.tinst); dfree(in.ipar);
+  }
+  inputs.clear();
+  add_input(points, np, tris, nbrs, nt, segs, ns, nullptr, nullptr, 0);
+  select_input(0);
+}
+
+int Refiner::add_input(const double *points, int np, const int *tris,
+                       const int *nbrs, int nt, const int *segs, int ns,
+                       const int *tri_inst, const InstPar *ipar, int n_inst) {
   if (np < 3 || nt < 1) throw std::runtime_error("input mesh is empty");
   if ((long long)nt * 4 >= (1LL << 31) / 2)
     throw std::runtime_error("too many triangles for 32-bit otri encoding");
@@ -962,17 +1351,40 @@ void Refiner::upload(const double *points, int np, const int *tris,
   for (long long i = 0; i < 2LL * ns; i++)
     if (segs[i] < 0 || segs[i] >= np)
       throw std::runtime_error("segment vertex index out of range");
-  np0 = np; nt0 = nt; ns0 = ns;
-  dfree(in_P); dfree(in_T); dfree(in_N); dfree(in_S);
-  dalloc(&in_P, (size_t)np);
-  dalloc(&in_T, 3 * (size_t)nt);
-  dalloc(&in_N, 3 * (size_t)nt);
-  dalloc(&in_S, 2 * (size_t)(ns > 0 ? ns : 1));
-  HC(hipMemcpyAsync(in_P, points, sizeof(double2) * np, hipMemcpyHostToDevice, stream));
-  HC(hipMemcpyAsync(in_T, tris, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, stream));
-  HC(hipMemcpyAsync(in_N, nbrs, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, stream));
-  if (ns) HC(hipMemcpyAsync(in_S, segs, sizeof(int) * 2 * ns, hipMemcpyHostToDevice, stream));
+  if (tri_inst)
+    for (long long i = 0; i < nt; i++)
+      if (tri_inst[i] < 0 || tri_inst[i] >= n_inst)
+        throw std::runtime_error("triangle instance index out of range");
+  InputSet in;
+  in.np = np; in.nt = nt; in.ns = ns;
+  in.n_inst = tri_inst ? n_inst : 0;
+  dalloc(&in.P, (size_t)np);
+  dalloc(&in.T, 3 * (size_t)nt);
+  dalloc(&in.N, 3 * (size_t)nt);
+  dalloc(&in.S, 2 * (size_t)(ns > 0 ? ns : 1));
+  HC(hipMemcpyAsync(in.P, points, sizeof(double2) * np, hipMemcpyHostToDevice, stream));
+  HC(hipMemcpyAsync(in.T, tris, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, stream));
+  HC(hipMemcpyAsync(in.N, nbrs, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, stream));
+  if (ns) HC(hipMemcpyAsync(in.S, segs, sizeof(int) * 2 * ns, hipMemcpyHostToDevice, stream));
+  if (tri_inst) {
+    dalloc(&in.tinst, (size_t)nt);
+    dalloc(&in.ipar, (size_t)n_inst);
+    HC(hipMemcpyAsync(in.tinst, tri_inst, sizeof(int) * nt, hipMemcpyHostToDevice, stream));
+    HC(hipMemcpyAsync(in.ipar, ipar, sizeof(InstPar) * n_inst, hipMemcpyHostToDevice, stream));
+  }
   HC(hipStreamSynchronize(stream));
+  inputs.push_back(in);
+  return (int)inputs.size() - 1;
+}
+
+void Refiner::select_input(int k) {
+  if (k < 0 || k >= (int)inputs.size()) throw std::runtime_error("no such input");
+  const InputSet &in = inputs[k];
+  np0 = in.np; nt0 = in.nt; ns0 = in.ns;
+  in_P = in.P; in_T = in.T; in_N = in.N; in_S = in.S;
+  in_tinst = in.tinst;
+  d_ipar = in.ipar;
+  n_inst = in.n_inst;
 }
 
 void Refiner::ensure_capacity(long long needP, long long needT, long long needS) {
@@ -989,6 +1401,7 @@ void Refiner::ensure_capacity(long long needP, long long needT, long long needS)
     dgrow(&m.TN, (size_t)nt, (size_t)c, stream);
     dgrow(&m.TS, (size_t)nt, (size_t)c, stream);
     dgrow(&m.tflag, (size_t)nt, (size_t)c, stream);
+    if (m.tinst) dgrow(&m.tinst, (size_t)nt, (size_t)c, stream);
     unsigned long long *nc;
     dalloc(&nc, (size_t)c);
     HC(hipMemsetAsync(nc, 0, sizeof(unsigned long long) * c, stream));
@@ -1010,7 +1423,7 @@ void Refiner::ensure_capacity(long long needP, long long needT, long long needS)
 
 // Kernel-time bookkeeping (options.profile): one event pair per launch.
 void Refiner::tbegin(int id) {
-  if (!opt.profile) return;
+  if (!opt.profile || capturing) return;
   hipEvent_t a, b;
   HC(hipEventCreate(&a));
   HC(hipEventCreate(&b));
@@ -1018,7 +1431,7 @@ void Refiner::tbegin(int id) {
   ktimes[id].pending.push_back({a, b});
 }
 void Refiner::tend(int id, long long items) {
-  if (!opt.profile) return;
+  if (!opt.profile || capturing) return;
   HC(hipEventRecord(ktimes[id].pending.back().second, stream));
   ktimes[id].launches++;
   ktimes[id].items += items;
@@ -1101,7 +1514,6 @@ int Refiner::cands_per_pass(int K) const {
 // -- the order Triangle's bad-triangle queue serves them
 // (triangle.cpp:6932-6968).  Histogram of priority buckets -> cut bucket ->
 // ordered compactions; no host-side sort.
-constexpr int NBUCKET = 8192;
 
 __global__ __launch_bounds__(256) void k_prio_hist(MeshD m, const int *ids,
                                                    int n, int *prio,
@@ -1154,22 +1566,11 @@ int Refiner::select_by_priority(int *ids, int nb, int cap, int **out_ids) {
   return got;
 }
 
-// One conflict pass; returns number of winners.
-int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
-                      bool *overflowed) {
-  PassD ps;
-  ps.ids = ids;
-  ps.nc = nc;
-  ps.K = K;
-  ps.R = 6 * K + 16;
-  int iters = std::max(1, opt.claim_iters);
-  if (epoch + iters + 1 >= (1ull << 11)) {  // epoch wrap: clear claims
-    HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
-    epoch = 0;
-  }
-  ps.epoch0 = epoch + 1;
-  ps.epoch = epoch + 1;
-  ps.iter = 0;
+// Pass descriptor common to both loops (counts and epochs filled later).
+PassD Refiner::pass_base(int Kc) const {
+  PassD ps{};
+  ps.K = Kc;
+  ps.R = 6 * Kc + 16;
   ps.cpt = c_pt;
   ps.cstatus = c_status;
   ps.cn = c_n;
@@ -1180,7 +1581,45 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   ps.ws = c_ws;
   ps.pws = c_pws;
   ps.cnt = d_cnt;
-  ps.learned_prio = learned_prio;
+  ps.feat = nullptr;
+  ps.ctl = nullptr;
+  ps.ids_sel = nullptr;
+  return ps;
+}
+
+// Claim iterations: claim -> win (-> block before the next iteration).  In
+// the host loop ps.epoch advances here; in the graph loop the kernels derive
+// it from the control block.
+void Refiner::enqueue_claims(PassD ps, int gc) {
+  int iters = std::max(1, opt.claim_iters);
+  for (int it = 0; it < iters; it++) {
+    if (!ps.ctl) ps.epoch = ++epoch;
+    ps.iter = it;
+    tbegin(K_CLAIM);
+    k_claim<<<gc, 256, 0, stream>>>(m, ps, c_act);
+    tend(K_CLAIM, ps.nc);
+    tbegin(K_WIN);
+    k_win<<<gc, 256, 0, stream>>>(m, ps, c_act);
+    if (it + 1 < iters) k_block<<<gc, 256, 0, stream>>>(m, ps, c_act);
+    tend(K_WIN, ps.nc);
+    HC(hipGetLastError());
+  }
+}
+
+// One conflict pass; returns number of winners.
+int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
+                      bool *overflowed) {
+  PassD ps = pass_base(K);
+  ps.ids = ids;
+  ps.nc = nc;
+  int iters = std::max(1, opt.claim_iters);
+  if (epoch + iters + 1 >= (1ull << 11)) {  // epoch wrap: clear claims
+    HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
+    epoch = 0;
+  }
+  ps.epoch0 = epoch + 1;
+  ps.epoch = epoch + 1;
+  ps.iter = 0;
   bool want_feat = !is_seg && (opt.priority == 2 || opt.log_training);
   if (want_feat && nc > feat_cap) {
     dfree(c_feat); dfree(c_score); dfree(c_deg);
@@ -1194,7 +1633,7 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
   // one candidate per wave64 block; the exact-predicate arena holds
   // pws_blocks*256 lanes, i.e. pws_blocks*4 blocks of 64
-  int gw = std::min(nc, pws_blocks * 4);
+  int gw = std::min(nc, std::min(opt.grid_cavity, pws_blocks * 4));
   size_t lds = sizeof(int) * (size_t)wave_lds_ints(K);
   if (is_seg) {
     tbegin(K_SEGCAV);
@@ -1206,12 +1645,12 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
     tend(K_BADCAV, nc);
   }
   HC(hipGetLastError());
-  int gc = std::min(nblk(nc), 4096);
+  int gc = std::min(nblk(nc), opt.grid_claim);
   if (!is_seg && opt.priority == 2) {
     if (!mlp_params) throw std::runtime_error("priority 2 needs a model (set_priority_model)");
     tbegin(K_MLP);
     mlp_forward_device(c_feat, nc, mlp_params, c_score, stream);
-    k_learned_prio<<<nblk(nc), 256, 0, stream>>>(ps, c_score);
+    k_learned_prio<<<std::min(nblk(nc), 4096), 256, 0, stream>>>(ps, c_score);
     tend(K_MLP, nc);
     HC(hipGetLastError());
   }
@@ -1235,26 +1674,16 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
       log_t.push_back(std::log2(1.0f + (float)hd[i]));
     }
   }
-  for (int it = 0; it < iters; it++) {
-    ps.epoch = ++epoch;
-    ps.iter = it;
-    tbegin(K_CLAIM);
-    k_claim<<<gc, 256, 0, stream>>>(m, ps, c_act);
-    tend(K_CLAIM, nc);
-    tbegin(K_WIN);
-    k_win<<<gc, 256, 0, stream>>>(m, ps, c_act);
-    if (it + 1 < iters) k_block<<<gc, 256, 0, stream>>>(m, ps, c_act);
-    tend(K_WIN, nc);
-    HC(hipGetLastError());
-  }
+  enqueue_claims(ps, gc);
   int nwin = compact(PredWin{c_win}, nc, nullptr, c_rank, nullptr);
   ensure_capacity(np + nwin, nt + 2LL * nwin, is_seg ? ns + nwin : ns);
   if (nwin > 0) {
     tbegin(K_APPLY);
     // apply keeps each winner's cavity in LDS: lds_k ints per thread
     int lds_k = std::min(K, (int)(BIG_LDS_BYTES / (sizeof(int) * CAV_BLOCK)));
-    k_apply<<<std::min(nblk(nc, CAV_BLOCK), 8192), CAV_BLOCK, sizeof(int) * lds_k * CAV_BLOCK,
-              stream>>>(m, pr, ps, np, nt, ns, is_seg ? 1 : 0, lds_k);
+    k_apply<<<std::min(nblk(nc, CAV_BLOCK), opt.grid_apply), CAV_BLOCK,
+              sizeof(int) * lds_k * CAV_BLOCK, stream>>>(m, pr, ps, np, nt, ns,
+                                                         is_seg ? 1 : 0, lds_k);
     tend(K_APPLY, nwin);
     HC(hipGetLastError());
   }
@@ -1288,19 +1717,28 @@ void Refiner::run() {
   long long wantP = std::max<long long>(16, 6LL * np0);
   long long wantT = std::max<long long>(16, 6LL * nt0);
   long long wantS = std::max<long long>(16, 8LL * ns0 + 16);
-  if (capP < wantP || capT < wantT || capS < wantS) {
+  bool want_inst = in_tinst != nullptr;
+  if (capP < wantP || capT < wantT || capS < wantS || (want_inst != (m.tinst != nullptr))) {
     free_mesh();
     capP = wantP; capT = wantT; capS = wantS;
     dalloc(&m.P, capP); dalloc(&m.pflag, capP); dalloc(&m.ppar, capP);
     dalloc(&m.TV, capT); dalloc(&m.TN, capT); dalloc(&m.TS, capT);
     dalloc(&m.tflag, capT); dalloc(&m.claim, capT);
+    if (want_inst) dalloc(&m.tinst, capT);
     dalloc(&m.SV, capS); dalloc(&m.SP, capS); dalloc(&m.ST, capS); dalloc(&m.senc, capS);
-    dalloc(&seg_in, (size_t)std::max(ns0, 1));
+  }
+  if (ns0 > seg_in_cap || !seg_in) {
+    dfree(seg_in);
+    seg_in_cap = std::max(ns0, 1);
+    dalloc(&seg_in, (size_t)seg_in_cap);
   }
   m.SEG = seg_in;
+  pr.ipar = d_ipar;
   HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
   tbegin(K_INIT);
   HC(hipMemcpyAsync(m.P, in_P, sizeof(double2) * np0, hipMemcpyDeviceToDevice, stream));
+  if (want_inst)
+    HC(hipMemcpyAsync(m.tinst, in_tinst, sizeof(int) * nt0, hipMemcpyDeviceToDevice, stream));
   k_init_points<<<nblk(np0), 256, 0, stream>>>(np0, m.pflag, m.ppar);
   k_init_tris<<<nblk(nt0), 256, 0, stream>>>(in_T, in_N, nt0, m.TV, m.TN, m.TS, m.tflag, m.claim, d_cnt);
   // segment hash: capacity power of two >= 2*ns + 16
@@ -1333,19 +1771,57 @@ void Refiner::run() {
   pass_index = 0;
   log_X.clear();
   log_t.clear();
-  int prev_win = -1;
   HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
   if (!c_ws || ws_ints < opt.cand_budget) {
     dfree(c_ws);
     dalloc(&c_ws, (size_t)opt.cand_budget);
     ws_ints = opt.cand_budget;
   }
-  if (!c_pws) {
-    pws_blocks = 2048;
+  int want_pws = (opt.grid_cavity + 3) / 4;
+  if (!c_pws || pws_blocks < want_pws) {
+    dfree(c_pws);
+    pws_blocks = std::max(want_pws, 1);
     dalloc(&c_pws, (size_t)pws_blocks * 256 * GCDT_INCIRCLE_WS);
   }
-  ensure_cand(std::min(cands_per_pass(K), std::max(4 * nt0, 1024)));
   st.status = 1;
+  if (opt.graph) loop_graph();
+  else loop_host();
+  st.final_cav_cap = K;
+  // --- export --------------------------------------------------------------
+  if (nt > newid_cap) {
+    dfree(newid);
+    newid_cap = nt;
+    dalloc(&newid, (size_t)newid_cap);
+  }
+  nt_out = compact(PredAlive{m.tflag}, nt, nullptr, newid, nullptr);
+  if (nt_out > o_cap) {
+    dfree(o_T); dfree(o_N); dfree(o_tstat); dfree(o_inst);
+    o_cap = nt_out;
+    dalloc(&o_T, 3 * (size_t)o_cap);
+    dalloc(&o_N, 3 * (size_t)o_cap);
+    dalloc(&o_tstat, (size_t)o_cap);
+    if (m.tinst) dalloc(&o_inst, (size_t)o_cap);
+  }
+  if (m.tinst && !o_inst) dalloc(&o_inst, (size_t)std::max(o_cap, 1));
+  tbegin(K_EXPORT);
+  k_export<<<nblk(nt), 256, 0, stream>>>(m, nt, newid, o_T, o_N, o_tstat, m.tinst ? o_inst : nullptr);
+  tend(K_EXPORT, nt);
+  HC(hipGetLastError());
+  HC(hipEventRecord(ev1, stream));
+  HC(hipStreamSynchronize(stream));
+  float ms = 0.f;
+  HC(hipEventElapsedTime(&ms, ev0, ev1));
+  st.device_ms = ms;
+  st.n_points = np;
+  st.n_triangles = nt_out;
+  st.n_subsegs = ns;
+  collect_times();
+}
+
+// Host-driven rounds: the host reads every count and decides every step.
+void Refiner::loop_host() {
+  int prev_win = -1;
+  ensure_cand(std::min(cands_per_pass(K), std::max(4 * nt0, 1024)));
   for (int round = 0; round < opt.max_rounds; round++) {
     st.rounds = round + 1;
     // segment phase
@@ -1363,7 +1839,7 @@ void Refiner::run() {
     }
     // bad phase
     tbegin(K_MARKBAD);
-    k_mark_bad<<<nblk(nt), 256, 0, stream>>>(m, pr, nt);
+    k_mark_bad<<<std::min(nblk(nt), 65536), 256, 0, stream>>>(m, pr, nt, nullptr);
     tend(K_MARKBAD, nt);
     ensure_tmp(nt);
     int nb = compact(PredBad{m.tflag}, nt, tmp_list, nullptr, nullptr);
@@ -1374,7 +1850,6 @@ void Refiner::run() {
     int *ids = tmp_list;
     int nc = select_by_priority(tmp_list, nb, cap, &ids);
     ensure_cand(nc);
-    if (learned_hook) learned_hook(this, ids, nc);
     long long prog = 0;
     bool of = false;
     prev_win = run_pass(ids, nc, false, &prog, &of);
@@ -1387,34 +1862,221 @@ void Refiner::run() {
     }
     if (prog == 0) { st.status = 2; break; }
   }
-  st.final_cav_cap = K;
-  // --- export --------------------------------------------------------------
-  if (nt > newid_cap) {
-    dfree(newid);
-    newid_cap = nt;
-    dalloc(&newid, (size_t)newid_cap);
+}
+
+// ---------------------------------------------------------------------------
+// Graph-driven rounds
+// ---------------------------------------------------------------------------
+// Buffers whose size depends on K and on the mesh capacity.
+void Refiner::size_pass_buffers() {
+  ensure_cand(cands_per_pass(K));
+  ensure_tmp((int)std::max(capT, capS));
+  if (capT > sel_cap) {
+    dfree(sel_prio); dfree(sel_tmp); dfree(sel_list);
+    sel_cap = (int)capT;
+    dalloc(&sel_prio, (size_t)sel_cap);
+    dalloc(&sel_tmp, (size_t)sel_cap);
+    dalloc(&sel_list, (size_t)sel_cap);
   }
-  nt_out = compact(PredAlive{m.tflag}, nt, nullptr, newid, nullptr);
-  if (nt_out > o_cap) {
-    dfree(o_T); dfree(o_N); dfree(o_tstat);
-    o_cap = nt_out;
-    dalloc(&o_T, 3 * (size_t)o_cap);
-    dalloc(&o_N, 3 * (size_t)o_cap);
-    dalloc(&o_tstat, (size_t)o_cap);
+  if (!d_hist) dalloc(&d_hist, NBUCKET);
+  if (opt.priority == 2 && feat_cap < c_cap) {
+    dfree(c_feat); dfree(c_score); dfree(c_deg);
+    feat_cap = c_cap;
+    dalloc(&c_feat, (size_t)feat_cap * FEAT);
+    dalloc(&c_score, (size_t)feat_cap);
+    dalloc(&c_deg, (size_t)feat_cap);
   }
-  tbegin(K_EXPORT);
-  k_export<<<nblk(nt), 256, 0, stream>>>(m, nt, newid, o_T, o_N, o_tstat);
-  tend(K_EXPORT, nt);
+}
+
+// Per-pass counters of the graph loop live in the control block.
+unsigned long long *Refiner::ctl_cnt_ptr() const {
+  return reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(d_ctl) +
+                                                offsetof(Ctl, cnt));
+}
+
+void Refiner::enqueue_seg_pass() {
+  k_gate_seg<<<1, 1, 0, stream>>>(d_ctl);
+  k_dcount<PredSenc><<<GC, CB, 0, stream>>>(PredSenc{m.senc}, d_ctl, C_NS, -1, g_cnt);
+  k_dscan<FinSeg><<<1, GC, 0, stream>>>(g_cnt, g_off, d_ctl, -1, FinSeg{});
+  k_dscatter<PredSenc><<<GC, CB, 0, stream>>>(PredSenc{m.senc}, d_ctl, C_NS, -1, g_off,
+                                              tmp_list, nullptr, nullptr);
+  PassD ps = pass_base(K);
+  ps.ids = tmp_list;
+  ps.ctl = d_ctl;
+  ps.cnt = ctl_cnt_ptr();
+  size_t lds = sizeof(int) * (size_t)wave_lds_ints(K);
+  k_cavity<true><<<opt.grid_cavity, 64, lds, stream>>>(m, pr, ps, opt.cav_cap_max);
+  enqueue_claims(ps, opt.grid_claim);
+  k_dcount<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_cnt);
+  k_dscan<FinWin><<<1, GC, 0, stream>>>(g_cnt, g_off, d_ctl, -1, FinWin{});
+  k_dscatter<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_off,
+                                             nullptr, c_rank, nullptr);
+  int lds_k = std::min(K, (int)(BIG_LDS_BYTES / (sizeof(int) * CAV_BLOCK)));
+  k_apply<<<opt.grid_apply, CAV_BLOCK, sizeof(int) * lds_k * CAV_BLOCK, stream>>>(
+      m, pr, ps, 0, 0, 0, 1, lds_k);
+  k_end_seg<<<1, 1, 0, stream>>>(d_ctl, std::max(1, opt.claim_iters));
   HC(hipGetLastError());
-  HC(hipEventRecord(ev1, stream));
-  HC(hipStreamSynchronize(stream));
-  float ms = 0.f;
-  HC(hipEventElapsedTime(&ms, ev0, ev1));
-  st.device_ms = ms;
-  st.n_points = np;
-  st.n_triangles = nt_out;
-  st.n_subsegs = ns;
-  collect_times();
+}
+
+void Refiner::enqueue_bad_pass() {
+  k_gate_bad<<<1, 1, 0, stream>>>(d_ctl);
+  k_mark_bad<<<4096, 256, 0, stream>>>(m, pr, 0, d_ctl);
+  k_dcount<PredBad><<<GC, CB, 0, stream>>>(PredBad{m.tflag}, d_ctl, C_NT, -1, g_cnt);
+  k_dscan<FinBad><<<1, GC, 0, stream>>>(g_cnt, g_off, d_ctl, -1, FinBad{d_hist});
+  k_dscatter<PredBad><<<GC, CB, 0, stream>>>(PredBad{m.tflag}, d_ctl, C_NT, -1, g_off,
+                                             tmp_list, nullptr, nullptr);
+  if (opt.priority == 1 || opt.priority == 2) {
+    k_dprio_hist<<<512, 256, 0, stream>>>(m, tmp_list, d_ctl, sel_prio, d_hist);
+    k_dcut<<<1, 1024, 0, stream>>>(d_hist, d_ctl);
+    PredPrioEqC peq{sel_prio, d_ctl};
+    k_dcount<PredPrioEqC><<<GC, CB, 0, stream>>>(peq, d_ctl, C_NB, C_SEL, g_cnt);
+    k_dscan<FinNone><<<1, GC, 0, stream>>>(g_cnt, g_off, d_ctl, C_SEL, FinNone{});
+    k_dscatter<PredPrioEqC><<<GC, CB, 0, stream>>>(peq, d_ctl, C_NB, C_SEL, g_off,
+                                                   sel_tmp, nullptr, nullptr);
+    k_dicut<<<1, 1, 0, stream>>>(sel_tmp, d_ctl);
+    PredPrioSelC psel{sel_prio, d_ctl};
+    k_dcount<PredPrioSelC><<<GC, CB, 0, stream>>>(psel, d_ctl, C_NB, C_SEL, g_cnt);
+    k_dscan<FinSel><<<1, GC, 0, stream>>>(g_cnt, g_off, d_ctl, C_SEL, FinSel{});
+    k_dscatter<PredPrioSelC><<<GC, CB, 0, stream>>>(psel, d_ctl, C_NB, C_SEL, g_off,
+                                                    sel_list, nullptr, tmp_list);
+  }
+  PassD ps = pass_base(K);
+  ps.ids = tmp_list;
+  ps.ids_sel = sel_list;
+  ps.ctl = d_ctl;
+  ps.cnt = ctl_cnt_ptr();
+  ps.feat = opt.priority == 2 ? c_feat : nullptr;
+  size_t lds = sizeof(int) * (size_t)wave_lds_ints(K);
+  k_cavity<false><<<opt.grid_cavity, 64, lds, stream>>>(m, pr, ps, opt.cav_cap_max);
+  if (opt.priority == 2) {
+    const int *ci = reinterpret_cast<const int *>(reinterpret_cast<const char *>(d_ctl) +
+                                                  offsetof(Ctl, i));
+    mlp_forward_device_n(c_feat, ci + C_NC, ci + C_GO, mlp_params, c_score, 4096, stream);
+    k_learned_prio<<<4096, 256, 0, stream>>>(ps, c_score);
+  }
+  enqueue_claims(ps, opt.grid_claim);
+  k_dcount<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_cnt);
+  k_dscan<FinWin><<<1, GC, 0, stream>>>(g_cnt, g_off, d_ctl, -1, FinWin{});
+  k_dscatter<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_off,
+                                             nullptr, c_rank, nullptr);
+  int lds_k = std::min(K, (int)(BIG_LDS_BYTES / (sizeof(int) * CAV_BLOCK)));
+  k_apply<<<opt.grid_apply, CAV_BLOCK, sizeof(int) * lds_k * CAV_BLOCK, stream>>>(
+      m, pr, ps, 0, 0, 0, 0, lds_k);
+  k_end_bad<<<1, 1, 0, stream>>>(d_ctl, std::max(1, opt.claim_iters));
+  HC(hipGetLastError());
+}
+
+void Refiner::capture_graph() {
+  if (gexec) { HC(hipGraphExecDestroy(gexec)); gexec = nullptr; }
+  if (graph) { HC(hipGraphDestroy(graph)); graph = nullptr; }
+  HC(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  capturing = true;
+  try {
+    for (int r = 0; r < opt.graph_rounds; r++) {
+      for (int q = 0; q < opt.graph_seg_passes; q++) enqueue_seg_pass();
+      enqueue_bad_pass();
+    }
+  } catch (...) {
+    capturing = false;
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  capturing = false;
+  HC(hipStreamEndCapture(stream, &graph));
+  HC(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+  g_K = K;
+}
+
+void Refiner::loop_graph() {
+  if (opt.priority == 2 && !mlp_params)
+    throw std::runtime_error("priority 2 needs a model (set_priority_model)");
+  Ctl &h = *h_ctl;
+  memset(&h, 0, sizeof(Ctl));
+  h.i[C_STATUS] = 1;
+  h.i[C_NP] = np;
+  h.i[C_NT] = nt;
+  h.i[C_NS] = ns;
+  h.i[C_K] = K;
+  h.i[C_KCAP] = K;
+  h.i[C_PREVWIN] = -1;
+  h.i[C_CAPP] = (int)std::min<long long>(capP, INT32_MAX);
+  h.i[C_CAPT] = (int)std::min<long long>(capT, INT32_MAX);
+  h.i[C_CAPS] = (int)std::min<long long>(capS, INT32_MAX);
+  h.i[C_MAXROUNDS] = opt.max_rounds;
+  h.i[C_KMAX] = opt.cav_cap_max;
+  h.i[C_CANDMIN] = opt.cand_min;
+  h.i[C_CANDFACTOR] = opt.cand_factor;
+  h.i[C_ITERS] = std::max(1, opt.claim_iters);
+  h.i[C_PRIORITY] = opt.priority;
+  h.acc[A_BUDGET] = (unsigned long long)opt.cand_budget;
+  size_pass_buffers();
+  HC(hipMemcpyAsync(d_ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, stream));
+  capture_graph();
+  const int iters = std::max(1, opt.claim_iters);
+  const long long passes = (long long)opt.graph_rounds * (opt.graph_seg_passes + 1);
+  const long long max_launches = 4LL * opt.max_rounds + 100000;
+  for (long long launch = 0;; launch++) {
+    if (launch > max_launches) throw std::runtime_error("graph loop made no progress");
+    if (h.i[C_EPOCH] + passes * iters + 1 >= (1LL << 11)) {  // epoch wrap
+      HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
+      h.i[C_EPOCH] = 0;
+      HC(hipMemcpyAsync(d_ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, stream));
+    }
+    HC(hipGraphLaunch(gexec, stream));
+    HC(hipMemcpyAsync(&h, d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+    HC(hipStreamSynchronize(stream));
+    if (h.i[C_DONE]) break;
+    bool recapture = false;
+    if (h.i[C_SUSPEND]) {  // the next pass needs more mesh capacity
+      np = h.i[C_NP];
+      nt = h.i[C_NT];
+      ns = h.i[C_NS];
+      long long nc = h.i[C_NC];
+      long long needT = nt + 2 * nc;
+      if (4 * needT >= (1LL << 31))
+        throw std::runtime_error("mesh exceeds the 32-bit otri encoding (2^29 triangles)");
+      ensure_capacity(np + nc, needT, ns + nc);
+      h.i[C_CAPP] = (int)std::min<long long>(capP, INT32_MAX);
+      h.i[C_CAPT] = (int)std::min<long long>(capT, INT32_MAX);
+      h.i[C_CAPS] = (int)std::min<long long>(capS, INT32_MAX);
+      h.i[C_SUSPEND] = 0;
+      recapture = true;
+    }
+    if (h.i[C_K] != g_K) {  // cavity capacity doubled
+      K = h.i[C_K];
+      h.i[C_KCAP] = K;
+      recapture = true;
+    }
+    if (recapture) {
+      size_pass_buffers();
+      HC(hipMemcpyAsync(d_ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, stream));
+      capture_graph();
+    }
+  }
+  np = h.i[C_NP];
+  nt = h.i[C_NT];
+  ns = h.i[C_NS];
+  K = h.i[C_K];
+  pass_index = h.i[C_PASS_INDEX];
+  st.status = h.i[C_STATUS];
+  st.rounds = h.i[C_ROUNDS];
+  st.seg_passes = h.i[C_SEG_PASSES];
+  st.bad_passes = h.i[C_BAD_PASSES];
+  st.rej_overflow = (long long)h.acc[A_REJ_OVF];
+  st.rej_encroach = (long long)h.acc[A_REJ_ENC];
+  st.rej_blocked = (long long)h.acc[A_REJ_BLK];
+  st.giveups = (long long)h.acc[A_GIVEUPS];
+  st.lost = (long long)h.acc[A_LOST];
+  st.pred_fp32 = (long long)h.acc[A_FP32];
+  st.pred_fp64 = (long long)h.acc[A_FP64];
+  st.pred_exact = (long long)h.acc[A_EXACT];
+  st.pred_dd = (long long)h.acc[A_DD];
+  st.big_cavities = (long long)h.acc[A_BIG];
+  st.seg_splits = (long long)h.acc[A_SEGSPLITS];
+  st.inserted = (long long)h.acc[A_INSERTED];
+  st.cand_evals = (long long)h.acc[A_CANDEVALS];
 }
 
 void Refiner::ensure_tmp(int n) {
@@ -1449,6 +2111,11 @@ void Refiner::download(MeshHost &out) {
   }
   if (ns) HC(hipMemcpyAsync(out.S.data(), m.SV, sizeof(int2) * ns, hipMemcpyDeviceToHost, stream));
   HC(hipMemcpyAsync(out.pstat.data(), o_pstat, np, hipMemcpyDeviceToHost, stream));
+  out.tinst.clear();
+  if (m.tinst && nt_out) {
+    out.tinst.resize((size_t)nt_out);
+    HC(hipMemcpyAsync(out.tinst.data(), o_inst, sizeof(int) * nt_out, hipMemcpyDeviceToHost, stream));
+  }
   HC(hipStreamSynchronize(stream));
 }
 

@@ -125,6 +125,17 @@ typedef struct gcdt_options {
                             the reference's single atomicMin claim); more
                             iterations let candidates that touch no winner
                             compete again, enlarging each independent set */
+  int graph;             /* 1 = device-driven round loop: every kernel reads
+                            its work count from a device control block, a
+                            block of rounds is one hipGraph launch and the
+                            host synchronises once per launch (same result
+                            as 0); 0 = host-driven passes (default) */
+  int graph_seg_passes;  /* segment passes captured per round (default 2) */
+  int graph_rounds;      /* rounds per graph launch (default 4) */
+  int grid_cavity;       /* wave64 blocks of the cavity kernels (default 8192,
+                            limit 16384) */
+  int grid_claim;        /* 256-lane blocks of the claim kernels (default 2048) */
+  int grid_apply;        /* 128-lane blocks of the apply kernel (default 4096) */
 } gcdt_options;
 
 typedef struct gcdt_stats {
@@ -180,6 +191,39 @@ int gcdt_session_download(gcdt_session *s, gcdt_mesh **out);
 int gcdt_session_kernel_times(gcdt_session *s, gcdt_kernel_time *rec,
                               int max_rec);
 void gcdt_session_destroy(gcdt_session *s);
+
+/* Batched refinement (no counterpart in the reference, which refines one
+ * CDT per GPU_Refine_Quality call, refine.cu:6500-7139): n_inst independent
+ * CDTs with their own theta (degrees), insertion policy (1 off-center) and
+ * mode (1 Ruppert), refined together.  Inputs are concatenated per instance
+ * with instance-local indices: np/nt/ns[n_inst] counts, points[2*sum np],
+ * tris and nbrs[3*sum nt], segs[2*sum ns] (each instance in the input
+ * contract of GPU_Refine_Quality).  Instances are packed into waves of at
+ * most wave_triangles input triangles (<= 0: 2^25); a wave is refined as the
+ * union of its instances, with every instance keeping its own parameters.
+ * All inputs stay resident on the device; gcdt_batch_run refines every wave
+ * (graph-driven when options.graph = 1).  Results are per instance, in
+ * instance order, with instance-local indices: input points first, then
+ * Steiner points. */
+typedef struct gcdt_batch gcdt_batch;
+int gcdt_batch_create(int n_inst, const int *np, const int *nt, const int *ns,
+                      const double *points, const int *tris, const int *nbrs,
+                      const int *segs, const double *theta_deg,
+                      const int *offcenter, const int *mode,
+                      const gcdt_options *opt, long long wave_triangles,
+                      gcdt_batch **out);
+int gcdt_batch_waves(const gcdt_batch *b);
+int gcdt_batch_run(gcdt_batch *b, gcdt_stats *stats);
+/* host wall time of the last run's refinement (ms), transfers excluded */
+double gcdt_batch_run_ms(const gcdt_batch *b);
+int gcdt_batch_set_priority_model(gcdt_batch *b, const float *params);
+/* Per-instance output counts (arrays of n_inst; any may be NULL). */
+int gcdt_batch_counts(gcdt_batch *b, int *np, int *nt, int *ns);
+/* Concatenated per-instance outputs (sizes from gcdt_batch_counts). */
+int gcdt_batch_copy(gcdt_batch *b, double *points, int *tris, int *nbrs,
+                    int *segs, unsigned char *pstatus, unsigned char *tstatus);
+void gcdt_batch_destroy(gcdt_batch *b);
+const char *gcdt_batch_last_error(void);
 
 /* Learned priority (options.priority = 2): load the MLP parameters
  * (gcdt_mlp_nparam() floats: W1[16][64] b1[64] W2[64][64] b2[64] W3[64] b3).

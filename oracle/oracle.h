@@ -69,6 +69,13 @@ typedef struct orc_result orc_result;
 orc_result *orc_refine(int np, const double *points, int nt, const int *tris,
                        const int *nbrs, int ns, const int *segs,
                        const orc_params *prm, orc_stats *st);
+orc_result *orc_refine_batch(int np, const double *points, int nt,
+                             const int *tris, const int *nbrs, int ns,
+                             const int *segs, const int *tri_inst, int n_inst,
+                             const double *theta_deg, const int *offcenter,
+                             const int *mode, const orc_params *prm,
+                             orc_stats *st);
+void orc_result_inst(const orc_result *r, int *tri_inst);
 void orc_result_counts(const orc_result *r, int *np, int *nt, int *ns);
 void orc_result_copy(const orc_result *r, double *points, int *tris, int *nbrs,
                      int *segs, uint8_t *pflag);

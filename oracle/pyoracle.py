@@ -74,6 +74,10 @@ def orc():
         L.orc_refine.restype = vp
         L.orc_refine.argtypes = [C.c_int, vp, C.c_int, vp, vp, C.c_int, vp,
                                  C.POINTER(OrcParams), C.POINTER(OrcStats)]
+        L.orc_refine_batch.restype = vp
+        L.orc_refine_batch.argtypes = [C.c_int, vp, C.c_int, vp, vp, C.c_int, vp, vp, C.c_int,
+                                       vp, vp, vp, C.POINTER(OrcParams), C.POINTER(OrcStats)]
+        L.orc_result_inst.argtypes = [vp, vp]
         L.orc_result_counts.argtypes = [vp, vp, vp, vp]
         L.orc_result_copy.argtypes = [vp, vp, vp, vp, vp, vp]
         L.orc_result_free.argtypes = [vp]
@@ -193,6 +197,86 @@ def oracle_refine(points, triangles, neighbors, segments, theta=25.0, offcenter=
     L.orc_result_copy(r, _p(Po), _p(To), _p(No), _p(So), _p(pf))
     L.orc_result_free(r)
     return Po, To, No, So, pf, st.as_dict()
+
+
+def split_instances(np_in, P, T, N, S, pf, tinst):
+    """Split a union mesh into its instances: input points of instance i are
+    the union's points [sum(np_in[:i]), sum(np_in[:i+1])); a Steiner point
+    belongs to the instance of a triangle that uses it.  Each instance keeps
+    its input points first, then its Steiner points in creation order, and
+    the relative order of its triangles and subsegments."""
+    k = len(np_in)
+    pin = np.concatenate([[0], np.cumsum(np_in)]).astype(np.int64)
+    np0 = int(pin[-1])
+    pinst = np.full(len(P), -1, np.int64)
+    for i in range(k):
+        pinst[pin[i]:pin[i + 1]] = i
+    ti = np.repeat(tinst, 3)
+    v = T.reshape(-1)
+    st = v >= np0
+    pinst[v[st]] = ti[st]
+    assert (pinst >= 0).all()
+    out = []
+    for i in range(k):
+        pid = np.nonzero(pinst == i)[0]
+        loc = np.full(len(P), -1, np.int64)
+        loc[pid] = np.arange(len(pid))
+        tid = np.nonzero(tinst == i)[0]
+        tl = np.full(len(T), -1, np.int64)
+        tl[tid] = np.arange(len(tid))
+        Ti = loc[T[tid]].astype(np.int32)
+        Ni = np.where(N[tid] < 0, -1, tl[np.maximum(N[tid], 0)]).astype(np.int32)
+        sid = np.nonzero(pinst[S[:, 0]] == i)[0]
+        Si = loc[S[sid]].astype(np.int32)
+        out.append((P[pid], Ti, Ni, Si, pf[pid]))
+    return out
+
+
+def oracle_refine_batch(instances, theta, offcenter, mode, **params):
+    """Batched refinement on the CPU: the union of `instances` ((P, T, N, S)
+    with local indices) refined with per-instance theta / offcenter / mode;
+    returns ([(P, T, N, S, pflag)] per instance, stats)."""
+    k = len(instances)
+    Ps, Ts, Ns, Ss, I = [], [], [], [], []
+    po = to = 0
+    for i, (P, T, N, S) in enumerate(instances):
+        P = np.asarray(P, np.float64).reshape(-1, 2)
+        T = np.asarray(T, np.int32).reshape(-1, 3)
+        N = np.asarray(N, np.int32).reshape(-1, 3)
+        S = np.asarray(S, np.int32).reshape(-1, 2)
+        Ps.append(P); Ts.append(T + po); Ns.append(np.where(N < 0, -1, N + to)); Ss.append(S + po)
+        I.append(np.full(len(T), i, np.int32))
+        po += len(P); to += len(T)
+    P = np.ascontiguousarray(np.concatenate(Ps))
+    T = np.ascontiguousarray(np.concatenate(Ts).astype(np.int32))
+    N = np.ascontiguousarray(np.concatenate(Ns).astype(np.int32))
+    S = np.ascontiguousarray(np.concatenate(Ss).astype(np.int32))
+    I = np.ascontiguousarray(np.concatenate(I))
+    th = np.ascontiguousarray(np.broadcast_to(np.asarray(theta, np.float64), (k,)))
+    off = np.ascontiguousarray(np.broadcast_to(np.asarray(offcenter, np.int32), (k,)))
+    md = np.ascontiguousarray(np.broadcast_to(np.asarray(mode, np.int32), (k,)))
+    d = dict(cav_cap0=32, cav_cap_max=2048, cand_budget=1 << 30, max_rounds=100000,
+             priority=1, cand_factor=4, cand_min=16384, claim_iters=1)
+    d.update(params)
+    prm = OrcParams(25.0, 1, 1, d["cav_cap0"], d["cav_cap_max"], d["cand_budget"],
+                    d["max_rounds"], d["priority"], d["cand_factor"], d["cand_min"],
+                    d["claim_iters"])
+    st = OrcStats()
+    L = orc()
+    r = L.orc_refine_batch(len(P), _p(P), len(T), _p(T), _p(N), len(S), _p(S), _p(I), k,
+                           _p(th), _p(off), _p(md), C.byref(prm), C.byref(st))
+    np_, nt, ns = C.c_int(), C.c_int(), C.c_int()
+    L.orc_result_counts(r, C.byref(np_), C.byref(nt), C.byref(ns))
+    Po = np.empty((np_.value, 2))
+    To = np.empty((nt.value, 3), np.int32)
+    No = np.empty((nt.value, 3), np.int32)
+    So = np.empty((ns.value, 2), np.int32)
+    pf = np.empty(np_.value, np.uint8)
+    Io = np.empty(nt.value, np.int32)
+    L.orc_result_copy(r, _p(Po), _p(To), _p(No), _p(So), _p(pf))
+    L.orc_result_inst(r, _p(Io))
+    L.orc_result_free(r)
+    return split_instances([len(x) for x in Ps], Po, To, No, So, pf, Io), st.as_dict()
 
 
 VIOLATION_NAMES = ["range", "neighbors", "orientation", "unsegmented_boundary",

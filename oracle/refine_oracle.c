@@ -69,6 +69,10 @@ typedef struct {
   int *SEG;
   double goodangle, lens, offconst;
   int offcenter, ruppert;
+  /* batched refinement: instance of every triangle and per-instance
+   * parameters (goodangle, lens, offconst, offcenter, ruppert) */
+  int *tinst;
+  const double *ipar;
 } Mesh;
 
 struct orc_result {
@@ -76,6 +80,7 @@ struct orc_result {
   double *P;
   int *T, *N, *S;
   uint8_t *pflag;
+  int *I; /* instance of every output triangle (batched refinement) */
 };
 
 static void *xrealloc(void *p, size_t n) {
@@ -102,6 +107,7 @@ static void grow_tris(Mesh *m, int need) {
   m->TN = xrealloc(m->TN, sizeof(int) * 3 * (size_t)c);
   m->TS = xrealloc(m->TS, sizeof(int) * 3 * (size_t)c);
   m->tflag = xrealloc(m->tflag, (size_t)c);
+  if (m->ipar) m->tinst = xrealloc(m->tinst, sizeof(int) * (size_t)c);
   m->capt = c;
 }
 
@@ -117,6 +123,17 @@ static void grow_segs(Mesh *m, int need) {
 }
 
 static const double *PT(const Mesh *m, int v) { return m->P + 2 * (size_t)v; }
+
+/* Make triangle t's instance parameters current (batched refinement). */
+static void use_inst(Mesh *m, int t) {
+  if (!m->ipar) return;
+  const double *q = m->ipar + 5 * (size_t)m->tinst[t];
+  m->goodangle = q[0];
+  m->lens = q[1];
+  m->offconst = q[2];
+  m->offcenter = (int)q[3];
+  m->ruppert = (int)q[4];
+}
 
 /* checkseg4encroach: does apex p encroach subsegment (a,b)?
  * refine.cu:115-139, triangle.cpp:7138-7153 */
@@ -461,6 +478,7 @@ static void apply_star(Mesh *m, const Cand *cd, int split_s, int p, int tbase,
   Bnd *B = malloc(sizeof(Bnd) * (size_t)k);
   int t0 = -1, j0 = -1;
   int open = 0;
+  const int inst = m->ipar ? m->tinst[L[0]] : 0;
   if (is_seg) {
     int t1, j1, t2, j2;
     seg_sides(m, split_s, &t1, &j1, &t2, &j2);
@@ -516,6 +534,7 @@ static void apply_star(Mesh *m, const Cand *cd, int split_s, int p, int tbase,
     m->TS[3 * st + 1] = (B[i].v == a) ? sA : (B[i].v == b) ? sB : -1;
     m->TS[3 * st + 2] = (B[i].u == a) ? sA : (B[i].u == b) ? sB : -1;
     m->tflag[st] = TF_ALIVE | TF_CHECK;
+    if (m->ipar) m->tinst[st] = inst;
   }
   if (open) { /* the second reserved slot stays unused */
     int st = tbase + 1;
@@ -602,13 +621,15 @@ static int run_pass(Mesh *m, const int *ids, int nc, int is_seg, int K,
     if (is_seg) {
       int s = ids[i];
       split_s = s;
-      if (!split_point(m, s, c->c)) { c->status = ST_GIVEUP; goto done; }
       int t1, j1, t2, j2;
       seg_sides(m, s, &t1, &j1, &t2, &j2);
+      use_inst(m, t1 >= 0 ? t1 : t2);
+      if (!split_point(m, s, c->c)) { c->status = ST_GIVEUP; goto done; }
       seed0 = t1 >= 0 ? t1 : t2;
       seed1 = t1 >= 0 ? t2 : -1;
     } else {
       int t0 = ids[i];
+      use_inst(m, t0);
       if (!steiner_point(m, t0, c->c)) { c->status = ST_GIVEUP; goto done; }
       int tl = -1, jl = -1, sg = -1;
       int w = walk(m, t0, c->c, &tl, &jl, &sg);
@@ -717,6 +738,7 @@ static int run_pass(Mesh *m, const int *ids, int nc, int is_seg, int K,
     m->P[2 * p + 1] = cd[i].c[1];
     m->pflag[p] = is_seg ? PF_SEGV : PF_FREE;
     m->ppar[p] = is_seg ? m->SP[cd[i].id] : -1;
+    use_inst(m, cd[i].L[0]);
     apply_star(m, &cd[i], is_seg ? cd[i].id : -1, p, tbase + 2 * rank,
                is_seg ? sbase + rank : -1, is_seg);
     rank++;
@@ -751,19 +773,23 @@ static void mark_bad(Mesh *m) {
     uint8_t f = m->tflag[t];
     if (!(f & TF_ALIVE) || !(f & TF_CHECK)) continue;
     f &= (uint8_t)~TF_CHECK;
+    use_inst(m, t);
     if (tri_is_bad(m, t)) f |= TF_BAD;
     else f &= (uint8_t)~TF_BAD;
     m->tflag[t] = f;
   }
 }
 
-orc_result *orc_refine(int np, const double *points, int nt, const int *tris,
-                       const int *nbrs, int ns, const int *segs,
-                       const orc_params *prm, orc_stats *st) {
+static orc_result *refine_impl(int np, const double *points, int nt,
+                               const int *tris, const int *nbrs, int ns,
+                               const int *segs, const int *tri_inst,
+                               const double *ipar, const orc_params *prm,
+                               orc_stats *st) {
   Mesh M;
   memset(&M, 0, sizeof(M));
   memset(st, 0, sizeof(*st));
   Mesh *m = &M;
+  m->ipar = ipar;
   double ga = cos(prm->theta_deg * 3.141592653589793238462643383279502884197169399375105820974944592308 / 180.0);
   m->offconst = (ga == 1.0) ? 0.0 : 0.475 * sqrt((1.0 + ga) / (1.0 - ga));
   m->goodangle = ga * ga;
@@ -776,6 +802,7 @@ orc_result *orc_refine(int np, const double *points, int nt, const int *tris,
   m->np = np;
   grow_tris(m, nt);
   memcpy(m->TV, tris, sizeof(int) * 3 * (size_t)nt);
+  if (ipar) memcpy(m->tinst, tri_inst, sizeof(int) * (size_t)nt);
   for (int t = 0; t < nt; t++) {
     for (int j = 0; j < 3; j++) {
       int u = nbrs[3 * t + j];
@@ -845,6 +872,7 @@ orc_result *orc_refine(int np, const double *points, int nt, const int *tris,
   for (int t = 0; t < nt; t++)
     for (int j = 0; j < 3; j++) {
       int s = m->TS[3 * t + j];
+      use_inst(m, t);
       if (s >= 0 && encroaches(m, m->SV[2 * s], m->SV[2 * s + 1], PT(m, m->TV[3 * t + j])))
         m->senc[s] = 1;
     }
@@ -929,6 +957,7 @@ orc_result *orc_refine(int np, const double *points, int nt, const int *tris,
   r->pflag = malloc((size_t)(m->np > 0 ? m->np : 1));
   memcpy(r->pflag, m->pflag, (size_t)m->np);
   r->T = malloc(sizeof(int) * 3 * (size_t)(nalive > 0 ? nalive : 1));
+  r->I = m->ipar ? malloc(sizeof(int) * (size_t)(nalive > 0 ? nalive : 1)) : NULL;
   r->N = malloc(sizeof(int) * 3 * (size_t)(nalive > 0 ? nalive : 1));
   for (int t = 0; t < m->nt; t++) {
     int q = newid[t];
@@ -938,6 +967,7 @@ orc_result *orc_refine(int np, const double *points, int nt, const int *tris,
       int nb = m->TN[3 * t + j];
       r->N[3 * q + j] = nb < 0 ? -1 : newid[nb >> 2];
     }
+    if (r->I) r->I[q] = m->tinst[t];
   }
   r->S = malloc(sizeof(int) * 2 * (size_t)(m->ns > 0 ? m->ns : 1));
   memcpy(r->S, m->SV, sizeof(int) * 2 * (size_t)m->ns);
@@ -945,7 +975,42 @@ orc_result *orc_refine(int np, const double *points, int nt, const int *tris,
   free(m->P); free(m->pflag); free(m->ppar);
   free(m->TV); free(m->TN); free(m->TS); free(m->tflag);
   free(m->SV); free(m->SP); free(m->ST); free(m->senc); free(m->SEG);
+  free(m->tinst);
   return r;
+}
+
+orc_result *orc_refine(int np, const double *points, int nt, const int *tris,
+                       const int *nbrs, int ns, const int *segs,
+                       const orc_params *prm, orc_stats *st) {
+  return refine_impl(np, points, nt, tris, nbrs, ns, segs, NULL, NULL, prm, st);
+}
+
+/* Batched refinement of the union of independent meshes: tri_inst[nt] gives
+ * every input triangle's instance, theta/offcenter/mode[n_inst] the
+ * instances' parameters (prm's own theta, offcenter and mode are unused). */
+orc_result *orc_refine_batch(int np, const double *points, int nt,
+                             const int *tris, const int *nbrs, int ns,
+                             const int *segs, const int *tri_inst, int n_inst,
+                             const double *theta_deg, const int *offcenter,
+                             const int *mode, const orc_params *prm,
+                             orc_stats *st) {
+  double *ip = malloc(sizeof(double) * 5 * (size_t)(n_inst > 0 ? n_inst : 1));
+  for (int i = 0; i < n_inst; i++) {
+    double ga = cos(theta_deg[i] * 3.141592653589793238462643383279502884197169399375105820974944592308 / 180.0);
+    double g2 = ga * ga;
+    ip[5 * i + 0] = g2;
+    ip[5 * i + 1] = (2.0 * g2 - 1.0) * (2.0 * g2 - 1.0);
+    ip[5 * i + 2] = (ga == 1.0) ? 0.0 : 0.475 * sqrt((1.0 + ga) / (1.0 - ga));
+    ip[5 * i + 3] = offcenter[i];
+    ip[5 * i + 4] = mode[i] == 1;
+  }
+  orc_result *r = refine_impl(np, points, nt, tris, nbrs, ns, segs, tri_inst, ip, prm, st);
+  free(ip);
+  return r;
+}
+
+void orc_result_inst(const orc_result *r, int *tri_inst) {
+  if (r->I) memcpy(tri_inst, r->I, sizeof(int) * (size_t)r->nt);
 }
 
 void orc_result_counts(const orc_result *r, int *np, int *nt, int *ns) {
@@ -965,7 +1030,7 @@ void orc_result_copy(const orc_result *r, double *points, int *tris, int *nbrs,
 
 void orc_result_free(orc_result *r) {
   if (!r) return;
-  free(r->P); free(r->T); free(r->N); free(r->S); free(r->pflag);
+  free(r->P); free(r->T); free(r->N); free(r->S); free(r->pflag); free(r->I);
   free(r);
 }
 
