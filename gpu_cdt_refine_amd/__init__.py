@@ -50,6 +50,26 @@ class Options(C.Structure):
     ]
 
 
+class TuneRecord(C.Structure):
+    _fields_ = [
+        ("graph_rounds", C.c_int),
+        ("graph_seg_passes", C.c_int),
+        ("grid_cavity", C.c_int),
+        ("grid_claim", C.c_int),
+        ("grid_apply", C.c_int),
+        ("ms", C.c_double),
+    ]
+
+
+TUNE_KEYS = ("graph_rounds", "graph_seg_passes", "grid_cavity", "grid_claim", "grid_apply")
+
+
+def _tune_result(n, best, rec):
+    trials = [dict({k: getattr(rec[i], k) for k in TUNE_KEYS}, ms=rec[i].ms)
+              for i in range(min(n, len(rec)))]
+    return {k: getattr(best, k) for k in TUNE_KEYS}, trials
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("rounds", C.c_int),
@@ -148,6 +168,12 @@ def lib():
         L.gcdt_batch_copy.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.gcdt_batch_copy.restype = C.c_int
         L.gcdt_batch_destroy.argtypes = [vp]
+        L.gcdt_session_autotune.argtypes = [vp, C.c_int, C.POINTER(Options),
+                                            C.POINTER(TuneRecord), C.c_int]
+        L.gcdt_session_autotune.restype = C.c_int
+        L.gcdt_batch_autotune.argtypes = [vp, C.c_int, C.POINTER(Options),
+                                          C.POINTER(TuneRecord), C.c_int]
+        L.gcdt_batch_autotune.restype = C.c_int
         L.gcdt_batch_last_error.restype = C.c_char_p
         if hasattr(L, "gcdt_generate_pslg"):
             L.gcdt_generate_pslg.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double,
@@ -278,6 +304,16 @@ class Batch:
         d["run_ms"] = lib().gcdt_batch_run_ms(self.h)
         return d
 
+    def autotune(self, reps=1):
+        """Tune launch shapes / graph structure on this batch (gcdt_batch_autotune);
+        returns (best settings, trials)."""
+        best = Options()
+        rec = (TuneRecord * 64)()
+        n = lib().gcdt_batch_autotune(self.h, int(reps), C.byref(best), rec, 64)
+        if n < 0:
+            raise RuntimeError("gcdt_batch_autotune: " + lib().gcdt_batch_last_error().decode())
+        return _tune_result(n, best, rec)
+
     def set_priority_model(self, params):
         p = np.ascontiguousarray(params, dtype=np.float32)
         if p.size != mlp_nparam():
@@ -341,6 +377,16 @@ class Session:
         if rc != 0:
             _err(rc, "gcdt_session_run")
         return st.as_dict()
+
+    def autotune(self, reps=1):
+        """Tune launch shapes / graph structure on this input
+        (gcdt_session_autotune); returns (best settings, trials)."""
+        best = Options()
+        rec = (TuneRecord * 64)()
+        n = lib().gcdt_session_autotune(self.h, int(reps), C.byref(best), rec, 64)
+        if n < 0:
+            _err(n, "gcdt_session_autotune")
+        return _tune_result(n, best, rec)
 
     def download(self):
         m = C.c_void_p()

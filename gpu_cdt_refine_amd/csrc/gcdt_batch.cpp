@@ -360,6 +360,28 @@ int gcdt_batch_set_priority_model(gcdt_batch *b, const float *params) {
   }
 }
 
+int gcdt_batch_autotune(gcdt_batch *b, int reps, gcdt_options *best,
+                        gcdt_tune_record *rec, int max_rec) {
+  g_batch_err.clear();
+  try {
+    gcdt_options o = b->r->opt;
+    int n = gcdt::autotune(o, reps, [b](const gcdt_options &x) {
+      b->r->opt = x;
+      gcdt_stats st;
+      if (gcdt_batch_run(b, &st) != 0) throw std::runtime_error(g_batch_err);
+      return gcdt_batch_run_ms(b);
+    }, rec, max_rec);
+    b->r->opt = o;
+    b->ran = false;
+    b->split = false;
+    if (best) *best = o;
+    return n;
+  } catch (const std::exception &e) {
+    g_batch_err = e.what();
+    return -2;
+  }
+}
+
 void gcdt_batch_destroy(gcdt_batch *b) {
   if (!b) return;
   delete b->r;

@@ -194,6 +194,26 @@ int gcdt_session_kernel_times(gcdt_session *s, gcdt_kernel_time *rec,
   return s->r->kernel_times(rec, max_rec);
 }
 
+int gcdt_session_autotune(gcdt_session *s, int reps, gcdt_options *best,
+                          gcdt_tune_record *rec, int max_rec) {
+  g_err.clear();
+  try {
+    Refiner *r = s->r;
+    gcdt_options o = r->opt;
+    int n = gcdt::autotune(o, reps, [r](const gcdt_options &x) {
+      r->opt = x;
+      r->run();
+      return r->stats().device_ms;
+    }, rec, max_rec);
+    r->opt = o;
+    if (best) *best = o;
+    return n;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -2;
+  }
+}
+
 void gcdt_session_destroy(gcdt_session *s) {
   if (!s) return;
   delete s->r;

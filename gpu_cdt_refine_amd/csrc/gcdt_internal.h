@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 #include <utility>
 #include <vector>
@@ -182,4 +183,13 @@ constexpr int FEAT = 16;  // features per candidate (learned priority)
 // Validates options; on failure err holds the reason.
 bool options_ok(const gcdt_options *o, std::string &err);
 
+}  // namespace gcdt
+
+namespace gcdt {
+// Coordinate-descent search over launch shapes and graph structure; `run`
+// refines once with the given options and returns its device time in ms.
+// Returns the number of trials written to rec (all trials if rec is NULL).
+int autotune(gcdt_options &opt, int reps,
+             const std::function<double(const gcdt_options &)> &run,
+             gcdt_tune_record *rec, int max_rec);
 }  // namespace gcdt

@@ -225,6 +225,24 @@ int gcdt_batch_copy(gcdt_batch *b, double *points, int *tris, int *nbrs,
 void gcdt_batch_destroy(gcdt_batch *b);
 const char *gcdt_batch_last_error(void);
 
+/* Autotuner (no counterpart in the reference, whose launch shapes are fixed
+ * at 256-thread blocks, refine.h:19).  Coordinate descent over the launch
+ * shapes of the persistent kernels (grid_cavity, grid_apply, grid_claim) and,
+ * when options.graph = 1, over the captured graph's structure (graph_rounds,
+ * graph_seg_passes), timing `reps` refinements of the resident input per
+ * setting (median device time).  These settings never change the refined
+ * mesh.  The best setting is kept for later runs and returned in *best;
+ * rec (may be NULL) receives up to max_rec trials; returns the number of
+ * trials, or a negative error code. */
+typedef struct gcdt_tune_record {
+  int graph_rounds, graph_seg_passes, grid_cavity, grid_claim, grid_apply;
+  double ms;
+} gcdt_tune_record;
+int gcdt_session_autotune(gcdt_session *s, int reps, gcdt_options *best,
+                          gcdt_tune_record *rec, int max_rec);
+int gcdt_batch_autotune(gcdt_batch *b, int reps, gcdt_options *best,
+                        gcdt_tune_record *rec, int max_rec);
+
 /* Learned priority (options.priority = 2): load the MLP parameters
  * (gcdt_mlp_nparam() floats: W1[16][64] b1[64] W2[64][64] b2[64] W3[64] b3).
  * The model predicts log2(1 + conflict degree) from 16 candidate features and

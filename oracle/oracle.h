@@ -76,6 +76,9 @@ orc_result *orc_refine_batch(int np, const double *points, int nt,
                              const int *mode, const orc_params *prm,
                              orc_stats *st);
 void orc_result_inst(const orc_result *r, int *tri_inst);
+int orc_cavity(int np, const double *points, int nt, const int *tris,
+               const int *nbrs, int ns, const int *segs, int t0,
+               const double *c, int cap, int *out);
 void orc_result_counts(const orc_result *r, int *np, int *nt, int *ns);
 void orc_result_copy(const orc_result *r, double *points, int *tris, int *nbrs,
                      int *segs, uint8_t *pflag);

@@ -78,6 +78,9 @@ def orc():
         L.orc_refine_batch.argtypes = [C.c_int, vp, C.c_int, vp, vp, C.c_int, vp, vp, C.c_int,
                                        vp, vp, vp, C.POINTER(OrcParams), C.POINTER(OrcStats)]
         L.orc_result_inst.argtypes = [vp, vp]
+        L.orc_cavity.argtypes = [C.c_int, vp, C.c_int, vp, vp, C.c_int, vp, C.c_int, vp,
+                                 C.c_int, vp]
+        L.orc_cavity.restype = C.c_int
         L.orc_result_counts.argtypes = [vp, vp, vp, vp]
         L.orc_result_copy.argtypes = [vp, vp, vp, vp, vp, vp]
         L.orc_result_free.argtypes = [vp]
@@ -277,6 +280,20 @@ def oracle_refine_batch(instances, theta, offcenter, mode, **params):
     L.orc_result_inst(r, _p(Io))
     L.orc_result_free(r)
     return split_instances([len(x) for x in Ps], Po, To, No, So, pf, Io), st.as_dict()
+
+
+def cavity(points, triangles, neighbors, segments, t0, c, cap=4096):
+    """Bowyer-Watson cavity of point c (oracle build_cavity, seeded by the
+    walk from triangle t0); sorted triangle ids, or None on failure."""
+    P = np.ascontiguousarray(points, np.float64).reshape(-1, 2)
+    T = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
+    N = np.ascontiguousarray(neighbors, np.int32).reshape(-1, 3)
+    S = np.ascontiguousarray(segments, np.int32).reshape(-1, 2)
+    cc = np.ascontiguousarray(c, np.float64)
+    out = np.empty(cap, np.int32)
+    n = orc().orc_cavity(len(P), _p(P), len(T), _p(T), _p(N), len(S), _p(S), int(t0), _p(cc),
+                         int(cap), _p(out))
+    return None if n < 0 else out[:n].copy()
 
 
 VIOLATION_NAMES = ["range", "neighbors", "orientation", "unsegmented_boundary",

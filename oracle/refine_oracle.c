@@ -780,15 +780,13 @@ static void mark_bad(Mesh *m) {
   }
 }
 
-static orc_result *refine_impl(int np, const double *points, int nt,
-                               const int *tris, const int *nbrs, int ns,
-                               const int *segs, const int *tri_inst,
-                               const double *ipar, const orc_params *prm,
-                               orc_stats *st) {
-  Mesh M;
-  memset(&M, 0, sizeof(M));
-  memset(st, 0, sizeof(*st));
-  Mesh *m = &M;
+/* Working mesh from an input CDT: otri neighbours, subsegments attached to
+ * their edges, initial encroachment (refine.cu:5557-5747, 3105-3178). */
+static void init_mesh(Mesh *m, int np, const double *points, int nt,
+                      const int *tris, const int *nbrs, int ns, const int *segs,
+                      const int *tri_inst, const double *ipar,
+                      const orc_params *prm) {
+  memset(m, 0, sizeof(*m));
   m->ipar = ipar;
   double ga = cos(prm->theta_deg * 3.141592653589793238462643383279502884197169399375105820974944592308 / 180.0);
   m->offconst = (ga == 1.0) ? 0.0 : 0.475 * sqrt((1.0 + ga) / (1.0 - ga));
@@ -876,7 +874,24 @@ static orc_result *refine_impl(int np, const double *points, int nt,
       if (s >= 0 && encroaches(m, m->SV[2 * s], m->SV[2 * s + 1], PT(m, m->TV[3 * t + j])))
         m->senc[s] = 1;
     }
+}
 
+static void free_mesh(Mesh *m) {
+  free(m->P); free(m->pflag); free(m->ppar);
+  free(m->TV); free(m->TN); free(m->TS); free(m->tflag);
+  free(m->SV); free(m->SP); free(m->ST); free(m->senc); free(m->SEG);
+  free(m->tinst);
+}
+
+static orc_result *refine_impl(int np, const double *points, int nt,
+                               const int *tris, const int *nbrs, int ns,
+                               const int *segs, const int *tri_inst,
+                               const double *ipar, const orc_params *prm,
+                               orc_stats *st) {
+  Mesh M;
+  memset(st, 0, sizeof(*st));
+  Mesh *m = &M;
+  init_mesh(m, np, points, nt, tris, nbrs, ns, segs, tri_inst, ipar, prm);
   int K = prm->cav_cap0;
   int prev_win = -1;
   int *ids = NULL;
@@ -972,10 +987,7 @@ static orc_result *refine_impl(int np, const double *points, int nt,
   r->S = malloc(sizeof(int) * 2 * (size_t)(m->ns > 0 ? m->ns : 1));
   memcpy(r->S, m->SV, sizeof(int) * 2 * (size_t)m->ns);
   free(newid);
-  free(m->P); free(m->pflag); free(m->ppar);
-  free(m->TV); free(m->TN); free(m->TS); free(m->tflag);
-  free(m->SV); free(m->SP); free(m->ST); free(m->senc); free(m->SEG);
-  free(m->tinst);
+  free_mesh(m);
   return r;
 }
 
@@ -1007,6 +1019,41 @@ orc_result *orc_refine_batch(int np, const double *points, int nt,
   orc_result *r = refine_impl(np, points, nt, tris, nbrs, ns, segs, tri_inst, ip, prm, st);
   free(ip);
   return r;
+}
+
+/* The Bowyer-Watson cavity of point c in an input CDT, grown from the
+ * triangle the walk from triangle t0 reaches (for tests against brute force):
+ * writes the cavity's triangles (ascending) to out and returns their number,
+ * or -1 if the walk or the cavity check fails, or the cavity exceeds cap. */
+int orc_cavity(int np, const double *points, int nt, const int *tris,
+               const int *nbrs, int ns, const int *segs, int t0,
+               const double *c, int cap, int *out) {
+  orc_params prm;
+  memset(&prm, 0, sizeof(prm));
+  prm.theta_deg = 20.0;
+  Mesh M;
+  init_mesh(&M, np, points, nt, tris, nbrs, ns, segs, NULL, NULL, &prm);
+  Cand cd;
+  memset(&cd, 0, sizeof(cd));
+  cd.c[0] = c[0];
+  cd.c[1] = c[1];
+  cd.L = malloc(sizeof(int) * (size_t)(2 * cap + 2));
+  int tl = -1, jl = -1, sg = -1, n = -1;
+  int w = walk(&M, t0, cd.c, &tl, &jl, &sg);
+  if (w == 0 || w == 1) {
+    int seed1 = -1;
+    if (w == 1 && M.TS[3 * tl + jl] < 0 && M.TN[3 * tl + jl] >= 0) seed1 = M.TN[3 * tl + jl] >> 2;
+    if (w == 0 || seed1 >= 0) {
+      int stt = build_cavity(&M, &cd, tl, seed1, -1, cap);
+      if (stt == ST_VALID) {
+        n = cd.n;
+        memcpy(out, cd.L, sizeof(int) * (size_t)n);
+      }
+    }
+  }
+  free(cd.L);
+  free_mesh(&M);
+  return n;
 }
 
 void orc_result_inst(const orc_result *r, int *tri_inst) {

@@ -96,3 +96,21 @@ def test_graph_session_repeatable(gpu, oracle):
     finally:
         s.close()
     _same(a, b)
+
+
+def test_autotune_keeps_the_mesh(gpu, oracle):
+    """The autotuner only changes launch shapes and the graph's structure:
+    the refined mesh after tuning is the mesh before."""
+    P, T, N, S = _input(oracle, 6000, 300, 61)
+    s = gpu.Session(P, T, N, S, theta=26.0, graph=1, claim_iters=2)
+    try:
+        s.run()
+        a = s.download()
+        best, trials = s.autotune(reps=1)
+        assert len(trials) >= 10
+        assert all(t["ms"] > 0 for t in trials)
+        s.run()
+        b = s.download()
+    finally:
+        s.close()
+    _same(a, b)

@@ -127,3 +127,25 @@ def test_oracle_deterministic(oracle):
     r2 = oracle.oracle_refine(P, T, N, S, theta=28.0)
     for a, b in zip(r1[:4], r2[:4]):
         assert np.array_equal(a, b)
+
+
+def test_cavity_equals_brute_force_conflict_set(oracle):
+    """In a Delaunay triangulation of a convex point set, the Bowyer-Watson
+    cavity of a point is exactly the set of triangles whose circumcircle
+    strictly contains it (exact incircle over every triangle)."""
+    rng = np.random.default_rng(5)
+    pts = rng.uniform(0, 100, size=(400, 2))
+    P, T, N, S = oracle.triangle_cdt(pts, np.zeros((0, 2), np.int32))
+    checked = 0
+    for k in range(60):
+        c = rng.uniform(20, 80, size=2)
+        if k % 3 == 0:  # on a circumcircle-heavy spot: near a vertex
+            c = P[rng.integers(len(P))] + rng.normal(scale=1e-3, size=2)
+        cav = oracle.cavity(P, T, N, S, int(rng.integers(len(T))), c)
+        if cav is None:
+            continue
+        brute = [t for t in range(len(T))
+                 if oracle.incircle(P[T[t, 0]], P[T[t, 1]], P[T[t, 2]], c) > 0]
+        assert cav.tolist() == brute, (k, c)
+        checked += 1
+    assert checked >= 50
