@@ -1,22 +1,30 @@
 """Benchmark: CDT quality refinement on one MI355X.
 
-Workload (the headline single-GPU configuration): 1,000,000 uniform FP64
-points in [0, 1e6]^2 plus 50,000 random non-crossing segments no longer than
-2% of the domain width, convex hull enclosed by segments, constrained
+Default workload (the headline single-GPU configuration): 1,000,000 uniform
+FP64 points in [0, 1e6]^2 plus 50,000 random non-crossing segments no longer
+than 2% of the domain width, convex hull enclosed by segments, constrained
 Delaunay triangulated on the host (untimed input preparation), then refined
 at theta = 25 deg with off-centers in Ruppert mode.
 
 A "step" is one full refinement of that CDT (GPU_Refine_Quality's work),
 starting from the CDT already resident in HBM and ending with the compacted
-output mesh in HBM.  value = output triangles / second of wall time.
+output mesh in HBM.  Two forms are timed in the same run:
+  plain   FP64-filtered exact predicates, one claim iteration, host-driven
+          passes (the reference's structure)
+  chosen  FP32 -> FP64 -> double-double -> exact predicate cascade, claim
+          iterations, graph-driven persistent loop with autotuned launch
+          shapes, learned claim priority (when the trained model is present)
+value = output triangles / second of wall time of the chosen form.
 
-Prints ONE JSON line (rank 0).  Also reports:
+Prints ONE JSON line.  Also reports:
   roofline      the dominant kernel's algorithmic bytes per launch / its
-                average HIP-event duration, against the 8 TB/s HBM peak
+                average HIP-event duration (events on the refinement stream),
+                against the 8 TB/s HBM peak
   whole_job     the whole refinement's algorithmic bytes (record-size model)
                 over its wall time
   cpu_baseline  the reference's Triangle "-q" path (compiled from its own
-                source into oracle/_ref) on the same CDT, one host core
+                source into oracle/_ref) on a bounded sample, one host core
+Other workloads (--workload learned | batch | precision) print their own line.
 """
 import argparse
 import json
@@ -36,44 +44,100 @@ B_POINT, B_TRI, B_SUBSEG = 16, 36, 12
 B_OUT_TRI, B_OUT_SEG = 24, 8
 B_PER_INSERT = 424  # 3 parent vertices + 4-triangle cavity read, 6-triangle star + point written
 
-# Algorithmic bytes per work item for each kernel:
+# Algorithmic bytes per work item of each timed kernel group:
 KERNEL_BYTES = {
-    # read parent record+vertices (36+48), 4 cavity records (144) + their new
-    # vertices (32), write cavity+ring list (10*4) and candidate record (40)
-    "bad_cavity": 36 + 48 + 144 + 32 + 40 + 40,
-    "seg_cavity": 12 + 32 + 72 + 144 + 32 + 40 + 40,
+    # parent record + 3 vertices, 4 cavity records + their opposite vertices,
+    # cavity + ring ids written, candidate record written
+    "bad_cavity": 36 + 48 + 4 * 36 + 4 * 8 + 40 + 40,
+    "seg_cavity": 12 + 32 + 2 * 36 + 4 * 36 + 4 * 8 + 40 + 40,
     "apply": B_PER_INSERT,
     "claim": 10 * 4 + 10 * 8,   # list read + 64-bit claim RMW
     "win": 10 * 4 + 10 * 8 + 4,
-    "mark_bad": 1 + 12 + 48 + 1,
+    "mark_bad": 1 + 16 + 3 * 16 + 1,
     "compact": 2 * 1,
-    "export": 1 + 4 + 24 + 24 + 12,
+    "export": 1 + 4 + 16 + 16 + 24,
     "init": 24 + 24 + 48 + 8,
 }
 
-
-def workload_config(args):
-    return {
-        "workload": "cdt-refine-1M-uniform-50k-segs",
-        "points": args.points,
-        "segments": args.segments,
-        "domain": args.domain,
-        "max_segment_length": args.domain * 0.02,
-        "theta_deg": args.theta,
-        "insert": "offcenter",
-        "mode": "ruppert",
-        "seed": args.seed,
-    }
+MODEL = os.path.join(ROOT, "gpu_cdt_refine_amd", "priority_mlp.npy")
 
 
-def make_input(args):
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_input(n, nseg, L, maxlen, dist, seed):
     import gpu_cdt_refine_amd as g
-    t0 = time.time()
-    P, S = g.generate_pslg(args.points, args.segments, L=args.domain,
-                           maxlen=0.02 * args.domain, dist=0, min_angle=5.0,
-                           seed=args.seed)
+    P, S = g.generate_pslg(n, nseg, L=L, maxlen=maxlen, dist=dist, min_angle=5.0, seed=seed)
     T, N, SS = g.build_cdt(P, S)
-    return P, T, N, SS, time.time() - t0
+    return P, T, N, SS
+
+
+def plain_options():
+    return dict(pred_mode=0, claim_iters=1, priority=1, graph=0)
+
+
+def chosen_options(args):
+    o = dict(pred_mode=1, claim_iters=args.claim_iters, priority=1, graph=1)
+    if args.learned and os.path.exists(MODEL):
+        o["priority"] = 2
+    return o
+
+
+def timed_runs(sess, steps, warmup, tag):
+    for i in range(warmup):
+        st = sess.run()
+        log(f"# {tag} warmup {i}: {st['device_ms']:.1f} ms device, {st['n_triangles']} tris, "
+            f"rounds {st['rounds']}")
+    walls, devs = [], []
+    for i in range(steps):
+        t0 = time.perf_counter()
+        st = sess.run()
+        walls.append(time.perf_counter() - t0)
+        devs.append(st["device_ms"])
+        log(f"# {tag} step {i}: wall {walls[-1] * 1e3:.1f} ms, device {st['device_ms']:.1f} ms")
+    return st, float(np.mean(walls)) * 1e3, float(np.mean(devs))
+
+
+def summary(st):
+    keys = ["n_points", "n_triangles", "n_subsegs", "rounds", "seg_passes", "bad_passes", "status",
+            "giveups", "cand_evals", "lost", "big_cavities", "inserted", "seg_splits",
+            "pred_fp32", "pred_fp64", "pred_dd", "pred_exact"]
+    return {k: st[k] for k in keys}
+
+
+def whole_job_bytes(npts, ntri, nseg, st):
+    n_ins = int(st["inserted"] + st["seg_splits"])
+    return (B_POINT * npts + B_TRI * ntri + B_SUBSEG * nseg + B_PER_INSERT * n_ins +
+            B_POINT * st["n_points"] + B_OUT_TRI * st["n_triangles"] + B_OUT_SEG * st["n_subsegs"])
+
+
+def profile_kernels(P, T, N, SS, theta, opts):
+    """One refinement with a HIP-event pair around every launch (host-driven
+    passes: the same kernels and work as the graph loop, same mesh)."""
+    import gpu_cdt_refine_amd as g
+    o = dict(opts)
+    o["graph"] = 0
+    prof = g.Session(P, T, N, SS, theta=theta, insert=g.Offcenter, mode=g.RUPPERT, profile=1, **o)
+    if o.get("priority") == 2:
+        prof.set_priority_model(np.load(MODEL))
+    prof.run()
+    kt = prof.kernel_times()
+    prof.close()
+    return kt
+
+
+def roofline(kt):
+    dom = max((k for k in kt if k in KERNEL_BYTES), key=lambda k: kt[k]["total_ms"])
+    d = kt[dom]
+    per_launch_ms = d["total_ms"] / max(1, d["launches"])
+    bytes_per_launch = KERNEL_BYTES[dom] * d["items"] / max(1, d["launches"])
+    achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "avg_launch_ms": per_launch_ms, "launches": d["launches"],
+            "bytes_per_item": KERNEL_BYTES[dom],
+            "items_per_launch": d["items"] / max(1, d["launches"])}
 
 
 def cpu_baseline(args, n_sample):
@@ -83,13 +147,11 @@ def cpu_baseline(args, n_sample):
     reference's CPU_Triangle_Quality switches (main.cpp:389-407)."""
     try:
         from oracle import pyoracle
-        import gpu_cdt_refine_amd as g
         if not os.path.exists(pyoracle.TRIREF_SO):
             return None
         L = args.domain * (n_sample / args.points) ** 0.5
-        P, S = g.generate_pslg(n_sample, int(args.segments * n_sample / args.points), L=L,
-                               maxlen=0.02 * args.domain, dist=0, min_angle=5.0, seed=args.seed)
-        T, N, SS = g.build_cdt(P, S)
+        P, T, N, SS = make_input(n_sample, int(args.segments * n_sample / args.points), L,
+                                 0.02 * args.domain, 0, args.seed)
         t0 = time.time()
         Po, To, No, So = pyoracle.triangle_refine(P, T, SS, args.theta, mode=1)
         dt = time.time() - t0
@@ -104,109 +166,246 @@ def cpu_baseline(args, n_sample):
         return {"error": str(e)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--points", type=int, default=1_000_000)
-    ap.add_argument("--segments", type=int, default=50_000)
-    ap.add_argument("--domain", type=float, default=1e6)
-    ap.add_argument("--theta", type=float, default=25.0)
-    ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--pred-mode", type=int, default=0)
-    ap.add_argument("--priority", type=int, default=1)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-points", type=int, default=200_000)
-    ap.add_argument("--claim-iters", type=int, default=1)
-    ap.add_argument("--no-profile-step", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is a design note only)"}))
-        sys.exit(2)
-
+def bench_refine(args):
     import gpu_cdt_refine_amd as g
-    P, T, N, SS, prep_s = make_input(args)
-    print(f"# input: {len(P)} points, {len(T)} triangles, {len(SS)} segments "
-          f"(prep {prep_s:.1f}s)", file=sys.stderr, flush=True)
-    sess = g.Session(P, T, N, SS, theta=args.theta, insert=g.Offcenter, mode=g.RUPPERT,
-                     pred_mode=args.pred_mode, priority=args.priority,
-                     claim_iters=args.claim_iters)
-    for i in range(args.warmup):
-        st = sess.run()
-        print(f"# warmup {i}: {st['device_ms']:.1f} ms device, {st['n_triangles']} tris, "
-              f"rounds {st['rounds']}", file=sys.stderr, flush=True)
-    walls, devs = [], []
-    for i in range(args.steps):
-        t0 = time.perf_counter()
-        st = sess.run()
-        walls.append(time.perf_counter() - t0)
-        devs.append(st["device_ms"])
-        print(f"# step {i}: wall {walls[-1]*1e3:.1f} ms, device {st['device_ms']:.1f} ms",
-              file=sys.stderr, flush=True)
-    ms = 1e3 * float(np.mean(walls))
+    t0 = time.time()
+    P, T, N, SS = make_input(args.points, args.segments, args.domain, 0.02 * args.domain, 0,
+                             args.seed)
+    log(f"# input: {len(P)} points, {len(T)} triangles, {len(SS)} segments "
+        f"(prep {time.time() - t0:.1f}s)")
+    kw = dict(theta=args.theta, insert=g.Offcenter, mode=g.RUPPERT)
+    # plain GPU path
+    sp = g.Session(P, T, N, SS, **kw, **plain_options())
+    st_p, ms_p, dev_p = timed_runs(sp, args.steps, args.warmup, "plain")
+    sp.close()
+    # chosen form
+    co = chosen_options(args)
+    sc = g.Session(P, T, N, SS, **kw, **co)
+    if co["priority"] == 2:
+        sc.set_priority_model(np.load(MODEL))
+    tune = None
+    if not args.no_autotune:
+        best, trials = sc.autotune(reps=1)
+        tune = {"best": best, "trials": len(trials)}
+        log(f"# autotune: {best}")
+    st, ms, dev = timed_runs(sc, args.steps, args.warmup, "chosen")
+    sc.close()
     tris = int(st["n_triangles"])
     value = tris / (ms / 1e3)
-
-    # algorithmic bytes of the whole refinement (record-size model)
-    n_ins = int(st["inserted"] + st["seg_splits"])
-    b_alg = (B_POINT * len(P) + B_TRI * len(T) + B_SUBSEG * len(SS) + B_PER_INSERT * n_ins +
-             B_POINT * st["n_points"] + B_OUT_TRI * tris + B_OUT_SEG * st["n_subsegs"])
+    b_alg = whole_job_bytes(len(P), len(T), len(SS), st)
     whole = {"bytes": b_alg, "achieved_GBs": b_alg / (ms / 1e3) / 1e9,
              "frac": (b_alg / (ms / 1e3) / 1e9) / HBM_PEAK_GBS}
-
-    roof = None
-    kt = None
+    roof, kt = None, None
     if not args.no_profile_step:
-        prof = g.Session(P, T, N, SS, theta=args.theta, insert=g.Offcenter, mode=g.RUPPERT,
-                         pred_mode=args.pred_mode, priority=args.priority,
-                         claim_iters=args.claim_iters, profile=1)
-        prof.run()
-        kt = prof.kernel_times()
-        prof.close()
-        dom = max((k for k in kt if k in KERNEL_BYTES), key=lambda k: kt[k]["total_ms"])
-        d = kt[dom]
-        per_launch_ms = d["total_ms"] / max(1, d["launches"])
-        bytes_per_launch = KERNEL_BYTES[dom] * d["items"] / max(1, d["launches"])
-        achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
-        roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "avg_launch_ms": per_launch_ms, "launches": d["launches"],
-                "bytes_per_item": KERNEL_BYTES[dom]}
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args, args.cpu_sample_points)
-
+        kt = profile_kernels(P, T, N, SS, args.theta, co)
+        roof = roofline(kt)
+    dropin = None
+    if not args.no_dropin:
+        # PCIe-inclusive: host CDT in, host mesh out (the drop-in's contract)
+        t1 = time.perf_counter()
+        r = g.refine(P, T, N, SS, **kw, **{k: v for k, v in co.items() if k != "priority"})
+        dt = time.perf_counter() - t1
+        dropin = {"value": len(r.triangles) / dt, "ms": dt * 1e3,
+                  "note": "host arrays in and out (upload, refine, download), priority 1"}
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args, args.cpu_sample_points)
     out = {
-        "metric": "output triangles/sec (refine, theta=25 deg, off-center, Ruppert)",
+        "metric": "output triangles/sec (refine, theta=25 deg, off-center, Ruppert): chosen form",
         "value": value,
         "unit": "triangles/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms,
-        "device_ms_per_step": float(np.mean(devs)),
+        "device_ms_per_step": dev,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded uniform PSLG, own CDT builder)",
-        "config": workload_config(args),
-        "output": {"points": int(st["n_points"]), "triangles": tris,
-                   "subsegments": int(st["n_subsegs"]), "rounds": st["rounds"],
-                   "seg_passes": st["seg_passes"], "bad_passes": st["bad_passes"],
-                   "status": st["status"], "giveups": st["giveups"],
-                   "cand_evals": st["cand_evals"], "lost": st["lost"],
-                   "big_cavities": st["big_cavities"], "inserted": st["inserted"],
-                   "seg_splits": st["seg_splits"]},
+        "data": "synthetic (seeded uniform PSLG, own CDT builder identical to Triangle pzQnc)",
+        "config": {"workload": "cdt-refine-1M-uniform-50k-segs", "points": args.points,
+                   "segments": args.segments, "domain": args.domain,
+                   "max_segment_length": args.domain * 0.02, "theta_deg": args.theta,
+                   "insert": "offcenter", "mode": "ruppert", "seed": args.seed,
+                   "chosen_options": co, "autotune": tune},
+        "plain": {"value": tris and st_p["n_triangles"] / (ms_p / 1e3), "ms_per_step": ms_p,
+                  "device_ms_per_step": dev_p, "output": summary(st_p)},
+        "speedup_vs_plain": ms_p / ms,
+        "steiner_delta_vs_plain": (st["n_points"] - st_p["n_points"]) / max(1, st_p["n_points"] - len(P)),
+        "output": summary(st),
         "roofline": roof,
         "whole_job": whole,
+        "dropin_pcie_inclusive": dropin,
         "cpu_baseline": cpu,
         "kernel_times_ms": {k: round(v["total_ms"], 3) for k, v in kt.items()} if kt else None,
     }
     print(json.dumps(out))
+
+
+def bench_learned(args):
+    """Clustered 2M-point PSLG, 100k segments, theta 28: plain priority vs the
+    learned priority, both with the chosen form's other settings."""
+    import gpu_cdt_refine_amd as g
+    n = args.points if args.points != 1_000_000 else 2_000_000
+    nseg = n // 20
+    P, T, N, SS = make_input(n, nseg, args.domain, 0.02 * args.domain, 1, args.seed + 1000)
+    log(f"# input: {len(P)} points, {len(T)} triangles, {len(SS)} segments")
+    res = {}
+    for name, prio in (("plain_priority", 1), ("learned_priority", 2)):
+        o = chosen_options(args)
+        o["priority"] = prio
+        s = g.Session(P, T, N, SS, theta=28.0, **o)
+        if prio == 2:
+            s.set_priority_model(np.load(MODEL))
+        st, ms, dev = timed_runs(s, args.steps, args.warmup, name)
+        s.close()
+        res[name] = {"ms_per_step": ms, "value": st["n_triangles"] / (ms / 1e3), "output": summary(st)}
+    a, b = res["plain_priority"], res["learned_priority"]
+    out = {"metric": "refine time, learned vs plain claim priority (2M clustered, theta=28)",
+           "value": b["value"], "unit": "triangles/s", "n_gpus": 1, "higher_is_better": True,
+           "config": {"workload": "cdt-refine-2M-clustered-100k-segs-theta28", "points": n,
+                      "segments": nseg, "seed": args.seed + 1000},
+           "speedup": a["ms_per_step"] / b["ms_per_step"],
+           "steiner_delta": (b["output"]["n_points"] - a["output"]["n_points"]) /
+                            max(1, a["output"]["n_points"] - len(P)),
+           "rounds": [a["output"]["rounds"], b["output"]["rounds"]], "runs": res}
+    print(json.dumps(out))
+
+
+def batch_instances(k, seed, max_points):
+    """Config-4 family: n log-uniform in [200, max_points], uniform or
+    Gaussian-clustered 50/50, 5% segments, theta U[20.7, 30], random mode."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(200), np.log(max_points), size=k)).astype(int)
+    dist = rng.integers(0, 2, size=k)
+    theta = rng.uniform(20.7, 30.0, size=k)
+    mode = rng.integers(0, 2, size=k)
+    return ns, dist, theta, mode
+
+
+def _prep_instance(a):
+    n, d, s = a
+    L = 1000.0 * (n / 10000) ** 0.5
+    return make_input(int(n), max(1, int(0.05 * n)), L, 0.05 * L, int(d), int(s))
+
+
+def bench_batch(args):
+    import gpu_cdt_refine_amd as g
+    from concurrent.futures import ProcessPoolExecutor
+    ns, dist, theta, mode = batch_instances(args.instances, args.seed, args.batch_max_points)
+    t0 = time.time()
+    jobs = [(int(n), int(d), 7000 + i) for i, (n, d) in enumerate(zip(ns, dist))]
+    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        ins = list(ex.map(_prep_instance, jobs, chunksize=8))
+    log(f"# {len(ins)} instances, {sum(len(x[0]) for x in ins)} points (prep {time.time() - t0:.1f}s)")
+    co = chosen_options(args)
+    co["priority"] = 1
+    b = g.Batch(ins, theta=theta, insert=1, mode=mode, **co)
+    best = None
+    if not args.no_autotune:
+        best, trials = b.autotune(reps=1)
+        log(f"# autotune: {best}")
+    for i in range(args.warmup):
+        b.run()
+    runs = []
+    for i in range(args.steps):
+        st = b.run()
+        runs.append(st["run_ms"])
+        log(f"# batch step {i}: {st['run_ms']:.1f} ms, {st['n_triangles']} tris")
+    ms = float(np.mean(runs))
+    res = b.results()
+    b.close()
+    out = {"metric": "batched refinement: output triangles/sec over all instances",
+           "value": st["n_triangles"] / (ms / 1e3), "unit": "triangles/s", "n_gpus": 1,
+           "ms_per_step": ms, "higher_is_better": True,
+           "config": {"workload": f"batch-{args.instances}-pslgs", "instances": args.instances,
+                      "max_points": args.batch_max_points, "waves": None, "autotune": best},
+           "output": summary(st)}
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle
+        sample = list(range(0, len(ins), max(1, len(ins) // 64)))
+        t1 = time.time()
+        tris = 0
+        for i in sample:
+            P, T, N, SS = ins[i]
+            Po, To, No, So = pyoracle.triangle_refine(P, T, SS, float(theta[i]), mode=int(mode[i]))
+            tris += len(To)
+        dt = time.time() - t1
+        out["cpu_baseline"] = {"value": tris / dt, "unit": "triangles/s", "cores": 1,
+                               "kind": "reference",
+                               "sample": f"Triangle -q on {len(sample)} of the instances, one core"}
+    print(json.dumps(out))
+
+
+def bench_precision(args):
+    """Precision stress: lattice + cocircular + uniform points, lattice-line
+    segments; stage hit rates of the predicate cascade and Steiner-point error
+    of the FP64 constructions against exact rational arithmetic."""
+    import gpu_cdt_refine_amd as g
+    from fractions import Fraction
+    from oracle import pyoracle
+    n = args.points
+    P, T, N, SS = make_input(n, n // 50, 1000.0, 20.0, 2, args.seed)
+    log(f"# input: {len(P)} points, {len(T)} triangles, {len(SS)} segments")
+    s = g.Session(P, T, N, SS, theta=25.0, pred_mode=1, claim_iters=args.claim_iters, graph=1)
+    st, ms, dev = timed_runs(s, args.steps, args.warmup, "precision")
+    r = s.download()
+    s.close()
+    tot_pred = st["pred_fp32"] + st["pred_fp64"] + st["pred_dd"] + st["pred_exact"]
+    # Steiner error on a sample of the input's skinny triangles
+    rng = np.random.default_rng(0)
+    idx = rng.choice(len(T), min(2000, len(T)), replace=False)
+    errs = []
+    for t in idx:
+        a, o, d = P[T[t, 0]], P[T[t, 1]], P[T[t, 2]]
+        c = pyoracle.steiner_point(a, o, d, 25.0, 1)
+        if c is None:
+            continue
+        ex = pyoracle.steiner_point_exact(a, o, d, 25.0, 1)
+        err = max(abs(Fraction(c[0]) - ex[0]), abs(Fraction(c[1]) - ex[1]))
+        scale = max(abs(float(ex[0])), abs(float(ex[1])), 1e-300)
+        errs.append((float(err), float(err) / (np.spacing(scale))))
+    errs = np.array(errs)
+    out = {"metric": "precision stress: predicate stage hit rates and Steiner error",
+           "value": st["n_triangles"] / (ms / 1e3), "unit": "triangles/s", "n_gpus": 1,
+           "ms_per_step": ms, "higher_is_better": True,
+           "config": {"workload": f"precision-stress-{n}", "points": n, "segments": len(SS)},
+           "stage_rates": {k: st[k] / max(1, tot_pred) for k in
+                           ("pred_fp32", "pred_fp64", "pred_dd", "pred_exact")},
+           "stage_counts": {k: st[k] for k in ("pred_fp32", "pred_fp64", "pred_dd", "pred_exact")},
+           "steiner_error": {"samples": int(len(errs)), "max_abs": float(errs[:, 0].max()),
+                             "median_ulps": float(np.median(errs[:, 1])),
+                             "max_ulps": float(errs[:, 1].max())},
+           "output": summary(st)}
+    print(json.dumps(out))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="refine", choices=["refine", "learned", "batch", "precision"])
+    ap.add_argument("--points", type=int, default=1_000_000)
+    ap.add_argument("--segments", type=int, default=50_000)
+    ap.add_argument("--domain", type=float, default=1e6)
+    ap.add_argument("--theta", type=float, default=25.0)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--claim-iters", type=int, default=3)
+    ap.add_argument("--learned", type=int, default=1)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile-step", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true")
+    ap.add_argument("--cpu-sample-points", type=int, default=200_000)
+    ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--batch-max-points", type=int, default=200_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is a design note only)"}))
+        sys.exit(2)
+    {"refine": bench_refine, "learned": bench_learned, "batch": bench_batch,
+     "precision": bench_precision}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -282,6 +282,52 @@ def oracle_refine_batch(instances, theta, offcenter, mode, **params):
     return split_instances([len(x) for x in Ps], Po, To, No, So, pf, Io), st.as_dict()
 
 
+def steiner_point(apex, org, dest, theta, offcenter=1):
+    """FP64 circumcenter / off-center as the refinement computes it
+    (bit-identical to the device construction); None if degenerate."""
+    a, o, d = (np.ascontiguousarray(x, np.float64) for x in (apex, org, dest))
+    out = np.empty(2)
+    ok = orc().orc_steiner_point(_p(a), _p(o), _p(d), float(theta), int(offcenter), _p(out))
+    return out if ok else None
+
+
+def steiner_point_exact(apex, org, dest, theta, offcenter=1):
+    """The same construction in exact rational arithmetic (the off-center
+    constant is the FP64 value the refinement uses): circumcenter relative to
+    org, off-center on the shortest edge's bisector when it is closer
+    (triangle.cpp:6523-6618)."""
+    from fractions import Fraction as F
+    ga = np.cos(theta * np.pi / 180.0)
+    offc = F(0.0) if ga == 1.0 else F(float(0.475 * np.sqrt((1.0 + ga) / (1.0 - ga))))
+    A = (F(float(apex[0])), F(float(apex[1])))
+    O = (F(float(org[0])), F(float(org[1])))
+    D = (F(float(dest[0])), F(float(dest[1])))
+    xdo, ydo = D[0] - O[0], D[1] - O[1]
+    xao, yao = A[0] - O[0], A[1] - O[1]
+    dodist = xdo * xdo + ydo * ydo
+    aodist = xao * xao + yao * yao
+    dadist = (D[0] - A[0]) ** 2 + (D[1] - A[1]) ** 2
+    det = xdo * yao - xao * ydo
+    den = F(1, 2) / det
+    dx = (yao * dodist - ydo * aodist) * den
+    dy = (xdo * aodist - xao * dodist) * den
+    if offcenter and offc > 0:
+        if dodist < aodist and dodist < dadist:
+            ox, oy = F(1, 2) * xdo - offc * ydo, F(1, 2) * ydo + offc * xdo
+            if ox * ox + oy * oy < dx * dx + dy * dy:
+                dx, dy = ox, oy
+        elif aodist < dadist:
+            ox, oy = F(1, 2) * xao + offc * yao, F(1, 2) * yao - offc * xao
+            if ox * ox + oy * oy < dx * dx + dy * dy:
+                dx, dy = ox, oy
+        else:
+            ox = F(1, 2) * (A[0] - D[0]) - offc * (A[1] - D[1])
+            oy = F(1, 2) * (A[1] - D[1]) + offc * (A[0] - D[0])
+            if ox * ox + oy * oy < (dx - xdo) ** 2 + (dy - ydo) ** 2:
+                dx, dy = xdo + ox, ydo + oy
+    return (O[0] + dx, O[1] + dy)
+
+
 def cavity(points, triangles, neighbors, segments, t0, c, cap=4096):
     """Bowyer-Watson cavity of point c (oracle build_cavity, seeded by the
     walk from triangle t0); sorted triangle ids, or None on failure."""

@@ -8,7 +8,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-FIXED_K = dict(cav_cap0=256, cav_cap_max=256)
+FIXED_K = dict(cav_cap0=256, cav_cap_max=256, cand_min=1 << 29)
 
 
 def _instances(oracle, specs):

@@ -149,3 +149,30 @@ def test_cavity_equals_brute_force_conflict_set(oracle):
         assert cav.tolist() == brute, (k, c)
         checked += 1
     assert checked >= 50
+
+
+def test_steiner_error_against_exact_rationals(oracle):
+    """FP64 Steiner points vs the same construction in exact rational
+    arithmetic, on the CDT of the precision-stress input (jittered lattice,
+    cocircular points): the error stays within a few ulps of the
+    coordinates' magnitude for well-shaped triangles and bounded relative to
+    the triangle's size for skinny ones."""
+    from fractions import Fraction
+    import gpu_cdt_refine_amd as g
+    P, S = g.generate_pslg(6000, 120, L=1000.0, maxlen=20.0, dist=2, min_angle=5.0, seed=3)
+    T, N, SS = g.build_cdt(P, S)
+    rng = np.random.default_rng(0)
+    worst_rel = 0.0
+    n = 0
+    for t in rng.choice(len(T), 400, replace=False):
+        a, o, d = P[T[t, 0]], P[T[t, 1]], P[T[t, 2]]
+        c = oracle.steiner_point(a, o, d, 25.0, 1)
+        if c is None:
+            continue
+        ex = oracle.steiner_point_exact(a, o, d, 25.0, 1)
+        size = max(np.hypot(*(o - a)), np.hypot(*(d - a)), np.hypot(*(d - o)))
+        err = max(abs(float(Fraction(c[0]) - ex[0])), abs(float(Fraction(c[1]) - ex[1])))
+        worst_rel = max(worst_rel, err / size)
+        n += 1
+    assert n > 300
+    assert worst_rel < 1e-9, worst_rel
