@@ -47,6 +47,7 @@ class Options(C.Structure):
         ("grid_cavity", C.c_int),
         ("grid_claim", C.c_int),
         ("grid_apply", C.c_int),
+        ("cavity_lanes", C.c_int),
     ]
 
 
@@ -57,11 +58,13 @@ class TuneRecord(C.Structure):
         ("grid_cavity", C.c_int),
         ("grid_claim", C.c_int),
         ("grid_apply", C.c_int),
+        ("cavity_lanes", C.c_int),
         ("ms", C.c_double),
     ]
 
 
-TUNE_KEYS = ("graph_rounds", "graph_seg_passes", "grid_cavity", "grid_claim", "grid_apply")
+TUNE_KEYS = ("graph_rounds", "graph_seg_passes", "grid_cavity", "grid_claim", "grid_apply",
+             "cavity_lanes")
 
 
 def _tune_result(n, best, rec):

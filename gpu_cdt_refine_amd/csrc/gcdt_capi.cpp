@@ -55,6 +55,7 @@ void gcdt_default_options(gcdt_options *o) {
   o->grid_cavity = 8192;
   o->grid_claim = 2048;
   o->grid_apply = 4096;
+  o->cavity_lanes = 32;
 }
 
 }  // extern "C"
@@ -77,6 +78,10 @@ bool options_ok(const gcdt_options *o, std::string &g_err) {
   if (o->grid_cavity < 1 || o->grid_cavity > 16384 || o->grid_claim < 1 || o->grid_claim > 65536 ||
       o->grid_apply < 1 || o->grid_apply > 65536) {
     g_err = "grid sizes must be in [1, 16384] (cavity) and [1, 65536] (claim, apply)";
+    return false;
+  }
+  if (o->cavity_lanes != 8 && o->cavity_lanes != 16 && o->cavity_lanes != 32 && o->cavity_lanes != 64) {
+    g_err = "cavity_lanes must be 8, 16, 32 or 64";
     return false;
   }
   if (o->graph && o->log_training) { g_err = "log_training needs the host-driven loop (graph = 0)"; return false; }

@@ -53,6 +53,7 @@ enum : int {
   CNT_VALID,
   CNT_BIG,
   CNT_DD,
+  CNT_SLOW,  // candidates deferred to the exact-predicate kernel
   CNT_N = 16
 };
 
@@ -155,6 +156,7 @@ struct PassD {
   // graph-driven loop: counts, epochs and the candidate list come from ctl
   Ctl *ctl;
   const int *ids_sel;  // selected candidates when ctl->i[C_USESEL]
+  int *slow;           // candidates whose filters could not decide a sign
 };
 
 __device__ __forceinline__ int pass_nc(const PassD &ps) {
@@ -368,33 +370,28 @@ __device__ __forceinline__ bool split_point(const MeshD &m, int s, double &px,
   return true;
 }
 
-__device__ __forceinline__ bool in_list(const int *L, int n, int x) {
-  for (int i = 0; i < n; i++)
-    if (L[i] == x) return true;
-  return false;
-}
-
-__device__ __forceinline__ bool is_bnd(const MeshD &m, int t, int j,
-                                       int split_s, const int *L, int n) {
-  int s = comp(m.TS[t], j);
-  int nb = comp(m.TN[t], j);
-  if (s >= 0 && s != split_s) return true;
-  if (nb < 0) return true;
-  return !in_list(L, n, nb >> 2);
-}
-
 // Orientation walk toward (cx,cy), stopping at subsegments
 // (refine.cu:3908-4184).  0 inside, 1 on edge jout, 2 on a vertex,
-// 3 blocked by subsegment seg, 4 failed.
+// 3 blocked by subsegment seg, 4 failed, 5 (FAST only) a sign the filters
+// could not decide.
+template <bool FAST>
 __device__ __forceinline__ int walk(const MeshD &m, int t, double cx,
                                     double cy, bool fp32, PredStats *st,
                                     int &tout, int &jout, int &seg) {
   for (int steps = 0; steps < GCDT_WALK_LIMIT; steps++) {
     int4 v = m.TV[t];
     double2 p0 = m.P[v.x], p1 = m.P[v.y], p2 = m.P[v.z];
-    int o0 = orient2d_sign(p1.x, p1.y, p2.x, p2.y, cx, cy, fp32, st);
-    int o1 = orient2d_sign(p2.x, p2.y, p0.x, p0.y, cx, cy, fp32, st);
-    int o2 = orient2d_sign(p0.x, p0.y, p1.x, p1.y, cx, cy, fp32, st);
+    int o0, o1, o2;
+    if (FAST) {
+      o0 = orient2d_filtered(p1.x, p1.y, p2.x, p2.y, cx, cy, fp32, st);
+      o1 = orient2d_filtered(p2.x, p2.y, p0.x, p0.y, cx, cy, fp32, st);
+      o2 = orient2d_filtered(p0.x, p0.y, p1.x, p1.y, cx, cy, fp32, st);
+      if (o0 == PRED_UNSURE || o1 == PRED_UNSURE || o2 == PRED_UNSURE) return 5;
+    } else {
+      o0 = orient2d_sign(p1.x, p1.y, p2.x, p2.y, cx, cy, fp32, st);
+      o1 = orient2d_sign(p2.x, p2.y, p0.x, p0.y, cx, cy, fp32, st);
+      o2 = orient2d_sign(p0.x, p0.y, p1.x, p1.y, cx, cy, fp32, st);
+    }
     int jn = o0 < 0 ? 0 : (o1 < 0 ? 1 : (o2 < 0 ? 2 : -1));
     if (jn < 0) {
       int zeros = (o0 == 0) + (o1 == 0) + (o2 == 0);
@@ -413,20 +410,6 @@ __device__ __forceinline__ int walk(const MeshD &m, int t, double cx,
     t = nb >> 2;
   }
   return 4;
-}
-
-__device__ __forceinline__ void next_bnd(const MeshD &m, int t, int j,
-                                         int split_s, const int *L, int n,
-                                         int &tn, int &jn) {
-  int cur = t, k = (j + 1) % 3;
-  for (int guard = 0; guard < 4 * n + 8; guard++) {
-    if (is_bnd(m, cur, k, split_s, L, n)) { tn = cur; jn = k; return; }
-    int nb = comp(m.TN[cur], k);
-    cur = nb >> 2;
-    k = ((nb & 3) + 1) % 3;
-  }
-  tn = t;  // unreachable for a disk-shaped cavity (checked before claims)
-  jn = j;
 }
 
 }  // namespace gcdt

@@ -91,6 +91,8 @@ class Refiner {
   void loop_host();
   void size_pass_buffers();
   unsigned long long *ctl_cnt_ptr() const;
+  int cavity_lanes(int Kc) const;
+  void launch_cavity(bool is_seg, int grid, const PassD &ps);
   int select_by_priority(int *ids, int nb, int cap, int **out_ids);
   void ensure_capacity(long long needP, long long needT, long long needS);
   void ensure_cand(int nc);
@@ -117,7 +119,7 @@ class Refiner {
   int c_cap = 0;
   double2 *c_pt = nullptr;
   int *c_status = nullptr, *c_n = nullptr, *c_r = nullptr, *c_prio = nullptr,
-      *c_win = nullptr, *c_rank = nullptr, *c_act = nullptr;
+      *c_win = nullptr, *c_rank = nullptr, *c_act = nullptr, *c_slow = nullptr;
   int *c_ws = nullptr;
   long long ws_ints = 0;
   double *c_pws = nullptr;

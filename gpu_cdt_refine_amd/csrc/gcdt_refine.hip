@@ -299,31 +299,70 @@ __device__ __forceinline__ void mark_enc(const MeshD &m, int s,
                                          unsigned long long *cnt) {
   if (atomicCAS(&m.senc[s], 0, 1) == 0) atomicAdd(&cnt[CNT_MARKS], 1ull);
 }
+// Marks subsegment s encroached; true if this call marked it.
+__device__ __forceinline__ bool mark_enc_local(const MeshD &m, int s) {
+  return atomicCAS(&m.senc[s], 0, 1) == 0;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+  return x;
+}
+
+// Per-lane event counts of the cavity kernel, summed over the wave and
+// added to the pass counters once per wave: a counter shared by every
+// candidate would otherwise serialise on one L2 address.
+enum { CC_MARKS = 0, CC_GIVEUPS, CC_OVERFLOW, CC_REJ_ENC, CC_REJ_BLOCKED, CC_VALID,
+       CC_FP32, CC_FP64, CC_DD, CC_EXACT, CC_N };
+struct CavCounts {
+  unsigned c[CC_N];
+  __device__ void flush(unsigned long long *cnt) {
+    const int dst[CC_N] = {CNT_MARKS, CNT_GIVEUPS, CNT_OVERFLOW, CNT_REJ_ENC, CNT_REJ_BLOCKED,
+                           CNT_VALID, CNT_FP32, CNT_FP64, CNT_DD, CNT_EXACT};
+    for (int k = 0; k < CC_N; k++) {
+      unsigned long long v = wave_sum(c[k]);
+      if ((threadIdx.x & 63) == 0 && v) atomicAdd(&cnt[dst[k]], v);
+    }
+  }
+};
 
 constexpr int BIG_LDS_BYTES = 160 * 1024;
 
 // ---------------------------------------------------------------------------
-// Candidate evaluation: one candidate per wave64 (block of 64 lanes).
+// Candidate evaluation: a wave64 evaluates 64/LG candidates at once, one per
+// group of LG lanes.
 //
-// Every lane computes the Steiner / split point and the walk (same loads,
-// broadcast); the Bowyer-Watson cavity is then grown level by level, the 64
-// lanes testing the three edges of every frontier triangle in parallel.
-// Membership is an LDS hash set (atomicCAS insert), the cavity list an LDS
-// array appended with atomicAdd and finally sorted, so the result is the
-// cavity *set* in ascending triangle order -- independent of timing.  The
-// boundary / disk / orientation checks, the ring and the encroachment test
-// run lane-parallel over the cavity's edges.
-// LDS per block: list (2K+2) | cavity hash (HC) | ring hash (HR) | 8 scalars.
+// The lanes of a group compute the Steiner / split point and the walk (same
+// loads, broadcast); the Bowyer-Watson cavity is then grown level by level,
+// the group's lanes testing the three edges of every frontier triangle in
+// parallel.  Membership is an LDS hash set (atomicCAS insert), the cavity
+// list an LDS array appended with atomicAdd and finally sorted, so the
+// result is the cavity *set* in ascending triangle order -- independent of
+// timing and of LG.  The boundary / disk / orientation checks, the ring and
+// the encroachment test run lane-parallel over the cavity's edges.  Loops
+// that contain a barrier run a wave-uniform number of times (groups with
+// nothing to do idle through them), so every barrier is reached by the whole
+// block.  Several candidates per wave keep several dependent-load chains in
+// flight per wave: the kernel is bound by memory latency, not bandwidth.
+// The boundary edges are also linked into the star's cyclic order here (the
+// successor of edge u->v is the boundary edge leaving v) and written to the
+// candidate's workspace, so the apply kernel only scatters records.
+// LDS per group: list (2K+2) | cavity hash (HC) | ring hash (HR) | 8 scalars |
+// boundary edges 6 x (K+2).
 // ---------------------------------------------------------------------------
 __host__ __device__ inline int pow2_at_least(int x) {
   int p = 1;
   while (p < x) p <<= 1;
   return p;
 }
-__host__ __device__ inline int cav_hash_cap(int K) { return pow2_at_least(2 * (K + 64)); }
-__host__ __device__ inline int ring_hash_cap(int K) { return pow2_at_least(4 * K + 8); }
-__host__ __device__ inline int wave_lds_ints(int K) {
-  return (2 * K + 2) + cav_hash_cap(K) + ring_hash_cap(K) + 8;
+// At most K + LG cavity inserts happen (see the overflow rule below).
+__host__ __device__ inline int cav_hash_cap(int K, int LG) { return pow2_at_least(2 * (K + LG)); }
+// Ring inserts come from boundary edges only: at most 3K of them.
+__host__ __device__ inline int ring_hash_cap(int K) { return pow2_at_least(3 * K + 8); }
+// list | cavity hash | ring hash | 8 scalars | boundary edges (origin,
+// destination, outer otri, subsegment, successor, star order) x (K+2)
+__host__ __device__ inline int group_lds_ints(int K, int LG) {
+  return (2 * K + 2) + cav_hash_cap(K, LG) + ring_hash_cap(K) + 8 + 6 * (K + 2);
 }
 
 __device__ __forceinline__ unsigned hmix(int x) {
@@ -350,99 +389,136 @@ __device__ __forceinline__ bool hset_has(const int *h, int H, int x) {
   }
 }
 
-enum { SC_N = 0, SC_R, SC_OVF, SC_GIVEUP, SC_NBND, SC_ENC, SC_DSEG };
+enum { SC_N = 0, SC_R, SC_OVF, SC_GIVEUP, SC_NBND, SC_ENC, SC_DSEG, SC_SLOW };
+constexpr int ST_NONE = -1;  // lane group without a candidate
+constexpr int ST_SLOW = 6;   // a sign the filters could not decide
 
-template <bool IS_SEG>
-__global__ __launch_bounds__(64) void k_cavity(MeshD m, ParamsD pr, PassD ps,
-                                               int Kmax) {
+// FAST: every candidate of the pass, with the FP32 / FP64 filters only; a
+// candidate that meets an undecided sign is abandoned without side effects
+// and listed in ps.slow.  !FAST: the listed candidates again, with the full
+// cascade (double-double, then exact expansions in the per-lane arena).  The
+// rare stages' registers thus never limit the occupancy of the hot kernel.
+template <bool IS_SEG, int LG, bool FAST>
+__device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
+                                            const PassD &ps, int Kmax) {
+  static_assert(LG >= 8 && 64 % LG == 0, "lane groups of 8..64 lanes");
+  constexpr int NG = 64 / LG;
   extern __shared__ int sm[];
   const int K = ps.K;
-  const int HC = cav_hash_cap(K), HR = ring_hash_cap(K);
-  int *L = sm;
+  const int HC = cav_hash_cap(K, LG), HR = ring_hash_cap(K);
+  const int lane = threadIdx.x, grp = lane / LG, gl = lane % LG;
+  int *L = sm + grp * group_lds_ints(K, LG);
   int *hc = L + (2 * K + 2);
   int *hr = hc + HC;
   int *sc = hr + HR;
-  const int lane = threadIdx.x;
+  int *eu = sc + 8, *ev = eu + (K + 2), *eo = ev + (K + 2), *es = eo + (K + 2);
+  int *esucc = es + (K + 2), *eord = esucc + (K + 2);
   const bool fp32 = pr.use_fp32 != 0;
-  PredStats st = {0, 0, 0, 0}, junk = {0, 0, 0, 0};
+  PredStats junk = {0, 0, 0, 0};
+  CavCounts cc;
+  for (int k = 0; k < CC_N; k++) cc.c[k] = 0u;
   double *pws = ps.pws + ((size_t)blockIdx.x * 64 + lane) * GCDT_INCIRCLE_WS;
   const int nc = pass_nc(ps);
   const int *ids = pass_ids(ps);
-  for (int i = blockIdx.x; i < nc; i += gridDim.x) {
-    // ---- point + seeds (uniform across the wave) ----
-    int status = ST_VALID, prio = 0, seed0 = -1, seed1 = -1, split_s = -1, t0 = -1;
+  // work items: the pass's candidates, or the deferred ones
+  const int nwork = FAST ? nc : (nc > 0 ? (int)ps.cnt[CNT_SLOW] : 0);
+  bool unsure = false;
+  auto in_circle = [&](double2 a, double2 b, double2 c, double x, double y, PredStats *q) {
+    if (!FAST) return incircle_sign(a.x, a.y, b.x, b.y, c.x, c.y, x, y, pws, fp32, q);
+    int sg = incircle_filtered(a.x, a.y, b.x, b.y, c.x, c.y, x, y, fp32, q);
+    if (sg == PRED_UNSURE) { unsure = true; sg = 0; }
+    return sg;
+  };
+  auto orient = [&](double ax, double ay, double2 b, double2 c, PredStats *q) {
+    if (!FAST) return orient2d_sign(ax, ay, b.x, b.y, c.x, c.y, fp32, q);
+    int sg = orient2d_filtered(ax, ay, b.x, b.y, c.x, c.y, fp32, q);
+    if (sg == PRED_UNSURE) { unsure = true; sg = 0; }
+    return sg;
+  };
+  for (int base = blockIdx.x * NG; base < nwork; base += gridDim.x * NG) {
+    const bool act = base + grp < nwork;
+    const int i = !act ? nc : (FAST ? base + grp : ps.slow[base + grp]);
+    PredStats cst = {0, 0, 0, 0};  // this candidate's predicate counts
+    // ---- point + seeds (uniform across the group) ----
+    int status = act ? ST_VALID : ST_NONE, prio = 0, seed0 = -1, seed1 = -1, split_s = -1,
+        t0 = -1;
     double cx = 0.0, cy = 0.0;
-    ParamsD lp;
-    if (IS_SEG) {
-      split_s = ids[i];
-      int t1, j1, t2, j2;
-      seg_sides(m, split_s, t1, j1, t2, j2);
-      lp = params_at(pr, m, t1 >= 0 ? t1 : t2);
-      if (!split_point(m, split_s, cx, cy)) {
-        status = ST_GIVEUP;
-      } else {
-        seed0 = t1 >= 0 ? t1 : t2;
-        seed1 = t1 >= 0 ? t2 : -1;
-      }
-    } else {
-      t0 = ids[i];
-      lp = params_at(pr, m, t0);
-      int4 v0 = m.TV[t0];
-      if (pr.priority == 1 || pr.priority == 2) prio = shortest_edge_bucket(m, v0);
-      if (!steiner_point(m, lp, v0, cx, cy)) {
-        status = ST_GIVEUP;
-      } else {
-        int tl = -1, jl = -1, sg = -1;
-        int w = walk(m, t0, cx, cy, fp32, lane == 0 ? &st : &junk, tl, jl, sg);
-        if (w == 3) {
-          status = ST_REJ_BLOCKED;
-          if (lane == 0) mark_enc(m, sg, ps.cnt);
-        } else if (w == 2 || w == 4) {
+    ParamsD lp = pr;
+    if (act) {
+      if (IS_SEG) {
+        split_s = ids[i];
+        int t1, j1, t2, j2;
+        seg_sides(m, split_s, t1, j1, t2, j2);
+        lp = params_at(pr, m, t1 >= 0 ? t1 : t2);
+        if (!split_point(m, split_s, cx, cy)) {
           status = ST_GIVEUP;
         } else {
-          seed0 = tl;
-          if (w == 1) {
-            int s = comp(m.TS[tl], jl);
-            if (s >= 0) {
-              status = ST_REJ_BLOCKED;
-              if (lane == 0) mark_enc(m, s, ps.cnt);
-            } else {
-              int nb = comp(m.TN[tl], jl);
-              if (nb < 0) status = ST_GIVEUP;
-              else seed1 = nb >> 2;
+          seed0 = t1 >= 0 ? t1 : t2;
+          seed1 = t1 >= 0 ? t2 : -1;
+        }
+      } else {
+        t0 = ids[i];
+        lp = params_at(pr, m, t0);
+        int4 v0 = m.TV[t0];
+        if (pr.priority == 1 || pr.priority == 2) prio = shortest_edge_bucket(m, v0);
+        if (!steiner_point(m, lp, v0, cx, cy)) {
+          status = ST_GIVEUP;
+        } else {
+          int tl = -1, jl = -1, sg = -1;
+          int w = walk<FAST>(m, t0, cx, cy, fp32, gl == 0 ? &cst : &junk, tl, jl, sg);
+          if (w == 5) {
+            status = ST_SLOW;
+          } else if (w == 3) {
+            status = ST_REJ_BLOCKED;
+            if (gl == 0 && mark_enc_local(m, sg)) cc.c[CC_MARKS]++;
+          } else if (w == 2 || w == 4) {
+            status = ST_GIVEUP;
+          } else {
+            seed0 = tl;
+            if (w == 1) {
+              int s = comp(m.TS[tl], jl);
+              if (s >= 0) {
+                status = ST_REJ_BLOCKED;
+                if (gl == 0 && mark_enc_local(m, s)) cc.c[CC_MARKS]++;
+              } else {
+                int nb = comp(m.TN[tl], jl);
+                if (nb < 0) status = ST_GIVEUP;
+                else seed1 = nb >> 2;
+              }
             }
           }
         }
       }
     }
-    int n = 0, r = 0;
-    if (status == ST_VALID) {
-      // ---- breadth-first growth of the cavity ----
-      for (int k = lane; k < HC; k += 64) hc[k] = -1;
-      for (int k = lane; k < HR; k += 64) hr[k] = -1;
-      if (lane < 8) sc[lane] = 0;
-      if (lane == 0) sc[SC_DSEG] = 0x7f7fffff;  // +FLT_MAX bits: no subsegment yet
-      __syncthreads();
-      if (lane == 0) {
-        L[0] = seed0;
-        hset_insert(hc, HC, seed0);
-        int c0 = 1;
-        if (seed1 >= 0) {
-          L[1] = seed1;
-          hset_insert(hc, HC, seed1);
-          c0 = 2;
-        }
-        sc[SC_N] = c0;
+    // ---- breadth-first growth of the cavity ----
+    const bool grow = status == ST_VALID;
+    for (int k = gl; k < HC; k += LG) hc[k] = -1;
+    for (int k = gl; k < HR; k += LG) hr[k] = -1;
+    if (gl < 8) sc[gl] = 0;
+    if (gl == 0) sc[SC_DSEG] = 0x7f7fffff;  // +FLT_MAX bits: no subsegment yet
+    __syncthreads();
+    if (grow && gl == 0) {
+      L[0] = seed0;
+      hset_insert(hc, HC, seed0);
+      int c0 = 1;
+      if (seed1 >= 0) {
+        L[1] = seed1;
+        hset_insert(hc, HC, seed1);
+        c0 = 2;
       }
+      sc[SC_N] = c0;
+    }
+    __syncthreads();
+    int fb = 0;
+    for (;;) {
+      int fe = sc[SC_N];
+      int ovf = sc[SC_OVF] | sc[SC_SLOW];
+      bool more = grow && !ovf && fb < fe;
       __syncthreads();
-      int fb = 0;
-      for (;;) {
-        int fe = sc[SC_N];
-        int ovf = sc[SC_OVF];
-        __syncthreads();
-        if (ovf || fb >= fe) break;
+      if (__ballot(more) == 0ull) break;  // wave-uniform exit
+      if (more) {
         int items = (fe - fb) * 3;
-        for (int it = lane; it < items; it += 64) {
+        for (int it = gl; it < items; it += LG) {
           int t = L[fb + it / 3], j = it % 3;
           if (comp(m.TS[t], j) >= 0) continue;  // subsegments are never crossed
           int nb = comp(m.TN[t], j);
@@ -451,10 +527,16 @@ __global__ __launch_bounds__(64) void k_cavity(MeshD m, ParamsD pr, PassD ps,
           if (hset_has(hc, HC, u)) continue;
           int4 w = m.TV[u];
           double2 a = m.P[w.x], b = m.P[w.y], c = m.P[w.z];
-          if (incircle_sign(a.x, a.y, b.x, b.y, c.x, c.y, cx, cy, pws, fp32, &st) > 0) {
+          int ins = in_circle(a, b, c, cx, cy, &cst);
+          if (FAST && unsure) {
+            sc[SC_SLOW] = 1;
+            unsure = false;
+            continue;
+          }
+          if (ins > 0) {
             // u is in the cavity.  Once K triangles are in, a triangle that is
             // still absent proves the cavity larger than K; inserts stay
-            // below K + 64, so the hash set never fills.
+            // below K + LG, so the hash set never fills.
             if (hset_has(hc, HC, u)) continue;
             if (sc[SC_N] >= K) {
               sc[SC_OVF] = 1;
@@ -465,125 +547,217 @@ __global__ __launch_bounds__(64) void k_cavity(MeshD m, ParamsD pr, PassD ps,
             }
           }
         }
-        __syncthreads();
-        fb = fe;
       }
+      __syncthreads();
+      if (more) fb = fe;
+    }
+    int n = 0, r = 0;
+    bool cav_ok = false;
+    if (grow) {
       n = sc[SC_N];
-      if (sc[SC_OVF] || n > K) {
-        status = (K >= Kmax) ? ST_GIVEUP : ST_OVERFLOW;
-      } else {
-        // ---- sort the cavity (rank sort into the free half of L) ----
-        int *tmp = L + K + 2;
-        for (int e = lane; e < n; e += 64) {
-          int v = L[e], rank = 0;
-          for (int x = 0; x < n; x++) rank += L[x] < v;
-          tmp[rank] = v;
+      if (FAST && sc[SC_SLOW]) status = ST_SLOW;
+      else if (sc[SC_OVF] || n > K) status = (K >= Kmax) ? ST_GIVEUP : ST_OVERFLOW;
+      else cav_ok = true;
+    }
+    // ---- sort the cavity (rank sort into the free half of L) ----
+    int *tmp = L + K + 2;
+    if (cav_ok)
+      for (int e = gl; e < n; e += LG) {
+        int v = L[e], rank = 0;
+        for (int x = 0; x < n; x++) rank += L[x] < v;
+        tmp[rank] = v;
+      }
+    __syncthreads();
+    if (cav_ok)
+      for (int e = gl; e < n; e += LG) L[e] = tmp[e];
+    __syncthreads();
+    // ---- disk and star checks over every cavity edge ----
+    if (cav_ok)
+      for (int it = gl; it < 3 * n; it += LG) {
+        int t = L[it / 3], j = it % 3;
+        int s = comp(m.TS[t], j);
+        int nb = comp(m.TN[t], j);
+        bool inside = nb >= 0 && hset_has(hc, HC, nb >> 2);
+        if (s >= 0 && s != split_s && inside) sc[SC_GIVEUP] = 1;
+        bool bnd = (s >= 0 && s != split_s) || nb < 0 || !inside;
+        if (bnd) {
+          int4 tv = m.TV[t];
+          int bpos = atomicAdd(&sc[SC_NBND], 1);
+          if (bpos < K + 2) {
+            eu[bpos] = comp(tv, (j + 1) % 3);
+            ev[bpos] = comp(tv, (j + 2) % 3);
+            eo[bpos] = nb;
+            es[bpos] = s;
+          }
+          if (!(s >= 0 && s == split_s)) {
+            double2 u = m.P[comp(tv, (j + 1) % 3)], v = m.P[comp(tv, (j + 2) % 3)];
+            int o = orient(cx, cy, u, v, &cst);
+            if (FAST && unsure) {
+              sc[SC_SLOW] = 1;
+              unsure = false;
+            } else if (o <= 0) {
+              sc[SC_GIVEUP] = 1;
+            }
+          }
+          if (nb >= 0 && !inside && hset_insert(hr, HR, nb >> 2)) {
+            int pos = atomicAdd(&sc[SC_R], 1);
+            if (pos < K + 2) L[n + pos] = nb >> 2;
+          }
         }
-        __syncthreads();
-        for (int e = lane; e < n; e += 64) L[e] = tmp[e];
-        __syncthreads();
-        // ---- disk and star checks over every cavity edge ----
-        for (int it = lane; it < 3 * n; it += 64) {
+        if (!IS_SEG && s >= 0) {
+          int2 sv = m.SV[s];
+          if (encroaches(m, lp, sv.x, sv.y, cx, cy)) sc[SC_ENC] = 1;
+          if (ps.feat) {  // distance of the Steiner point to this subsegment
+            double2 A = m.P[sv.x], B = m.P[sv.y];
+            float ex = (float)(B.x - A.x), ey = (float)(B.y - A.y);
+            float px = (float)(cx - A.x), py = (float)(cy - A.y);
+            float len2 = ex * ex + ey * ey;
+            float tt = len2 > 0.f ? fminf(fmaxf((px * ex + py * ey) / len2, 0.f), 1.f) : 0.f;
+            float ddx = px - tt * ex, ddy = py - tt * ey;
+            atomicMin(&sc[SC_DSEG], __float_as_int(sqrtf(ddx * ddx + ddy * ddy)));
+          }
+        }
+      }
+    __syncthreads();
+    if (cav_ok) {
+      r = sc[SC_R];
+      if (FAST && sc[SC_SLOW]) {
+        status = ST_SLOW;
+      } else if (sc[SC_GIVEUP] || sc[SC_NBND] != n + 2 || r > n + 2) {
+        status = ST_GIVEUP;
+      } else if (!IS_SEG && sc[SC_ENC]) {
+        // Ruppert/Chew rule: a Steiner point that encroaches a subsegment on
+        // its cavity boundary is not inserted; every such subsegment is marked.
+        for (int it = gl; it < 3 * n; it += LG) {
           int t = L[it / 3], j = it % 3;
           int s = comp(m.TS[t], j);
-          int nb = comp(m.TN[t], j);
-          bool inside = nb >= 0 && hset_has(hc, HC, nb >> 2);
-          if (s >= 0 && s != split_s && inside) sc[SC_GIVEUP] = 1;
-          bool bnd = (s >= 0 && s != split_s) || nb < 0 || !inside;
-          if (bnd) {
-            atomicAdd(&sc[SC_NBND], 1);
-            if (!(s >= 0 && s == split_s)) {
-              int4 tv = m.TV[t];
-              double2 u = m.P[comp(tv, (j + 1) % 3)], v = m.P[comp(tv, (j + 2) % 3)];
-              if (orient2d_sign(cx, cy, u.x, u.y, v.x, v.y, fp32, &st) <= 0) sc[SC_GIVEUP] = 1;
-            }
-            if (nb >= 0 && !inside && hset_insert(hr, HR, nb >> 2)) {
-              int pos = atomicAdd(&sc[SC_R], 1);
-              if (pos < K + 2) L[n + pos] = nb >> 2;
-            }
-          }
-          if (!IS_SEG && s >= 0) {
-            int2 sv = m.SV[s];
-            if (encroaches(m, lp, sv.x, sv.y, cx, cy)) sc[SC_ENC] = 1;
-            if (ps.feat) {  // distance of the Steiner point to this subsegment
-              double2 A = m.P[sv.x], B = m.P[sv.y];
-              float ex = (float)(B.x - A.x), ey = (float)(B.y - A.y);
-              float px = (float)(cx - A.x), py = (float)(cy - A.y);
-              float len2 = ex * ex + ey * ey;
-              float tt = len2 > 0.f ? fminf(fmaxf((px * ex + py * ey) / len2, 0.f), 1.f) : 0.f;
-              float ddx = px - tt * ex, ddy = py - tt * ey;
-              atomicMin(&sc[SC_DSEG], __float_as_int(sqrtf(ddx * ddx + ddy * ddy)));
-            }
-          }
+          if (s < 0) continue;
+          int2 sv = m.SV[s];
+          if (encroaches(m, lp, sv.x, sv.y, cx, cy) && mark_enc_local(m, s)) cc.c[CC_MARKS]++;
         }
-        __syncthreads();
-        r = sc[SC_R];
-        if (sc[SC_GIVEUP] || sc[SC_NBND] != n + 2 || r > n + 2) {
-          status = ST_GIVEUP;
-        } else if (!IS_SEG && sc[SC_ENC]) {
-          // Ruppert/Chew rule: a Steiner point that encroaches a subsegment on
-          // its cavity boundary is not inserted; every such subsegment is marked.
-          for (int it = lane; it < 3 * n; it += 64) {
-            int t = L[it / 3], j = it % 3;
-            int s = comp(m.TS[t], j);
-            if (s < 0) continue;
-            int2 sv = m.SV[s];
-            if (encroaches(m, lp, sv.x, sv.y, cx, cy)) mark_enc(m, s, ps.cnt);
-          }
-          status = ST_REJ_ENC;
+        status = ST_REJ_ENC;
+      }
+    }
+    // ---- star order of the boundary (valid candidates) ----
+    // The n+2 boundary edges form one cycle.  Closed star: it starts at the
+    // edge with the smallest origin vertex; a hull subsegment split (open
+    // star) starts after the split edge and leaves it out.
+    const bool star = status == ST_VALID;
+    const int nbe = n + 2;
+    if (star)
+      for (int e = gl; e < nbe; e += LG) {
+        int v = ev[e], f = 0;
+        while (f < nbe && eu[f] != v) f++;
+        esucc[e] = f < nbe ? f : 0;
+      }
+    __syncthreads();
+    int nstar = 0;
+    if (star) {
+      int start = 0;
+      bool open = false;
+      for (int e = 0; e < nbe; e++) {
+        if (IS_SEG && es[e] == split_s) {  // only a hull split edge is on the boundary
+          open = true;
+          start = esucc[e];
+          break;
         }
+        if (eu[e] < eu[start]) start = e;
+      }
+      nstar = open ? nbe - 1 : nbe;
+      if (gl == 0) {
+        int e = start;
+        for (int q = 0; q < nstar; q++) {
+          eord[q] = e;
+          e = esucc[e];
+        }
+      }
+    }
+    __syncthreads();
+    if (star) {
+      int *G = ps.ws + (size_t)i * ps.R + (2 * K + 2);
+      for (int q = gl; q < nstar; q += LG) {
+        int e = eord[q];
+        G[q] = eu[e];
+        G[(K + 2) + q] = ev[e];
+        G[2 * (K + 2) + q] = eo[e];
+        G[3 * (K + 2) + q] = es[e];
       }
     }
     // ---- outputs ----
     if (status == ST_VALID) {
       int *G = ps.ws + (size_t)i * ps.R;
-      for (int q = lane; q < n + r; q += 64) G[q] = L[q];
+      for (int q = gl; q < n + r; q += LG) G[q] = L[q];
     }
-    if (!IS_SEG && ps.feat && lane < FEAT) {
+    if (status == ST_SLOW) {  // defer: no output, no side effect, no counts
+      if (gl == 0) ps.slow[atomicAdd(&ps.cnt[CNT_SLOW], 1ull)] = i;
+    } else {
+      cc.c[CC_FP32] += (unsigned)cst.fp32_hits;
+      cc.c[CC_FP64] += (unsigned)cst.fp64_hits;
+      cc.c[CC_DD] += (unsigned)cst.dd_hits;
+      cc.c[CC_EXACT] += (unsigned)cst.exact_calls;
+    }
+    if (!IS_SEG && ps.feat && act && status != ST_SLOW) {
       // per-candidate features of the learned priority
-      float fv = 0.f;
+      float l0 = 0.f, l1 = 0.f, l2 = 0.f, area2 = 0.f;
       if (status == ST_VALID) {
         int4 v0 = m.TV[t0];
         double2 A = m.P[v0.x], O = m.P[v0.y], D = m.P[v0.z];
-        float l0 = (float)((O.x - D.x) * (O.x - D.x) + (O.y - D.y) * (O.y - D.y));
-        float l1 = (float)((D.x - A.x) * (D.x - A.x) + (D.y - A.y) * (D.y - A.y));
-        float l2 = (float)((A.x - O.x) * (A.x - O.x) + (A.y - O.y) * (A.y - O.y));
-        float sh2 = fminf(l0, fminf(l1, l2));
-        float area2 = fabsf((float)((O.x - A.x) * (D.y - A.y) - (O.y - A.y) * (D.x - A.x)));
-        float R = sqrtf(l0 * l1 * l2) / fmaxf(2.f * area2, 1e-30f);
-        float sh = sqrtf(sh2);
-        // smallest angle from the law of sines opposite the shortest edge
-        float sina = fminf(sh / fmaxf(2.f * R, 1e-30f), 1.f);
-        float dseg = __int_as_float(sc[SC_DSEG]);
-        if (lane == 0) fv = asinf(sina) * (180.f / 3.14159265f) / 60.f;
-        else if (lane == 1) fv = log2f(fmaxf(R / fmaxf(sh, 1e-30f), 1e-6f)) / 4.f;
-        else if (lane == 2) fv = (float)n / 16.f;
-        else if (lane == 3) fv = (float)r / 16.f;
-        else if (lane == 4) fv = dseg > 1e30f ? 1.f : fminf(dseg / fmaxf(sh, 1e-30f), 4.f) / 4.f;
-        else if (lane == 5) fv = (float)pass_index_of(ps) / 256.f;
-        else if (lane == 7) fv = 1.f;
+        l0 = (float)((O.x - D.x) * (O.x - D.x) + (O.y - D.y) * (O.y - D.y));
+        l1 = (float)((D.x - A.x) * (D.x - A.x) + (D.y - A.y) * (D.y - A.y));
+        l2 = (float)((A.x - O.x) * (A.x - O.x) + (A.y - O.y) * (A.y - O.y));
+        area2 = fabsf((float)((O.x - A.x) * (D.y - A.y) - (O.y - A.y) * (D.x - A.x)));
       }
-      ps.feat[(size_t)i * FEAT + lane] = fv;
+      float sh2 = fminf(l0, fminf(l1, l2));
+      float R = sqrtf(l0 * l1 * l2) / fmaxf(2.f * area2, 1e-30f);
+      float sh = sqrtf(sh2);
+      // smallest angle from the law of sines opposite the shortest edge
+      float sina = fminf(sh / fmaxf(2.f * R, 1e-30f), 1.f);
+      float dseg = __int_as_float(sc[SC_DSEG]);
+      for (int f = gl; f < FEAT; f += LG) {
+        float fv = 0.f;
+        if (status == ST_VALID) {
+          if (f == 0) fv = asinf(sina) * (180.f / 3.14159265f) / 60.f;
+          else if (f == 1) fv = log2f(fmaxf(R / fmaxf(sh, 1e-30f), 1e-6f)) / 4.f;
+          else if (f == 2) fv = (float)n / 16.f;
+          else if (f == 3) fv = (float)r / 16.f;
+          else if (f == 4) fv = dseg > 1e30f ? 1.f : fminf(dseg / fmaxf(sh, 1e-30f), 4.f) / 4.f;
+          else if (f == 5) fv = (float)pass_index_of(ps) / 256.f;
+          else if (f == 7) fv = 1.f;
+        }
+        ps.feat[(size_t)i * FEAT + f] = fv;
+      }
     }
-    if (lane == 0) {
+    if (act && gl == 0 && status != ST_SLOW) {
       ps.cstatus[i] = status;
       ps.cpt[i] = make_double2(cx, cy);
       ps.cn[i] = status == ST_VALID ? n : 0;
       ps.cr[i] = status == ST_VALID ? r : 0;
       ps.cprio[i] = prio;
-      if (status == ST_VALID) atomicAdd(&ps.cnt[CNT_VALID], 1ull);
-      else if (status == ST_OVERFLOW) atomicAdd(&ps.cnt[CNT_OVERFLOW], 1ull);
-      else if (status == ST_REJ_ENC) atomicAdd(&ps.cnt[CNT_REJ_ENC], 1ull);
-      else if (status == ST_REJ_BLOCKED) atomicAdd(&ps.cnt[CNT_REJ_BLOCKED], 1ull);
+      if (status == ST_VALID) cc.c[CC_VALID]++;
+      else if (status == ST_OVERFLOW) cc.c[CC_OVERFLOW]++;
+      else if (status == ST_REJ_ENC) cc.c[CC_REJ_ENC]++;
+      else if (status == ST_REJ_BLOCKED) cc.c[CC_REJ_BLOCKED]++;
       else if (status == ST_GIVEUP) {
-        atomicAdd(&ps.cnt[CNT_GIVEUPS], 1ull);
+        cc.c[CC_GIVEUPS]++;
         if (IS_SEG) m.senc[split_s] = 2;
         else m.tflag[t0] |= TF_GIVEUP;
       }
     }
-    __syncthreads();  // LDS is reused by the next candidate
+    __syncthreads();  // LDS is reused by the next candidates
   }
-  flush_stats(ps.cnt, st);
+  cc.flush(ps.cnt);
+}
+
+// The filtered kernel is held to 5 waves per SIMD (<= 96 VGPRs, no spills);
+// the exact kernel runs on few candidates and keeps its full register file.
+template <bool IS_SEG, int LG>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_cavity_fast(
+    MeshD m, ParamsD pr, PassD ps, int Kmax) {
+  cavity_body<IS_SEG, LG, true>(m, pr, ps, Kmax);
+}
+template <bool IS_SEG, int LG>
+__global__ __launch_bounds__(64) void k_cavity_slow(MeshD m, ParamsD pr, PassD ps, int Kmax) {
+  cavity_body<IS_SEG, LG, false>(m, pr, ps, Kmax);
 }
 
 // Claim key: higher wins.  Epoch in the top 11 bits makes older claims lose
@@ -658,8 +832,7 @@ __global__ __launch_bounds__(256) void k_block(MeshD m, PassD ps, int *cact) {
 __global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
                                                      PassD ps, int pbase,
                                                      int tbase, int sbase,
-                                                     int is_seg, int lds_k) {
-  extern __shared__ int lds[];
+                                                     int is_seg) {
   const int nc = pass_nc(ps);
   const int *ids = pass_ids(ps);
   if (ps.ctl) {
@@ -678,11 +851,6 @@ __global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
     double2 c = ps.cpt[i];
     int n = ps.cn[i];
     const int *L = ps.ws + (size_t)i * ps.R;
-    if (n <= lds_k) {  // membership scans below run on an LDS copy
-      int *Ls = lds + threadIdx.x * lds_k;
-      for (int q = 0; q < n; q++) Ls[q] = L[q];
-      L = Ls;
-    }
     int k = n + 2;
     const int inst = m.tinst ? m.tinst[L[0]] : 0;
     const ParamsD lp = params_at(pr, m, L[0]);
@@ -690,49 +858,20 @@ __global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
     int *Bv = Bu + (ps.K + 2);
     int *Bo = Bv + (ps.K + 2);
     int *Bs = Bo + (ps.K + 2);
-    int t0 = -1, j0 = -1;
+    // the boundary edges in star order (origin Bu, destination Bv, outer
+    // otri Bo, subsegment Bs) were written by the cavity kernel
     bool open = false;
     int a = -1, b = -1, par = -1;
     if (is_seg) {
       int t1, j1, t2, j2;
       seg_sides(m, split_s, t1, j1, t2, j2);
-      if (t1 < 0 || t2 < 0) {
-        open = true;
-        int ts = t1 >= 0 ? t1 : t2, js = t1 >= 0 ? j1 : j2;
-        next_bnd(m, ts, js, split_s, L, n, t0, j0);
-      }
+      open = t1 < 0 || t2 < 0;  // hull subsegment: the star is an open fan
       int2 sv = m.SV[split_s];
       a = sv.x;
       b = sv.y;
       par = m.SP[split_s];
     }
-    if (!open) {
-      int best = -1;
-      for (int idx = 0; idx < n; idx++) {
-        int t = L[idx];
-        int4 tv = m.TV[t];
-        for (int j = 0; j < 3; j++) {
-          if (!is_bnd(m, t, j, split_s, L, n)) continue;
-          int u = comp(tv, (j + 1) % 3);
-          if (best < 0 || u < best) { best = u; t0 = t; j0 = j; }
-        }
-      }
-    }
     int nstar = open ? k - 1 : k;
-    {
-      int t = t0, j = j0;
-      for (int q = 0; q < nstar; q++) {
-        int4 tv = m.TV[t];
-        Bu[q] = comp(tv, (j + 1) % 3);
-        Bv[q] = comp(tv, (j + 2) % 3);
-        Bo[q] = comp(m.TN[t], j);
-        Bs[q] = comp(m.TS[t], j);
-        int tn, jn;
-        next_bnd(m, t, j, split_s, L, n, tn, jn);
-        t = tn;
-        j = jn;
-      }
-    }
     // point
     m.P[p] = c;
     m.pflag[p] = is_seg ? PF_SEGV : PF_FREE;
@@ -1265,12 +1404,18 @@ Refiner::Refiner(const gcdt_options &o) : opt(o) {
   HC(hipEventCreate(&ev0));
   HC(hipEventCreate(&ev1));
   // cavity and apply kernels take up to BIG_LDS_BYTES of dynamic LDS
-  HC(hipFuncSetAttribute((const void *)k_cavity<true>,
-                         hipFuncAttributeMaxDynamicSharedMemorySize, BIG_LDS_BYTES));
-  HC(hipFuncSetAttribute((const void *)k_cavity<false>,
-                         hipFuncAttributeMaxDynamicSharedMemorySize, BIG_LDS_BYTES));
-  HC(hipFuncSetAttribute((const void *)k_apply,
-                         hipFuncAttributeMaxDynamicSharedMemorySize, BIG_LDS_BYTES));
+  const void *cav[] = {
+      (const void *)k_cavity_fast<true, 8>,  (const void *)k_cavity_fast<false, 8>,
+      (const void *)k_cavity_fast<true, 16>, (const void *)k_cavity_fast<false, 16>,
+      (const void *)k_cavity_fast<true, 32>, (const void *)k_cavity_fast<false, 32>,
+      (const void *)k_cavity_fast<true, 64>, (const void *)k_cavity_fast<false, 64>,
+      (const void *)k_cavity_slow<true, 8>,  (const void *)k_cavity_slow<false, 8>,
+      (const void *)k_cavity_slow<true, 16>, (const void *)k_cavity_slow<false, 16>,
+      (const void *)k_cavity_slow<true, 32>, (const void *)k_cavity_slow<false, 32>,
+      (const void *)k_cavity_slow<true, 64>, (const void *)k_cavity_slow<false, 64>};
+  for (const void *f : cav)
+    HC(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, BIG_LDS_BYTES));
+
   dalloc(&d_total, 4);
   dalloc(&d_cnt, CNT_N);
   dalloc(&d_ctl, 1);
@@ -1296,7 +1441,7 @@ Refiner::~Refiner() {
   dfree(seg_in);
   dfree(c_pt); dfree(c_status); dfree(c_n); dfree(c_r);
   dfree(c_prio); dfree(c_win); dfree(c_rank); dfree(c_ws); dfree(c_pws);
-  dfree(c_act); dfree(tmp_list);
+  dfree(c_act); dfree(c_slow); dfree(tmp_list);
   dfree(sel_prio); dfree(sel_tmp); dfree(sel_list); dfree(d_hist);
   dfree(c_feat); dfree(c_score); dfree(c_deg); dfree(t_cnt); dfree(mlp_params);
   dfree(cb_cnt); dfree(cb_off); dfree(d_total); dfree(d_cnt);
@@ -1490,8 +1635,9 @@ void Refiner::ensure_cand(int nc) {
   if (nc <= c_cap) return;
   int c = std::max(nc, c_cap * 2);
   dfree(c_pt); dfree(c_status); dfree(c_n); dfree(c_r);
-  dfree(c_prio); dfree(c_win); dfree(c_rank); dfree(c_act);
+  dfree(c_prio); dfree(c_win); dfree(c_rank); dfree(c_act); dfree(c_slow);
   dalloc(&c_act, (size_t)c);
+  dalloc(&c_slow, (size_t)c);
   dalloc(&c_pt, (size_t)c);
   dalloc(&c_status, (size_t)c);
   dalloc(&c_n, (size_t)c);
@@ -1584,6 +1730,7 @@ PassD Refiner::pass_base(int Kc) const {
   ps.feat = nullptr;
   ps.ctl = nullptr;
   ps.ids_sel = nullptr;
+  ps.slow = c_slow;
   return ps;
 }
 
@@ -1633,17 +1780,11 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
   // one candidate per wave64 block; the exact-predicate arena holds
   // pws_blocks*256 lanes, i.e. pws_blocks*4 blocks of 64
-  int gw = std::min(nc, std::min(opt.grid_cavity, pws_blocks * 4));
-  size_t lds = sizeof(int) * (size_t)wave_lds_ints(K);
-  if (is_seg) {
-    tbegin(K_SEGCAV);
-    k_cavity<true><<<gw, 64, lds, stream>>>(m, pr, ps, opt.cav_cap_max);
-    tend(K_SEGCAV, nc);
-  } else {
-    tbegin(K_BADCAV);
-    k_cavity<false><<<gw, 64, lds, stream>>>(m, pr, ps, opt.cav_cap_max);
-    tend(K_BADCAV, nc);
-  }
+  int lg = cavity_lanes(K);
+  int gw = std::min((nc + 64 / lg - 1) / (64 / lg), std::min(opt.grid_cavity, pws_blocks * 4));
+  tbegin(is_seg ? K_SEGCAV : K_BADCAV);
+  launch_cavity(is_seg, gw, ps);
+  tend(is_seg ? K_SEGCAV : K_BADCAV, nc);
   HC(hipGetLastError());
   int gc = std::min(nblk(nc), opt.grid_claim);
   if (!is_seg && opt.priority == 2) {
@@ -1679,11 +1820,8 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   ensure_capacity(np + nwin, nt + 2LL * nwin, is_seg ? ns + nwin : ns);
   if (nwin > 0) {
     tbegin(K_APPLY);
-    // apply keeps each winner's cavity in LDS: lds_k ints per thread
-    int lds_k = std::min(K, (int)(BIG_LDS_BYTES / (sizeof(int) * CAV_BLOCK)));
-    k_apply<<<std::min(nblk(nc, CAV_BLOCK), opt.grid_apply), CAV_BLOCK,
-              sizeof(int) * lds_k * CAV_BLOCK, stream>>>(m, pr, ps, np, nt, ns,
-                                                         is_seg ? 1 : 0, lds_k);
+    k_apply<<<std::min(nblk(nc, CAV_BLOCK), opt.grid_apply), CAV_BLOCK, 0, stream>>>(
+        m, pr, ps, np, nt, ns, is_seg ? 1 : 0);
     tend(K_APPLY, nwin);
     HC(hipGetLastError());
   }
@@ -1888,6 +2026,54 @@ void Refiner::size_pass_buffers() {
   }
 }
 
+// Lanes per candidate in the cavity kernel: the configured group size,
+// widened until the groups' LDS fits one block.
+int Refiner::cavity_lanes(int Kc) const {
+  int lg = opt.cavity_lanes;
+  while (lg < 64 && (size_t)(64 / lg) * group_lds_ints(Kc, lg) * sizeof(int) > BIG_LDS_BYTES)
+    lg *= 2;
+  return lg;
+}
+
+template <bool IS_SEG, bool FAST>
+static void launch_cavity_t(int lg, int grid, size_t lds, hipStream_t s, const MeshD &m,
+                            const ParamsD &pr, const PassD &ps, int kmax) {
+  if (FAST) {
+    switch (lg) {
+      case 8: k_cavity_fast<IS_SEG, 8><<<grid, 64, lds, s>>>(m, pr, ps, kmax); break;
+      case 16: k_cavity_fast<IS_SEG, 16><<<grid, 64, lds, s>>>(m, pr, ps, kmax); break;
+      case 32: k_cavity_fast<IS_SEG, 32><<<grid, 64, lds, s>>>(m, pr, ps, kmax); break;
+      default: k_cavity_fast<IS_SEG, 64><<<grid, 64, lds, s>>>(m, pr, ps, kmax); break;
+    }
+  } else {
+    switch (lg) {
+      case 8: k_cavity_slow<IS_SEG, 8><<<grid, 64, lds, s>>>(m, pr, ps, kmax); break;
+      case 16: k_cavity_slow<IS_SEG, 16><<<grid, 64, lds, s>>>(m, pr, ps, kmax); break;
+      case 32: k_cavity_slow<IS_SEG, 32><<<grid, 64, lds, s>>>(m, pr, ps, kmax); break;
+      default: k_cavity_slow<IS_SEG, 64><<<grid, 64, lds, s>>>(m, pr, ps, kmax); break;
+    }
+  }
+}
+
+// Filtered-predicate pass over all candidates, then the exact-predicate pass
+// over the candidates it deferred (their count stays on the device).
+void Refiner::launch_cavity(bool is_seg, int grid, const PassD &ps) {
+  int lg = cavity_lanes(ps.K);
+  size_t lds = sizeof(int) * (size_t)(64 / lg) * group_lds_ints(ps.K, lg);
+  if (lds > (size_t)BIG_LDS_BYTES)
+    throw std::runtime_error("cavity capacity " + std::to_string(ps.K) + " needs more LDS than a block has");
+  if (grid < 1) grid = 1;
+  int slow_grid = std::min(grid, std::min(1024, pws_blocks * 4));
+  if (is_seg) {
+    launch_cavity_t<true, true>(lg, grid, lds, stream, m, pr, ps, opt.cav_cap_max);
+    launch_cavity_t<true, false>(lg, slow_grid, lds, stream, m, pr, ps, opt.cav_cap_max);
+  } else {
+    launch_cavity_t<false, true>(lg, grid, lds, stream, m, pr, ps, opt.cav_cap_max);
+    launch_cavity_t<false, false>(lg, slow_grid, lds, stream, m, pr, ps, opt.cav_cap_max);
+  }
+  HC(hipGetLastError());
+}
+
 // Per-pass counters of the graph loop live in the control block.
 unsigned long long *Refiner::ctl_cnt_ptr() const {
   return reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(d_ctl) +
@@ -1904,16 +2090,13 @@ void Refiner::enqueue_seg_pass() {
   ps.ids = tmp_list;
   ps.ctl = d_ctl;
   ps.cnt = ctl_cnt_ptr();
-  size_t lds = sizeof(int) * (size_t)wave_lds_ints(K);
-  k_cavity<true><<<opt.grid_cavity, 64, lds, stream>>>(m, pr, ps, opt.cav_cap_max);
+  launch_cavity(true, opt.grid_cavity, ps);
   enqueue_claims(ps, opt.grid_claim);
   k_dcount<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_cnt);
   k_dscan<FinWin><<<1, GC, 0, stream>>>(g_cnt, g_off, d_ctl, -1, FinWin{});
   k_dscatter<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_off,
                                              nullptr, c_rank, nullptr);
-  int lds_k = std::min(K, (int)(BIG_LDS_BYTES / (sizeof(int) * CAV_BLOCK)));
-  k_apply<<<opt.grid_apply, CAV_BLOCK, sizeof(int) * lds_k * CAV_BLOCK, stream>>>(
-      m, pr, ps, 0, 0, 0, 1, lds_k);
+  k_apply<<<opt.grid_apply, CAV_BLOCK, 0, stream>>>(m, pr, ps, 0, 0, 0, 1);
   k_end_seg<<<1, 1, 0, stream>>>(d_ctl, std::max(1, opt.claim_iters));
   HC(hipGetLastError());
 }
@@ -1946,8 +2129,7 @@ void Refiner::enqueue_bad_pass() {
   ps.ctl = d_ctl;
   ps.cnt = ctl_cnt_ptr();
   ps.feat = opt.priority == 2 ? c_feat : nullptr;
-  size_t lds = sizeof(int) * (size_t)wave_lds_ints(K);
-  k_cavity<false><<<opt.grid_cavity, 64, lds, stream>>>(m, pr, ps, opt.cav_cap_max);
+  launch_cavity(false, opt.grid_cavity, ps);
   if (opt.priority == 2) {
     const int *ci = reinterpret_cast<const int *>(reinterpret_cast<const char *>(d_ctl) +
                                                   offsetof(Ctl, i));
@@ -1959,9 +2141,7 @@ void Refiner::enqueue_bad_pass() {
   k_dscan<FinWin><<<1, GC, 0, stream>>>(g_cnt, g_off, d_ctl, -1, FinWin{});
   k_dscatter<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_off,
                                              nullptr, c_rank, nullptr);
-  int lds_k = std::min(K, (int)(BIG_LDS_BYTES / (sizeof(int) * CAV_BLOCK)));
-  k_apply<<<opt.grid_apply, CAV_BLOCK, sizeof(int) * lds_k * CAV_BLOCK, stream>>>(
-      m, pr, ps, 0, 0, 0, 0, lds_k);
+  k_apply<<<opt.grid_apply, CAV_BLOCK, 0, stream>>>(m, pr, ps, 0, 0, 0, 0);
   k_end_bad<<<1, 1, 0, stream>>>(d_ctl, std::max(1, opt.claim_iters));
   HC(hipGetLastError());
 }

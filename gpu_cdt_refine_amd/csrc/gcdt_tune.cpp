@@ -35,6 +35,7 @@ int autotune(gcdt_options &opt, int reps,
              gcdt_tune_record *rec, int max_rec) {
   if (reps < 1) reps = 1;
   const std::vector<Knob> knobs = {
+      {&gcdt_options::cavity_lanes, {8, 16, 32, 64}, false},
       {&gcdt_options::grid_cavity, {2048, 4096, 8192, 16384}, false},
       {&gcdt_options::grid_apply, {1024, 2048, 4096, 8192}, false},
       {&gcdt_options::grid_claim, {1024, 2048, 4096}, false},
@@ -49,6 +50,7 @@ int autotune(gcdt_options &opt, int reps,
       rec[n].grid_cavity = o.grid_cavity;
       rec[n].grid_claim = o.grid_claim;
       rec[n].grid_apply = o.grid_apply;
+      rec[n].cavity_lanes = o.cavity_lanes;
       rec[n].ms = ms;
     }
     n++;

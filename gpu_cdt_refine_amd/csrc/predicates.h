@@ -499,9 +499,27 @@ GCDT_HD int orient2d_dd_sign(double ax, double ay, double bx, double by,
 
 // Sign-exact incircle: >0 if d strictly inside circle(a,b,c), a,b,c CCW.
 // use_fp32 enables the FP32 semi-static stage.  stats may be null.
-GCDT_HD int incircle_sign(double ax, double ay, double bx, double by,
-                          double cx, double cy, double dx, double dy,
-                          double *ws, bool use_fp32, PredStats *stats) {
+// Rare stages, out of line so the hot filters keep a small register
+// footprint: 0 = exact zero; stats->dd_hits / exact_calls counted here.
+GCDT_HD_NOINLINE int incircle_slow(double ax, double ay, double bx, double by,
+                                   double cx, double cy, double dx, double dy,
+                                   double *ws, PredStats *stats) {
+  int sd = incircle_dd_sign(ax, ay, bx, by, cx, cy, dx, dy);
+  if (sd != 0) {
+    if (stats) stats->dd_hits++;
+    return sd;
+  }
+  if (stats) stats->exact_calls++;
+  double s = incircle_exact(ax, ay, bx, by, cx, cy, dx, dy, ws);
+  return s > 0.0 ? 1 : (s < 0.0 ? -1 : 0);
+}
+
+// Filter stages only: the sign, or PRED_UNSURE when neither filter decides.
+constexpr int PRED_UNSURE = 2;
+
+GCDT_HD int incircle_filtered(double ax, double ay, double bx, double by,
+                              double cx, double cy, double dx, double dy,
+                              bool use_fp32, PredStats *stats) {
   if (use_fp32) {
     int s = incircle_fp32_sign(ax, ay, bx, by, cx, cy, dx, dy);
     if (s != 0) {
@@ -515,19 +533,32 @@ GCDT_HD int incircle_sign(double ax, double ay, double bx, double by,
     if (stats) stats->fp64_hits++;
     return det > 0.0 ? 1 : -1;
   }
-  int sd = incircle_dd_sign(ax, ay, bx, by, cx, cy, dx, dy);
+  return PRED_UNSURE;
+}
+
+GCDT_HD int incircle_sign(double ax, double ay, double bx, double by,
+                          double cx, double cy, double dx, double dy,
+                          double *ws, bool use_fp32, PredStats *stats) {
+  int s = incircle_filtered(ax, ay, bx, by, cx, cy, dx, dy, use_fp32, stats);
+  if (s != PRED_UNSURE) return s;
+  return incircle_slow(ax, ay, bx, by, cx, cy, dx, dy, ws, stats);
+}
+
+GCDT_HD_NOINLINE int orient2d_slow(double ax, double ay, double bx, double by,
+                                   double cx, double cy, PredStats *stats) {
+  int sd = orient2d_dd_sign(ax, ay, bx, by, cx, cy);
   if (sd != 0) {
     if (stats) stats->dd_hits++;
     return sd;
   }
   if (stats) stats->exact_calls++;
-  double s = incircle_exact(ax, ay, bx, by, cx, cy, dx, dy, ws);
+  double s = orient2d_exact(ax, ay, bx, by, cx, cy);
   return s > 0.0 ? 1 : (s < 0.0 ? -1 : 0);
 }
 
-GCDT_HD int orient2d_sign(double ax, double ay, double bx, double by,
-                          double cx, double cy, bool use_fp32,
-                          PredStats *stats) {
+GCDT_HD int orient2d_filtered(double ax, double ay, double bx, double by,
+                              double cx, double cy, bool use_fp32,
+                              PredStats *stats) {
   if (use_fp32) {
     int s = orient2d_fp32_sign(ax, ay, bx, by, cx, cy);
     if (s != 0) {
@@ -543,14 +574,15 @@ GCDT_HD int orient2d_sign(double ax, double ay, double bx, double by,
     if (stats) stats->fp64_hits++;
     return det > 0.0 ? 1 : -1;
   }
-  int sd = orient2d_dd_sign(ax, ay, bx, by, cx, cy);
-  if (sd != 0) {
-    if (stats) stats->dd_hits++;
-    return sd;
-  }
-  if (stats) stats->exact_calls++;
-  double s = orient2d_exact(ax, ay, bx, by, cx, cy);
-  return s > 0.0 ? 1 : (s < 0.0 ? -1 : 0);
+  return PRED_UNSURE;
+}
+
+GCDT_HD int orient2d_sign(double ax, double ay, double bx, double by,
+                          double cx, double cy, bool use_fp32,
+                          PredStats *stats) {
+  int s = orient2d_filtered(ax, ay, bx, by, cx, cy, use_fp32, stats);
+  if (s != PRED_UNSURE) return s;
+  return orient2d_slow(ax, ay, bx, by, cx, cy, stats);
 }
 
 }  // namespace gcdt

@@ -136,6 +136,10 @@ typedef struct gcdt_options {
                             limit 16384) */
   int grid_claim;        /* 256-lane blocks of the claim kernels (default 2048) */
   int grid_apply;        /* 128-lane blocks of the apply kernel (default 4096) */
+  int cavity_lanes;      /* lanes per candidate in the cavity kernel: 8, 16,
+                            32 or 64 (a wave evaluates 64/cavity_lanes
+                            candidates at once; default 32; widened when the
+                            groups' LDS would not fit) */
 } gcdt_options;
 
 typedef struct gcdt_stats {
@@ -227,15 +231,16 @@ const char *gcdt_batch_last_error(void);
 
 /* Autotuner (no counterpart in the reference, whose launch shapes are fixed
  * at 256-thread blocks, refine.h:19).  Coordinate descent over the launch
- * shapes of the persistent kernels (grid_cavity, grid_apply, grid_claim) and,
- * when options.graph = 1, over the captured graph's structure (graph_rounds,
- * graph_seg_passes), timing `reps` refinements of the resident input per
+ * shapes of the persistent kernels (cavity_lanes, grid_cavity, grid_apply,
+ * grid_claim) and, when options.graph = 1, over the captured graph's
+ * structure (graph_rounds, graph_seg_passes), timing `reps` refinements of the resident input per
  * setting (median device time).  These settings never change the refined
  * mesh.  The best setting is kept for later runs and returned in *best;
  * rec (may be NULL) receives up to max_rec trials; returns the number of
  * trials, or a negative error code. */
 typedef struct gcdt_tune_record {
-  int graph_rounds, graph_seg_passes, grid_cavity, grid_claim, grid_apply;
+  int graph_rounds, graph_seg_passes, grid_cavity, grid_claim, grid_apply,
+      cavity_lanes;
   double ms;
 } gcdt_tune_record;
 int gcdt_session_autotune(gcdt_session *s, int reps, gcdt_options *best,
