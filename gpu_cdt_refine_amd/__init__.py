@@ -48,6 +48,8 @@ class Options(C.Structure):
         ("grid_claim", C.c_int),
         ("grid_apply", C.c_int),
         ("cavity_lanes", C.c_int),
+        ("cav_lds_cap", C.c_int),
+        ("learned_shift", C.c_int),
     ]
 
 
@@ -59,12 +61,14 @@ class TuneRecord(C.Structure):
         ("grid_claim", C.c_int),
         ("grid_apply", C.c_int),
         ("cavity_lanes", C.c_int),
+        ("cav_lds_cap", C.c_int),
+        ("graph", C.c_int),
         ("ms", C.c_double),
     ]
 
 
 TUNE_KEYS = ("graph_rounds", "graph_seg_passes", "grid_cavity", "grid_claim", "grid_apply",
-             "cavity_lanes")
+             "cavity_lanes", "cav_lds_cap", "graph")
 
 
 def _tune_result(n, best, rec):

@@ -55,7 +55,9 @@ void gcdt_default_options(gcdt_options *o) {
   o->grid_cavity = 8192;
   o->grid_claim = 2048;
   o->grid_apply = 4096;
-  o->cavity_lanes = 32;
+  o->cavity_lanes = 16;
+  o->cav_lds_cap = 32;
+  o->learned_shift = 2;
 }
 
 }  // extern "C"
@@ -82,6 +84,14 @@ bool options_ok(const gcdt_options *o, std::string &g_err) {
   }
   if (o->cavity_lanes != 8 && o->cavity_lanes != 16 && o->cavity_lanes != 32 && o->cavity_lanes != 64) {
     g_err = "cavity_lanes must be 8, 16, 32 or 64";
+    return false;
+  }
+  if (o->learned_shift < 0 || o->learned_shift > 12) {
+    g_err = "learned_shift must be in [0, 12]";
+    return false;
+  }
+  if (o->cav_lds_cap < 4 || o->cav_lds_cap > 2048) {
+    g_err = "cav_lds_cap must be in [4, 2048]";
     return false;
   }
   if (o->graph && o->log_training) { g_err = "log_training needs the host-driven loop (graph = 0)"; return false; }

@@ -141,7 +141,8 @@ struct Ctl {
 struct PassD {
   const int *ids;
   int nc;
-  int K;  // cavity capacity (triangles)
+  int K;   // cavity capacity (triangles)
+  int Kf;  // cavity capacity of the filtered kernel's LDS fast path
   int R;  // workspace ints per candidate = 6K+16
   unsigned long long epoch;
   double2 *cpt;

@@ -391,20 +391,25 @@ __device__ __forceinline__ bool hset_has(const int *h, int H, int x) {
 
 enum { SC_N = 0, SC_R, SC_OVF, SC_GIVEUP, SC_NBND, SC_ENC, SC_DSEG, SC_SLOW };
 constexpr int ST_NONE = -1;  // lane group without a candidate
-constexpr int ST_SLOW = 6;   // a sign the filters could not decide
+constexpr int ST_SLOW = 6;   // deferred: an undecided sign, or a cavity beyond
+                             // the fast kernel's LDS capacity
 
-// FAST: every candidate of the pass, with the FP32 / FP64 filters only; a
-// candidate that meets an undecided sign is abandoned without side effects
-// and listed in ps.slow.  !FAST: the listed candidates again, with the full
-// cascade (double-double, then exact expansions in the per-lane arena).  The
-// rare stages' registers thus never limit the occupancy of the hot kernel.
+// FAST: every candidate of the pass, with the FP32 / FP64 filters only and
+// LDS sized for cavities of at most Kf = min(K, ps.Kf) triangles; a candidate
+// that meets an undecided sign or a larger cavity is abandoned without side
+// effects and listed in ps.slow.  !FAST: the listed candidates again, with
+// the full cascade (double-double, then exact expansions in the per-lane
+// arena) and LDS for K triangles.  Neither the rare stages' registers nor the
+// largest cavities' LDS thus limit the occupancy of the hot kernel.
 template <bool IS_SEG, int LG, bool FAST>
 __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
                                             const PassD &ps, int Kmax) {
   static_assert(LG >= 8 && 64 % LG == 0, "lane groups of 8..64 lanes");
   constexpr int NG = 64 / LG;
   extern __shared__ int sm[];
-  const int K = ps.K;
+  // K: the cavity capacity this kernel handles (LDS layout); ps.K: the
+  // pass's capacity (workspace layout, overflow rule)
+  const int K = FAST ? min(ps.K, ps.Kf) : ps.K;
   const int HC = cav_hash_cap(K, LG), HR = ring_hash_cap(K);
   const int lane = threadIdx.x, grp = lane / LG, gl = lane % LG;
   int *L = sm + grp * group_lds_ints(K, LG);
@@ -556,6 +561,7 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
     if (grow) {
       n = sc[SC_N];
       if (FAST && sc[SC_SLOW]) status = ST_SLOW;
+      else if ((sc[SC_OVF] || n > K) && K < ps.K) status = ST_SLOW;  // larger cavity
       else if (sc[SC_OVF] || n > K) status = (K >= Kmax) ? ST_GIVEUP : ST_OVERFLOW;
       else cav_ok = true;
     }
@@ -674,13 +680,13 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
     }
     __syncthreads();
     if (star) {
-      int *G = ps.ws + (size_t)i * ps.R + (2 * K + 2);
+      int *G = ps.ws + (size_t)i * ps.R + (2 * ps.K + 2);
       for (int q = gl; q < nstar; q += LG) {
         int e = eord[q];
         G[q] = eu[e];
-        G[(K + 2) + q] = ev[e];
-        G[2 * (K + 2) + q] = eo[e];
-        G[3 * (K + 2) + q] = es[e];
+        G[(ps.K + 2) + q] = ev[e];
+        G[2 * (ps.K + 2) + q] = eo[e];
+        G[3 * (ps.K + 2) + q] = es[e];
       }
     }
     // ---- outputs ----
@@ -994,16 +1000,19 @@ __global__ void k_predicates(const double *q, int n, int pred_mode, int *o,
   flush_stats(cnt, st);
 }
 
-// Learned priority: cprio = bucket * 256 + quantised predicted conflict
-// degree (smaller predicted degree wins ties inside a bucket).
-__global__ void k_learned_prio(PassD ps, const float *score) {
+// Learned priority: cprio = (bucket >> shift) * 256 + (255 - quantised
+// predicted conflict degree).  Inside a (coarsened) shortest-edge bucket the
+// candidate whose cavity and ring overlap the most other candidates' regions
+// wins: it sits inside a cluster of bad triangles and its insertion clears
+// the most of it, so clusters resolve in fewer rounds.
+__global__ void k_learned_prio(PassD ps, const float *score, int shift) {
   const int nc = pass_nc(ps);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc;
        i += gridDim.x * blockDim.x) {
     if (ps.cstatus[i] != ST_VALID) continue;
     float q = score[i] * 32.f;
-    int qi = q != q ? 255 : (int)fminf(fmaxf(q, 0.f), 255.f);
-    ps.cprio[i] = (ps.cprio[i] << 8) | qi;
+    int qi = q != q ? 0 : (int)fminf(fmaxf(q, 0.f), 255.f);
+    ps.cprio[i] = ((ps.cprio[i] >> shift) << 8) | (255 - qi);
   }
 }
 
@@ -1716,6 +1725,7 @@ int Refiner::select_by_priority(int *ids, int nb, int cap, int **out_ids) {
 PassD Refiner::pass_base(int Kc) const {
   PassD ps{};
   ps.K = Kc;
+  ps.Kf = opt.cav_lds_cap;
   ps.R = 6 * Kc + 16;
   ps.cpt = c_pt;
   ps.cstatus = c_status;
@@ -1780,7 +1790,7 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
   // one candidate per wave64 block; the exact-predicate arena holds
   // pws_blocks*256 lanes, i.e. pws_blocks*4 blocks of 64
-  int lg = cavity_lanes(K);
+  int lg = cavity_lanes(std::min(K, opt.cav_lds_cap));
   int gw = std::min((nc + 64 / lg - 1) / (64 / lg), std::min(opt.grid_cavity, pws_blocks * 4));
   tbegin(is_seg ? K_SEGCAV : K_BADCAV);
   launch_cavity(is_seg, gw, ps);
@@ -1791,7 +1801,7 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
     if (!mlp_params) throw std::runtime_error("priority 2 needs a model (set_priority_model)");
     tbegin(K_MLP);
     mlp_forward_device(c_feat, nc, mlp_params, c_score, stream);
-    k_learned_prio<<<std::min(nblk(nc), 4096), 256, 0, stream>>>(ps, c_score);
+    k_learned_prio<<<std::min(nblk(nc), 4096), 256, 0, stream>>>(ps, c_score, opt.learned_shift);
     tend(K_MLP, nc);
     HC(hipGetLastError());
   }
@@ -2058,18 +2068,20 @@ static void launch_cavity_t(int lg, int grid, size_t lds, hipStream_t s, const M
 // Filtered-predicate pass over all candidates, then the exact-predicate pass
 // over the candidates it deferred (their count stays on the device).
 void Refiner::launch_cavity(bool is_seg, int grid, const PassD &ps) {
-  int lg = cavity_lanes(ps.K);
-  size_t lds = sizeof(int) * (size_t)(64 / lg) * group_lds_ints(ps.K, lg);
-  if (lds > (size_t)BIG_LDS_BYTES)
+  int kf = std::min(ps.K, ps.Kf);
+  int lg = cavity_lanes(kf), lgs = cavity_lanes(ps.K);
+  size_t lds = sizeof(int) * (size_t)(64 / lg) * group_lds_ints(kf, lg);
+  size_t lds_slow = sizeof(int) * (size_t)(64 / lgs) * group_lds_ints(ps.K, lgs);
+  if (lds_slow > (size_t)BIG_LDS_BYTES)
     throw std::runtime_error("cavity capacity " + std::to_string(ps.K) + " needs more LDS than a block has");
   if (grid < 1) grid = 1;
   int slow_grid = std::min(grid, std::min(1024, pws_blocks * 4));
   if (is_seg) {
     launch_cavity_t<true, true>(lg, grid, lds, stream, m, pr, ps, opt.cav_cap_max);
-    launch_cavity_t<true, false>(lg, slow_grid, lds, stream, m, pr, ps, opt.cav_cap_max);
+    launch_cavity_t<true, false>(lgs, slow_grid, lds_slow, stream, m, pr, ps, opt.cav_cap_max);
   } else {
     launch_cavity_t<false, true>(lg, grid, lds, stream, m, pr, ps, opt.cav_cap_max);
-    launch_cavity_t<false, false>(lg, slow_grid, lds, stream, m, pr, ps, opt.cav_cap_max);
+    launch_cavity_t<false, false>(lgs, slow_grid, lds_slow, stream, m, pr, ps, opt.cav_cap_max);
   }
   HC(hipGetLastError());
 }
@@ -2134,7 +2146,7 @@ void Refiner::enqueue_bad_pass() {
     const int *ci = reinterpret_cast<const int *>(reinterpret_cast<const char *>(d_ctl) +
                                                   offsetof(Ctl, i));
     mlp_forward_device_n(c_feat, ci + C_NC, ci + C_GO, mlp_params, c_score, 4096, stream);
-    k_learned_prio<<<4096, 256, 0, stream>>>(ps, c_score);
+    k_learned_prio<<<4096, 256, 0, stream>>>(ps, c_score, opt.learned_shift);
   }
   enqueue_claims(ps, opt.grid_claim);
   k_dcount<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_cnt);

@@ -35,7 +35,9 @@ int autotune(gcdt_options &opt, int reps,
              gcdt_tune_record *rec, int max_rec) {
   if (reps < 1) reps = 1;
   const std::vector<Knob> knobs = {
+      {&gcdt_options::graph, {0, 1}, false},
       {&gcdt_options::cavity_lanes, {8, 16, 32, 64}, false},
+      {&gcdt_options::cav_lds_cap, {16, 32, 64}, false},
       {&gcdt_options::grid_cavity, {2048, 4096, 8192, 16384}, false},
       {&gcdt_options::grid_apply, {1024, 2048, 4096, 8192}, false},
       {&gcdt_options::grid_claim, {1024, 2048, 4096}, false},
@@ -51,6 +53,8 @@ int autotune(gcdt_options &opt, int reps,
       rec[n].grid_claim = o.grid_claim;
       rec[n].grid_apply = o.grid_apply;
       rec[n].cavity_lanes = o.cavity_lanes;
+      rec[n].cav_lds_cap = o.cav_lds_cap;
+      rec[n].graph = o.graph;
       rec[n].ms = ms;
     }
     n++;
@@ -59,7 +63,8 @@ int autotune(gcdt_options &opt, int reps,
   double best_ms = median_ms(best, reps, run);
   log(best, best_ms);
   for (const Knob &k : knobs) {
-    if (k.graph_only && !opt.graph) continue;
+    if (k.graph_only && !best.graph) continue;
+    if (k.field == &gcdt_options::graph && opt.log_training) continue;  // host loop only
     int keep = best.*(k.field);
     for (int v : k.values) {
       if (v == keep) continue;

@@ -109,7 +109,9 @@ typedef struct gcdt_options {
   int max_rounds;        /* refinement round cap, default 100000 */
   int priority;          /* 1 = shortest-edge bucket then index (default);
                             0 = lowest index only (refine.cu:4159);
-                            2 = learned priority (gcdt_set_priority_model) */
+                            2 = learned priority (gcdt_session_set_priority_model):
+                            coarsened bucket, then the MLP's predicted
+                            conflict degree, higher first (learned_shift) */
   int pred_mode;         /* 0 = FP64 filter + exact; 1 = FP32 semi-static,
                             then FP64, then exact */
   int profile;           /* 1 = time every kernel launch with HIP events */
@@ -138,8 +140,16 @@ typedef struct gcdt_options {
   int grid_apply;        /* 128-lane blocks of the apply kernel (default 4096) */
   int cavity_lanes;      /* lanes per candidate in the cavity kernel: 8, 16,
                             32 or 64 (a wave evaluates 64/cavity_lanes
-                            candidates at once; default 32; widened when the
+                            candidates at once; default 16; widened when the
                             groups' LDS would not fit) */
+  int cav_lds_cap;       /* cavities of up to this many triangles are built
+                            in the filtered kernel's LDS; larger ones (and
+                            undecided signs) go to the exact kernel, which
+                            holds cav_cap triangles (default 32) */
+  int learned_shift;     /* learned priority: claims compare shortest-edge
+                            buckets coarsened by 2^learned_shift half-octaves,
+                            then the predicted conflict degree (higher first);
+                            0 keeps Triangle's half-octave order (default 2) */
 } gcdt_options;
 
 typedef struct gcdt_stats {
@@ -231,16 +241,16 @@ const char *gcdt_batch_last_error(void);
 
 /* Autotuner (no counterpart in the reference, whose launch shapes are fixed
  * at 256-thread blocks, refine.h:19).  Coordinate descent over the launch
- * shapes of the persistent kernels (cavity_lanes, grid_cavity, grid_apply,
- * grid_claim) and, when options.graph = 1, over the captured graph's
- * structure (graph_rounds, graph_seg_passes), timing `reps` refinements of the resident input per
+ * shapes of the persistent kernels (cavity_lanes, cav_lds_cap, grid_cavity,
+ * grid_apply, grid_claim), the round driver (graph 0 / 1) and, for the graph
+ * loop, the captured graph's structure (graph_rounds, graph_seg_passes), timing `reps` refinements of the resident input per
  * setting (median device time).  These settings never change the refined
  * mesh.  The best setting is kept for later runs and returned in *best;
  * rec (may be NULL) receives up to max_rec trials; returns the number of
  * trials, or a negative error code. */
 typedef struct gcdt_tune_record {
   int graph_rounds, graph_seg_passes, grid_cavity, grid_claim, grid_apply,
-      cavity_lanes;
+      cavity_lanes, cav_lds_cap, graph;
   double ms;
 } gcdt_tune_record;
 int gcdt_session_autotune(gcdt_session *s, int reps, gcdt_options *best,
