@@ -12,8 +12,9 @@ output mesh in HBM.  Two forms are timed in the same run:
   plain   FP64-filtered exact predicates, one claim iteration, host-driven
           passes (the reference's structure)
   chosen  FP32 -> FP64 -> double-double -> exact predicate cascade, claim
-          iterations, graph-driven persistent loop with autotuned launch
-          shapes, learned claim priority (when the trained model is present)
+          iterations, learned claim priority (when the trained model is
+          present), and the round driver (host- or graph-driven) and launch
+          shapes chosen by the autotuner on this input (untimed)
 value = output triangles / second of wall time of the chosen form.
 
 Prints ONE JSON line.  Also reports:
@@ -127,14 +128,31 @@ def profile_kernels(P, T, N, SS, theta, opts):
     return kt
 
 
-def roofline(kt):
+TRAFFIC = os.path.join(ROOT, "profiles", "traffic_bad_cavity_refine_1M.json")
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated), when one exists for this
+    workload; None otherwise."""
+    try:
+        t = json.load(open(TRAFFIC))
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel or t.get("workload") != workload:
+        return None
+    return t["traffic_bytes_per_launch"]
+
+
+def roofline(kt, workload):
     dom = max((k for k in kt if k in KERNEL_BYTES), key=lambda k: kt[k]["total_ms"])
     d = kt[dom]
     per_launch_ms = d["total_ms"] / max(1, d["launches"])
     bytes_per_launch = KERNEL_BYTES[dom] * d["items"] / max(1, d["launches"])
     achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
     return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(dom, workload),
             "avg_launch_ms": per_launch_ms, "launches": d["launches"],
             "bytes_per_item": KERNEL_BYTES[dom],
             "items_per_launch": d["items"] / max(1, d["launches"])}
@@ -174,10 +192,19 @@ def bench_refine(args):
     log(f"# input: {len(P)} points, {len(T)} triangles, {len(SS)} segments "
         f"(prep {time.time() - t0:.1f}s)")
     kw = dict(theta=args.theta, insert=g.Offcenter, mode=g.RUPPERT)
+    if args.calibration_copy:
+        # a plain 256 MiB device copy (256 MiB read + written): the reference
+        # dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counts
+        import torch
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        torch.empty_like(a).copy_(a)
+        torch.cuda.synchronize()
     # plain GPU path
-    sp = g.Session(P, T, N, SS, **kw, **plain_options())
-    st_p, ms_p, dev_p = timed_runs(sp, args.steps, args.warmup, "plain")
-    sp.close()
+    st_p, ms_p, dev_p = None, None, None
+    if not args.no_plain:
+        sp = g.Session(P, T, N, SS, **kw, **plain_options())
+        st_p, ms_p, dev_p = timed_runs(sp, args.steps, args.warmup, "plain")
+        sp.close()
     # chosen form
     co = chosen_options(args)
     sc = g.Session(P, T, N, SS, **kw, **co)
@@ -198,7 +225,7 @@ def bench_refine(args):
     roof, kt = None, None
     if not args.no_profile_step:
         kt = profile_kernels(P, T, N, SS, args.theta, co)
-        roof = roofline(kt)
+        roof = roofline(kt, "cdt-refine-1M-uniform-50k-segs")
     dropin = None
     if not args.no_dropin:
         # PCIe-inclusive: host CDT in, host mesh out (the drop-in's contract)
@@ -227,10 +254,12 @@ def bench_refine(args):
                    "max_segment_length": args.domain * 0.02, "theta_deg": args.theta,
                    "insert": "offcenter", "mode": "ruppert", "seed": args.seed,
                    "chosen_options": co, "autotune": tune},
-        "plain": {"value": tris and st_p["n_triangles"] / (ms_p / 1e3), "ms_per_step": ms_p,
-                  "device_ms_per_step": dev_p, "output": summary(st_p)},
-        "speedup_vs_plain": ms_p / ms,
-        "steiner_delta_vs_plain": (st["n_points"] - st_p["n_points"]) / max(1, st_p["n_points"] - len(P)),
+        "plain": None if st_p is None else {
+            "value": st_p["n_triangles"] / (ms_p / 1e3), "ms_per_step": ms_p,
+            "device_ms_per_step": dev_p, "output": summary(st_p)},
+        "speedup_vs_plain": None if st_p is None else ms_p / ms,
+        "steiner_delta_vs_plain": None if st_p is None else
+            (st["n_points"] - st_p["n_points"]) / max(1, st_p["n_points"] - len(P)),
         "output": summary(st),
         "roofline": roof,
         "whole_job": whole,
@@ -396,6 +425,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile-step", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="start with a 256 MiB device copy (counter calibration)")
     ap.add_argument("--cpu-sample-points", type=int, default=200_000)
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--batch-max-points", type=int, default=200_000)

@@ -1,19 +1,22 @@
 // Device-side mesh model and per-element refinement steps for gfx950.
 //
-// Mesh layout in HBM (structure of arrays, one 16-byte record per triangle
-// per array so a triangle's vertices, neighbours or subsegments are one
-// aligned 128-bit load):
+// Mesh layout in HBM.  Triangles are 128-byte records, one L2 line each:
+// every kernel that visits a triangle needs its vertices, neighbours,
+// subsegments and vertex coordinates together, and the visits are random
+// gathers, so one line per triangle replaces six (three 16-byte arrays and
+// three point lookups).  Per-triangle flags and claims, which are scanned or
+// updated alone, stay in their own arrays.
+//   T     TriRec[capT]    V (v0, v1, v2, instance) counter-clockwise,
+//                         N neighbour across edge j as otri = tri<<2 | edge
+//                         or -1, S subsegment on edge j or -1, X[3] the
+//                         vertices' coordinates (copies of P)
 //   P     double2[capP]   point coordinates
 //   pflag uint8[capP]     PF_* bits;   ppar int[capP]: parent input segment
-//   TV    int4[capT]      (v0, v1, v2, -) counter-clockwise
-//   TN    int4[capT]      neighbour across edge j as otri = tri<<2 | edge, -1
-//   TS    int4[capT]      subsegment on edge j, -1
 //   tflag uint8[capT]     TF_* bits
 //   SV    int2[capS]      subsegment endpoints (org, dest)
 //   SP    int[capS]       parent input segment;  ST int[capS]: an adjacent otri
 //   senc  int[capS]       0 free, 1 encroached, 2 given up
 //   claim uint64[capT]    epoch-tagged conflict claims (atomicMax)
-//   tinst int[capT]       instance of each triangle (batched refinement only)
 // Edge j of triangle t is (v[(j+1)%3], v[(j+2)%3]); its apex is v[j]
 // (otri encoding as the reference's encode_tri, refine.cu:95-113).
 //
@@ -63,14 +66,23 @@ struct MeshD {
   double2 *P;
   uint8_t *pflag;
   int *ppar;
-  int4 *TV, *TN, *TS;
+  struct TriRec *T;
   uint8_t *tflag;
   int2 *SV;
   int *SP, *ST, *senc;
   const int2 *SEG;
   unsigned long long *claim;
-  int *tinst;  // nullptr: one instance, parameters from ParamsD
+  int batched;  // 0: one instance, parameters from ParamsD; 1: T[t].V.w
 };
+
+struct __align__(128) TriRec {
+  int4 V;          // vertices; w = instance (batched refinement), else 0
+  int4 N;          // neighbour otri across edge j (w unused)
+  int4 S;          // subsegment on edge j (w unused)
+  double2 X[3];    // coordinates of V.x, V.y, V.z
+  int4 pad[2];
+};
+static_assert(sizeof(TriRec) == 128, "one L2 line per triangle");
 
 // Quality parameters of one PSLG instance (the reference's per-call theta,
 // insertpolicy and mode; refine.cu:6557-6567 derives them from theta).
@@ -86,14 +98,14 @@ struct ParamsD {
   double lens;       // (2 cos^2(theta) - 1)^2
   double offconst;   // 0.475 sqrt((1+cos)/(1-cos)), kept in double
   int offcenter, ruppert, use_fp32, priority;
-  const InstPar *ipar;  // per-instance parameters when MeshD::tinst is set
+  const InstPar *ipar;  // per-instance parameters when MeshD::batched is set
 };
 
 // Parameters that apply to triangle t (its instance's, in a batch).
 __device__ __forceinline__ ParamsD params_at(const ParamsD &pr, const MeshD &m,
                                              int t) {
-  if (!m.tinst) return pr;
-  InstPar q = pr.ipar[m.tinst[t]];
+  if (!m.batched) return pr;
+  InstPar q = pr.ipar[m.T[t].V.w];
   ParamsD r = pr;
   r.goodangle = q.goodangle;
   r.lens = q.lens;
@@ -182,8 +194,12 @@ constexpr int CAV_BLOCK = 128;  // threads per block of the apply kernel
 __device__ __forceinline__ int comp(const int4 &v, int j) {
   return j == 0 ? v.x : (j == 1 ? v.y : v.z);
 }
-__device__ __forceinline__ int *compp(int4 *base, int t, int j) {
-  return reinterpret_cast<int *>(base + t) + j;
+// Component j of a triangle's neighbour / subsegment vector, for writes.
+__device__ __forceinline__ int *nbr_slot(TriRec *T, int t, int j) {
+  return &T[t].N.x + j;
+}
+__device__ __forceinline__ int *seg_slot(TriRec *T, int t, int j) {
+  return &T[t].S.x + j;
 }
 
 // checkseg4encroach (refine.cu:115-139, triangle.cpp:7138-7153).
@@ -204,8 +220,9 @@ __device__ __forceinline__ bool encroaches(const MeshD &m, const ParamsD &pr,
 // testtriangle + Miller-Pav-Walkington (refine.cu:376-497,
 // triangle.cpp:7220-7384); org = v1, dest = v2, apex = v0.
 __device__ __forceinline__ bool tri_is_bad(const MeshD &m, const ParamsD &pr,
-                                           int t, int4 v) {
-  double2 O = m.P[v.y], D = m.P[v.z], A = m.P[v.x];
+                                           const TriRec &r) {
+  const int4 v = r.V;
+  double2 O = r.X[1], D = r.X[2], A = r.X[0];
   double dxod = O.x - D.x, dyod = O.y - D.y;
   double dxda = D.x - A.x, dyda = D.y - A.y;
   double dxao = A.x - O.x, dyao = A.y - O.y;
@@ -214,29 +231,30 @@ __device__ __forceinline__ bool tri_is_bad(const MeshD &m, const ParamsD &pr,
   double destlen = dxao * dxao + dyao * dyao;
   double angle;
   int e, b1, b2;
+  double2 B1, B2;
   if ((apexlen < orglen) && (apexlen < destlen)) {
     angle = dxda * dxao + dyda * dyao;
     angle = angle * angle / (orglen * destlen);
-    e = 0; b1 = v.y; b2 = v.z;
+    e = 0; b1 = v.y; b2 = v.z; B1 = O; B2 = D;
   } else if (orglen < destlen) {
     angle = dxod * dxao + dyod * dyao;
     angle = angle * angle / (apexlen * destlen);
-    e = 1; b1 = v.z; b2 = v.x;
+    e = 1; b1 = v.z; b2 = v.x; B1 = D; B2 = A;
   } else {
     angle = dxod * dxda + dyod * dyda;
     angle = angle * angle / (apexlen * orglen);
-    e = 2; b1 = v.x; b2 = v.y;
+    e = 2; b1 = v.x; b2 = v.y; B1 = A; B2 = O;
   }
   if (!(angle > pr.goodangle)) return false;
   if ((m.pflag[b1] & PF_SEGV) && (m.pflag[b2] & PF_SEGV) &&
-      comp(m.TS[t], e) < 0) {
+      comp(r.S, e) < 0) {
     int s1 = m.ppar[b1], s2 = m.ppar[b2];
     int2 g1 = m.SEG[s1], g2 = m.SEG[s2];
     int common = -1;
     if (g1.x == g2.x || g1.x == g2.y) common = g1.x;
     else if (g1.y == g2.x || g1.y == g2.y) common = g1.y;
     if (common >= 0) {
-      double2 J = m.P[common], B1 = m.P[b1], B2 = m.P[b2];
+      double2 J = m.P[common];
       double dist1 = (B1.x - J.x) * (B1.x - J.x) + (B1.y - J.y) * (B1.y - J.y);
       double dist2 = (B2.x - J.x) * (B2.x - J.x) + (B2.y - J.y) * (B2.y - J.y);
       if ((dist1 < 1.001 * dist2) && (dist1 > 0.999 * dist2)) return false;
@@ -247,10 +265,10 @@ __device__ __forceinline__ bool tri_is_bad(const MeshD &m, const ParamsD &pr,
 
 // Circumcenter / Ungor off-center (refine.cu:3780-3868,
 // triangle.cpp:6523-6618).  false if numerically degenerate.
-__device__ __forceinline__ bool steiner_point(const MeshD &m,
-                                              const ParamsD &pr, int4 v,
-                                              double &cx, double &cy) {
-  double2 A = m.P[v.x], O = m.P[v.y], D = m.P[v.z];
+__device__ __forceinline__ bool steiner_point(const ParamsD &pr,
+                                              const TriRec &r, double &cx,
+                                              double &cy) {
+  double2 A = r.X[0], O = r.X[1], D = r.X[2];
   double xdo = D.x - O.x, ydo = D.y - O.y;
   double xao = A.x - O.x, yao = A.y - O.y;
   double dodist = xdo * xdo + ydo * ydo;
@@ -301,8 +319,8 @@ __device__ __forceinline__ int edge_bucket(double key) {
   return b;
 }
 
-__device__ __forceinline__ int shortest_edge_bucket(const MeshD &m, int4 v) {
-  double2 A = m.P[v.x], O = m.P[v.y], D = m.P[v.z];
+__device__ __forceinline__ int shortest_edge_bucket(const TriRec &r) {
+  double2 A = r.X[0], O = r.X[1], D = r.X[2];
   double l0 = (O.x - D.x) * (O.x - D.x) + (O.y - D.y) * (O.y - D.y);
   double l1 = (D.x - A.x) * (D.x - A.x) + (D.y - A.y) * (D.y - A.y);
   double l2 = (A.x - O.x) * (A.x - O.x) + (A.y - O.y) * (A.y - O.y);
@@ -317,9 +335,8 @@ __device__ __forceinline__ void seg_sides(const MeshD &m, int s, int &t1,
   int a = m.SV[s].x;
   int ot = m.ST[s];
   int t = ot >> 2, j = ot & 3;
-  int4 tv = m.TV[t];
-  int u = comp(tv, (j + 1) % 3);
-  int n = comp(m.TN[t], j);
+  int u = comp(m.T[t].V, (j + 1) % 3);
+  int n = comp(m.T[t].N, j);
   int tb = -1, jb = -1;
   if (n >= 0) { tb = n >> 2; jb = n & 3; }
   if (u == a) { t1 = t; j1 = j; t2 = tb; j2 = jb; }
@@ -335,12 +352,12 @@ __device__ __forceinline__ bool split_point(const MeshD &m, int s, double &px,
   seg_sides(m, s, t1, j1, t2, j2);
   bool acuteorg = false, acutedest = false;
   if (t1 >= 0) {
-    int4 ts = m.TS[t1];
+    int4 ts = m.T[t1].S;
     if (comp(ts, (j1 + 2) % 3) >= 0) acuteorg = true;
     if (comp(ts, (j1 + 1) % 3) >= 0) acutedest = true;
   }
   if (t2 >= 0) {
-    int4 ts = m.TS[t2];
+    int4 ts = m.T[t2].S;
     if (comp(ts, (j2 + 1) % 3) >= 0) acuteorg = true;
     if (comp(ts, (j2 + 2) % 3) >= 0) acutedest = true;
   }
@@ -380,8 +397,8 @@ __device__ __forceinline__ int walk(const MeshD &m, int t, double cx,
                                     double cy, bool fp32, PredStats *st,
                                     int &tout, int &jout, int &seg) {
   for (int steps = 0; steps < GCDT_WALK_LIMIT; steps++) {
-    int4 v = m.TV[t];
-    double2 p0 = m.P[v.x], p1 = m.P[v.y], p2 = m.P[v.z];
+    const TriRec &r = m.T[t];
+    double2 p0 = r.X[0], p1 = r.X[1], p2 = r.X[2];
     int o0, o1, o2;
     if (FAST) {
       o0 = orient2d_filtered(p1.x, p1.y, p2.x, p2.y, cx, cy, fp32, st);
@@ -404,9 +421,9 @@ __device__ __forceinline__ int walk(const MeshD &m, int t, double cx,
       }
       return 2;
     }
-    int s = comp(m.TS[t], jn);
+    int s = comp(r.S, jn);
     if (s >= 0) { seg = s; tout = t; return 3; }
-    int nb = comp(m.TN[t], jn);
+    int nb = comp(r.N, jn);
     if (nb < 0) return 4;
     t = nb >> 2;
   }

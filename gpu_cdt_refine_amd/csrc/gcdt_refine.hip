@@ -154,13 +154,12 @@ __global__ __launch_bounds__(CB) void k_scatter(Pred pred, int n,
 // ---------------------------------------------------------------------------
 // Initialisation kernels
 // ---------------------------------------------------------------------------
-__global__ void k_init_tris(const int *tris, const int *nbrs, int nt,
-                            int4 *TV, int4 *TN, int4 *TS, uint8_t *tflag,
+__global__ void k_init_tris(const int *tris, const int *nbrs, const int *tinst,
+                            const double2 *P, int nt, TriRec *T, uint8_t *tflag,
                             unsigned long long *claim, unsigned long long *cnt) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
   int v0 = tris[3 * t], v1 = tris[3 * t + 1], v2 = tris[3 * t + 2];
-  TV[t] = make_int4(v0, v1, v2, 0);
   int code[3];
   int vv[3] = {v0, v1, v2};
   for (int j = 0; j < 3; j++) {
@@ -174,8 +173,15 @@ __global__ void k_init_tris(const int *tris, const int *nbrs, int nt,
       if (code[j] < 0) atomicAdd(&cnt[CNT_ERRORS], 1ull);
     }
   }
-  TN[t] = make_int4(code[0], code[1], code[2], 0);
-  TS[t] = make_int4(-1, -1, -1, 0);
+  TriRec r;
+  r.V = make_int4(v0, v1, v2, tinst ? tinst[t] : 0);
+  r.N = make_int4(code[0], code[1], code[2], 0);
+  r.S = make_int4(-1, -1, -1, 0);
+  r.X[0] = P[v0];
+  r.X[1] = P[v1];
+  r.X[2] = P[v2];
+  r.pad[0] = r.pad[1] = make_int4(0, 0, 0, 0);
+  T[t] = r;
   tflag[t] = TF_ALIVE | TF_CHECK;
   claim[t] = 0ull;
 }
@@ -221,7 +227,7 @@ __global__ void k_attach_segs(MeshD m, int nt, const unsigned long long *hkey,
                               const int *hval, unsigned long long hmask) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
-  int4 tv = m.TV[t];
+  int4 tv = m.T[t].V;
   for (int j = 0; j < 3; j++) {
     int a = comp(tv, (j + 1) % 3), b = comp(tv, (j + 2) % 3);
     unsigned long long key = edge_key(a, b);
@@ -229,7 +235,7 @@ __global__ void k_attach_segs(MeshD m, int nt, const unsigned long long *hkey,
     while (hkey[h] != ~0ull) {
       if (hkey[h] == key) {
         int s = hval[h];
-        *compp(m.TS, t, j) = s;
+        *seg_slot(m.T, t, j) = s;
         int code = (t << 2) | j;
         if (a == m.SEG[s].x) atomicExch(&m.ST[s], code);
         else atomicCAS(&m.ST[s], -1, code);
@@ -243,12 +249,12 @@ __global__ void k_attach_segs(MeshD m, int nt, const unsigned long long *hkey,
 __global__ void k_init_enc(MeshD m, ParamsD pr, int nt) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
-  int4 ts = m.TS[t];
-  int4 tv = m.TV[t];
+  const TriRec &r = m.T[t];
+  int4 ts = r.S;
   for (int j = 0; j < 3; j++) {
     int s = comp(ts, j);
     if (s < 0) continue;
-    double2 p = m.P[comp(tv, j)];
+    double2 p = r.X[j];
     int2 sv = m.SV[s];
     if (encroaches(m, params_at(pr, m, t), sv.x, sv.y, p.x, p.y)) m.senc[s] = 1;
   }
@@ -278,7 +284,7 @@ __global__ __launch_bounds__(256) void k_mark_bad(MeshD m, ParamsD pr, int nt,
     uint8_t f = m.tflag[t];
     if (!(f & TF_ALIVE) || !(f & TF_CHECK)) continue;
     f &= (uint8_t)~TF_CHECK;
-    if (tri_is_bad(m, params_at(pr, m, t), t, m.TV[t])) f |= TF_BAD;
+    if (tri_is_bad(m, params_at(pr, m, t), m.T[t])) f |= TF_BAD;
     else f &= (uint8_t)~TF_BAD;
     m.tflag[t] = f;
   }
@@ -316,13 +322,21 @@ enum { CC_MARKS = 0, CC_GIVEUPS, CC_OVERFLOW, CC_REJ_ENC, CC_REJ_BLOCKED, CC_VAL
        CC_FP32, CC_FP64, CC_DD, CC_EXACT, CC_N };
 struct CavCounts {
   unsigned c[CC_N];
+  __device__ void add(unsigned long long *cnt, int k, int dst) {
+    unsigned long long v = wave_sum(c[k]);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&cnt[dst], v);
+  }
   __device__ void flush(unsigned long long *cnt) {
-    const int dst[CC_N] = {CNT_MARKS, CNT_GIVEUPS, CNT_OVERFLOW, CNT_REJ_ENC, CNT_REJ_BLOCKED,
-                           CNT_VALID, CNT_FP32, CNT_FP64, CNT_DD, CNT_EXACT};
-    for (int k = 0; k < CC_N; k++) {
-      unsigned long long v = wave_sum(c[k]);
-      if ((threadIdx.x & 63) == 0 && v) atomicAdd(&cnt[dst[k]], v);
-    }
+    add(cnt, CC_MARKS, CNT_MARKS);
+    add(cnt, CC_GIVEUPS, CNT_GIVEUPS);
+    add(cnt, CC_OVERFLOW, CNT_OVERFLOW);
+    add(cnt, CC_REJ_ENC, CNT_REJ_ENC);
+    add(cnt, CC_REJ_BLOCKED, CNT_REJ_BLOCKED);
+    add(cnt, CC_VALID, CNT_VALID);
+    add(cnt, CC_FP32, CNT_FP32);
+    add(cnt, CC_FP64, CNT_FP64);
+    add(cnt, CC_DD, CNT_DD);
+    add(cnt, CC_EXACT, CNT_EXACT);
   }
 };
 
@@ -419,7 +433,6 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
   int *eu = sc + 8, *ev = eu + (K + 2), *eo = ev + (K + 2), *es = eo + (K + 2);
   int *esucc = es + (K + 2), *eord = esucc + (K + 2);
   const bool fp32 = pr.use_fp32 != 0;
-  PredStats junk = {0, 0, 0, 0};
   CavCounts cc;
   for (int k = 0; k < CC_N; k++) cc.c[k] = 0u;
   double *pws = ps.pws + ((size_t)blockIdx.x * 64 + lane) * GCDT_INCIRCLE_WS;
@@ -464,13 +477,21 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
       } else {
         t0 = ids[i];
         lp = params_at(pr, m, t0);
-        int4 v0 = m.TV[t0];
-        if (pr.priority == 1 || pr.priority == 2) prio = shortest_edge_bucket(m, v0);
-        if (!steiner_point(m, lp, v0, cx, cy)) {
+        const TriRec &r0 = m.T[t0];
+        if (pr.priority == 1 || pr.priority == 2) prio = shortest_edge_bucket(r0);
+        if (!steiner_point(lp, r0, cx, cy)) {
           status = ST_GIVEUP;
         } else {
           int tl = -1, jl = -1, sg = -1;
-          int w = walk<FAST>(m, t0, cx, cy, fp32, gl == 0 ? &cst : &junk, tl, jl, sg);
+          // the walk's signs are counted once per group (every lane walks)
+          PredStats wst = {0, 0, 0, 0};
+          int w = walk<FAST>(m, t0, cx, cy, fp32, &wst, tl, jl, sg);
+          if (gl == 0) {
+            cst.fp32_hits += wst.fp32_hits;
+            cst.fp64_hits += wst.fp64_hits;
+            cst.dd_hits += wst.dd_hits;
+            cst.exact_calls += wst.exact_calls;
+          }
           if (w == 5) {
             status = ST_SLOW;
           } else if (w == 3) {
@@ -481,12 +502,12 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
           } else {
             seed0 = tl;
             if (w == 1) {
-              int s = comp(m.TS[tl], jl);
+              int s = comp(m.T[tl].S, jl);
               if (s >= 0) {
                 status = ST_REJ_BLOCKED;
                 if (gl == 0 && mark_enc_local(m, s)) cc.c[CC_MARKS]++;
               } else {
-                int nb = comp(m.TN[tl], jl);
+                int nb = comp(m.T[tl].N, jl);
                 if (nb < 0) status = ST_GIVEUP;
                 else seed1 = nb >> 2;
               }
@@ -525,13 +546,13 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
         int items = (fe - fb) * 3;
         for (int it = gl; it < items; it += LG) {
           int t = L[fb + it / 3], j = it % 3;
-          if (comp(m.TS[t], j) >= 0) continue;  // subsegments are never crossed
-          int nb = comp(m.TN[t], j);
+          if (comp(m.T[t].S, j) >= 0) continue;  // subsegments are never crossed
+          int nb = comp(m.T[t].N, j);
           if (nb < 0) continue;
           int u = nb >> 2;
           if (hset_has(hc, HC, u)) continue;
-          int4 w = m.TV[u];
-          double2 a = m.P[w.x], b = m.P[w.y], c = m.P[w.z];
+          const TriRec &ru = m.T[u];
+          double2 a = ru.X[0], b = ru.X[1], c = ru.X[2];
           int ins = in_circle(a, b, c, cx, cy, &cst);
           if (FAST && unsure) {
             sc[SC_SLOW] = 1;
@@ -581,13 +602,14 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
     if (cav_ok)
       for (int it = gl; it < 3 * n; it += LG) {
         int t = L[it / 3], j = it % 3;
-        int s = comp(m.TS[t], j);
-        int nb = comp(m.TN[t], j);
+        const TriRec &rt = m.T[t];
+        int s = comp(rt.S, j);
+        int nb = comp(rt.N, j);
         bool inside = nb >= 0 && hset_has(hc, HC, nb >> 2);
         if (s >= 0 && s != split_s && inside) sc[SC_GIVEUP] = 1;
         bool bnd = (s >= 0 && s != split_s) || nb < 0 || !inside;
         if (bnd) {
-          int4 tv = m.TV[t];
+          int4 tv = rt.V;
           int bpos = atomicAdd(&sc[SC_NBND], 1);
           if (bpos < K + 2) {
             eu[bpos] = comp(tv, (j + 1) % 3);
@@ -596,7 +618,7 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
             es[bpos] = s;
           }
           if (!(s >= 0 && s == split_s)) {
-            double2 u = m.P[comp(tv, (j + 1) % 3)], v = m.P[comp(tv, (j + 2) % 3)];
+            double2 u = rt.X[(j + 1) % 3], v = rt.X[(j + 2) % 3];
             int o = orient(cx, cy, u, v, &cst);
             if (FAST && unsure) {
               sc[SC_SLOW] = 1;
@@ -636,7 +658,7 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
         // its cavity boundary is not inserted; every such subsegment is marked.
         for (int it = gl; it < 3 * n; it += LG) {
           int t = L[it / 3], j = it % 3;
-          int s = comp(m.TS[t], j);
+          int s = comp(m.T[t].S, j);
           if (s < 0) continue;
           int2 sv = m.SV[s];
           if (encroaches(m, lp, sv.x, sv.y, cx, cy) && mark_enc_local(m, s)) cc.c[CC_MARKS]++;
@@ -706,8 +728,8 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
       // per-candidate features of the learned priority
       float l0 = 0.f, l1 = 0.f, l2 = 0.f, area2 = 0.f;
       if (status == ST_VALID) {
-        int4 v0 = m.TV[t0];
-        double2 A = m.P[v0.x], O = m.P[v0.y], D = m.P[v0.z];
+        const TriRec &r0 = m.T[t0];
+        double2 A = r0.X[0], O = r0.X[1], D = r0.X[2];
         l0 = (float)((O.x - D.x) * (O.x - D.x) + (O.y - D.y) * (O.y - D.y));
         l1 = (float)((D.x - A.x) * (D.x - A.x) + (D.y - A.y) * (D.y - A.y));
         l2 = (float)((A.x - O.x) * (A.x - O.x) + (A.y - O.y) * (A.y - O.y));
@@ -858,7 +880,7 @@ __global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
     int n = ps.cn[i];
     const int *L = ps.ws + (size_t)i * ps.R;
     int k = n + 2;
-    const int inst = m.tinst ? m.tinst[L[0]] : 0;
+    const int inst = m.T[L[0]].V.w;
     const ParamsD lp = params_at(pr, m, L[0]);
     int *Bu = ps.ws + (size_t)i * ps.R + (2 * ps.K + 2);
     int *Bv = Bu + (ps.K + 2);
@@ -889,25 +911,33 @@ __global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
       int pv = (q > 0) ? q - 1 : (open ? -1 : nstar - 1);
       int snx = nx < 0 ? -1 : (nx < n ? L[nx] : tb + (nx - n));
       int spv = pv < 0 ? -1 : (pv < n ? L[pv] : tb + (pv - n));
-      m.TV[st] = make_int4(p, Bu[q], Bv[q], 0);
-      m.TN[st] = make_int4(Bo[q], snx >= 0 ? (snx << 2) | 2 : -1,
-                           spv >= 0 ? (spv << 2) | 1 : -1, 0);
-      m.TS[st] = make_int4(Bs[q], (Bv[q] == a) ? sA : (Bv[q] == b) ? sB : -1,
-                           (Bu[q] == a) ? sA : (Bu[q] == b) ? sB : -1, 0);
+      TriRec r;
+      r.V = make_int4(p, Bu[q], Bv[q], inst);
+      r.N = make_int4(Bo[q], snx >= 0 ? (snx << 2) | 2 : -1, spv >= 0 ? (spv << 2) | 1 : -1, 0);
+      r.S = make_int4(Bs[q], (Bv[q] == a) ? sA : (Bv[q] == b) ? sB : -1,
+                      (Bu[q] == a) ? sA : (Bu[q] == b) ? sB : -1, 0);
+      r.X[0] = c;
+      r.X[1] = m.P[Bu[q]];
+      r.X[2] = m.P[Bv[q]];
+      r.pad[0] = r.pad[1] = make_int4(0, 0, 0, 0);
+      m.T[st] = r;
       m.tflag[st] = TF_ALIVE | TF_CHECK;
-      if (m.tinst) m.tinst[st] = inst;
     }
     if (open) {
       int st = tb + 1;
-      m.TV[st] = make_int4(-1, -1, -1, 0);
-      m.TN[st] = make_int4(-1, -1, -1, 0);
-      m.TS[st] = make_int4(-1, -1, -1, 0);
+      TriRec r;
+      r.V = make_int4(-1, -1, -1, 0);
+      r.N = make_int4(-1, -1, -1, 0);
+      r.S = make_int4(-1, -1, -1, 0);
+      r.X[0] = r.X[1] = r.X[2] = make_double2(0.0, 0.0);
+      r.pad[0] = r.pad[1] = make_int4(0, 0, 0, 0);
+      m.T[st] = r;
       m.tflag[st] = 0;
     }
     for (int q = 0; q < nstar; q++) {
       int st = q < n ? L[q] : tb + (q - n);
       int o = Bo[q];
-      if (o >= 0) *compp(m.TN, o >> 2, o & 3) = (st << 2) | 0;
+      if (o >= 0) *nbr_slot(m.T, o >> 2, o & 3) = (st << 2) | 0;
       int s = Bs[q];
       if (s >= 0) {
         m.ST[s] = (st << 2) | 0;
@@ -965,7 +995,7 @@ __global__ void k_export(MeshD m, int nt, const int *newid, int *otri,
   uint8_t f = m.tflag[t];
   if (!(f & TF_ALIVE)) return;
   int q = newid[t];
-  int4 tv = m.TV[t], tn = m.TN[t];
+  int4 tv = m.T[t].V, tn = m.T[t].N;
   otri[3 * q] = tv.x;
   otri[3 * q + 1] = tv.y;
   otri[3 * q + 2] = tv.z;
@@ -974,7 +1004,7 @@ __global__ void k_export(MeshD m, int nt, const int *newid, int *otri,
   onbr[3 * q + 2] = tn.z < 0 ? -1 : newid[tn.z >> 2];
   // TStatus bits: 0 non-null, 1 bad, 2 check
   otstat[q] = (uint8_t)(1 | ((f & TF_BAD) ? 2 : 0) | ((f & TF_CHECK) ? 4 : 0));
-  if (oinst) oinst[q] = m.tinst[t];
+  if (oinst) oinst[q] = tv.w;
 }
 
 __global__ void k_pstatus(const uint8_t *pflag, int np, uint8_t *out) {
@@ -1327,7 +1357,7 @@ __global__ __launch_bounds__(256) void k_dprio_hist(MeshD m, const int *ids,
   __syncthreads();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
-    int p = shortest_edge_bucket(m, m.TV[ids[i]]);
+    int p = shortest_edge_bucket(m.T[ids[i]]);
     prio[i] = p;
     atomicAdd(&h[p], 1u);
   }
@@ -1473,9 +1503,8 @@ Refiner::~Refiner() {
 
 void Refiner::free_mesh() {
   dfree(m.P); dfree(m.pflag); dfree(m.ppar);
-  dfree(m.TV); dfree(m.TN); dfree(m.TS); dfree(m.tflag); dfree(m.claim);
+  dfree(m.T); dfree(m.tflag); dfree(m.claim);
   dfree(m.SV); dfree(m.SP); dfree(m.ST); dfree(m.senc);
-  dfree(m.tinst);
   capP = capT = capS = 0;
 }
 
@@ -1551,11 +1580,8 @@ void Refiner::ensure_capacity(long long needP, long long needT, long long needS)
   }
   if (needT > capT) {
     long long c = std::max<long long>(needT, capT + capT / 2);
-    dgrow(&m.TV, (size_t)nt, (size_t)c, stream);
-    dgrow(&m.TN, (size_t)nt, (size_t)c, stream);
-    dgrow(&m.TS, (size_t)nt, (size_t)c, stream);
+    dgrow(&m.T, (size_t)nt, (size_t)c, stream);
     dgrow(&m.tflag, (size_t)nt, (size_t)c, stream);
-    if (m.tinst) dgrow(&m.tinst, (size_t)nt, (size_t)c, stream);
     unsigned long long *nc;
     dalloc(&nc, (size_t)c);
     HC(hipMemsetAsync(nc, 0, sizeof(unsigned long long) * c, stream));
@@ -1678,7 +1704,7 @@ __global__ __launch_bounds__(256) void k_prio_hist(MeshD m, const int *ids,
   __syncthreads();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
-    int p = shortest_edge_bucket(m, m.TV[ids[i]]);
+    int p = shortest_edge_bucket(m.T[ids[i]]);
     prio[i] = p;
     atomicAdd(&h[p], 1u);
   }
@@ -1865,14 +1891,13 @@ void Refiner::run() {
   long long wantP = std::max<long long>(16, 6LL * np0);
   long long wantT = std::max<long long>(16, 6LL * nt0);
   long long wantS = std::max<long long>(16, 8LL * ns0 + 16);
-  bool want_inst = in_tinst != nullptr;
-  if (capP < wantP || capT < wantT || capS < wantS || (want_inst != (m.tinst != nullptr))) {
+  const bool want_inst = in_tinst != nullptr;
+  if (capP < wantP || capT < wantT || capS < wantS) {
     free_mesh();
     capP = wantP; capT = wantT; capS = wantS;
     dalloc(&m.P, capP); dalloc(&m.pflag, capP); dalloc(&m.ppar, capP);
-    dalloc(&m.TV, capT); dalloc(&m.TN, capT); dalloc(&m.TS, capT);
+    dalloc(&m.T, capT);
     dalloc(&m.tflag, capT); dalloc(&m.claim, capT);
-    if (want_inst) dalloc(&m.tinst, capT);
     dalloc(&m.SV, capS); dalloc(&m.SP, capS); dalloc(&m.ST, capS); dalloc(&m.senc, capS);
   }
   if (ns0 > seg_in_cap || !seg_in) {
@@ -1885,10 +1910,10 @@ void Refiner::run() {
   HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
   tbegin(K_INIT);
   HC(hipMemcpyAsync(m.P, in_P, sizeof(double2) * np0, hipMemcpyDeviceToDevice, stream));
-  if (want_inst)
-    HC(hipMemcpyAsync(m.tinst, in_tinst, sizeof(int) * nt0, hipMemcpyDeviceToDevice, stream));
+  m.batched = want_inst ? 1 : 0;
   k_init_points<<<nblk(np0), 256, 0, stream>>>(np0, m.pflag, m.ppar);
-  k_init_tris<<<nblk(nt0), 256, 0, stream>>>(in_T, in_N, nt0, m.TV, m.TN, m.TS, m.tflag, m.claim, d_cnt);
+  k_init_tris<<<nblk(nt0), 256, 0, stream>>>(in_T, in_N, in_tinst, in_P, nt0, m.T, m.tflag,
+                                              m.claim, d_cnt);
   // segment hash: capacity power of two >= 2*ns + 16
   unsigned long long hc = 1;
   while (hc < 2ull * (unsigned long long)ns0 + 16) hc <<= 1;
@@ -1948,11 +1973,11 @@ void Refiner::run() {
     dalloc(&o_T, 3 * (size_t)o_cap);
     dalloc(&o_N, 3 * (size_t)o_cap);
     dalloc(&o_tstat, (size_t)o_cap);
-    if (m.tinst) dalloc(&o_inst, (size_t)o_cap);
+    if (m.batched) dalloc(&o_inst, (size_t)o_cap);
   }
-  if (m.tinst && !o_inst) dalloc(&o_inst, (size_t)std::max(o_cap, 1));
+  if (m.batched && !o_inst) dalloc(&o_inst, (size_t)std::max(o_cap, 1));
   tbegin(K_EXPORT);
-  k_export<<<nblk(nt), 256, 0, stream>>>(m, nt, newid, o_T, o_N, o_tstat, m.tinst ? o_inst : nullptr);
+  k_export<<<nblk(nt), 256, 0, stream>>>(m, nt, newid, o_T, o_N, o_tstat, m.batched ? o_inst : nullptr);
   tend(K_EXPORT, nt);
   HC(hipGetLastError());
   HC(hipEventRecord(ev1, stream));
@@ -2304,7 +2329,7 @@ void Refiner::download(MeshHost &out) {
   if (ns) HC(hipMemcpyAsync(out.S.data(), m.SV, sizeof(int2) * ns, hipMemcpyDeviceToHost, stream));
   HC(hipMemcpyAsync(out.pstat.data(), o_pstat, np, hipMemcpyDeviceToHost, stream));
   out.tinst.clear();
-  if (m.tinst && nt_out) {
+  if (m.batched && nt_out) {
     out.tinst.resize((size_t)nt_out);
     HC(hipMemcpyAsync(out.tinst.data(), o_inst, sizeof(int) * nt_out, hipMemcpyDeviceToHost, stream));
   }
