@@ -79,7 +79,8 @@ def plain_options():
 
 
 def chosen_options(args):
-    o = dict(pred_mode=1, claim_iters=args.claim_iters, priority=1, graph=1)
+    # the autotuner may switch to the graph-driven loop (graph=1)
+    o = dict(pred_mode=1, claim_iters=args.claim_iters, priority=1, graph=0)
     if args.learned and os.path.exists(MODEL):
         o["priority"] = 2
     return o
@@ -184,7 +185,19 @@ def cpu_baseline(args, n_sample):
         return {"error": str(e)}
 
 
+def calibration_copy():
+    """A plain 256 MiB device copy (256 MiB read + 256 MiB written): the
+    reference dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counts.
+    Runs before the library touches the device."""
+    import torch
+    a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    torch.empty_like(a).copy_(a)
+    torch.cuda.synchronize()
+
+
 def bench_refine(args):
+    if args.calibration_copy:
+        calibration_copy()
     import gpu_cdt_refine_amd as g
     t0 = time.time()
     P, T, N, SS = make_input(args.points, args.segments, args.domain, 0.02 * args.domain, 0,
@@ -192,13 +205,6 @@ def bench_refine(args):
     log(f"# input: {len(P)} points, {len(T)} triangles, {len(SS)} segments "
         f"(prep {time.time() - t0:.1f}s)")
     kw = dict(theta=args.theta, insert=g.Offcenter, mode=g.RUPPERT)
-    if args.calibration_copy:
-        # a plain 256 MiB device copy (256 MiB read + written): the reference
-        # dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counts
-        import torch
-        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
-        torch.empty_like(a).copy_(a)
-        torch.cuda.synchronize()
     # plain GPU path
     st_p, ms_p, dev_p = None, None, None
     if not args.no_plain:
