@@ -317,6 +317,13 @@ def batch_instances(k, seed, max_points):
     return ns, dist, theta, mode
 
 
+def _triangle_q_count(a):
+    from oracle import pyoracle
+    P, T, SS, theta, mode = a
+    Po, To, No, So = pyoracle.triangle_refine(P, T, SS, theta, mode=mode)
+    return len(To)
+
+
 def _prep_instance(a):
     n, d, s = a
     L = 1000.0 * (n / 10000) ** 0.5
@@ -333,12 +340,14 @@ def bench_batch(args):
         ins = list(ex.map(_prep_instance, jobs, chunksize=8))
     log(f"# {len(ins)} instances, {sum(len(x[0]) for x in ins)} points (prep {time.time() - t0:.1f}s)")
     co = chosen_options(args)
-    co["priority"] = 1
     b = g.Batch(ins, theta=theta, insert=1, mode=mode, **co)
+    if co["priority"] == 2:
+        b.set_priority_model(np.load(MODEL))
     best = None
     if not args.no_autotune:
         best, trials = b.autotune(reps=1)
         log(f"# autotune: {best}")
+    n_waves = b.waves()
     for i in range(args.warmup):
         b.run()
     runs = []
@@ -353,21 +362,35 @@ def bench_batch(args):
            "value": st["n_triangles"] / (ms / 1e3), "unit": "triangles/s", "n_gpus": 1,
            "ms_per_step": ms, "higher_is_better": True,
            "config": {"workload": f"batch-{args.instances}-pslgs", "instances": args.instances,
-                      "max_points": args.batch_max_points, "waves": None, "autotune": best},
+                      "input_points": int(sum(len(x[0]) for x in ins)),
+                      "max_points": args.batch_max_points, "waves": n_waves, "autotune": best},
            "output": summary(st)}
+    # exact invariants on a few instances (the oracle's checker)
+    from oracle import pyoracle
+    checked = []
+    for i in range(0, len(ins), max(1, len(ins) // 4)):
+        P, T, N, SS = ins[i]
+        r = res[i]
+        tot, viol, mina, ex = pyoracle.check_mesh(r.points, r.triangles, r.neighbors, r.segments,
+                                                  P, SS, float(theta[i]))
+        checked.append({"instance": i, "violations": int(tot), "min_angle": mina})
+    out["checked_instances"] = checked
     if not args.no_cpu_baseline:
-        from oracle import pyoracle
-        sample = list(range(0, len(ins), max(1, len(ins) // 64)))
+        # the reference's Triangle -q on a sample of the instances, one process
+        # per core (Triangle is sequential and keeps global state)
+        sample = list(range(0, len(ins), max(1, len(ins) // 48)))
+        jobs = [(ins[i][0], ins[i][1], ins[i][3], float(theta[i]), int(mode[i])) for i in sample]
+        cores = min(16, os.cpu_count() or 1)
         t1 = time.time()
-        tris = 0
-        for i in sample:
-            P, T, N, SS = ins[i]
-            Po, To, No, So = pyoracle.triangle_refine(P, T, SS, float(theta[i]), mode=int(mode[i]))
-            tris += len(To)
+        with ProcessPoolExecutor(max_workers=cores) as ex:
+            tris = sum(ex.map(_triangle_q_count, jobs))
         dt = time.time() - t1
-        out["cpu_baseline"] = {"value": tris / dt, "unit": "triangles/s", "cores": 1,
+        gpu_tris_sample = sum(len(res[i].triangles) for i in sample)
+        out["cpu_baseline"] = {"value": tris / dt, "unit": "triangles/s", "cores": cores,
                                "kind": "reference",
-                               "sample": f"Triangle -q on {len(sample)} of the instances, one core"}
+                               "sample": f"Triangle 1.6 -q on {len(sample)} of the {len(ins)} "
+                                         f"instances ({tris} output triangles; the GPU made "
+                                         f"{gpu_tris_sample} on them), {cores} processes, {dt:.1f} s"}
     print(json.dumps(out))
 
 
@@ -381,7 +404,10 @@ def bench_precision(args):
     n = args.points
     P, T, N, SS = make_input(n, n // 50, 1000.0, 20.0, 2, args.seed)
     log(f"# input: {len(P)} points, {len(T)} triangles, {len(SS)} segments")
-    s = g.Session(P, T, N, SS, theta=25.0, pred_mode=1, claim_iters=args.claim_iters, graph=1)
+    co = chosen_options(args)
+    s = g.Session(P, T, N, SS, theta=25.0, **co)
+    if co["priority"] == 2:
+        s.set_priority_model(np.load(MODEL))
     st, ms, dev = timed_runs(s, args.steps, args.warmup, "precision")
     r = s.download()
     s.close()

@@ -523,12 +523,13 @@ extern "C" int gcdt_generate_pslg(int dist, int n, int nseg, double L,
   int i = 0;
   // lattice bookkeeping for the precision-stress mix (dist == 2)
   int side = 0;
+  int n_lattice = 0;     // points [0, n_lattice) are lattice points
   std::vector<int> lat;  // lattice cell -> point index or -1
   if (dist == 2) {
     // Precision stress: thirds of (a) a random third of the cells of a
     // side x side integer lattice (side = 1000 for 10^6 points) scaled to
-    // [0,L]^2, with 1e-12 relative jitter, (b) points on up to 2000 random
-    // circles, cocircular in exact rational arithmetic before rounding to
+    // [0,L]^2, with 1e-12 relative jitter, (b) points on random circles, 16
+    // per circle, cocircular in exact rational arithmetic before rounding to
     // FP64 (x = cx + r(1-u^2)/(1+u^2), y = cy +- 2ru/(1+u^2), u = k/1000),
     // and (c) uniform background points.
     int nl = n / 3, nc = n / 3;
@@ -547,8 +548,9 @@ extern "C" int gcdt_generate_pslg(int dist, int n, int nseg, double L,
       y += y * 1e-12 * (2 * rng.uniform() - 1);
       if (accept(i, x, y)) lat[c] = i++;
     }
-    int ncirc = std::max(1, std::min(2000, nc / 8));
-    int per = std::max(3, nc / ncirc);
+    n_lattice = i;
+    int per = 16;
+    int ncirc = std::max(1, nc / per);
     for (int c = 0; c < ncirc && i < nl + nc; c++) {
       double r = L * (0.002 + 0.02 * rng.uniform());
       double cx = r + (L - 2 * r) * rng.uniform(), cy = r + (L - 2 * r) * rng.uniform();
@@ -617,6 +619,11 @@ extern "C" int gcdt_generate_pslg(int dist, int n, int nseg, double L,
       j = (int)rng.below((uint64_t)n);
     }
     if (i == j) continue;
+    // lattice points are joined only along lattice lines to their next
+    // lattice neighbour: a longer lattice-to-lattice segment would pass
+    // within the jitter (~1e-10) of the lattice points between its ends,
+    // a feature no refinement resolves in double precision
+    if (k >= nlat_seg && i < n_lattice && j < n_lattice) continue;
     double dx = points[2 * j] - points[2 * i], dy = points[2 * j + 1] - points[2 * i + 1];
     double len2 = dx * dx + dy * dy;
     if (maxlen > 0 && len2 > maxlen * maxlen) continue;

@@ -319,14 +319,18 @@ __device__ __forceinline__ int edge_bucket(double key) {
   return b;
 }
 
-__device__ __forceinline__ int shortest_edge_bucket(const TriRec &r) {
+__device__ __forceinline__ double shortest_edge_sq(const TriRec &r) {
   double2 A = r.X[0], O = r.X[1], D = r.X[2];
   double l0 = (O.x - D.x) * (O.x - D.x) + (O.y - D.y) * (O.y - D.y);
   double l1 = (D.x - A.x) * (D.x - A.x) + (D.y - A.y) * (D.y - A.y);
   double l2 = (A.x - O.x) * (A.x - O.x) + (A.y - O.y) * (A.y - O.y);
   double mn = l0 < l1 ? l0 : l1;
   if (l2 < mn) mn = l2;
-  return edge_bucket(mn);
+  return mn;
+}
+
+__device__ __forceinline__ int shortest_edge_bucket(const TriRec &r) {
+  return edge_bucket(shortest_edge_sq(r));
 }
 
 // Sides of subsegment s: (t1,j1) holds org->dest, (t2,j2) dest->org.

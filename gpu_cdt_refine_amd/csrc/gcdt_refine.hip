@@ -461,6 +461,7 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
     int status = act ? ST_VALID : ST_NONE, prio = 0, seed0 = -1, seed1 = -1, split_s = -1,
         t0 = -1;
     double cx = 0.0, cy = 0.0;
+    double near2 = 0.0;  // a cavity vertex closer than this coincides with the point
     ParamsD lp = pr;
     if (act) {
       if (IS_SEG) {
@@ -478,7 +479,9 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
         t0 = ids[i];
         lp = params_at(pr, m, t0);
         const TriRec &r0 = m.T[t0];
-        if (pr.priority == 1 || pr.priority == 2) prio = shortest_edge_bucket(r0);
+        const double sh2 = shortest_edge_sq(r0);
+        near2 = 1e-12 * sh2;
+        if (pr.priority == 1 || pr.priority == 2) prio = edge_bucket(sh2);
         if (!steiner_point(lp, r0, cx, cy)) {
           status = ST_GIVEUP;
         } else {
@@ -610,6 +613,14 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
         bool bnd = (s >= 0 && s != split_s) || nb < 0 || !inside;
         if (bnd) {
           int4 tv = rt.V;
+          if (!IS_SEG) {
+            // Steiner point on an existing vertex up to rounding: give up
+            // (in exact arithmetic the empty circumcircle keeps every vertex
+            // at least half the shortest edge away)
+            double2 q = rt.X[(j + 1) % 3];
+            double dx = cx - q.x, dy = cy - q.y;
+            if (dx * dx + dy * dy < near2) sc[SC_GIVEUP] = 1;
+          }
           int bpos = atomicAdd(&sc[SC_NBND], 1);
           if (bpos < K + 2) {
             eu[bpos] = comp(tv, (j + 1) % 3);
@@ -1570,16 +1581,24 @@ void Refiner::select_input(int k) {
   n_inst = in.n_inst;
 }
 
+// Largest mesh the 32-bit encodings hold: triangle ids travel as
+// otri = tri << 2 | edge in an int, point and subsegment ids as ints.
+constexpr long long MAX_TRIANGLES = (1LL << 29) - 1;
+constexpr long long MAX_POINTS = (1LL << 31) - 1;
+
 void Refiner::ensure_capacity(long long needP, long long needT, long long needS) {
+  if (needT > MAX_TRIANGLES || needP > MAX_POINTS || needS > MAX_POINTS)
+    throw std::runtime_error("the refined mesh would exceed 2^29 triangles / 2^31 points "
+                             "(32-bit encodings); the refinement does not converge on this input");
   if (needP > capP) {
-    long long c = std::max<long long>(needP, capP + capP / 2);
+    long long c = std::min(std::max<long long>(needP, capP + capP / 2), MAX_POINTS);
     dgrow(&m.P, (size_t)np, (size_t)c, stream);
     dgrow(&m.pflag, (size_t)np, (size_t)c, stream);
     dgrow(&m.ppar, (size_t)np, (size_t)c, stream);
     capP = c;
   }
   if (needT > capT) {
-    long long c = std::max<long long>(needT, capT + capT / 2);
+    long long c = std::min(std::max<long long>(needT, capT + capT / 2), MAX_TRIANGLES);
     dgrow(&m.T, (size_t)nt, (size_t)c, stream);
     dgrow(&m.tflag, (size_t)nt, (size_t)c, stream);
     unsigned long long *nc;
@@ -1592,7 +1611,7 @@ void Refiner::ensure_capacity(long long needP, long long needT, long long needS)
     capT = c;
   }
   if (needS > capS) {
-    long long c = std::max<long long>(needS, capS + capS / 2);
+    long long c = std::min(std::max<long long>(needS, capS + capS / 2), MAX_POINTS);
     dgrow(&m.SV, (size_t)ns, (size_t)c, stream);
     dgrow(&m.SP, (size_t)ns, (size_t)c, stream);
     dgrow(&m.ST, (size_t)ns, (size_t)c, stream);
@@ -1888,9 +1907,9 @@ void Refiner::run() {
   HC(hipEventRecord(ev0, stream));
   // --- initialise working mesh from the uploaded input --------------------
   np = np0; nt = nt0; ns = ns0;
-  long long wantP = std::max<long long>(16, 6LL * np0);
-  long long wantT = std::max<long long>(16, 6LL * nt0);
-  long long wantS = std::max<long long>(16, 8LL * ns0 + 16);
+  long long wantP = std::min(std::max<long long>(16, 6LL * np0), MAX_POINTS);
+  long long wantT = std::min(std::max<long long>(16, 6LL * nt0), MAX_TRIANGLES);
+  long long wantS = std::min(std::max<long long>(16, 8LL * ns0 + 16), MAX_POINTS);
   const bool want_inst = in_tinst != nullptr;
   if (capP < wantP || capT < wantT || capS < wantS) {
     free_mesh();
@@ -2251,10 +2270,7 @@ void Refiner::loop_graph() {
       nt = h.i[C_NT];
       ns = h.i[C_NS];
       long long nc = h.i[C_NC];
-      long long needT = nt + 2 * nc;
-      if (4 * needT >= (1LL << 31))
-        throw std::runtime_error("mesh exceeds the 32-bit otri encoding (2^29 triangles)");
-      ensure_capacity(np + nc, needT, ns + nc);
+      ensure_capacity(np + nc, nt + 2 * nc, ns + nc);
       h.i[C_CAPP] = (int)std::min<long long>(capP, INT32_MAX);
       h.i[C_CAPT] = (int)std::min<long long>(capT, INT32_MAX);
       h.i[C_CAPS] = (int)std::min<long long>(capS, INT32_MAX);

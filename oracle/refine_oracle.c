@@ -309,7 +309,7 @@ static int edge_bucket(double key) {
   return b;
 }
 
-static int shortest_edge_bucket(const Mesh *m, int t) {
+static double shortest_edge_sq(const Mesh *m, int t) {
   const int *v = m->TV + 3 * (size_t)t;
   const double *A = PT(m, v[0]), *O = PT(m, v[1]), *D = PT(m, v[2]);
   double l0 = (O[0] - D[0]) * (O[0] - D[0]) + (O[1] - D[1]) * (O[1] - D[1]);
@@ -317,7 +317,11 @@ static int shortest_edge_bucket(const Mesh *m, int t) {
   double l2 = (A[0] - O[0]) * (A[0] - O[0]) + (A[1] - O[1]) * (A[1] - O[1]);
   double mn = l0 < l1 ? l0 : l1;
   if (l2 < mn) mn = l2;
-  return edge_bucket(mn);
+  return mn;
+}
+
+static int shortest_edge_bucket(const Mesh *m, int t) {
+  return edge_bucket(shortest_edge_sq(m, t));
 }
 
 static int cmp_int_c(const void *a, const void *b) {
@@ -646,6 +650,22 @@ static int run_pass(Mesh *m, const int *ids, int nc, int is_seg, int K,
     }
     c->status = build_cavity(m, c, seed0, seed1, split_s, K);
     if (c->status == ST_OVERFLOW && K >= Kmax) c->status = ST_GIVEUP;
+    if (c->status == ST_VALID && !is_seg) {
+      /* A Steiner point within 1e-6 of the triangle's shortest edge of a
+       * cavity vertex coincides with it up to rounding (in exact arithmetic
+       * the empty circumcircle keeps every vertex at least half the shortest
+       * edge away): the triangle is given up instead of seeding a cascade of
+       * rounding-scale triangles (nearly cocircular inputs). */
+      double near2 = 1e-12 * shortest_edge_sq(m, c->id);
+      for (int idx = 0; idx < c->n && c->status == ST_VALID; idx++) {
+        const int *v = m->TV + 3 * (size_t)c->L[idx];
+        for (int j = 0; j < 3; j++) {
+          const double *q = PT(m, v[j]);
+          double dx = c->c[0] - q[0], dy = c->c[1] - q[1];
+          if (dx * dx + dy * dy < near2) { c->status = ST_GIVEUP; break; }
+        }
+      }
+    }
     if (c->status == ST_VALID && !is_seg) {
       /* Ruppert/Chew rule: a Steiner point that encroaches a subsegment on
        * its cavity boundary is not inserted; every such subsegment is marked. */
