@@ -195,6 +195,12 @@ def calibration_copy():
     torch.cuda.synchronize()
 
 
+def refine_workload_name(args):
+    return (f"cdt-refine-{args.points // 1000000}M-uniform-{args.segments // 1000}k-segs"
+            if args.points % 1000000 == 0 else
+            f"cdt-refine-{args.points}-uniform-{args.segments}-segs")
+
+
 def bench_refine(args):
     if args.calibration_copy:
         calibration_copy()
@@ -231,7 +237,7 @@ def bench_refine(args):
     roof, kt = None, None
     if not args.no_profile_step:
         kt = profile_kernels(P, T, N, SS, args.theta, co)
-        roof = roofline(kt, "cdt-refine-1M-uniform-50k-segs")
+        roof = roofline(kt, refine_workload_name(args))
     dropin = None
     if not args.no_dropin:
         # PCIe-inclusive: host CDT in, host mesh out (the drop-in's contract)
@@ -255,7 +261,7 @@ def bench_refine(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded uniform PSLG, own CDT builder identical to Triangle pzQnc)",
-        "config": {"workload": "cdt-refine-1M-uniform-50k-segs", "points": args.points,
+        "config": {"workload": refine_workload_name(args), "points": args.points,
                    "segments": args.segments, "domain": args.domain,
                    "max_segment_length": args.domain * 0.02, "theta_deg": args.theta,
                    "insert": "offcenter", "mode": "ruppert", "seed": args.seed,
@@ -285,24 +291,34 @@ def bench_learned(args):
     P, T, N, SS = make_input(n, nseg, args.domain, 0.02 * args.domain, 1, args.seed + 1000)
     log(f"# input: {len(P)} points, {len(T)} triangles, {len(SS)} segments")
     res = {}
-    for name, prio in (("plain_priority", 1), ("learned_priority", 2)):
-        o = chosen_options(args)
-        o["priority"] = prio
-        s = g.Session(P, T, N, SS, theta=28.0, **o)
+    runs = [("plain_path", None), ("plain_priority", 1), ("learned_priority", 2)]
+    for name, prio in runs:
+        if prio is None:
+            o = plain_options()
+        else:
+            o = chosen_options(args)
+            o["priority"] = prio
+        s = g.Session(P, T, N, SS, theta=args.learned_theta, **o)
         if prio == 2:
             s.set_priority_model(np.load(MODEL))
         st, ms, dev = timed_runs(s, args.steps, args.warmup, name)
         s.close()
         res[name] = {"ms_per_step": ms, "value": st["n_triangles"] / (ms / 1e3), "output": summary(st)}
-    a, b = res["plain_priority"], res["learned_priority"]
-    out = {"metric": "refine time, learned vs plain claim priority (2M clustered, theta=28)",
+    p, a, b = res["plain_path"], res["plain_priority"], res["learned_priority"]
+    th = f"{args.learned_theta:g}"
+    out = {"metric": f"refine time, learned vs plain claim priority (2M clustered, theta={th})",
            "value": b["value"], "unit": "triangles/s", "n_gpus": 1, "higher_is_better": True,
-           "config": {"workload": "cdt-refine-2M-clustered-100k-segs-theta28", "points": n,
-                      "segments": nseg, "seed": args.seed + 1000},
+           "config": {"workload": f"cdt-refine-{n // 1000}k-clustered-{nseg // 1000}k-segs-theta{th}",
+                      "points": n, "segments": nseg, "seed": args.seed + 1000,
+                      "theta_deg": args.learned_theta},
            "speedup": a["ms_per_step"] / b["ms_per_step"],
            "steiner_delta": (b["output"]["n_points"] - a["output"]["n_points"]) /
                             max(1, a["output"]["n_points"] - len(P)),
-           "rounds": [a["output"]["rounds"], b["output"]["rounds"]], "runs": res}
+           "speedup_vs_plain_path": p["ms_per_step"] / b["ms_per_step"],
+           "steiner_delta_vs_plain_path": (b["output"]["n_points"] - p["output"]["n_points"]) /
+                                          max(1, p["output"]["n_points"] - len(P)),
+           "rounds": [p["output"]["rounds"], a["output"]["rounds"], b["output"]["rounds"]],
+           "runs": res}
     print(json.dumps(out))
 
 
@@ -453,6 +469,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--claim-iters", type=int, default=3)
     ap.add_argument("--learned", type=int, default=1)
+    ap.add_argument("--learned-theta", type=float, default=28.0)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile-step", action="store_true")

@@ -189,7 +189,6 @@ __device__ __forceinline__ int pass_index_of(const PassD &ps) {
   return ps.ctl ? ps.ctl->i[C_PASS_INDEX] : ps.pass_index;
 }
 
-constexpr int CAV_BLOCK = 128;  // threads per block of the apply kernel
 
 __device__ __forceinline__ int comp(const int4 &v, int j) {
   return j == 0 ? v.x : (j == 1 ? v.y : v.z);
@@ -202,11 +201,10 @@ __device__ __forceinline__ int *seg_slot(TriRec *T, int t, int j) {
   return &T[t].S.x + j;
 }
 
-// checkseg4encroach (refine.cu:115-139, triangle.cpp:7138-7153).
-__device__ __forceinline__ bool encroaches(const MeshD &m, const ParamsD &pr,
-                                           int a, int b, double px,
-                                           double py) {
-  double2 A = m.P[a], B = m.P[b];
+// checkseg4encroach (refine.cu:115-139, triangle.cpp:7138-7153): does
+// (px, py) encroach the subsegment with endpoints A, B?
+__device__ __forceinline__ bool encroaches_xy(const ParamsD &pr, double2 A, double2 B,
+                                              double px, double py) {
   double dot = (A.x - px) * (B.x - px) + (A.y - py) * (B.y - py);
   if (dot < 0.0) {
     if (pr.ruppert) return true;
@@ -215,6 +213,11 @@ __device__ __forceinline__ bool encroaches(const MeshD &m, const ParamsD &pr,
     if (dot * dot >= pr.lens * la * lb) return true;
   }
   return false;
+}
+__device__ __forceinline__ bool encroaches(const MeshD &m, const ParamsD &pr,
+                                           int a, int b, double px,
+                                           double py) {
+  return encroaches_xy(pr, m.P[a], m.P[b], px, py);
 }
 
 // testtriangle + Miller-Pav-Walkington (refine.cu:376-497,

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstddef>
 #include <chrono>
 #include <cmath>
@@ -809,69 +810,104 @@ __device__ __forceinline__ unsigned long long claim_key(unsigned long long epoch
          (unsigned long long)(0xffffffffu - (unsigned)i);
 }
 
+// Per-candidate list kernels (claims, win test, block test, apply): a group of
+// G lanes serves one candidate, one entry of its cavity + ring list (or one
+// triangle of its star) per lane, so the candidate's loads and atomics are in
+// flight together instead of one dependent memory round trip per entry.
+// Group decisions are ballots over the group's lanes.  Groups are aligned
+// within the wave and every lane of a group follows the same candidate, so a
+// group reaches each ballot / shuffle with all of its lanes.
+constexpr int LIST_BLOCK = 256;
+constexpr int CLAIM_LANES = 16;
+constexpr int APPLY_LANES = 8;
+
+template <int G>
+__device__ __forceinline__ unsigned long long group_bits(unsigned long long b) {
+  static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "power-of-two groups");
+  const int base = (threadIdx.x & 63) & ~(G - 1);
+  return G == 64 ? b : (b >> base) & ((1ull << (G == 64 ? 0 : G)) - 1ull);
+}
+
 // Candidate i takes part in claim iteration ps.iter when it is valid, has not
 // won and touches no winner's region (cact != 0).
-__global__ __launch_bounds__(256) void k_claim(MeshD m, PassD ps, int *cact) {
+template <int G>
+__global__ __launch_bounds__(LIST_BLOCK) void k_claim(MeshD m, PassD ps, int *cact) {
   const int nc = pass_nc(ps);
   const unsigned long long ep = pass_epoch(ps);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc;
-       i += gridDim.x * blockDim.x) {
+  const int gl = threadIdx.x % G, per = blockDim.x / G;
+  for (int i = blockIdx.x * per + threadIdx.x / G; i < nc; i += gridDim.x * per) {
+    bool on;
     if (ps.iter == 0) {
-      cact[i] = ps.cstatus[i] == ST_VALID;
-      ps.cwin[i] = 0;
+      on = ps.cstatus[i] == ST_VALID;
+      if (gl == 0) {
+        cact[i] = on ? 1 : 0;
+        ps.cwin[i] = 0;
+      }
+    } else {
+      on = cact[i] != 0;
     }
-    if (!cact[i]) continue;
-    unsigned long long key = claim_key(ep, ps.cprio[i], i);
+    if (!on) continue;
+    const unsigned long long key = claim_key(ep, ps.cprio[i], i);
     const int *L = ps.ws + (size_t)i * ps.R;
-    int k = ps.cn[i] + ps.cr[i];
-    for (int q = 0; q < k; q++) atomicMax(&m.claim[L[q]], key);
+    const int k = ps.cn[i] + ps.cr[i];
+    for (int q = gl; q < k; q += G) atomicMax(&m.claim[L[q]], key);
   }
 }
 
-__global__ __launch_bounds__(256) void k_win(MeshD m, PassD ps, const int *cact) {
+// A candidate wins when every triangle of its cavity and ring holds its key.
+template <int G>
+__global__ __launch_bounds__(LIST_BLOCK) void k_win(MeshD m, PassD ps, const int *cact) {
   const int nc = pass_nc(ps);
   const unsigned long long ep = pass_epoch(ps);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc;
-       i += gridDim.x * blockDim.x) {
+  const int gl = threadIdx.x % G, per = blockDim.x / G;
+  for (int i = blockIdx.x * per + threadIdx.x / G; i < nc; i += gridDim.x * per) {
     if (!cact[i]) continue;
-    unsigned long long key = claim_key(ep, ps.cprio[i], i);
+    const unsigned long long key = claim_key(ep, ps.cprio[i], i);
     const int *L = ps.ws + (size_t)i * ps.R;
-    int k = ps.cn[i] + ps.cr[i];
-    int w = 1;
-    for (int q = 0; q < k && w; q++)
-      if (m.claim[L[q]] != key) w = 0;
-    if (w) ps.cwin[i] = 1;
+    const int k = ps.cn[i] + ps.cr[i];
+    bool lose = false;
+    for (int q = gl; q < k; q += G) lose |= m.claim[L[q]] != key;
+    if (group_bits<G>(__ballot(lose)) == 0ull && gl == 0) ps.cwin[i] = 1;
   }
 }
 
 // After a claim iteration: winners and candidates touching a winner's region
 // (a triangle whose claim comes from this pass and from a winner) drop out.
-__global__ __launch_bounds__(256) void k_block(MeshD m, PassD ps, int *cact) {
+template <int G>
+__global__ __launch_bounds__(LIST_BLOCK) void k_block(MeshD m, PassD ps, int *cact) {
   const int nc = pass_nc(ps);
   const unsigned long long ep0 = pass_epoch0(ps);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc;
-       i += gridDim.x * blockDim.x) {
+  const int gl = threadIdx.x % G, per = blockDim.x / G;
+  for (int i = blockIdx.x * per + threadIdx.x / G; i < nc; i += gridDim.x * per) {
     if (!cact[i]) continue;
-    if (ps.cwin[i]) { cact[i] = 0; continue; }
+    if (ps.cwin[i]) {
+      if (gl == 0) cact[i] = 0;
+      continue;
+    }
     const int *L = ps.ws + (size_t)i * ps.R;
-    int k = ps.cn[i] + ps.cr[i];
-    for (int q = 0; q < k; q++) {
-      unsigned long long c = m.claim[L[q]];
+    const int k = ps.cn[i] + ps.cr[i];
+    bool hit = false;
+    for (int q = gl; q < k; q += G) {
+      const unsigned long long c = m.claim[L[q]];
       if ((c >> 53) >= ep0) {
-        int j = (int)(0xffffffffu - (unsigned)(c & 0xffffffffull));
-        if (ps.cwin[j]) { cact[i] = 0; break; }
+        const int j = (int)(0xffffffffu - (unsigned)(c & 0xffffffffull));
+        hit |= ps.cwin[j] != 0;
       }
     }
+    if (group_bits<G>(__ballot(hit)) != 0ull && gl == 0) cact[i] = 0;
   }
 }
 
 // Retriangulate a winner's cavity as the star of its new vertex; same star
 // order and slot rule as the oracle.  Also writes the new point, and for a
-// segment split the two subsegment halves.
-__global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
-                                                     PassD ps, int pbase,
-                                                     int tbase, int sbase,
-                                                     int is_seg) {
+// segment split the two subsegment halves.  Lane q of the group writes star
+// triangle q and repairs the outer neighbour and subsegment across its
+// boundary edge.  The group's records are read (uniform loads) before any of
+// its lanes writes, and no lane reads a record another lane writes: the new
+// point's coordinates come from the candidate, not from P.
+template <int G>
+__global__ __launch_bounds__(LIST_BLOCK) void k_apply(MeshD m, ParamsD pr, PassD ps, int pbase,
+                                                      int tbase, int sbase, int is_seg) {
   const int nc = pass_nc(ps);
   const int *ids = pass_ids(ps);
   if (ps.ctl) {
@@ -879,119 +915,123 @@ __global__ __launch_bounds__(CAV_BLOCK) void k_apply(MeshD m, ParamsD pr,
     tbase = ps.ctl->i[C_NT];
     sbase = ps.ctl->i[C_NS];
   }
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc;
-       i += gridDim.x * blockDim.x) {
+  const int gl = threadIdx.x % G, per = blockDim.x / G;
+  for (int i = blockIdx.x * per + threadIdx.x / G; i < nc; i += gridDim.x * per) {
     if (!ps.cwin[i]) continue;
-    int rank = ps.crank[i];
-    int p = pbase + rank;
-    int tb = tbase + 2 * rank;
-    int id = ids[i];
-    int split_s = is_seg ? id : -1;
-    double2 c = ps.cpt[i];
-    int n = ps.cn[i];
+    const int rank = ps.crank[i];
+    const int p = pbase + rank;
+    const int tb = tbase + 2 * rank;
+    const int split_s = is_seg ? ids[i] : -1;
+    const double2 c = ps.cpt[i];
+    const int n = ps.cn[i];
     const int *L = ps.ws + (size_t)i * ps.R;
-    int k = n + 2;
-    const int inst = m.T[L[0]].V.w;
-    const ParamsD lp = params_at(pr, m, L[0]);
-    int *Bu = ps.ws + (size_t)i * ps.R + (2 * ps.K + 2);
-    int *Bv = Bu + (ps.K + 2);
-    int *Bo = Bv + (ps.K + 2);
-    int *Bs = Bo + (ps.K + 2);
     // the boundary edges in star order (origin Bu, destination Bv, outer
     // otri Bo, subsegment Bs) were written by the cavity kernel
+    const int *Bu = L + (2 * ps.K + 2);
+    const int *Bv = Bu + (ps.K + 2);
+    const int *Bo = Bv + (ps.K + 2);
+    const int *Bs = Bo + (ps.K + 2);
+    const int inst = m.T[L[0]].V.w;
+    const ParamsD lp = params_at(pr, m, L[0]);
     bool open = false;
     int a = -1, b = -1, par = -1;
+    double2 Pa = make_double2(0.0, 0.0), Pb = Pa;
     if (is_seg) {
       int t1, j1, t2, j2;
       seg_sides(m, split_s, t1, j1, t2, j2);
       open = t1 < 0 || t2 < 0;  // hull subsegment: the star is an open fan
-      int2 sv = m.SV[split_s];
+      const int2 sv = m.SV[split_s];
       a = sv.x;
       b = sv.y;
       par = m.SP[split_s];
+      Pa = m.P[a];
+      Pb = m.P[b];
     }
-    int nstar = open ? k - 1 : k;
-    // point
-    m.P[p] = c;
-    m.pflag[p] = is_seg ? PF_SEGV : PF_FREE;
-    m.ppar[p] = is_seg ? par : -1;
-    int sA = split_s, sB = is_seg ? sbase + rank : -1;
-    for (int q = 0; q < nstar; q++) {
-      int st = q < n ? L[q] : tb + (q - n);
-      int nx = (q + 1 < nstar) ? q + 1 : (open ? -1 : 0);
-      int pv = (q > 0) ? q - 1 : (open ? -1 : nstar - 1);
-      int snx = nx < 0 ? -1 : (nx < n ? L[nx] : tb + (nx - n));
-      int spv = pv < 0 ? -1 : (pv < n ? L[pv] : tb + (pv - n));
+    const int nstar = open ? n + 1 : n + 2;
+    const int sA = split_s, sB = is_seg ? sbase + rank : -1;
+    // first star triangle (in star order) at each end of the split, packed
+    // as (q << 32 | otri) so that a minimum finds it
+    long long kA = LLONG_MAX, kB = LLONG_MAX;
+    bool encA = false, encB = false;
+    for (int q = gl; q < nstar; q += G) {
+      const int st = q < n ? L[q] : tb + (q - n);
+      const int nx = (q + 1 < nstar) ? q + 1 : (open ? -1 : 0);
+      const int pv = (q > 0) ? q - 1 : (open ? -1 : nstar - 1);
+      const int snx = nx < 0 ? -1 : (nx < n ? L[nx] : tb + (nx - n));
+      const int spv = pv < 0 ? -1 : (pv < n ? L[pv] : tb + (pv - n));
+      const int u = Bu[q], v = Bv[q], o = Bo[q], s = Bs[q];
+      const double2 Pu = m.P[u], Pv = m.P[v];
       TriRec r;
-      r.V = make_int4(p, Bu[q], Bv[q], inst);
-      r.N = make_int4(Bo[q], snx >= 0 ? (snx << 2) | 2 : -1, spv >= 0 ? (spv << 2) | 1 : -1, 0);
-      r.S = make_int4(Bs[q], (Bv[q] == a) ? sA : (Bv[q] == b) ? sB : -1,
-                      (Bu[q] == a) ? sA : (Bu[q] == b) ? sB : -1, 0);
+      r.V = make_int4(p, u, v, inst);
+      r.N = make_int4(o, snx >= 0 ? (snx << 2) | 2 : -1, spv >= 0 ? (spv << 2) | 1 : -1, 0);
+      r.S = make_int4(s, (v == a) ? sA : (v == b) ? sB : -1, (u == a) ? sA : (u == b) ? sB : -1,
+                      0);
       r.X[0] = c;
-      r.X[1] = m.P[Bu[q]];
-      r.X[2] = m.P[Bv[q]];
+      r.X[1] = Pu;
+      r.X[2] = Pv;
       r.pad[0] = r.pad[1] = make_int4(0, 0, 0, 0);
       m.T[st] = r;
       m.tflag[st] = TF_ALIVE | TF_CHECK;
-    }
-    if (open) {
-      int st = tb + 1;
-      TriRec r;
-      r.V = make_int4(-1, -1, -1, 0);
-      r.N = make_int4(-1, -1, -1, 0);
-      r.S = make_int4(-1, -1, -1, 0);
-      r.X[0] = r.X[1] = r.X[2] = make_double2(0.0, 0.0);
-      r.pad[0] = r.pad[1] = make_int4(0, 0, 0, 0);
-      m.T[st] = r;
-      m.tflag[st] = 0;
-    }
-    for (int q = 0; q < nstar; q++) {
-      int st = q < n ? L[q] : tb + (q - n);
-      int o = Bo[q];
       if (o >= 0) *nbr_slot(m.T, o >> 2, o & 3) = (st << 2) | 0;
-      int s = Bs[q];
       if (s >= 0) {
         m.ST[s] = (st << 2) | 0;
         if (is_seg && m.senc[s] == 0) {
-          int2 sv = m.SV[s];
+          const int2 sv = m.SV[s];
           if (encroaches(m, lp, sv.x, sv.y, c.x, c.y)) m.senc[s] = 1;
+        }
+      }
+      if (is_seg) {
+        const long long hq = (long long)q << 32;
+        if (u == a) {
+          kA = min(kA, hq | ((st << 2) | 2));
+          encA |= encroaches_xy(lp, Pa, c, Pv.x, Pv.y);
+        }
+        if (v == a) {
+          kA = min(kA, hq | ((st << 2) | 1));
+          encA |= encroaches_xy(lp, Pa, c, Pu.x, Pu.y);
+        }
+        if (u == b) {
+          kB = min(kB, hq | ((st << 2) | 2));
+          encB |= encroaches_xy(lp, c, Pb, Pv.x, Pv.y);
+        }
+        if (v == b) {
+          kB = min(kB, hq | ((st << 2) | 1));
+          encB |= encroaches_xy(lp, c, Pb, Pu.x, Pu.y);
         }
       }
     }
     if (is_seg) {
-      m.SV[sA] = make_int2(a, p);
-      m.SV[sB] = make_int2(p, b);
-      m.SP[sB] = par;
-      int stA = -1, stB = -1;
-      bool encA = false, encB = false;
-      for (int q = 0; q < nstar; q++) {
-        int st = q < n ? L[q] : tb + (q - n);
-        int u = Bu[q], v = Bv[q];
-        if (u == a) {
-          if (stA < 0) stA = (st << 2) | 2;
-          double2 pv = m.P[v];
-          encA |= encroaches(m, lp, a, p, pv.x, pv.y);
-        }
-        if (v == a) {
-          if (stA < 0) stA = (st << 2) | 1;
-          double2 pu = m.P[u];
-          encA |= encroaches(m, lp, a, p, pu.x, pu.y);
-        }
-        if (u == b) {
-          if (stB < 0) stB = (st << 2) | 2;
-          double2 pv = m.P[v];
-          encB |= encroaches(m, lp, p, b, pv.x, pv.y);
-        }
-        if (v == b) {
-          if (stB < 0) stB = (st << 2) | 1;
-          double2 pu = m.P[u];
-          encB |= encroaches(m, lp, p, b, pu.x, pu.y);
-        }
+      for (int off = G / 2; off > 0; off >>= 1) {
+        kA = min(kA, __shfl_xor(kA, off, G));
+        kB = min(kB, __shfl_xor(kB, off, G));
       }
-      m.ST[sA] = stA;
-      m.ST[sB] = stB;
-      m.senc[sA] = encA ? 1 : 0;
-      m.senc[sB] = encB ? 1 : 0;
+      const bool eA = group_bits<G>(__ballot(encA)) != 0ull;
+      const bool eB = group_bits<G>(__ballot(encB)) != 0ull;
+      if (gl == 0) {
+        m.SV[sA] = make_int2(a, p);
+        m.SV[sB] = make_int2(p, b);
+        m.SP[sB] = par;
+        m.ST[sA] = kA == LLONG_MAX ? -1 : (int)(kA & 0xffffffffll);
+        m.ST[sB] = kB == LLONG_MAX ? -1 : (int)(kB & 0xffffffffll);
+        m.senc[sA] = eA ? 1 : 0;
+        m.senc[sB] = eB ? 1 : 0;
+      }
+    }
+    if (gl == 0) {
+      m.P[p] = c;
+      m.pflag[p] = is_seg ? PF_SEGV : PF_FREE;
+      m.ppar[p] = is_seg ? par : -1;
+      if (open) {  // the second new slot stays empty
+        const int st = tb + 1;
+        TriRec r;
+        r.V = make_int4(-1, -1, -1, 0);
+        r.N = make_int4(-1, -1, -1, 0);
+        r.S = make_int4(-1, -1, -1, 0);
+        r.X[0] = r.X[1] = r.X[2] = make_double2(0.0, 0.0);
+        r.pad[0] = r.pad[1] = make_int4(0, 0, 0, 0);
+        m.T[st] = r;
+        m.tflag[st] = 0;
+      }
     }
   }
 }
@@ -1798,11 +1838,11 @@ void Refiner::enqueue_claims(PassD ps, int gc) {
     if (!ps.ctl) ps.epoch = ++epoch;
     ps.iter = it;
     tbegin(K_CLAIM);
-    k_claim<<<gc, 256, 0, stream>>>(m, ps, c_act);
+    k_claim<CLAIM_LANES><<<gc, LIST_BLOCK, 0, stream>>>(m, ps, c_act);
     tend(K_CLAIM, ps.nc);
     tbegin(K_WIN);
-    k_win<<<gc, 256, 0, stream>>>(m, ps, c_act);
-    if (it + 1 < iters) k_block<<<gc, 256, 0, stream>>>(m, ps, c_act);
+    k_win<CLAIM_LANES><<<gc, LIST_BLOCK, 0, stream>>>(m, ps, c_act);
+    if (it + 1 < iters) k_block<CLAIM_LANES><<<gc, LIST_BLOCK, 0, stream>>>(m, ps, c_act);
     tend(K_WIN, ps.nc);
     HC(hipGetLastError());
   }
@@ -1841,7 +1881,7 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   launch_cavity(is_seg, gw, ps);
   tend(is_seg ? K_SEGCAV : K_BADCAV, nc);
   HC(hipGetLastError());
-  int gc = std::min(nblk(nc), opt.grid_claim);
+  int gc = std::min(nblk((long long)nc * CLAIM_LANES, LIST_BLOCK), opt.grid_claim);
   if (!is_seg && opt.priority == 2) {
     if (!mlp_params) throw std::runtime_error("priority 2 needs a model (set_priority_model)");
     tbegin(K_MLP);
@@ -1875,8 +1915,8 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   ensure_capacity(np + nwin, nt + 2LL * nwin, is_seg ? ns + nwin : ns);
   if (nwin > 0) {
     tbegin(K_APPLY);
-    k_apply<<<std::min(nblk(nc, CAV_BLOCK), opt.grid_apply), CAV_BLOCK, 0, stream>>>(
-        m, pr, ps, np, nt, ns, is_seg ? 1 : 0);
+    k_apply<APPLY_LANES><<<std::min(nblk((long long)nc * APPLY_LANES, LIST_BLOCK), opt.grid_apply),
+                           LIST_BLOCK, 0, stream>>>(m, pr, ps, np, nt, ns, is_seg ? 1 : 0);
     tend(K_APPLY, nwin);
     HC(hipGetLastError());
   }
@@ -2152,7 +2192,7 @@ void Refiner::enqueue_seg_pass() {
   k_dscan<FinWin><<<1, GC, 0, stream>>>(g_cnt, g_off, d_ctl, -1, FinWin{});
   k_dscatter<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_off,
                                              nullptr, c_rank, nullptr);
-  k_apply<<<opt.grid_apply, CAV_BLOCK, 0, stream>>>(m, pr, ps, 0, 0, 0, 1);
+  k_apply<APPLY_LANES><<<opt.grid_apply, LIST_BLOCK, 0, stream>>>(m, pr, ps, 0, 0, 0, 1);
   k_end_seg<<<1, 1, 0, stream>>>(d_ctl, std::max(1, opt.claim_iters));
   HC(hipGetLastError());
 }
@@ -2197,7 +2237,7 @@ void Refiner::enqueue_bad_pass() {
   k_dscan<FinWin><<<1, GC, 0, stream>>>(g_cnt, g_off, d_ctl, -1, FinWin{});
   k_dscatter<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_off,
                                              nullptr, c_rank, nullptr);
-  k_apply<<<opt.grid_apply, CAV_BLOCK, 0, stream>>>(m, pr, ps, 0, 0, 0, 0);
+  k_apply<APPLY_LANES><<<opt.grid_apply, LIST_BLOCK, 0, stream>>>(m, pr, ps, 0, 0, 0, 0);
   k_end_bad<<<1, 1, 0, stream>>>(d_ctl, std::max(1, opt.claim_iters));
   HC(hipGetLastError());
 }
