@@ -50,6 +50,7 @@ class Options(C.Structure):
         ("cavity_lanes", C.c_int),
         ("cav_lds_cap", C.c_int),
         ("learned_shift", C.c_int),
+        ("mixed", C.c_int),
     ]
 
 

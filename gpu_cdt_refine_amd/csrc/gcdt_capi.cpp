@@ -58,6 +58,7 @@ void gcdt_default_options(gcdt_options *o) {
   o->cavity_lanes = 16;
   o->cav_lds_cap = 32;
   o->learned_shift = 2;
+  o->mixed = 0;
 }
 
 }  // extern "C"
@@ -95,6 +96,9 @@ bool options_ok(const gcdt_options *o, std::string &g_err) {
     return false;
   }
   if (o->graph && o->log_training) { g_err = "log_training needs the host-driven loop (graph = 0)"; return false; }
+  if (o->mixed != 0 && o->mixed != 1) { g_err = "mixed must be 0 or 1"; return false; }
+  if (o->mixed && o->graph) { g_err = "mixed rounds run in the host-driven loop (graph = 0)"; return false; }
+  if (o->mixed && o->log_training) { g_err = "log_training needs separate segment and bad passes (mixed = 0)"; return false; }
   return true;
 }
 }  // namespace gcdt

@@ -56,7 +56,8 @@ enum : int {
   CNT_VALID,
   CNT_BIG,
   CNT_DD,
-  CNT_SLOW,  // candidates deferred to the exact-predicate kernel
+  CNT_SLOW,    // candidates deferred to the exact-predicate kernel
+  CNT_SEGWIN,  // winners that split a subsegment
   CNT_N = 16
 };
 
@@ -153,6 +154,8 @@ struct Ctl {
 struct PassD {
   const int *ids;
   int nc;
+  int nseg;  // candidates [0, nseg) split subsegments, the rest are bad
+             // triangles; -1: every candidate of the pass is a subsegment
   int K;   // cavity capacity (triangles)
   int Kf;  // cavity capacity of the filtered kernel's LDS fast path
   int R;  // workspace ints per candidate = 6K+16
@@ -175,6 +178,9 @@ struct PassD {
 __device__ __forceinline__ int pass_nc(const PassD &ps) {
   if (!ps.ctl) return ps.nc;
   return ps.ctl->i[C_GO] ? ps.ctl->i[C_NC] : 0;
+}
+__device__ __forceinline__ int pass_nseg(const PassD &ps) {
+  return ps.nseg < 0 ? pass_nc(ps) : ps.nseg;
 }
 __device__ __forceinline__ const int *pass_ids(const PassD &ps) {
   return (ps.ctl && ps.ctl->i[C_USESEL]) ? ps.ids_sel : ps.ids;

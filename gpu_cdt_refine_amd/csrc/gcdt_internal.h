@@ -80,8 +80,8 @@ class Refiner {
  private:
   template <class Pred>
   int compact(Pred pred, int n, int *out_list, int *out_rank, const int *src);
-  int run_pass(const int *ids, int nc, bool is_seg, long long *progress,
-               bool *overflowed);
+  int run_pass(const int *ids, int nc, int nseg, long long *progress, bool *overflowed,
+               int *nwin_seg);
   PassD pass_base(int Kc) const;
   void enqueue_claims(PassD ps, int gc);
   void enqueue_seg_pass();
@@ -89,6 +89,8 @@ class Refiner {
   void capture_graph();
   void loop_graph();
   void loop_host();
+  void loop_mixed();
+  void ensure_mix(long long n);
   void size_pass_buffers();
   unsigned long long *ctl_cnt_ptr() const;
   int cavity_lanes(int Kc) const;
@@ -126,6 +128,8 @@ class Refiner {
   int pws_blocks = 0;
   int *tmp_list = nullptr;
   int tmp_list_cap = 0;
+  int *mix_list = nullptr;  // mixed pass: subsegment ids, then bad triangle ids
+  long long mix_cap = 0;
   int *sel_prio = nullptr, *sel_tmp = nullptr, *sel_list = nullptr;
   int sel_cap = 0;
   unsigned *d_hist = nullptr;

@@ -438,9 +438,12 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
   for (int k = 0; k < CC_N; k++) cc.c[k] = 0u;
   double *pws = ps.pws + ((size_t)blockIdx.x * 64 + lane) * GCDT_INCIRCLE_WS;
   const int nc = pass_nc(ps);
+  const int nseg = pass_nseg(ps);
   const int *ids = pass_ids(ps);
-  // work items: the pass's candidates, or the deferred ones
-  const int nwork = FAST ? nc : (nc > 0 ? (int)ps.cnt[CNT_SLOW] : 0);
+  // work items: this kernel's kind of the pass's candidates (subsegments
+  // [0, nseg), bad triangles [nseg, nc)), or the deferred ones of any kind
+  const int lo = IS_SEG ? 0 : nseg;
+  const int nwork = FAST ? (IS_SEG ? nseg : nc - nseg) : (nc > 0 ? (int)ps.cnt[CNT_SLOW] : 0);
   bool unsure = false;
   auto in_circle = [&](double2 a, double2 b, double2 c, double x, double y, PredStats *q) {
     if (!FAST) return incircle_sign(a.x, a.y, b.x, b.y, c.x, c.y, x, y, pws, fp32, q);
@@ -455,8 +458,15 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
     return sg;
   };
   for (int base = blockIdx.x * NG; base < nwork; base += gridDim.x * NG) {
-    const bool act = base + grp < nwork;
-    const int i = !act ? nc : (FAST ? base + grp : ps.slow[base + grp]);
+    bool act = base + grp < nwork;
+    int i = nc;
+    if (act) {
+      i = FAST ? lo + base + grp : ps.slow[base + grp];
+      if (!FAST && (i < nseg) != IS_SEG) {  // deferred candidate of the other kind
+        act = false;
+        i = nc;
+      }
+    }
     PredStats cst = {0, 0, 0, 0};  // this candidate's predicate counts
     // ---- point + seeds (uniform across the group) ----
     int status = act ? ST_VALID : ST_NONE, prio = 0, seed0 = -1, seed1 = -1, split_s = -1,
@@ -907,8 +917,9 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_block(MeshD m, PassD ps, int *ca
 // point's coordinates come from the candidate, not from P.
 template <int G>
 __global__ __launch_bounds__(LIST_BLOCK) void k_apply(MeshD m, ParamsD pr, PassD ps, int pbase,
-                                                      int tbase, int sbase, int is_seg) {
+                                                      int tbase, int sbase) {
   const int nc = pass_nc(ps);
+  const int nseg = pass_nseg(ps);
   const int *ids = pass_ids(ps);
   if (ps.ctl) {
     pbase = ps.ctl->i[C_NP];
@@ -918,6 +929,14 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_apply(MeshD m, ParamsD pr, PassD
   const int gl = threadIdx.x % G, per = blockDim.x / G;
   for (int i = blockIdx.x * per + threadIdx.x / G; i < nc; i += gridDim.x * per) {
     if (!ps.cwin[i]) continue;
+    // winners split subsegments first (candidates [0, nseg)), so a segment
+    // winner's rank is also its rank among the new subsegments
+    const bool is_seg = i < nseg;
+    {
+      const unsigned long long wb = __ballot(gl == 0 && is_seg);
+      if (wb != 0ull && (int)(threadIdx.x & 63) == __builtin_ctzll(wb))
+        atomicAdd(&ps.cnt[CNT_SEGWIN], (unsigned long long)__popcll(wb));
+    }
     const int rank = ps.crank[i];
     const int p = pbase + rank;
     const int tb = tbase + 2 * rank;
@@ -1088,7 +1107,7 @@ __global__ void k_predicates(const double *q, int n, int pred_mode, int *o,
 // the most of it, so clusters resolve in fewer rounds.
 __global__ void k_learned_prio(PassD ps, const float *score, int shift) {
   const int nc = pass_nc(ps);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+  for (int i = pass_nseg(ps) + blockIdx.x * blockDim.x + threadIdx.x; i < nc;
        i += gridDim.x * blockDim.x) {
     if (ps.cstatus[i] != ST_VALID) continue;
     float q = score[i] * 32.f;
@@ -1531,7 +1550,7 @@ Refiner::~Refiner() {
   dfree(seg_in);
   dfree(c_pt); dfree(c_status); dfree(c_n); dfree(c_r);
   dfree(c_prio); dfree(c_win); dfree(c_rank); dfree(c_ws); dfree(c_pws);
-  dfree(c_act); dfree(c_slow); dfree(tmp_list);
+  dfree(c_act); dfree(c_slow); dfree(tmp_list); dfree(mix_list);
   dfree(sel_prio); dfree(sel_tmp); dfree(sel_list); dfree(d_hist);
   dfree(c_feat); dfree(c_score); dfree(c_deg); dfree(t_cnt); dfree(mlp_params);
   dfree(cb_cnt); dfree(cb_off); dfree(d_total); dfree(d_cnt);
@@ -1848,12 +1867,16 @@ void Refiner::enqueue_claims(PassD ps, int gc) {
   }
 }
 
-// One conflict pass; returns number of winners.
-int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
-                      bool *overflowed) {
+// One conflict pass over ids[0..nc): candidates below nseg split
+// subsegments, the rest are bad triangles.  Returns the number of winners,
+// *nwin_seg of them subsegment splits.
+int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, bool *overflowed,
+                      int *nwin_seg) {
+  const int nbad = nc - nseg;
   PassD ps = pass_base(K);
   ps.ids = ids;
   ps.nc = nc;
+  ps.nseg = nseg;
   int iters = std::max(1, opt.claim_iters);
   if (epoch + iters + 1 >= (1ull << 11)) {  // epoch wrap: clear claims
     HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
@@ -1862,7 +1885,7 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   ps.epoch0 = epoch + 1;
   ps.epoch = epoch + 1;
   ps.iter = 0;
-  bool want_feat = !is_seg && (opt.priority == 2 || opt.log_training);
+  bool want_feat = nbad > 0 && (opt.priority == 2 || opt.log_training);
   if (want_feat && nc > feat_cap) {
     dfree(c_feat); dfree(c_score); dfree(c_deg);
     feat_cap = std::max(nc, 2 * feat_cap);
@@ -1873,24 +1896,33 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   ps.feat = want_feat ? c_feat : nullptr;
   ps.pass_index = pass_index;
   HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
-  // one candidate per wave64 block; the exact-predicate arena holds
+  // 64/lg candidates per wave64 block; the exact-predicate arena holds
   // pws_blocks*256 lanes, i.e. pws_blocks*4 blocks of 64
   int lg = cavity_lanes(std::min(K, opt.cav_lds_cap));
-  int gw = std::min((nc + 64 / lg - 1) / (64 / lg), std::min(opt.grid_cavity, pws_blocks * 4));
-  tbegin(is_seg ? K_SEGCAV : K_BADCAV);
-  launch_cavity(is_seg, gw, ps);
-  tend(is_seg ? K_SEGCAV : K_BADCAV, nc);
+  auto cav_grid = [&](int n) {
+    return std::min((n + 64 / lg - 1) / (64 / lg), std::min(opt.grid_cavity, pws_blocks * 4));
+  };
+  if (nseg > 0) {
+    tbegin(K_SEGCAV);
+    launch_cavity(true, cav_grid(nseg), ps);
+    tend(K_SEGCAV, nseg);
+  }
+  if (nbad > 0) {
+    tbegin(K_BADCAV);
+    launch_cavity(false, cav_grid(nbad), ps);
+    tend(K_BADCAV, nbad);
+  }
   HC(hipGetLastError());
   int gc = std::min(nblk((long long)nc * CLAIM_LANES, LIST_BLOCK), opt.grid_claim);
-  if (!is_seg && opt.priority == 2) {
+  if (nbad > 0 && opt.priority == 2) {
     if (!mlp_params) throw std::runtime_error("priority 2 needs a model (set_priority_model)");
     tbegin(K_MLP);
-    mlp_forward_device(c_feat, nc, mlp_params, c_score, stream);
-    k_learned_prio<<<std::min(nblk(nc), 4096), 256, 0, stream>>>(ps, c_score, opt.learned_shift);
-    tend(K_MLP, nc);
+    mlp_forward_device(c_feat + (size_t)nseg * FEAT, nbad, mlp_params, c_score + nseg, stream);
+    k_learned_prio<<<std::min(nblk(nbad), 4096), 256, 0, stream>>>(ps, c_score, opt.learned_shift);
+    tend(K_MLP, nbad);
     HC(hipGetLastError());
   }
-  if (!is_seg && opt.log_training) {  // conflict logs for training
+  if (nbad > 0 && opt.log_training) {  // conflict logs for training (pure bad passes)
     if (nt > t_cnt_cap) {
       dfree(t_cnt);
       t_cnt_cap = std::max<long long>(nt, capT);
@@ -1904,7 +1936,7 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
     HC(hipMemcpyAsync(hf.data(), c_feat, sizeof(float) * hf.size(), hipMemcpyDeviceToHost, stream));
     HC(hipMemcpyAsync(hd.data(), c_deg, sizeof(int) * nc, hipMemcpyDeviceToHost, stream));
     HC(hipStreamSynchronize(stream));
-    for (int i = 0; i < nc; i++) {
+    for (int i = nseg; i < nc; i++) {
       if (hd[i] < 0) continue;
       log_X.insert(log_X.end(), hf.begin() + (size_t)i * FEAT, hf.begin() + (size_t)(i + 1) * FEAT);
       log_t.push_back(std::log2(1.0f + (float)hd[i]));
@@ -1912,18 +1944,19 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   }
   enqueue_claims(ps, gc);
   int nwin = compact(PredWin{c_win}, nc, nullptr, c_rank, nullptr);
-  ensure_capacity(np + nwin, nt + 2LL * nwin, is_seg ? ns + nwin : ns);
+  ensure_capacity(np + nwin, nt + 2LL * nwin, nseg > 0 ? ns + nwin : ns);
   if (nwin > 0) {
     tbegin(K_APPLY);
     k_apply<APPLY_LANES><<<std::min(nblk((long long)nc * APPLY_LANES, LIST_BLOCK), opt.grid_apply),
-                           LIST_BLOCK, 0, stream>>>(m, pr, ps, np, nt, ns, is_seg ? 1 : 0);
+                           LIST_BLOCK, 0, stream>>>(m, pr, ps, np, nt, ns);
     tend(K_APPLY, nwin);
     HC(hipGetLastError());
   }
   read_counters();
+  const int nws = (int)h_cnt[CNT_SEGWIN];
   np += nwin;
   nt += 2 * nwin;
-  if (is_seg) ns += nwin;
+  ns += nws;
   st.rej_overflow += h_cnt[CNT_OVERFLOW];
   st.rej_encroach += h_cnt[CNT_REJ_ENC];
   st.rej_blocked += h_cnt[CNT_REJ_BLOCKED];
@@ -1934,10 +1967,11 @@ int Refiner::run_pass(const int *ids, int nc, bool is_seg, long long *progress,
   st.pred_exact += h_cnt[CNT_EXACT];
   st.pred_dd += h_cnt[CNT_DD];
   st.big_cavities += h_cnt[CNT_BIG];
-  if (is_seg) st.seg_splits += nwin;
-  else st.inserted += nwin;
+  st.seg_splits += nws;
+  st.inserted += nwin - nws;
   *progress = nwin + (long long)h_cnt[CNT_MARKS] + (long long)h_cnt[CNT_GIVEUPS];
   *overflowed = h_cnt[CNT_OVERFLOW] > 0;
+  *nwin_seg = nws;
   return nwin;
 }
 
@@ -2017,6 +2051,7 @@ void Refiner::run() {
   }
   st.status = 1;
   if (opt.graph) loop_graph();
+  else if (opt.mixed) loop_mixed();
   else loop_host();
   st.final_cav_cap = K;
   // --- export --------------------------------------------------------------
@@ -2065,7 +2100,8 @@ void Refiner::loop_host() {
       ensure_cand(nc);
       long long prog = 0;
       bool of = false;
-      run_pass(tmp_list, nc, true, &prog, &of);
+      int nws = 0;
+      run_pass(tmp_list, nc, nc, &prog, &of, &nws);
       st.seg_passes++;
       if (of && K < opt.cav_cap_max) K = std::min(2 * K, opt.cav_cap_max);
     }
@@ -2084,7 +2120,8 @@ void Refiner::loop_host() {
     ensure_cand(nc);
     long long prog = 0;
     bool of = false;
-    prev_win = run_pass(ids, nc, false, &prog, &of);
+    int nws = 0;
+    prev_win = run_pass(ids, nc, 0, &prog, &of, &nws);
     pass_index++;
     st.bad_passes++;
     st.cand_evals += nc;
@@ -2094,6 +2131,60 @@ void Refiner::loop_host() {
     }
     if (prog == 0) { st.status = 2; break; }
   }
+}
+
+// Mixed rounds (options.mixed): one pass per round over the encroached
+// subsegments followed by the selected bad triangles.
+void Refiner::loop_mixed() {
+  int prev_win = -1;
+  ensure_cand(std::min(cands_per_pass(K), std::max(4 * nt0, 1024)));
+  for (int round = 0; round < opt.max_rounds; round++) {
+    st.rounds = round + 1;
+    tbegin(K_MARKBAD);
+    k_mark_bad<<<std::min(nblk(nt), 65536), 256, 0, stream>>>(m, pr, nt, nullptr);
+    tend(K_MARKBAD, nt);
+    const int cap = cands_per_pass(K);
+    ensure_mix(std::min<long long>((long long)ns + nt, (long long)cap + 1));
+    ensure_tmp((int)std::max(ns, nt));
+    int ne = compact(PredSenc{m.senc}, ns, tmp_list, nullptr, nullptr);
+    ne = std::min(ne, cap);
+    if (ne > 0)
+      HC(hipMemcpyAsync(mix_list, tmp_list, sizeof(int) * (size_t)ne, hipMemcpyDeviceToDevice, stream));
+    int nb = compact(PredBad{m.tflag}, nt, tmp_list, nullptr, nullptr);
+    if (ne == 0 && nb == 0) { st.status = 0; break; }
+    int capb = cap - ne;
+    if (opt.cand_factor > 0 && prev_win >= 0)
+      capb = (int)std::min<long long>(capb, std::max<long long>(opt.cand_min, (long long)opt.cand_factor * prev_win));
+    int *bids = tmp_list;
+    int nbc = nb > 0 && capb > 0 ? select_by_priority(tmp_list, nb, capb, &bids) : 0;
+    if (nbc > 0)
+      HC(hipMemcpyAsync(mix_list + ne, bids, sizeof(int) * (size_t)nbc, hipMemcpyDeviceToDevice, stream));
+    const int nc = ne + nbc;
+    ensure_cand(nc);
+    long long prog = 0;
+    bool of = false;
+    int nws = 0;
+    int nwin = run_pass(mix_list, nc, ne, &prog, &of, &nws);
+    if (ne > 0) st.seg_passes++;
+    if (nbc > 0) {
+      pass_index++;
+      st.bad_passes++;
+      st.cand_evals += nbc;
+      prev_win = nwin - nws;
+    }
+    if (of && K < opt.cav_cap_max) {
+      K = std::min(2 * K, opt.cav_cap_max);
+      prog++;
+    }
+    if (prog == 0) { st.status = 2; break; }
+  }
+}
+
+void Refiner::ensure_mix(long long n) {
+  if (n <= mix_cap) return;
+  dfree(mix_list);
+  mix_cap = std::max<long long>(n, 2 * mix_cap);
+  dalloc(&mix_list, (size_t)mix_cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -2184,6 +2275,7 @@ void Refiner::enqueue_seg_pass() {
                                               tmp_list, nullptr, nullptr);
   PassD ps = pass_base(K);
   ps.ids = tmp_list;
+  ps.nseg = -1;
   ps.ctl = d_ctl;
   ps.cnt = ctl_cnt_ptr();
   launch_cavity(true, opt.grid_cavity, ps);
@@ -2192,7 +2284,7 @@ void Refiner::enqueue_seg_pass() {
   k_dscan<FinWin><<<1, GC, 0, stream>>>(g_cnt, g_off, d_ctl, -1, FinWin{});
   k_dscatter<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_off,
                                              nullptr, c_rank, nullptr);
-  k_apply<APPLY_LANES><<<opt.grid_apply, LIST_BLOCK, 0, stream>>>(m, pr, ps, 0, 0, 0, 1);
+  k_apply<APPLY_LANES><<<opt.grid_apply, LIST_BLOCK, 0, stream>>>(m, pr, ps, 0, 0, 0);
   k_end_seg<<<1, 1, 0, stream>>>(d_ctl, std::max(1, opt.claim_iters));
   HC(hipGetLastError());
 }
@@ -2237,7 +2329,7 @@ void Refiner::enqueue_bad_pass() {
   k_dscan<FinWin><<<1, GC, 0, stream>>>(g_cnt, g_off, d_ctl, -1, FinWin{});
   k_dscatter<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_off,
                                              nullptr, c_rank, nullptr);
-  k_apply<APPLY_LANES><<<opt.grid_apply, LIST_BLOCK, 0, stream>>>(m, pr, ps, 0, 0, 0, 0);
+  k_apply<APPLY_LANES><<<opt.grid_apply, LIST_BLOCK, 0, stream>>>(m, pr, ps, 0, 0, 0);
   k_end_bad<<<1, 1, 0, stream>>>(d_ctl, std::max(1, opt.claim_iters));
   HC(hipGetLastError());
 }

@@ -150,6 +150,14 @@ typedef struct gcdt_options {
                             buckets coarsened by 2^learned_shift half-octaves,
                             then the predicted conflict degree (higher first);
                             0 keeps Triangle's half-octave order (default 2) */
+  int mixed;             /* 0: a round is segment-split passes until no
+                            subsegment is encroached, then one bad-triangle
+                            pass (the reference's round, refine.cu:6700-7100);
+                            1: a round is ONE pass in which the encroached
+                            subsegments and the selected bad triangles compete
+                            for their regions, subsegments first in candidate
+                            order (they win every tie).  Host-driven loop only
+                            (graph = 0).  Default 0 */
 } gcdt_options;
 
 typedef struct gcdt_stats {

@@ -53,6 +53,12 @@ typedef struct {
   int claim_iters;    /* conflict-resolution rounds per pass (>= 1): after the
                          first, valid candidates whose region touches no winner
                          claim again among themselves */
+  int mixed;          /* 0: a round is segment passes until no subsegment is
+                         encroached, then one bad-triangle pass (refine.cu:
+                         6700-7100 round structure); 1: a round is ONE pass in
+                         which the encroached subsegments (first in candidate
+                         order, so they win every tie) and the selected bad
+                         triangles compete together */
 } orc_params;
 
 typedef struct {

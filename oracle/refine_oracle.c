@@ -606,18 +606,20 @@ static int cands_per_pass(long long budget, int K) {
   return (int)c;
 }
 
-/* One conflict-resolution pass over candidates ids[0..nc). is_seg selects
- * the segment-split or the bad-triangle flavour.  Returns the number of
- * winners; sets *overflowed if any cavity exceeded K. */
-static int run_pass(Mesh *m, const int *ids, int nc, int is_seg, int K,
+/* One conflict-resolution pass over candidates ids[0..nc): candidates below
+ * nseg split subsegments, the rest are bad triangles.  Returns the number of
+ * winners (*nwin_seg of them segment splits); sets *overflowed if any cavity
+ * exceeded K. */
+static int run_pass(Mesh *m, const int *ids, int nc, int nseg, int K,
                     int Kmax, int priority, int claim_iters, orc_stats *st,
-                    int *overflowed, long long *progress) {
+                    int *overflowed, long long *progress, int *nwin_seg) {
   long long marks = 0, gave = st->giveups;
   Cand *cd = calloc((size_t)nc, sizeof(Cand));
   int *pool = malloc(sizeof(int) * (size_t)(2 * K + 2));
   *overflowed = 0;
   for (int i = 0; i < nc; i++) {
     Cand *c = &cd[i];
+    const int is_seg = i < nseg;
     c->id = ids[i];
     c->prio = (priority == 1 && !is_seg) ? shortest_edge_bucket(m, ids[i]) : 0;
     c->L = pool; /* temporary; copied out below if valid */
@@ -747,12 +749,16 @@ static int run_pass(Mesh *m, const int *ids, int nc, int is_seg, int K,
   free(active);
   /* slot bases (winners take points/triangles/subsegments in candidate order) */
   int pbase = m->np, tbase = m->nt, sbase = m->ns;
+  int nws = 0;
+  for (int i = 0; i < nseg; i++) nws += win[i];
   grow_points(m, pbase + nwin);
   grow_tris(m, tbase + 2 * nwin);
-  if (is_seg) grow_segs(m, sbase + nwin);
+  grow_segs(m, sbase + nws);
   int rank = 0;
+  /* segment winners come first, so their rank is also their subsegment rank */
   for (int i = 0; i < nc; i++) {
     if (!win[i]) continue;
+    const int is_seg = i < nseg;
     int p = pbase + rank;
     m->P[2 * p] = cd[i].c[0];
     m->P[2 * p + 1] = cd[i].c[1];
@@ -765,9 +771,10 @@ static int run_pass(Mesh *m, const int *ids, int nc, int is_seg, int K,
   }
   m->np = pbase + nwin;
   m->nt = tbase + 2 * nwin;
-  if (is_seg) m->ns = sbase + nwin;
-  if (is_seg) st->seg_splits += nwin;
-  else st->inserted += nwin;
+  m->ns = sbase + nws;
+  st->seg_splits += nws;
+  st->inserted += nwin - nws;
+  *nwin_seg = nws;
   *progress = nwin + marks + (st->giveups - gave);
   if (getenv("ORC_TRACE")) {
     long long nval = 0, csum = 0, cmax = 0;
@@ -777,8 +784,8 @@ static int run_pass(Mesh *m, const int *ids, int nc, int is_seg, int K,
         csum += cd[i].n;
         if (cd[i].n > cmax) cmax = cd[i].n;
       }
-    fprintf(stderr, "pass %s nc=%d valid=%lld win=%d marks=%lld avgcav=%.2f maxcav=%lld K=%d\n",
-            is_seg ? "seg" : "bad", nc, nval, nwin, marks, nval ? (double)csum / nval : 0.0, cmax, K);
+    fprintf(stderr, "pass nseg=%d nc=%d valid=%lld win=%d (seg %d) marks=%lld avgcav=%.2f maxcav=%lld K=%d\n",
+            nseg, nc, nval, nwin, nws, marks, nval ? (double)csum / nval : 0.0, cmax, K);
   }
   for (int i = 0; i < nc; i++) free(cd[i].L);
   free(cd);
@@ -903,6 +910,72 @@ static void free_mesh(Mesh *m) {
   free(m->tinst);
 }
 
+/* Keep the cap smallest (priority, index) of the bad triangles ids[0..nb),
+ * in index order (priority 1); otherwise the first cap.  Returns the count. */
+static int select_bad(Mesh *m, int *ids, int nb, int cap, int priority) {
+  if (nb <= cap) return nb;
+  if (priority == 1) {
+    long long *key = malloc(sizeof(long long) * (size_t)nb);
+    for (int i = 0; i < nb; i++)
+      key[i] = ((long long)shortest_edge_bucket(m, ids[i]) << 32) | ids[i];
+    qsort(key, (size_t)nb, sizeof(long long), cmp_ll);
+    for (int i = 0; i < cap; i++) ids[i] = (int)(key[i] & 0xffffffffLL);
+    qsort(ids, (size_t)cap, sizeof(int), cmp_int);
+    free(key);
+  }
+  return cap;
+}
+
+/* One round of the mixed form: the encroached subsegments (capped by the
+ * workspace) followed by the selected bad triangles, in one pass.  Returns 1
+ * when the refinement ends (st->status set). */
+static int mixed_round(Mesh *m, const orc_params *prm, orc_stats *st, int *K, int *prev_win,
+                       int **ids, int *idcap) {
+  mark_bad(m);
+  int ne = 0, nb = 0;
+  for (int s = 0; s < m->ns; s++) ne += m->senc[s] == 1;
+  for (int t = 0; t < m->nt; t++)
+    nb += (m->tflag[t] & (TF_ALIVE | TF_BAD | TF_GIVEUP)) == (TF_ALIVE | TF_BAD);
+  if (ne == 0 && nb == 0) { st->status = 0; return 1; }
+  if (ne + nb > *idcap) {
+    *idcap = ne + nb;
+    *ids = xrealloc(*ids, sizeof(int) * (size_t)*idcap);
+  }
+  int *L = *ids;
+  int cap = cands_per_pass(prm->cand_budget, *K);
+  int k = 0;
+  for (int s = 0; s < m->ns && k < cap; s++)
+    if (m->senc[s] == 1) L[k++] = s;
+  ne = k;
+  int capb = cap - ne;
+  if (prm->cand_factor > 0 && *prev_win >= 0) {
+    long long lim = (long long)prm->cand_factor * *prev_win;
+    if (lim < prm->cand_min) lim = prm->cand_min;
+    if (lim < capb) capb = (int)lim;
+  }
+  int *B = L + ne;
+  nb = 0;
+  for (int t = 0; t < m->nt; t++)
+    if ((m->tflag[t] & (TF_ALIVE | TF_BAD | TF_GIVEUP)) == (TF_ALIVE | TF_BAD)) B[nb++] = t;
+  nb = select_bad(m, B, nb, capb, prm->priority);
+  int of = 0, nws = 0;
+  long long prog = 0;
+  int nw = run_pass(m, L, ne + nb, ne, *K, prm->cav_cap_max, prm->priority, prm->claim_iters, st,
+                    &of, &prog, &nws);
+  if (ne > 0) st->seg_passes++;
+  if (nb > 0) {
+    st->bad_passes++;
+    st->cand_evals += nb;
+    *prev_win = nw - nws;
+  }
+  if (of && *K < prm->cav_cap_max) {
+    *K = *K * 2 > prm->cav_cap_max ? prm->cav_cap_max : *K * 2;
+    prog++;
+  }
+  if (prog == 0) { st->status = 2; return 1; }
+  return 0;
+}
+
 static orc_result *refine_impl(int np, const double *points, int nt,
                                const int *tris, const int *nbrs, int ns,
                                const int *segs, const int *tri_inst,
@@ -919,6 +992,10 @@ static orc_result *refine_impl(int np, const double *points, int nt,
   st->status = 1;
   for (int round = 0; round < prm->max_rounds; round++) {
     st->rounds = round + 1;
+    if (prm->mixed) {
+      if (mixed_round(m, prm, st, &K, &prev_win, &ids, &idcap)) break;
+      continue;
+    }
     /* segment phase */
     for (;;) {
       int ne = 0;
@@ -929,9 +1006,10 @@ static orc_result *refine_impl(int np, const double *points, int nt,
       for (int s = 0; s < m->ns; s++) if (m->senc[s] == 1) ids[ne++] = s;
       int cap = cands_per_pass(prm->cand_budget, K);
       if (ne > cap) ne = cap;
-      int of = 0;
+      int of = 0, nws = 0;
       long long prog = 0;
-      run_pass(m, ids, ne, 1, K, prm->cav_cap_max, prm->priority, prm->claim_iters, st, &of, &prog);
+      run_pass(m, ids, ne, ne, K, prm->cav_cap_max, prm->priority, prm->claim_iters, st, &of,
+               &prog, &nws);
       st->seg_passes++;
       if (of && K < prm->cav_cap_max) K = K * 2 > prm->cav_cap_max ? prm->cav_cap_max : K * 2;
     }
@@ -951,22 +1029,11 @@ static orc_result *refine_impl(int np, const double *points, int nt,
       if (lim < prm->cand_min) lim = prm->cand_min;
       if (lim < cap) cap = (int)lim;
     }
-    if (nb > cap) {
-      if (prm->priority == 1) {
-        /* keep the cap smallest (priority, index) candidates, in index order */
-        long long *key = malloc(sizeof(long long) * (size_t)nb);
-        for (int i = 0; i < nb; i++)
-          key[i] = ((long long)shortest_edge_bucket(m, ids[i]) << 32) | ids[i];
-        qsort(key, (size_t)nb, sizeof(long long), cmp_ll);
-        for (int i = 0; i < cap; i++) ids[i] = (int)(key[i] & 0xffffffffLL);
-        qsort(ids, (size_t)cap, sizeof(int), cmp_int);
-        free(key);
-      }
-      nb = cap;
-    }
-    int of = 0;
+    nb = select_bad(m, ids, nb, cap, prm->priority);
+    int of = 0, nws = 0;
     long long prog = 0;
-    int nw = run_pass(m, ids, nb, 0, K, prm->cav_cap_max, prm->priority, prm->claim_iters, st, &of, &prog);
+    int nw = run_pass(m, ids, nb, 0, K, prm->cav_cap_max, prm->priority, prm->claim_iters, st, &of,
+                      &prog, &nws);
     st->bad_passes++;
     st->cand_evals += nb;
     prev_win = nw;

@@ -176,3 +176,24 @@ def test_steiner_error_against_exact_rationals(oracle):
         n += 1
     assert n > 300
     assert worst_rel < 1e-9, worst_rel
+
+
+@pytest.mark.parametrize("theta,off,mode,ci", [(25, 1, 1, 1), (25, 1, 1, 3), (30, 0, 0, 2),
+                                               (20.7, 1, 0, 1)])
+def test_mixed_rounds_invariants(oracle, theta, off, mode, ci):
+    """The mixed form (encroached subsegments and bad triangles in one pass per
+    round) meets the same invariants, needs fewer passes than segment phases +
+    bad passes, and its Steiner count stays within 5% of the classic round's."""
+    P, T, N, S = _cdt(oracle, 3000, 150, 8)
+    kw = dict(theta=theta, offcenter=off, mode=mode, claim_iters=ci)
+    Pc, Tc, Nc, Sc, pfc, stc = oracle.oracle_refine(P, T, N, S, **kw)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, mixed=1, **kw)
+    assert st["status"] == 0 and st["giveups"] == 0
+    tot, viol, mina, ex = oracle.check_mesh(Po, To, No, So, P, S, theta)
+    assert tot == 0, viol
+    assert mina >= theta - 1e-7
+    assert np.array_equal(Po[:len(P)].view(np.uint64), P.view(np.uint64))
+    assert st["rounds"] < stc["seg_passes"] + stc["bad_passes"]
+    assert abs((len(Po) - len(Pc)) / (len(Pc) - len(P))) < 0.05
+    assert st["seg_splits"] + len(S) == len(So)
+    assert st["inserted"] + st["seg_splits"] + len(P) == len(Po)
