@@ -45,12 +45,19 @@ B_POINT, B_TRI, B_SUBSEG = 16, 36, 12
 B_OUT_TRI, B_OUT_SEG = 24, 8
 B_PER_INSERT = 424  # 3 parent vertices + 4-triangle cavity read, 6-triangle star + point written
 
-# Algorithmic bytes per work item of each timed kernel group:
+# Cavity kernels: algorithmic bytes are counted from the units the kernels
+# report -- every triangle record examined (walk step, seed, conflict test)
+# needs its 3 vertex ids, 3 neighbours, 3 subsegments and 3 vertex
+# coordinates (36 + 48 B); every candidate reads its id and writes status,
+# point, cavity / ring sizes and priority (36 B); every workspace int
+# written (cavity + ring list, ordered star boundary) is 4 B; under the
+# learned priority a bad candidate also writes 8 float features (32 B).
+B_EXAM, B_CAND, B_WS, B_FEAT = 84, 36, 4, 32
+
+# Algorithmic bytes per work item of the other timed kernel groups:
 KERNEL_BYTES = {
-    # parent record + 3 vertices, 4 cavity records + their opposite vertices,
-    # cavity + ring ids written, candidate record written
-    "bad_cavity": 36 + 48 + 4 * 36 + 4 * 8 + 40 + 40,
-    "seg_cavity": 12 + 32 + 2 * 36 + 4 * 36 + 4 * 8 + 40 + 40,
+    "bad_cavity": None,
+    "seg_cavity": None,
     "apply": B_PER_INSERT,
     "claim": 10 * 4 + 10 * 8,   # list read + 64-bit claim RMW
     "win": 10 * 4 + 10 * 8 + 4,
@@ -123,10 +130,10 @@ def profile_kernels(P, T, N, SS, theta, opts):
     prof = g.Session(P, T, N, SS, theta=theta, insert=g.Offcenter, mode=g.RUPPERT, profile=1, **o)
     if o.get("priority") == 2:
         prof.set_priority_model(np.load(MODEL))
-    prof.run()
+    st = prof.run()
     kt = prof.kernel_times()
     prof.close()
-    return kt
+    return kt, st
 
 
 TRAFFIC = os.path.join(ROOT, "profiles", "traffic_bad_cavity_refine_1M.json")
@@ -145,18 +152,37 @@ def measured_traffic(kernel, workload):
     return t["traffic_bytes_per_launch"]
 
 
-def roofline(kt, workload):
+def kernel_bytes(name, d, st, learned):
+    """Algorithmic bytes of all launches of kernel group `name`, and a
+    description of the model."""
+    if name in ("bad_cavity", "seg_cavity"):
+        kind = "bad" if name == "bad_cavity" else "seg"
+        e, w, c = st[f"tri_examined_{kind}"], st[f"ws_ints_{kind}"], d["items"]
+        feat = B_FEAT * c if (learned and kind == "bad") else 0
+        return (B_EXAM * e + B_CAND * c + B_WS * w + feat,
+                {"bytes_per_examined_triangle": B_EXAM, "bytes_per_candidate": B_CAND
+                 + (B_FEAT if feat else 0), "bytes_per_workspace_int": B_WS,
+                 "examined_triangles_per_launch": e / max(1, d["launches"]),
+                 "candidates_per_launch": c / max(1, d["launches"]),
+                 "workspace_ints_per_launch": w / max(1, d["launches"])})
+    return KERNEL_BYTES[name] * d["items"], {"bytes_per_item": KERNEL_BYTES[name],
+                                             "items_per_launch": d["items"] / max(1, d["launches"])}
+
+
+def roofline(kt, st, workload, learned):
     dom = max((k for k in kt if k in KERNEL_BYTES), key=lambda k: kt[k]["total_ms"])
     d = kt[dom]
     per_launch_ms = d["total_ms"] / max(1, d["launches"])
-    bytes_per_launch = KERNEL_BYTES[dom] * d["items"] / max(1, d["launches"])
+    total, model = kernel_bytes(dom, d, st, learned)
+    bytes_per_launch = total / max(1, d["launches"])
     achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
-    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom, workload),
-            "avg_launch_ms": per_launch_ms, "launches": d["launches"],
-            "bytes_per_item": KERNEL_BYTES[dom],
-            "items_per_launch": d["items"] / max(1, d["launches"])}
+    out = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+           "traffic": measured_traffic(dom, workload),
+           "algorithmic_bytes_per_launch": bytes_per_launch,
+           "avg_launch_ms": per_launch_ms, "launches": d["launches"]}
+    out.update(model)
+    return out
 
 
 def cpu_baseline(args, n_sample):
@@ -236,8 +262,15 @@ def bench_refine(args):
              "frac": (b_alg / (ms / 1e3) / 1e9) / HBM_PEAK_GBS}
     roof, kt = None, None
     if not args.no_profile_step:
-        kt = profile_kernels(P, T, N, SS, args.theta, co)
-        roof = roofline(kt, refine_workload_name(args))
+        # per-launch events need the host-driven passes; the launch shapes
+        # are the tuned ones
+        po = dict(co)
+        if tune:
+            po.update({k: v for k, v in tune["best"].items()
+                       if k in ("grid_cavity", "grid_claim", "grid_apply", "cavity_lanes",
+                                "cav_lds_cap")})
+        kt, kst = profile_kernels(P, T, N, SS, args.theta, po)
+        roof = roofline(kt, kst, refine_workload_name(args), co["priority"] == 2)
     dropin = None
     if not args.no_dropin:
         # PCIe-inclusive: host CDT in, host mesh out (the drop-in's contract)

@@ -102,6 +102,10 @@ class Stats(C.Structure):
         ("cand_evals", C.c_longlong),
         ("big_cavities", C.c_longlong),
         ("pred_dd", C.c_longlong),
+        ("tri_examined_seg", C.c_longlong),
+        ("tri_examined_bad", C.c_longlong),
+        ("ws_ints_seg", C.c_longlong),
+        ("ws_ints_bad", C.c_longlong),
     ]
 
     def as_dict(self):

@@ -299,6 +299,10 @@ int gcdt_batch_run(gcdt_batch *b, gcdt_stats *stats) {
       tot.n_subsegs += s.n_subsegs;
       tot.cand_evals += s.cand_evals;
       tot.big_cavities += s.big_cavities;
+      tot.tri_examined_seg += s.tri_examined_seg;
+      tot.tri_examined_bad += s.tri_examined_bad;
+      tot.ws_ints_seg += s.ws_ints_seg;
+      tot.ws_ints_bad += s.ws_ints_bad;
       b->run_ms[w] = std::chrono::duration<double, std::milli>(
                          std::chrono::steady_clock::now() - t0).count();
       // the Refiner's buffers are reused by the next wave: fetch this one

@@ -58,7 +58,11 @@ enum : int {
   CNT_DD,
   CNT_SLOW,    // candidates deferred to the exact-predicate kernel
   CNT_SEGWIN,  // winners that split a subsegment
-  CNT_N = 16
+  CNT_EXAM_SEG,  // triangle records examined by the segment cavity kernels
+  CNT_EXAM_BAD,  // ... by the bad-triangle cavity kernels
+  CNT_WS_SEG,    // ints written to candidate workspaces (segment kernels)
+  CNT_WS_BAD,    // ... (bad-triangle kernels)
+  CNT_N = 24
 };
 
 #define GCDT_WALK_LIMIT (1 << 22)
@@ -142,7 +146,8 @@ enum : int {
 enum : int {
   A_REJ_OVF = 0, A_REJ_ENC, A_REJ_BLK, A_GIVEUPS, A_LOST, A_FP32, A_FP64,
   A_EXACT, A_DD, A_BIG, A_SEGSPLITS, A_INSERTED, A_CANDEVALS, A_BUDGET,
-  A_NA = 16
+  A_EXAM_SEG, A_EXAM_BAD, A_WS_SEG, A_WS_BAD,
+  A_NA = 20
 };
 struct Ctl {
   int i[C_NI];
@@ -167,6 +172,7 @@ struct PassD {
   unsigned long long *cnt;
   unsigned long long epoch0;  // first claim epoch of this pass
   int iter;                   // claim iteration
+  int iters;                  // claim iterations of the pass
   float *feat;                // optional: FEAT features per candidate
   int pass_index;             // bad-pass counter (feature)
   // graph-driven loop: counts, epochs and the candidate list come from ctl
@@ -190,6 +196,11 @@ __device__ __forceinline__ unsigned long long pass_epoch0(const PassD &ps) {
 }
 __device__ __forceinline__ unsigned long long pass_epoch(const PassD &ps) {
   return ps.ctl ? pass_epoch0(ps) + (unsigned long long)ps.iter : ps.epoch;
+}
+// Winners' regions are stamped with the epoch after the pass's last claim
+// epoch: no claim of the pass can displace them.
+__device__ __forceinline__ unsigned long long pass_sticky(const PassD &ps) {
+  return pass_epoch0(ps) + (unsigned long long)ps.iters;
 }
 __device__ __forceinline__ int pass_index_of(const PassD &ps) {
   return ps.ctl ? ps.ctl->i[C_PASS_INDEX] : ps.pass_index;
@@ -408,8 +419,10 @@ __device__ __forceinline__ bool split_point(const MeshD &m, int s, double &px,
 template <bool FAST>
 __device__ __forceinline__ int walk(const MeshD &m, int t, double cx,
                                     double cy, bool fp32, PredStats *st,
-                                    int &tout, int &jout, int &seg) {
+                                    int &tout, int &jout, int &seg, int &nsteps) {
+  nsteps = 0;
   for (int steps = 0; steps < GCDT_WALK_LIMIT; steps++) {
+    nsteps = steps + 1;  // records read
     const TriRec &r = m.T[t];
     double2 p0 = r.X[0], p1 = r.X[1], p2 = r.X[2];
     int o0, o1, o2;

@@ -320,14 +320,15 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
 // added to the pass counters once per wave: a counter shared by every
 // candidate would otherwise serialise on one L2 address.
 enum { CC_MARKS = 0, CC_GIVEUPS, CC_OVERFLOW, CC_REJ_ENC, CC_REJ_BLOCKED, CC_VALID,
-       CC_FP32, CC_FP64, CC_DD, CC_EXACT, CC_N };
+       CC_FP32, CC_FP64, CC_DD, CC_EXACT, CC_EXAM, CC_N };
 struct CavCounts {
   unsigned c[CC_N];
   __device__ void add(unsigned long long *cnt, int k, int dst) {
     unsigned long long v = wave_sum(c[k]);
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(&cnt[dst], v);
   }
-  __device__ void flush(unsigned long long *cnt) {
+  __device__ void flush(unsigned long long *cnt, bool is_seg) {
+    add(cnt, CC_EXAM, is_seg ? CNT_EXAM_SEG : CNT_EXAM_BAD);
     add(cnt, CC_MARKS, CNT_MARKS);
     add(cnt, CC_GIVEUPS, CNT_GIVEUPS);
     add(cnt, CC_OVERFLOW, CNT_OVERFLOW);
@@ -479,6 +480,7 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
         split_s = ids[i];
         int t1, j1, t2, j2;
         seg_sides(m, split_s, t1, j1, t2, j2);
+        if (gl == 0) cc.c[CC_EXAM] += (t1 >= 0) + (t2 >= 0);
         lp = params_at(pr, m, t1 >= 0 ? t1 : t2);
         if (!split_point(m, split_s, cx, cy)) {
           status = ST_GIVEUP;
@@ -499,8 +501,10 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
           int tl = -1, jl = -1, sg = -1;
           // the walk's signs are counted once per group (every lane walks)
           PredStats wst = {0, 0, 0, 0};
-          int w = walk<FAST>(m, t0, cx, cy, fp32, &wst, tl, jl, sg);
+          int wsteps = 0;
+          int w = walk<FAST>(m, t0, cx, cy, fp32, &wst, tl, jl, sg, wsteps);
           if (gl == 0) {
+            cc.c[CC_EXAM] += (unsigned)wsteps;  // the first step reads the bad triangle
             cst.fp32_hits += wst.fp32_hits;
             cst.fp64_hits += wst.fp64_hits;
             cst.dd_hits += wst.dd_hits;
@@ -567,6 +571,7 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
           if (hset_has(hc, HC, u)) continue;
           const TriRec &ru = m.T[u];
           double2 a = ru.X[0], b = ru.X[1], c = ru.X[2];
+          cc.c[CC_EXAM]++;
           int ins = in_circle(a, b, c, cx, cy, &cst);
           if (FAST && unsure) {
             sc[SC_SLOW] = 1;
@@ -795,7 +800,7 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
     }
     __syncthreads();  // LDS is reused by the next candidates
   }
-  cc.flush(ps.cnt);
+  cc.flush(ps.cnt, IS_SEG);
 }
 
 // The filtered kernel is held to 5 waves per SIMD (<= 96 VGPRs, no spills);
@@ -838,37 +843,64 @@ __device__ __forceinline__ unsigned long long group_bits(unsigned long long b) {
   return G == 64 ? b : (b >> base) & ((1ull << (G == 64 ? 0 : G)) - 1ull);
 }
 
-// Candidate i takes part in claim iteration ps.iter when it is valid, has not
-// won and touches no winner's region (cact != 0).
+// Claim iteration ps.iter.  Iteration 0: every valid candidate claims its
+// cavity and ring.  Later iterations: a candidate still in play (cact) whose
+// region touches a winner's stamped region drops out (it is blocked, the
+// oracle's rule); the others claim again among themselves.  Stamps carry an
+// epoch above every claim of the pass, so concurrent claims never hide them.
 template <int G>
 __global__ __launch_bounds__(LIST_BLOCK) void k_claim(MeshD m, PassD ps, int *cact) {
   const int nc = pass_nc(ps);
   const unsigned long long ep = pass_epoch(ps);
+  const unsigned long long sticky = pass_sticky(ps);
   const int gl = threadIdx.x % G, per = blockDim.x / G;
+  const int nseg = pass_nseg(ps);
+  unsigned ws_seg = 0, ws_bad = 0;  // workspace ints the cavity kernels wrote
   for (int i = blockIdx.x * per + threadIdx.x / G; i < nc; i += gridDim.x * per) {
-    bool on;
+    const int *L = ps.ws + (size_t)i * ps.R;
     if (ps.iter == 0) {
-      on = ps.cstatus[i] == ST_VALID;
+      const bool on = ps.cstatus[i] == ST_VALID;
       if (gl == 0) {
         cact[i] = on ? 1 : 0;
         ps.cwin[i] = 0;
       }
+      if (!on) continue;
+      if (gl == 0) {  // cavity + ring list, then the star boundary (4 ints per edge)
+        const unsigned w = (unsigned)(ps.cn[i] + ps.cr[i] + 4 * (ps.cn[i] + 2));
+        if (i < nseg) ws_seg += w;
+        else ws_bad += w;
+      }
     } else {
-      on = cact[i] != 0;
+      if (!cact[i]) continue;
+      const int k = ps.cn[i] + ps.cr[i];
+      bool hit = false;
+      for (int q = gl; q < k; q += G) hit |= (m.claim[L[q]] >> 53) == sticky;
+      if (group_bits<G>(__ballot(hit)) != 0ull) {
+        if (gl == 0) cact[i] = 0;
+        continue;
+      }
     }
-    if (!on) continue;
     const unsigned long long key = claim_key(ep, ps.cprio[i], i);
-    const int *L = ps.ws + (size_t)i * ps.R;
     const int k = ps.cn[i] + ps.cr[i];
     for (int q = gl; q < k; q += G) atomicMax(&m.claim[L[q]], key);
   }
+  if (ps.iter == 0) {
+    const unsigned long long a = wave_sum(ws_seg), b = wave_sum(ws_bad);
+    if ((threadIdx.x & 63) == 0) {
+      if (a) atomicAdd(&ps.cnt[CNT_WS_SEG], a);
+      if (b) atomicAdd(&ps.cnt[CNT_WS_BAD], b);
+    }
+  }
 }
 
-// A candidate wins when every triangle of its cavity and ring holds its key.
+// A candidate wins when every triangle of its cavity and ring holds its key;
+// a winner then stamps its region.  A stamp only replaces the winner's own
+// key, which no other candidate's test could match either.
 template <int G>
 __global__ __launch_bounds__(LIST_BLOCK) void k_win(MeshD m, PassD ps, const int *cact) {
   const int nc = pass_nc(ps);
   const unsigned long long ep = pass_epoch(ps);
+  const unsigned long long sticky = pass_sticky(ps);
   const int gl = threadIdx.x % G, per = blockDim.x / G;
   for (int i = blockIdx.x * per + threadIdx.x / G; i < nc; i += gridDim.x * per) {
     if (!cact[i]) continue;
@@ -877,34 +909,12 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_win(MeshD m, PassD ps, const int
     const int k = ps.cn[i] + ps.cr[i];
     bool lose = false;
     for (int q = gl; q < k; q += G) lose |= m.claim[L[q]] != key;
-    if (group_bits<G>(__ballot(lose)) == 0ull && gl == 0) ps.cwin[i] = 1;
-  }
-}
-
-// After a claim iteration: winners and candidates touching a winner's region
-// (a triangle whose claim comes from this pass and from a winner) drop out.
-template <int G>
-__global__ __launch_bounds__(LIST_BLOCK) void k_block(MeshD m, PassD ps, int *cact) {
-  const int nc = pass_nc(ps);
-  const unsigned long long ep0 = pass_epoch0(ps);
-  const int gl = threadIdx.x % G, per = blockDim.x / G;
-  for (int i = blockIdx.x * per + threadIdx.x / G; i < nc; i += gridDim.x * per) {
-    if (!cact[i]) continue;
-    if (ps.cwin[i]) {
-      if (gl == 0) cact[i] = 0;
-      continue;
+    if (group_bits<G>(__ballot(lose)) != 0ull) continue;
+    if (gl == 0) ps.cwin[i] = 1;
+    if (ps.iter + 1 < ps.iters) {
+      const unsigned long long stamp = claim_key(sticky, ps.cprio[i], i);
+      for (int q = gl; q < k; q += G) m.claim[L[q]] = stamp;
     }
-    const int *L = ps.ws + (size_t)i * ps.R;
-    const int k = ps.cn[i] + ps.cr[i];
-    bool hit = false;
-    for (int q = gl; q < k; q += G) {
-      const unsigned long long c = m.claim[L[q]];
-      if ((c >> 53) >= ep0) {
-        const int j = (int)(0xffffffffu - (unsigned)(c & 0xffffffffull));
-        hit |= ps.cwin[j] != 0;
-      }
-    }
-    if (group_bits<G>(__ballot(hit)) != 0ull && gl == 0) cact[i] = 0;
   }
 }
 
@@ -1374,6 +1384,10 @@ __device__ void ctl_accumulate(Ctl *c, int nwin) {
   c->acc[A_EXACT] += n[CNT_EXACT];
   c->acc[A_DD] += n[CNT_DD];
   c->acc[A_BIG] += n[CNT_BIG];
+  c->acc[A_EXAM_SEG] += n[CNT_EXAM_SEG];
+  c->acc[A_EXAM_BAD] += n[CNT_EXAM_BAD];
+  c->acc[A_WS_SEG] += n[CNT_WS_SEG];
+  c->acc[A_WS_BAD] += n[CNT_WS_BAD];
 }
 
 __global__ void k_end_seg(Ctl *c, int iters) {
@@ -1385,7 +1399,7 @@ __global__ void k_end_seg(Ctl *c, int iters) {
   c->i[C_NT] += 2 * nwin;
   c->i[C_NS] += nwin;
   c->i[C_SEG_PASSES]++;
-  c->i[C_EPOCH] += iters;
+  c->i[C_EPOCH] += iters + 1;  // claim epochs + the winners' stamp
   int K = c->i[C_K];
   if (c->cnt[CNT_OVERFLOW] > 0 && K < c->i[C_KMAX]) c->i[C_K] = min(2 * K, c->i[C_KMAX]);
 }
@@ -1401,7 +1415,7 @@ __global__ void k_end_bad(Ctl *c, int iters) {
   c->i[C_PREVWIN] = nwin;
   c->i[C_PASS_INDEX]++;
   c->i[C_BAD_PASSES]++;
-  c->i[C_EPOCH] += iters;
+  c->i[C_EPOCH] += iters + 1;  // claim epochs + the winners' stamp
   long long prog = nwin + (long long)c->cnt[CNT_MARKS] + (long long)c->cnt[CNT_GIVEUPS];
   int K = c->i[C_K];
   if (c->cnt[CNT_OVERFLOW] > 0 && K < c->i[C_KMAX]) {
@@ -1845,6 +1859,7 @@ PassD Refiner::pass_base(int Kc) const {
   ps.ctl = nullptr;
   ps.ids_sel = nullptr;
   ps.slow = c_slow;
+  ps.iters = std::max(1, opt.claim_iters);
   return ps;
 }
 
@@ -1853,6 +1868,7 @@ PassD Refiner::pass_base(int Kc) const {
 // it from the control block.
 void Refiner::enqueue_claims(PassD ps, int gc) {
   int iters = std::max(1, opt.claim_iters);
+  ps.iters = iters;
   for (int it = 0; it < iters; it++) {
     if (!ps.ctl) ps.epoch = ++epoch;
     ps.iter = it;
@@ -1861,10 +1877,10 @@ void Refiner::enqueue_claims(PassD ps, int gc) {
     tend(K_CLAIM, ps.nc);
     tbegin(K_WIN);
     k_win<CLAIM_LANES><<<gc, LIST_BLOCK, 0, stream>>>(m, ps, c_act);
-    if (it + 1 < iters) k_block<CLAIM_LANES><<<gc, LIST_BLOCK, 0, stream>>>(m, ps, c_act);
     tend(K_WIN, ps.nc);
     HC(hipGetLastError());
   }
+  if (!ps.ctl) ++epoch;  // the winners' stamp epoch
 }
 
 // One conflict pass over ids[0..nc): candidates below nseg split
@@ -1878,7 +1894,7 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
   ps.nc = nc;
   ps.nseg = nseg;
   int iters = std::max(1, opt.claim_iters);
-  if (epoch + iters + 1 >= (1ull << 11)) {  // epoch wrap: clear claims
+  if (epoch + iters + 2 >= (1ull << 11)) {  // epoch wrap: clear claims
     HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
     epoch = 0;
   }
@@ -1967,6 +1983,10 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
   st.pred_exact += h_cnt[CNT_EXACT];
   st.pred_dd += h_cnt[CNT_DD];
   st.big_cavities += h_cnt[CNT_BIG];
+  st.tri_examined_seg += h_cnt[CNT_EXAM_SEG];
+  st.tri_examined_bad += h_cnt[CNT_EXAM_BAD];
+  st.ws_ints_seg += h_cnt[CNT_WS_SEG];
+  st.ws_ints_bad += h_cnt[CNT_WS_BAD];
   st.seg_splits += nws;
   st.inserted += nwin - nws;
   *progress = nwin + (long long)h_cnt[CNT_MARKS] + (long long)h_cnt[CNT_GIVEUPS];
@@ -2387,7 +2407,7 @@ void Refiner::loop_graph() {
   const long long max_launches = 4LL * opt.max_rounds + 100000;
   for (long long launch = 0;; launch++) {
     if (launch > max_launches) throw std::runtime_error("graph loop made no progress");
-    if (h.i[C_EPOCH] + passes * iters + 1 >= (1LL << 11)) {  // epoch wrap
+    if (h.i[C_EPOCH] + passes * (iters + 1) + 1 >= (1LL << 11)) {  // epoch wrap
       HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
       h.i[C_EPOCH] = 0;
       HC(hipMemcpyAsync(d_ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, stream));
@@ -2439,6 +2459,10 @@ void Refiner::loop_graph() {
   st.pred_exact = (long long)h.acc[A_EXACT];
   st.pred_dd = (long long)h.acc[A_DD];
   st.big_cavities = (long long)h.acc[A_BIG];
+  st.tri_examined_seg = (long long)h.acc[A_EXAM_SEG];
+  st.tri_examined_bad = (long long)h.acc[A_EXAM_BAD];
+  st.ws_ints_seg = (long long)h.acc[A_WS_SEG];
+  st.ws_ints_bad = (long long)h.acc[A_WS_BAD];
   st.seg_splits = (long long)h.acc[A_SEGSPLITS];
   st.inserted = (long long)h.acc[A_INSERTED];
   st.cand_evals = (long long)h.acc[A_CANDEVALS];

@@ -172,6 +172,11 @@ typedef struct gcdt_stats {
   long long cand_evals;  /* bad-pass candidates evaluated */
   long long big_cavities; /* cavity evaluations that left the LDS fast path */
   long long pred_dd;     /* signs decided by the double-double stage */
+  long long tri_examined_seg, tri_examined_bad; /* triangle records the cavity
+                            kernels examined (walk steps, seeds, conflict
+                            tests), segment / bad-triangle candidates */
+  long long ws_ints_seg, ws_ints_bad; /* ints written to candidate workspaces
+                            (cavity + ring lists, ordered star boundary) */
 } gcdt_stats;
 
 typedef struct gcdt_mesh gcdt_mesh;       /* host-side result */
