@@ -12,9 +12,11 @@ output mesh in HBM.  Two forms are timed in the same run:
   plain   FP64-filtered exact predicates, one claim iteration, host-driven
           passes (the reference's structure)
   chosen  FP32 -> FP64 -> double-double -> exact predicate cascade, claim
-          iterations, learned claim priority (when the trained model is
-          present), and the round driver (host- or graph-driven) and launch
-          shapes chosen by the autotuner on this input (untimed)
+          iterations with shared rings (winners' rings may overlap, their
+          cavities may not meet another winner's region), learned claim
+          priority (when the trained model is present), and the round driver
+          (host- or graph-driven) and launch shapes chosen by the autotuner
+          on this input (untimed)
 value = output triangles / second of wall time of the chosen form.
 
 Prints ONE JSON line.  Also reports:
@@ -86,10 +88,14 @@ def plain_options():
 
 
 def chosen_options(args):
-    # the autotuner may switch to the graph-driven loop (graph=1)
-    o = dict(pred_mode=1, claim_iters=args.claim_iters, priority=1, graph=0)
+    # FP32 filter, claim iterations, shared rings, learned priority inside
+    # 2-half-octave buckets (learned_shift 1 keeps the Steiner count within a
+    # few percent of the plain path); the autotuner may switch to the
+    # graph-driven loop (graph=1)
+    o = dict(pred_mode=1, claim_iters=args.claim_iters, priority=1, graph=0, shared_ring=1)
     if args.learned and os.path.exists(MODEL):
         o["priority"] = 2
+        o["learned_shift"] = args.learned_shift
     return o
 
 
@@ -503,6 +509,7 @@ def main():
     ap.add_argument("--claim-iters", type=int, default=3)
     ap.add_argument("--learned", type=int, default=1)
     ap.add_argument("--learned-theta", type=float, default=28.0)
+    ap.add_argument("--learned-shift", type=int, default=1)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile-step", action="store_true")

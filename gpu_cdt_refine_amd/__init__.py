@@ -51,6 +51,7 @@ class Options(C.Structure):
         ("cav_lds_cap", C.c_int),
         ("learned_shift", C.c_int),
         ("mixed", C.c_int),
+        ("shared_ring", C.c_int),
     ]
 
 

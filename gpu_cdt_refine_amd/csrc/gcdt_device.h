@@ -76,7 +76,8 @@ struct MeshD {
   int2 *SV;
   int *SP, *ST, *senc;
   const int2 *SEG;
-  unsigned long long *claim;
+  unsigned long long *claim;  // per triangle t: [2t] any claim (cavity or
+                              // ring), [2t+1] cavity claim (shared rings)
   int batched;  // 0: one instance, parameters from ParamsD; 1: T[t].V.w
 };
 
@@ -173,6 +174,8 @@ struct PassD {
   unsigned long long epoch0;  // first claim epoch of this pass
   int iter;                   // claim iteration
   int iters;                  // claim iterations of the pass
+  int shared_ring;            // winners' rings may overlap (cavities may not
+                              // meet another winner's cavity or ring)
   float *feat;                // optional: FEAT features per candidate
   int pass_index;             // bad-pass counter (feature)
   // graph-driven loop: counts, epochs and the candidate list come from ctl

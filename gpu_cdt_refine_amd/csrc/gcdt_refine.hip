@@ -184,7 +184,8 @@ __global__ void k_init_tris(const int *tris, const int *nbrs, const int *tinst,
   r.pad[0] = r.pad[1] = make_int4(0, 0, 0, 0);
   T[t] = r;
   tflag[t] = TF_ALIVE | TF_CHECK;
-  claim[t] = 0ull;
+  claim[2 * t] = 0ull;
+  claim[2 * t + 1] = 0ull;
 }
 
 __device__ __forceinline__ unsigned long long edge_key(int a, int b) {
@@ -660,10 +661,20 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
           }
         }
         if (!IS_SEG && s >= 0) {
-          int2 sv = m.SV[s];
-          if (encroaches(m, lp, sv.x, sv.y, cx, cy)) sc[SC_ENC] = 1;
+          // The subsegment is this edge.  Ruppert's test (dot < 0) is exactly
+          // symmetric in the endpoints, so the record's coordinates serve;
+          // Chew's lens product is not, so it takes the subsegment's order.
+          double2 A, B;
+          if (lp.ruppert) {
+            A = rt.X[(j + 1) % 3];
+            B = rt.X[(j + 2) % 3];
+          } else {
+            const int2 sv = m.SV[s];
+            A = m.P[sv.x];
+            B = m.P[sv.y];
+          }
+          if (encroaches_xy(lp, A, B, cx, cy)) sc[SC_ENC] = 1;
           if (ps.feat) {  // distance of the Steiner point to this subsegment
-            double2 A = m.P[sv.x], B = m.P[sv.y];
             float ex = (float)(B.x - A.x), ey = (float)(B.y - A.y);
             float px = (float)(cx - A.x), py = (float)(cy - A.y);
             float len2 = ex * ex + ey * ey;
@@ -685,10 +696,17 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
         // its cavity boundary is not inserted; every such subsegment is marked.
         for (int it = gl; it < 3 * n; it += LG) {
           int t = L[it / 3], j = it % 3;
-          int s = comp(m.T[t].S, j);
+          const TriRec &rt = m.T[t];
+          int s = comp(rt.S, j);
           if (s < 0) continue;
-          int2 sv = m.SV[s];
-          if (encroaches(m, lp, sv.x, sv.y, cx, cy) && mark_enc_local(m, s)) cc.c[CC_MARKS]++;
+          bool enc;
+          if (lp.ruppert) {
+            enc = encroaches_xy(lp, rt.X[(j + 1) % 3], rt.X[(j + 2) % 3], cx, cy);
+          } else {
+            const int2 sv = m.SV[s];
+            enc = encroaches(m, lp, sv.x, sv.y, cx, cy);
+          }
+          if (enc && mark_enc_local(m, s)) cc.c[CC_MARKS]++;
         }
         status = ST_REJ_ENC;
       }
@@ -872,17 +890,25 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_claim(MeshD m, PassD ps, int *ca
       }
     } else {
       if (!cact[i]) continue;
-      const int k = ps.cn[i] + ps.cr[i];
+      const int n = ps.cn[i], k = n + ps.cr[i];
       bool hit = false;
-      for (int q = gl; q < k; q += G) hit |= (m.claim[L[q]] >> 53) == sticky;
+      for (int q = gl; q < k; q += G) {
+        // shared rings: a ring entry is blocked only by a winner's cavity
+        const int w = (ps.shared_ring && q >= n) ? 1 : 0;
+        hit |= (m.claim[2 * (size_t)L[q] + w] >> 53) == sticky;
+      }
       if (group_bits<G>(__ballot(hit)) != 0ull) {
         if (gl == 0) cact[i] = 0;
         continue;
       }
     }
     const unsigned long long key = claim_key(ep, ps.cprio[i], i);
-    const int k = ps.cn[i] + ps.cr[i];
-    for (int q = gl; q < k; q += G) atomicMax(&m.claim[L[q]], key);
+    const int n = ps.cn[i], k = n + ps.cr[i];
+    for (int q = gl; q < k; q += G) {
+      unsigned long long *c = &m.claim[2 * (size_t)L[q]];
+      atomicMax(c, key);
+      if (ps.shared_ring && q < n) atomicMax(c + 1, key);
+    }
   }
   if (ps.iter == 0) {
     const unsigned long long a = wave_sum(ws_seg), b = wave_sum(ws_bad);
@@ -906,14 +932,24 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_win(MeshD m, PassD ps, const int
     if (!cact[i]) continue;
     const unsigned long long key = claim_key(ep, ps.cprio[i], i);
     const int *L = ps.ws + (size_t)i * ps.R;
-    const int k = ps.cn[i] + ps.cr[i];
+    const int n = ps.cn[i], k = n + ps.cr[i];
     bool lose = false;
-    for (int q = gl; q < k; q += G) lose |= m.claim[L[q]] != key;
+    for (int q = gl; q < k; q += G) {
+      const unsigned long long *c = &m.claim[2 * (size_t)L[q]];
+      // a cavity triangle must hold the key; a shared ring triangle must hold
+      // no better candidate's cavity
+      if (ps.shared_ring && q >= n) lose |= c[1] > key;
+      else lose |= c[0] != key;
+    }
     if (group_bits<G>(__ballot(lose)) != 0ull) continue;
     if (gl == 0) ps.cwin[i] = 1;
     if (ps.iter + 1 < ps.iters) {
       const unsigned long long stamp = claim_key(sticky, ps.cprio[i], i);
-      for (int q = gl; q < k; q += G) m.claim[L[q]] = stamp;
+      for (int q = gl; q < k; q += G) {
+        unsigned long long *c = &m.claim[2 * (size_t)L[q]];
+        c[0] = stamp;
+        if (ps.shared_ring && q < n) c[1] = stamp;
+      }
     }
   }
 }
@@ -1005,8 +1041,15 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_apply(MeshD m, ParamsD pr, PassD
       if (s >= 0) {
         m.ST[s] = (st << 2) | 0;
         if (is_seg && m.senc[s] == 0) {
-          const int2 sv = m.SV[s];
-          if (encroaches(m, lp, sv.x, sv.y, c.x, c.y)) m.senc[s] = 1;
+          // s is the boundary edge (u, v); see the cavity kernel on endpoint order
+          bool enc;
+          if (lp.ruppert) {
+            enc = encroaches_xy(lp, Pu, Pv, c.x, c.y);
+          } else {
+            const int2 sv = m.SV[s];
+            enc = encroaches(m, lp, sv.x, sv.y, c.x, c.y);
+          }
+          if (enc) m.senc[s] = 1;
         }
       }
       if (is_seg) {
@@ -1675,9 +1718,9 @@ void Refiner::ensure_capacity(long long needP, long long needT, long long needS)
     dgrow(&m.T, (size_t)nt, (size_t)c, stream);
     dgrow(&m.tflag, (size_t)nt, (size_t)c, stream);
     unsigned long long *nc;
-    dalloc(&nc, (size_t)c);
-    HC(hipMemsetAsync(nc, 0, sizeof(unsigned long long) * c, stream));
-    if (nt) HC(hipMemcpyAsync(nc, m.claim, sizeof(unsigned long long) * nt, hipMemcpyDeviceToDevice, stream));
+    dalloc(&nc, 2 * (size_t)c);
+    HC(hipMemsetAsync(nc, 0, sizeof(unsigned long long) * 2 * c, stream));
+    if (nt) HC(hipMemcpyAsync(nc, m.claim, sizeof(unsigned long long) * 2 * nt, hipMemcpyDeviceToDevice, stream));
     HC(hipStreamSynchronize(stream));
     dfree(m.claim);
     m.claim = nc;
@@ -1860,6 +1903,7 @@ PassD Refiner::pass_base(int Kc) const {
   ps.ids_sel = nullptr;
   ps.slow = c_slow;
   ps.iters = std::max(1, opt.claim_iters);
+  ps.shared_ring = opt.shared_ring;
   return ps;
 }
 
@@ -1895,7 +1939,7 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
   ps.nseg = nseg;
   int iters = std::max(1, opt.claim_iters);
   if (epoch + iters + 2 >= (1ull << 11)) {  // epoch wrap: clear claims
-    HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
+    HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * 2 * capT, stream));
     epoch = 0;
   }
   ps.epoch0 = epoch + 1;
@@ -2010,7 +2054,7 @@ void Refiner::run() {
     capP = wantP; capT = wantT; capS = wantS;
     dalloc(&m.P, capP); dalloc(&m.pflag, capP); dalloc(&m.ppar, capP);
     dalloc(&m.T, capT);
-    dalloc(&m.tflag, capT); dalloc(&m.claim, capT);
+    dalloc(&m.tflag, capT); dalloc(&m.claim, 2 * (size_t)capT);
     dalloc(&m.SV, capS); dalloc(&m.SP, capS); dalloc(&m.ST, capS); dalloc(&m.senc, capS);
   }
   if (ns0 > seg_in_cap || !seg_in) {
@@ -2057,7 +2101,7 @@ void Refiner::run() {
   pass_index = 0;
   log_X.clear();
   log_t.clear();
-  HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
+  HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * 2 * capT, stream));
   if (!c_ws || ws_ints < opt.cand_budget) {
     dfree(c_ws);
     dalloc(&c_ws, (size_t)opt.cand_budget);
@@ -2408,7 +2452,7 @@ void Refiner::loop_graph() {
   for (long long launch = 0;; launch++) {
     if (launch > max_launches) throw std::runtime_error("graph loop made no progress");
     if (h.i[C_EPOCH] + passes * (iters + 1) + 1 >= (1LL << 11)) {  // epoch wrap
-      HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * capT, stream));
+      HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * 2 * capT, stream));
       h.i[C_EPOCH] = 0;
       HC(hipMemcpyAsync(d_ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, stream));
     }

@@ -158,6 +158,12 @@ typedef struct gcdt_options {
                             for their regions, subsegments first in candidate
                             order (they win every tie).  Host-driven loop only
                             (graph = 0).  Default 0 */
+  int shared_ring;       /* 0: the winners of a pass own disjoint cavity +
+                            ring regions (the reference's rule); 1: a winner's
+                            cavity may not meet another winner's cavity or
+                            ring, but rings may overlap -- the condition under
+                            which simultaneous star insertions equal a
+                            sequential order.  Default 0 */
 } gcdt_options;
 
 typedef struct gcdt_stats {

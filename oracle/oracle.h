@@ -59,6 +59,9 @@ typedef struct {
                          which the encroached subsegments (first in candidate
                          order, so they win every tie) and the selected bad
                          triangles compete together */
+  int shared_ring;    /* 0: winners' cavity + ring regions are disjoint;
+                         1: only a cavity may not meet another winner's cavity
+                         or ring (rings may overlap) */
 } orc_params;
 
 typedef struct {

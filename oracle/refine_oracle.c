@@ -611,7 +611,7 @@ static int cands_per_pass(long long budget, int K) {
  * winners (*nwin_seg of them segment splits); sets *overflowed if any cavity
  * exceeded K. */
 static int run_pass(Mesh *m, const int *ids, int nc, int nseg, int K,
-                    int Kmax, int priority, int claim_iters, orc_stats *st,
+                    int Kmax, int priority, int claim_iters, int shared_ring, orc_stats *st,
                     int *overflowed, long long *progress, int *nwin_seg) {
   long long marks = 0, gave = st->giveups;
   Cand *cd = calloc((size_t)nc, sizeof(Cand));
@@ -705,21 +705,29 @@ static int run_pass(Mesh *m, const int *ids, int nc, int nseg, int K,
   /* claims: the smallest (priority, candidate index) wins each triangle.
    * Repeated claim_iters times: candidates that neither won nor touch a
    * winner's region compete again among themselves. */
+  /* claim[t]: smallest key of a candidate holding t in its cavity or ring;
+   * cavc[t] (shared rings): smallest key holding t in its cavity.
+   * owner / cowner: t belongs to a winner's region / a winner's cavity. */
   long long *claim = malloc(sizeof(long long) * (size_t)m->nt);
+  long long *cavc = shared_ring ? malloc(sizeof(long long) * (size_t)m->nt) : NULL;
   int *owner = malloc(sizeof(int) * (size_t)m->nt);
-  for (int t = 0; t < m->nt; t++) owner[t] = -1;
+  int *cowner = malloc(sizeof(int) * (size_t)m->nt);
+  for (int t = 0; t < m->nt; t++) owner[t] = cowner[t] = -1;
   int nwin = 0;
   int *win = calloc((size_t)nc, sizeof(int));
   int *active = malloc(sizeof(int) * (size_t)(nc > 0 ? nc : 1));
   for (int i = 0; i < nc; i++) active[i] = cd[i].status == ST_VALID;
   for (int it = 0; it < claim_iters; it++) {
     for (int t = 0; t < m->nt; t++) claim[t] = 0x7fffffffffffffffLL;
+    if (cavc)
+      for (int t = 0; t < m->nt; t++) cavc[t] = 0x7fffffffffffffffLL;
     for (int i = 0; i < nc; i++) {
       if (!active[i]) continue;
       long long key = ((long long)cd[i].prio << 32) | i;
       for (int q = 0; q < cd[i].n + cd[i].r; q++) {
         int t = cd[i].L[q];
         if (claim[t] > key) claim[t] = key;
+        if (cavc && q < cd[i].n && cavc[t] > key) cavc[t] = key;
       }
     }
     int newwin = 0;
@@ -727,18 +735,27 @@ static int run_pass(Mesh *m, const int *ids, int nc, int nseg, int K,
       if (!active[i]) continue;
       long long key = ((long long)cd[i].prio << 32) | i;
       int ok = 1;
-      for (int q = 0; q < cd[i].n + cd[i].r && ok; q++)
-        if (claim[cd[i].L[q]] != key) ok = 0;
+      for (int q = 0; q < cd[i].n + cd[i].r && ok; q++) {
+        int t = cd[i].L[q];
+        if (cavc && q >= cd[i].n) ok = cavc[t] >= key;  /* ring: no better cavity */
+        else ok = claim[t] == key;
+      }
       if (ok) { win[i] = 1; newwin++; }
     }
     for (int i = 0; i < nc; i++)
       if (win[i] && active[i])
-        for (int q = 0; q < cd[i].n + cd[i].r; q++) owner[cd[i].L[q]] = i;
+        for (int q = 0; q < cd[i].n + cd[i].r; q++) {
+          owner[cd[i].L[q]] = i;
+          if (q < cd[i].n) cowner[cd[i].L[q]] = i;
+        }
     for (int i = 0; i < nc; i++) {
       if (!active[i]) continue;
       if (win[i]) { active[i] = 0; continue; }
-      for (int q = 0; q < cd[i].n + cd[i].r; q++)
-        if (owner[cd[i].L[q]] >= 0) { active[i] = 0; break; }
+      for (int q = 0; q < cd[i].n + cd[i].r; q++) {
+        int t = cd[i].L[q];
+        int hit = (cavc && q >= cd[i].n) ? cowner[t] >= 0 : owner[t] >= 0;
+        if (hit) { active[i] = 0; break; }
+      }
     }
     nwin += newwin;
     if (newwin == 0) break;
@@ -746,6 +763,8 @@ static int run_pass(Mesh *m, const int *ids, int nc, int nseg, int K,
   for (int i = 0; i < nc; i++)
     if (cd[i].status == ST_VALID && !win[i]) st->lost++;
   free(owner);
+  free(cowner);
+  free(cavc);
   free(active);
   /* slot bases (winners take points/triangles/subsegments in candidate order) */
   int pbase = m->np, tbase = m->nt, sbase = m->ns;
@@ -960,8 +979,8 @@ static int mixed_round(Mesh *m, const orc_params *prm, orc_stats *st, int *K, in
   nb = select_bad(m, B, nb, capb, prm->priority);
   int of = 0, nws = 0;
   long long prog = 0;
-  int nw = run_pass(m, L, ne + nb, ne, *K, prm->cav_cap_max, prm->priority, prm->claim_iters, st,
-                    &of, &prog, &nws);
+  int nw = run_pass(m, L, ne + nb, ne, *K, prm->cav_cap_max, prm->priority, prm->claim_iters,
+                    prm->shared_ring, st, &of, &prog, &nws);
   if (ne > 0) st->seg_passes++;
   if (nb > 0) {
     st->bad_passes++;
@@ -1008,8 +1027,8 @@ static orc_result *refine_impl(int np, const double *points, int nt,
       if (ne > cap) ne = cap;
       int of = 0, nws = 0;
       long long prog = 0;
-      run_pass(m, ids, ne, ne, K, prm->cav_cap_max, prm->priority, prm->claim_iters, st, &of,
-               &prog, &nws);
+      run_pass(m, ids, ne, ne, K, prm->cav_cap_max, prm->priority, prm->claim_iters, prm->shared_ring,
+               st, &of, &prog, &nws);
       st->seg_passes++;
       if (of && K < prm->cav_cap_max) K = K * 2 > prm->cav_cap_max ? prm->cav_cap_max : K * 2;
     }
@@ -1032,8 +1051,8 @@ static orc_result *refine_impl(int np, const double *points, int nt,
     nb = select_bad(m, ids, nb, cap, prm->priority);
     int of = 0, nws = 0;
     long long prog = 0;
-    int nw = run_pass(m, ids, nb, 0, K, prm->cav_cap_max, prm->priority, prm->claim_iters, st, &of,
-                      &prog, &nws);
+    int nw = run_pass(m, ids, nb, 0, K, prm->cav_cap_max, prm->priority, prm->claim_iters, prm->shared_ring,
+                      st, &of, &prog, &nws);
     st->bad_passes++;
     st->cand_evals += nb;
     prev_win = nw;

@@ -197,3 +197,20 @@ def test_mixed_rounds_invariants(oracle, theta, off, mode, ci):
     assert abs((len(Po) - len(Pc)) / (len(Pc) - len(P))) < 0.05
     assert st["seg_splits"] + len(S) == len(So)
     assert st["inserted"] + st["seg_splits"] + len(P) == len(Po)
+
+
+@pytest.mark.parametrize("theta,off,mode,ci", [(25, 1, 1, 1), (25, 1, 1, 3), (30, 0, 0, 2)])
+def test_shared_ring_invariants(oracle, theta, off, mode, ci):
+    """Winners whose rings overlap (cavities disjoint from every other
+    winner's cavity and ring): the mesh meets every invariant, needs no more
+    passes than disjoint regions, and its Steiner count stays within 5%."""
+    P, T, N, S = _cdt(oracle, 3000, 150, 9)
+    kw = dict(theta=theta, offcenter=off, mode=mode, claim_iters=ci)
+    Pc, Tc, Nc, Sc, pfc, stc = oracle.oracle_refine(P, T, N, S, **kw)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, shared_ring=1, **kw)
+    assert st["status"] == 0 and st["giveups"] == 0
+    tot, viol, mina, ex = oracle.check_mesh(Po, To, No, So, P, S, theta)
+    assert tot == 0, viol
+    assert mina >= theta - 1e-7
+    assert st["seg_passes"] + st["bad_passes"] <= stc["seg_passes"] + stc["bad_passes"]
+    assert abs((len(Po) - len(Pc)) / (len(Pc) - len(P))) < 0.05
