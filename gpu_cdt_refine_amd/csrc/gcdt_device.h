@@ -76,8 +76,8 @@ struct MeshD {
   int2 *SV;
   int *SP, *ST, *senc;
   const int2 *SEG;
-  unsigned long long *claim;  // per triangle t: [2t] any claim (cavity or
-                              // ring), [2t+1] cavity claim (shared rings)
+  unsigned long long *claim;  // per triangle t: [2t] claims (shared rings:
+                              // cavity claims), [2t+1] ring claims (shared rings)
   int batched;  // 0: one instance, parameters from ParamsD; 1: T[t].V.w
 };
 

@@ -893,9 +893,12 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_claim(MeshD m, PassD ps, int *ca
       const int n = ps.cn[i], k = n + ps.cr[i];
       bool hit = false;
       for (int q = gl; q < k; q += G) {
+        const unsigned long long *c = &m.claim[2 * (size_t)L[q]];
         // shared rings: a ring entry is blocked only by a winner's cavity
-        const int w = (ps.shared_ring && q >= n) ? 1 : 0;
-        hit |= (m.claim[2 * (size_t)L[q] + w] >> 53) == sticky;
+        // (stamped in word 0), a cavity entry also by a winner's ring (word 1)
+        if (!ps.shared_ring) hit |= (c[0] >> 53) == sticky;
+        else if (q >= n) hit |= (c[0] >> 53) == sticky;
+        else hit |= ((c[0] >> 53) == sticky) | ((c[1] >> 53) == sticky);
       }
       if (group_bits<G>(__ballot(hit)) != 0ull) {
         if (gl == 0) cact[i] = 0;
@@ -904,11 +907,10 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_claim(MeshD m, PassD ps, int *ca
     }
     const unsigned long long key = claim_key(ep, ps.cprio[i], i);
     const int n = ps.cn[i], k = n + ps.cr[i];
-    for (int q = gl; q < k; q += G) {
-      unsigned long long *c = &m.claim[2 * (size_t)L[q]];
-      atomicMax(c, key);
-      if (ps.shared_ring && q < n) atomicMax(c + 1, key);
-    }
+    // shared rings: cavity entries claim word 0, ring entries word 1;
+    // otherwise every entry claims word 0
+    for (int q = gl; q < k; q += G)
+      atomicMax(&m.claim[2 * (size_t)L[q] + ((ps.shared_ring && q >= n) ? 1 : 0)], key);
   }
   if (ps.iter == 0) {
     const unsigned long long a = wave_sum(ws_seg), b = wave_sum(ws_bad);
@@ -936,20 +938,21 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_win(MeshD m, PassD ps, const int
     bool lose = false;
     for (int q = gl; q < k; q += G) {
       const unsigned long long *c = &m.claim[2 * (size_t)L[q]];
-      // a cavity triangle must hold the key; a shared ring triangle must hold
-      // no better candidate's cavity
-      if (ps.shared_ring && q >= n) lose |= c[1] > key;
-      else lose |= c[0] != key;
+      if (!ps.shared_ring) {
+        lose |= c[0] != key;  // every entry must hold the key
+      } else if (q < n) {
+        // cavity: the best cavity claim, and no better ring claim
+        lose |= (c[0] != key) | (c[1] > key);
+      } else {
+        lose |= c[0] > key;  // ring: no better candidate's cavity
+      }
     }
     if (group_bits<G>(__ballot(lose)) != 0ull) continue;
     if (gl == 0) ps.cwin[i] = 1;
     if (ps.iter + 1 < ps.iters) {
       const unsigned long long stamp = claim_key(sticky, ps.cprio[i], i);
-      for (int q = gl; q < k; q += G) {
-        unsigned long long *c = &m.claim[2 * (size_t)L[q]];
-        c[0] = stamp;
-        if (ps.shared_ring && q < n) c[1] = stamp;
-      }
+      for (int q = gl; q < k; q += G)
+        m.claim[2 * (size_t)L[q] + ((ps.shared_ring && q >= n) ? 1 : 0)] = stamp;
     }
   }
 }
