@@ -52,6 +52,7 @@ class Options(C.Structure):
         ("learned_shift", C.c_int),
         ("mixed", C.c_int),
         ("shared_ring", C.c_int),
+        ("mix_below", C.c_int),
     ]
 
 

@@ -60,6 +60,7 @@ void gcdt_default_options(gcdt_options *o) {
   o->learned_shift = 2;
   o->mixed = 0;
   o->shared_ring = 0;
+  o->mix_below = 0;
 }
 
 }  // extern "C"
@@ -99,6 +100,8 @@ bool options_ok(const gcdt_options *o, std::string &g_err) {
   if (o->graph && o->log_training) { g_err = "log_training needs the host-driven loop (graph = 0)"; return false; }
   if (o->mixed != 0 && o->mixed != 1) { g_err = "mixed must be 0 or 1"; return false; }
   if (o->shared_ring != 0 && o->shared_ring != 1) { g_err = "shared_ring must be 0 or 1"; return false; }
+  if (o->mix_below < 0) { g_err = "mix_below must be >= 0"; return false; }
+  if (o->mix_below > 0 && o->graph) { g_err = "mix_below needs the host-driven loop (graph = 0)"; return false; }
   if (o->mixed && o->graph) { g_err = "mixed rounds run in the host-driven loop (graph = 0)"; return false; }
   if (o->mixed && o->log_training) { g_err = "log_training needs separate segment and bad passes (mixed = 0)"; return false; }
   return true;

@@ -90,6 +90,7 @@ class Refiner {
   void loop_graph();
   void loop_host();
   void loop_mixed();
+  bool bad_pass(int ne, int *prev_win);
   void ensure_mix(long long n);
   void size_pass_buffers();
   unsigned long long *ctl_cnt_ptr() const;

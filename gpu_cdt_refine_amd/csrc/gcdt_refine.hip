@@ -2158,11 +2158,12 @@ void Refiner::loop_host() {
   ensure_cand(std::min(cands_per_pass(K), std::max(4 * nt0, 1024)));
   for (int round = 0; round < opt.max_rounds; round++) {
     st.rounds = round + 1;
-    // segment phase
+    // segment phase: until at most mix_below subsegments are encroached
+    int ne;
     for (;;) {
       ensure_tmp(ns);
-      int ne = compact(PredSenc{m.senc}, ns, tmp_list, nullptr, nullptr);
-      if (ne == 0) break;
+      ne = compact(PredSenc{m.senc}, ns, tmp_list, nullptr, nullptr);
+      if (ne <= opt.mix_below) break;
       int nc = std::min(ne, cands_per_pass(K));
       ensure_cand(nc);
       long long prog = 0;
@@ -2172,31 +2173,7 @@ void Refiner::loop_host() {
       st.seg_passes++;
       if (of && K < opt.cav_cap_max) K = std::min(2 * K, opt.cav_cap_max);
     }
-    // bad phase
-    tbegin(K_MARKBAD);
-    k_mark_bad<<<std::min(nblk(nt), 65536), 256, 0, stream>>>(m, pr, nt, nullptr);
-    tend(K_MARKBAD, nt);
-    ensure_tmp(nt);
-    int nb = compact(PredBad{m.tflag}, nt, tmp_list, nullptr, nullptr);
-    if (nb == 0) { st.status = 0; break; }
-    int cap = cands_per_pass(K);
-    if (opt.cand_factor > 0 && prev_win >= 0)
-      cap = (int)std::min<long long>(cap, std::max<long long>(opt.cand_min, (long long)opt.cand_factor * prev_win));
-    int *ids = tmp_list;
-    int nc = select_by_priority(tmp_list, nb, cap, &ids);
-    ensure_cand(nc);
-    long long prog = 0;
-    bool of = false;
-    int nws = 0;
-    prev_win = run_pass(ids, nc, 0, &prog, &of, &nws);
-    pass_index++;
-    st.bad_passes++;
-    st.cand_evals += nc;
-    if (of && K < opt.cav_cap_max) {
-      K = std::min(2 * K, opt.cav_cap_max);
-      prog++;
-    }
-    if (prog == 0) { st.status = 2; break; }
+    if (bad_pass(ne, &prev_win)) break;
   }
 }
 
@@ -2207,44 +2184,63 @@ void Refiner::loop_mixed() {
   ensure_cand(std::min(cands_per_pass(K), std::max(4 * nt0, 1024)));
   for (int round = 0; round < opt.max_rounds; round++) {
     st.rounds = round + 1;
-    tbegin(K_MARKBAD);
-    k_mark_bad<<<std::min(nblk(nt), 65536), 256, 0, stream>>>(m, pr, nt, nullptr);
-    tend(K_MARKBAD, nt);
-    const int cap = cands_per_pass(K);
-    ensure_mix(std::min<long long>((long long)ns + nt, (long long)cap + 1));
-    ensure_tmp((int)std::max(ns, nt));
+    ensure_tmp(ns);
     int ne = compact(PredSenc{m.senc}, ns, tmp_list, nullptr, nullptr);
-    ne = std::min(ne, cap);
-    if (ne > 0)
-      HC(hipMemcpyAsync(mix_list, tmp_list, sizeof(int) * (size_t)ne, hipMemcpyDeviceToDevice, stream));
-    int nb = compact(PredBad{m.tflag}, nt, tmp_list, nullptr, nullptr);
-    if (ne == 0 && nb == 0) { st.status = 0; break; }
-    int capb = cap - ne;
-    if (opt.cand_factor > 0 && prev_win >= 0)
-      capb = (int)std::min<long long>(capb, std::max<long long>(opt.cand_min, (long long)opt.cand_factor * prev_win));
-    int *bids = tmp_list;
-    int nbc = nb > 0 && capb > 0 ? select_by_priority(tmp_list, nb, capb, &bids) : 0;
+    if (bad_pass(ne, &prev_win)) break;
+  }
+}
+
+// A round's bad pass: the ne encroached subsegments listed in tmp_list (none
+// in the reference's round) first, then the selected bad triangles.
+// Returns true when the refinement has ended (st.status set).
+bool Refiner::bad_pass(int ne, int *prev_win) {
+  const int cap = cands_per_pass(K);
+  ne = std::min(ne, cap);
+  ensure_mix(std::min<long long>((long long)ns + nt, (long long)cap + 1));
+  if (ne > 0)
+    HC(hipMemcpyAsync(mix_list, tmp_list, sizeof(int) * (size_t)ne, hipMemcpyDeviceToDevice, stream));
+  tbegin(K_MARKBAD);
+  k_mark_bad<<<std::min(nblk(nt), 65536), 256, 0, stream>>>(m, pr, nt, nullptr);
+  tend(K_MARKBAD, nt);
+  ensure_tmp(nt);
+  int nb = compact(PredBad{m.tflag}, nt, tmp_list, nullptr, nullptr);
+  if (ne == 0 && nb == 0) {
+    st.status = 0;
+    return true;
+  }
+  int capb = cap - ne;
+  if (opt.cand_factor > 0 && *prev_win >= 0)
+    capb = (int)std::min<long long>(capb, std::max<long long>(opt.cand_min, (long long)opt.cand_factor * *prev_win));
+  int *bids = tmp_list;
+  int nbc = nb > 0 && capb > 0 ? select_by_priority(tmp_list, nb, capb, &bids) : 0;
+  const int *ids = bids;
+  if (ne > 0) {
     if (nbc > 0)
       HC(hipMemcpyAsync(mix_list + ne, bids, sizeof(int) * (size_t)nbc, hipMemcpyDeviceToDevice, stream));
-    const int nc = ne + nbc;
-    ensure_cand(nc);
-    long long prog = 0;
-    bool of = false;
-    int nws = 0;
-    int nwin = run_pass(mix_list, nc, ne, &prog, &of, &nws);
-    if (ne > 0) st.seg_passes++;
-    if (nbc > 0) {
-      pass_index++;
-      st.bad_passes++;
-      st.cand_evals += nbc;
-      prev_win = nwin - nws;
-    }
-    if (of && K < opt.cav_cap_max) {
-      K = std::min(2 * K, opt.cav_cap_max);
-      prog++;
-    }
-    if (prog == 0) { st.status = 2; break; }
+    ids = mix_list;
   }
+  const int nc = ne + nbc;
+  ensure_cand(nc);
+  long long prog = 0;
+  bool of = false;
+  int nws = 0;
+  int nwin = run_pass(ids, nc, ne, &prog, &of, &nws);
+  if (ne > 0) st.seg_passes++;
+  if (nbc > 0) {
+    pass_index++;
+    st.bad_passes++;
+    st.cand_evals += nbc;
+    *prev_win = nwin - nws;
+  }
+  if (of && K < opt.cav_cap_max) {
+    K = std::min(2 * K, opt.cav_cap_max);
+    prog++;
+  }
+  if (prog == 0) {
+    st.status = 2;
+    return true;
+  }
+  return false;
 }
 
 void Refiner::ensure_mix(long long n) {

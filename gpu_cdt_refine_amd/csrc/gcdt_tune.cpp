@@ -64,7 +64,8 @@ int autotune(gcdt_options &opt, int reps,
   log(best, best_ms);
   for (const Knob &k : knobs) {
     if (k.graph_only && !best.graph) continue;
-    if (k.field == &gcdt_options::graph && (opt.log_training || opt.mixed)) continue;  // host loop only
+    if (k.field == &gcdt_options::graph && (opt.log_training || opt.mixed || opt.mix_below))
+      continue;  // host loop only
     int keep = best.*(k.field);
     for (int v : k.values) {
       if (v == keep) continue;

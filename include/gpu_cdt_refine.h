@@ -164,6 +164,11 @@ typedef struct gcdt_options {
                             ring, but rings may overlap -- the condition under
                             which simultaneous star insertions equal a
                             sequential order.  Default 0 */
+  int mix_below;         /* the segment phase of a round ends once at most
+                            this many subsegments are encroached; those join
+                            the round's bad pass, first in candidate order.
+                            0 is the reference's round.  Host-driven loop
+                            only (graph = 0).  Default 0 */
 } gcdt_options;
 
 typedef struct gcdt_stats {

@@ -62,6 +62,9 @@ typedef struct {
   int shared_ring;    /* 0: winners' cavity + ring regions are disjoint;
                          1: only a cavity may not meet another winner's cavity
                          or ring (rings may overlap) */
+  int mix_below;      /* classic rounds: the segment phase ends once at most
+                         this many subsegments are encroached; they join the
+                         round's bad pass (0: the reference's round) */
 } orc_params;
 
 typedef struct {

@@ -1019,7 +1019,7 @@ static orc_result *refine_impl(int np, const double *points, int nt,
     for (;;) {
       int ne = 0;
       for (int s = 0; s < m->ns; s++) if (m->senc[s] == 1) ne++;
-      if (ne == 0) break;
+      if (ne <= prm->mix_below) break;
       if (ne > idcap) { idcap = ne; ids = xrealloc(ids, sizeof(int) * (size_t)idcap); }
       ne = 0;
       for (int s = 0; s < m->ns; s++) if (m->senc[s] == 1) ids[ne++] = s;
@@ -1032,35 +1032,9 @@ static orc_result *refine_impl(int np, const double *points, int nt,
       st->seg_passes++;
       if (of && K < prm->cav_cap_max) K = K * 2 > prm->cav_cap_max ? prm->cav_cap_max : K * 2;
     }
-    /* bad-triangle phase */
-    mark_bad(m);
-    int nb = 0;
-    for (int t = 0; t < m->nt; t++)
-      if ((m->tflag[t] & (TF_ALIVE | TF_BAD | TF_GIVEUP)) == (TF_ALIVE | TF_BAD)) nb++;
-    if (nb == 0) { st->status = 0; break; }
-    if (nb > idcap) { idcap = nb; ids = xrealloc(ids, sizeof(int) * (size_t)idcap); }
-    nb = 0;
-    for (int t = 0; t < m->nt; t++)
-      if ((m->tflag[t] & (TF_ALIVE | TF_BAD | TF_GIVEUP)) == (TF_ALIVE | TF_BAD)) ids[nb++] = t;
-    int cap = cands_per_pass(prm->cand_budget, K);
-    if (prm->cand_factor > 0 && prev_win >= 0) {
-      long long lim = (long long)prm->cand_factor * prev_win;
-      if (lim < prm->cand_min) lim = prm->cand_min;
-      if (lim < cap) cap = (int)lim;
-    }
-    nb = select_bad(m, ids, nb, cap, prm->priority);
-    int of = 0, nws = 0;
-    long long prog = 0;
-    int nw = run_pass(m, ids, nb, 0, K, prm->cav_cap_max, prm->priority, prm->claim_iters, prm->shared_ring,
-                      st, &of, &prog, &nws);
-    st->bad_passes++;
-    st->cand_evals += nb;
-    prev_win = nw;
-    if (of && K < prm->cav_cap_max) {
-      K = K * 2 > prm->cav_cap_max ? prm->cav_cap_max : K * 2;
-      prog++;
-    }
-    if (prog == 0) { st->status = 2; break; } /* nothing can change any more */
+    /* bad-triangle phase (with the at most mix_below subsegments still
+     * encroached, first in candidate order) */
+    if (mixed_round(m, prm, st, &K, &prev_win, &ids, &idcap)) break;
   }
   st->final_cav_cap = K;
   free(ids);

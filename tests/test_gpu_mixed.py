@@ -108,3 +108,34 @@ def test_mixed_needs_host_loop(gpu, oracle):
     P, T, N, S = _input(oracle, 500, 20, 79)
     with pytest.raises(RuntimeError, match="mixed"):
         gpu.refine(P, T, N, S, mixed=1, graph=1)
+
+
+MIX_BELOW_CASES = [
+    ("mb50", dict(n=4000, ns=300, seed=91), dict(theta=25.0, mix_below=50)),
+    ("mb500-shared-ring", dict(n=4000, ns=300, seed=92),
+     dict(theta=28.0, mix_below=500, shared_ring=1, claim_iters=3, pred_mode=1)),
+    ("mb200-index-priority", dict(n=4000, ns=300, seed=93),
+     dict(theta=25.0, mix_below=200, priority=0, cand_min=64, cand_factor=2)),
+    ("mb1000-large", dict(n=20000, ns=1000, seed=94),
+     dict(theta=25.0, mix_below=1000, shared_ring=1, claim_iters=3)),
+]
+
+
+@pytest.mark.parametrize("name,inp,kw", MIX_BELOW_CASES, ids=[c[0] for c in MIX_BELOW_CASES])
+def test_mix_below_equals_oracle(gpu, oracle, name, inp, kw):
+    """Segment phases that stop at mix_below encroached subsegments, which
+    then join the round's bad pass."""
+    print(f"mix_below case {name}", flush=True)
+    P, T, N, S = _input(oracle, **inp)
+    r = gpu.refine(P, T, N, S, **kw)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, **_oracle_kw(kw))
+    for k in ORACLE_KEYS:
+        assert r.stats[k] == st[k], k
+    assert r.stats["status"] == 0
+    assert np.array_equal(r.points.view(np.uint64), Po.view(np.uint64))
+    assert np.array_equal(r.triangles, To)
+    assert np.array_equal(r.neighbors, No)
+    assert np.array_equal(r.segments, So)
+    tot, viol, mina, ex = oracle.check_mesh(r.points, r.triangles, r.neighbors, r.segments,
+                                            P, S, kw["theta"])
+    assert tot == 0, viol

@@ -214,3 +214,18 @@ def test_shared_ring_invariants(oracle, theta, off, mode, ci):
     assert mina >= theta - 1e-7
     assert st["seg_passes"] + st["bad_passes"] <= stc["seg_passes"] + stc["bad_passes"]
     assert abs((len(Po) - len(Pc)) / (len(Pc) - len(P))) < 0.05
+
+
+@pytest.mark.parametrize("mix_below", [20, 200])
+def test_mix_below_invariants(oracle, mix_below):
+    """Rounds whose segment phase hands its last few encroached subsegments
+    to the bad pass: same invariants, fewer passes, Steiner count within 5%."""
+    P, T, N, S = _cdt(oracle, 3000, 300, 10, maxlen=400.0)
+    kw = dict(theta=25.0, claim_iters=3, shared_ring=1)
+    Pc, Tc, Nc, Sc, pfc, stc = oracle.oracle_refine(P, T, N, S, **kw)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, mix_below=mix_below, **kw)
+    assert st["status"] == 0
+    tot, viol, mina, ex = oracle.check_mesh(Po, To, No, So, P, S, 25.0)
+    assert tot == 0, viol
+    assert mina >= 25.0 - 1e-7
+    assert abs((len(Po) - len(Pc)) / (len(Pc) - len(P))) < 0.05
