@@ -62,6 +62,7 @@ enum : int {
   CNT_EXAM_BAD,  // ... by the bad-triangle cavity kernels
   CNT_WS_SEG,    // ints written to candidate workspaces (segment kernels)
   CNT_WS_BAD,    // ... (bad-triangle kernels)
+  CNT_NWIN,      // winners of the pass
   CNT_N = 24
 };
 

@@ -79,7 +79,8 @@ class Refiner {
 
  private:
   template <class Pred>
-  int compact(Pred pred, int n, int *out_list, int *out_rank, const int *src);
+  int compact(Pred pred, int n, int *out_list, int *out_rank, const int *src,
+              unsigned long long *total64 = nullptr);
   int run_pass(const int *ids, int nc, int nseg, long long *progress, bool *overflowed,
                int *nwin_seg);
   PassD pass_base(int Kc) const;
@@ -143,6 +144,8 @@ class Refiner {
   int *cb_cnt = nullptr, *cb_off = nullptr;
   int cb_cap = 0;
   int *d_total = nullptr;
+  int *h_total = nullptr;       // pinned, host-coherent: compaction totals
+  int *d_host_total = nullptr;  // its device address
   unsigned long long *d_cnt = nullptr;
   int *h_scalar = nullptr;
   unsigned long long *h_cnt = nullptr;

@@ -95,8 +95,12 @@ __global__ __launch_bounds__(CB) void k_count(Pred pred, int n, int *blockcnt) {
 }
 
 // Exclusive scan of block counts (single block); total -> *total.
+// Exclusive scan of the per-block counts.  The total also goes to
+// host_total (pinned, host-coherent memory: the host reads it after a stream
+// synchronisation, no copy) and / or total64 (a pass counter) when given.
 __global__ __launch_bounds__(1024) void k_scan(const int *blockcnt, int nb,
-                                               int *blockoff, int *total) {
+                                               int *blockoff, int *total, int *host_total,
+                                               unsigned long long *total64) {
   __shared__ int sh[1024];
   __shared__ int carry;
   if (threadIdx.x == 0) carry = 0;
@@ -117,7 +121,14 @@ __global__ __launch_bounds__(1024) void k_scan(const int *blockcnt, int nb,
     if (threadIdx.x == 1023) carry += sh[1023];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *total = carry;
+  if (threadIdx.x == 0) {
+    *total = carry;
+    if (total64) *total64 = (unsigned long long)carry;
+    if (host_total) {
+      __hip_atomic_store(host_total, carry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __threadfence_system();
+    }
+  }
 }
 
 template <class Pred>
@@ -1569,6 +1580,9 @@ Refiner::Refiner(const gcdt_options &o) : opt(o) {
   HC(hipSetDevice(opt.device));
   HC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   HC(hipHostMalloc((void **)&h_scalar, 64 * sizeof(int), hipHostMallocDefault));
+  // compaction totals are written here by the scan kernel
+  HC(hipHostMalloc((void **)&h_total, 64 * sizeof(int), hipHostMallocCoherent | hipHostMallocMapped));
+  HC(hipHostGetDevicePointer((void **)&d_host_total, h_total, 0));
   HC(hipHostMalloc((void **)&h_cnt, CNT_N * sizeof(unsigned long long), hipHostMallocDefault));
   HC(hipEventCreate(&ev0));
   HC(hipEventCreate(&ev1));
@@ -1622,6 +1636,7 @@ Refiner::~Refiner() {
   if (graph) (void)hipGraphDestroy(graph);
   if (h_ctl) (void)hipHostFree(h_ctl);
   if (h_scalar) (void)hipHostFree(h_scalar);
+  if (h_total) (void)hipHostFree(h_total);
   if (h_cnt) (void)hipHostFree(h_cnt);
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
@@ -1778,8 +1793,11 @@ const char *Refiner::kname(int id) {
 
 template <class Pred>
 int Refiner::compact(Pred pred, int n, int *out_list, int *out_rank,
-                     const int *src) {
-  if (n <= 0) return 0;
+                     const int *src, unsigned long long *total64) {
+  if (n <= 0) {
+    if (total64) HC(hipMemsetAsync(total64, 0, sizeof(unsigned long long), stream));
+    return 0;
+  }
   int nb = (n + CT - 1) / CT;
   if (nb > cb_cap) {
     dfree(cb_cnt);
@@ -1790,13 +1808,14 @@ int Refiner::compact(Pred pred, int n, int *out_list, int *out_rank,
   }
   tbegin(K_COMPACT);
   k_count<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_cnt);
-  k_scan<<<1, 1024, 0, stream>>>(cb_cnt, nb, cb_off, d_total);
+  k_scan<<<1, 1024, 0, stream>>>(cb_cnt, nb, cb_off, d_total, total64 ? nullptr : d_host_total,
+                                  total64);
   k_scatter<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_off, out_list, out_rank, src);
   tend(K_COMPACT, n);
   HC(hipGetLastError());
-  HC(hipMemcpyAsync(h_scalar, d_total, sizeof(int), hipMemcpyDeviceToHost, stream));
+  if (total64) return -1;  // the count is read later with the pass counters
   HC(hipStreamSynchronize(stream));
-  return h_scalar[0];
+  return __atomic_load_n(h_total, __ATOMIC_ACQUIRE);
 }
 
 void Refiner::read_counters() {
@@ -1936,6 +1955,9 @@ void Refiner::enqueue_claims(PassD ps, int gc) {
 int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, bool *overflowed,
                       int *nwin_seg) {
   const int nbad = nc - nseg;
+  // room for every candidate winning: the winners' count is only read back
+  // with the pass counters, after the apply kernel
+  ensure_capacity(np + (long long)nc, nt + 2LL * nc, ns + (long long)nseg);
   PassD ps = pass_base(K);
   ps.ids = ids;
   ps.nc = nc;
@@ -2006,17 +2028,16 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
     }
   }
   enqueue_claims(ps, gc);
-  int nwin = compact(PredWin{c_win}, nc, nullptr, c_rank, nullptr);
-  ensure_capacity(np + nwin, nt + 2LL * nwin, nseg > 0 ? ns + nwin : ns);
-  if (nwin > 0) {
-    tbegin(K_APPLY);
-    k_apply<APPLY_LANES><<<std::min(nblk((long long)nc * APPLY_LANES, LIST_BLOCK), opt.grid_apply),
-                           LIST_BLOCK, 0, stream>>>(m, pr, ps, np, nt, ns);
-    tend(K_APPLY, nwin);
-    HC(hipGetLastError());
-  }
+  compact(PredWin{c_win}, nc, nullptr, c_rank, nullptr, d_cnt + CNT_NWIN);
+  tbegin(K_APPLY);
+  k_apply<APPLY_LANES><<<std::min(nblk((long long)nc * APPLY_LANES, LIST_BLOCK), opt.grid_apply),
+                         LIST_BLOCK, 0, stream>>>(m, pr, ps, np, nt, ns);
+  tend(K_APPLY, 0);
+  HC(hipGetLastError());
   read_counters();
+  const int nwin = (int)h_cnt[CNT_NWIN];
   const int nws = (int)h_cnt[CNT_SEGWIN];
+  if (opt.profile && !capturing) ktimes[K_APPLY].items += nwin;  // winners applied
   np += nwin;
   nt += 2 * nwin;
   ns += nws;
