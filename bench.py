@@ -88,11 +88,13 @@ def plain_options():
 
 
 def chosen_options(args):
-    # FP32 filter, claim iterations, shared rings, learned priority inside
+    # FP32 filter, claim iterations, shared rings, a bad pass evaluating up to
+    # 3x the previous pass's winners, learned priority inside
     # 2-half-octave buckets (learned_shift 1 keeps the Steiner count within a
     # few percent of the plain path); the autotuner may switch to the
     # graph-driven loop (graph=1)
-    o = dict(pred_mode=1, claim_iters=args.claim_iters, priority=1, graph=0, shared_ring=1)
+    o = dict(pred_mode=1, claim_iters=args.claim_iters, priority=1, graph=0, shared_ring=1,
+             cand_factor=args.cand_factor)
     if args.learned and os.path.exists(MODEL):
         o["priority"] = 2
         o["learned_shift"] = args.learned_shift
@@ -510,6 +512,7 @@ def main():
     ap.add_argument("--learned", type=int, default=1)
     ap.add_argument("--learned-theta", type=float, default=28.0)
     ap.add_argument("--learned-shift", type=int, default=1)
+    ap.add_argument("--cand-factor", type=int, default=3)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile-step", action="store_true")
