@@ -148,6 +148,11 @@ def lib():
         L.gcdt_session_kernel_times.argtypes = [vp, C.POINTER(KernelTime), C.c_int]
         L.gcdt_session_kernel_times.restype = C.c_int
         L.gcdt_session_destroy.argtypes = [vp]
+        L.gcdt_session_check.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.gcdt_session_check.restype = C.c_int
+        L.gcdt_check_mesh.argtypes = [vp, C.c_int, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int,
+                                      C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.gcdt_check_mesh.restype = C.c_int
         L.gcdt_last_error.restype = C.c_char_p
         L.gcdt_device_predicates.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
         L.gcdt_device_predicates.restype = C.c_int
@@ -372,6 +377,36 @@ class Batch:
             pass
 
 
+CHECK_NAMES = ["range", "symmetry", "orientation", "boundary", "delaunay", "subsegment",
+               "coverage", "moved", "quality", "manifold"]
+DEVICE_CHECKS = ["range", "symmetry", "orientation", "boundary", "delaunay", "subsegment",
+                 "moved", "manifold"]
+
+
+def _check_dict(v):
+    return {name: int(v[i]) for i, name in enumerate(CHECK_NAMES) if name in DEVICE_CHECKS}
+
+
+def check_mesh(points, triangles, neighbors, segments, points_in=None):
+    """Invariant checks on the GPU with exact predicates (gcdt_check_mesh):
+    violation counts per category -- index range, neighbour symmetry,
+    orientation, boundary edges that are not subsegments, Delaunay across
+    non-subsegment edges, subsegments that are not edges, input points moved
+    or missing, edges in three or more triangles -- plus "exact_signs", the
+    number of incircle signs left to the exact stage (not a violation)."""
+    P, T, N, S = _as_inputs(points, triangles, neighbors, segments)
+    Pin = np.zeros((0, 2)) if points_in is None else np.ascontiguousarray(points_in, np.float64).reshape(-1, 2)
+    v = (C.c_longlong * 10)()
+    ex = C.c_longlong(0)
+    rc = lib().gcdt_check_mesh(_ptr(P), len(P), _ptr(T), _ptr(N), len(T), _ptr(S), len(S),
+                               _ptr(Pin), len(Pin), v, C.byref(ex))
+    if rc != 0:
+        _err(rc, "gcdt_check_mesh")
+    d = _check_dict(v)
+    d["exact_signs"] = ex.value  # signs the filters left to the exact stage
+    return d
+
+
 class Session:
     """Device-resident CDT: upload once, refine repeatedly."""
 
@@ -425,6 +460,18 @@ class Session:
         if n:
             L.gcdt_session_logs(self.h, _ptr(X), _ptr(t), n)
         return X, t
+
+    def check(self):
+        """Invariant counts of the last refined mesh, checked in HBM
+        (gcdt_session_check); see check_mesh."""
+        v = (C.c_longlong * 10)()
+        ex = C.c_longlong(0)
+        rc = lib().gcdt_session_check(self.h, v, C.byref(ex))
+        if rc != 0:
+            _err(rc, "gcdt_session_check")
+        d = _check_dict(v)
+        d["exact_signs"] = ex.value
+        return d
 
     def kernel_times(self):
         rec = (KernelTime * 32)()

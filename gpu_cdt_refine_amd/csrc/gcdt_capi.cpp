@@ -181,6 +181,55 @@ int gcdt_session_run(gcdt_session *s, gcdt_stats *stats) {
   }
 }
 
+int gcdt_session_check(gcdt_session *s, long long *viol, long long *exact_signs) {
+  g_err.clear();
+  try {
+    if (!s || !viol) throw std::runtime_error("session and viol must be non-NULL");
+    if (s->r->input_count() != 1) throw std::runtime_error("session holds no single refined mesh");
+    gcdt::check_device_mesh(s->r->device_mesh(), s->r->stream, viol, exact_signs);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -2;
+  }
+}
+
+int gcdt_check_mesh(const double *points, int np, const int *tris, const int *nbrs, int nt,
+                    const int *segs, int ns, const double *points_in, int np_in,
+                    long long *viol, long long *exact_signs) {
+  g_err.clear();
+  void *d[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipStream_t stream = nullptr;
+  try {
+    if (!viol || np < 0 || nt < 0 || ns < 0 || np_in < 0 || (np > 0 && !points) ||
+        (nt > 0 && (!tris || !nbrs)) || (ns > 0 && !segs) || (np_in > 0 && !points_in))
+      throw std::runtime_error("invalid mesh arrays");
+    auto up = [&](int k, const void *src, size_t bytes) {
+      if (hipMalloc(&d[k], bytes ? bytes : 1) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+      if (bytes && hipMemcpy(d[k], src, bytes, hipMemcpyHostToDevice) != hipSuccess)
+        throw std::runtime_error("upload failed");
+    };
+    up(0, points, sizeof(double) * 2 * (size_t)np);
+    up(1, tris, sizeof(int) * 3 * (size_t)nt);
+    up(2, nbrs, sizeof(int) * 3 * (size_t)nt);
+    up(3, segs, sizeof(int) * 2 * (size_t)ns);
+    up(4, points_in, sizeof(double) * 2 * (size_t)np_in);
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
+      throw std::runtime_error("stream creation failed");
+    gcdt::Refiner::DeviceMesh dm{(const double2 *)d[0], np, (const int *)d[1], (const int *)d[2], nt,
+                                 (const int2 *)d[3], ns, (const double2 *)d[4], np_in};
+    gcdt::check_device_mesh(dm, stream, viol, exact_signs);
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    for (void *p : d) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+    return -2;
+  }
+  for (void *p : d) (void)hipFree(p);
+  (void)hipStreamDestroy(stream);
+  return 0;
+}
+
 int gcdt_session_download(gcdt_session *s, gcdt_mesh **out) {
   g_err.clear();
   try {

@@ -61,6 +61,19 @@ class Refiner {
   // device copies of the last run's output (for batches kept across waves)
   const double2 *out_points() const { return m.P; }
   int out_np() const { return np; }
+  // the last run's exported mesh in device memory (triangles / neighbours in
+  // the output numbering, points, subsegments) and the first input's points
+  struct DeviceMesh {
+    const double2 *P;
+    int np;
+    const int *T, *N;
+    int nt;
+    const int2 *S;
+    int ns;
+    const double2 *P_in;
+    int np_in;
+  };
+  DeviceMesh device_mesh() const;
   void run();
   void download(MeshHost &out);
   int kernel_times(gcdt_kernel_time *rec, int max_rec) const;
@@ -192,6 +205,12 @@ void mlp_forward_device_n(const float *dX, const int *dn, const int *dgo,
 constexpr int FEAT = 16;  // features per candidate (learned priority)
 // Validates options; on failure err holds the reason.
 bool options_ok(const gcdt_options *o, std::string &err);
+
+// Device invariant checks of a mesh in device memory (gcdt_check.hip);
+// viol[10] receives the per-category counts of gcdt_check_mesh, exact_signs
+// (if non-null) the number of signs left to the exact stage.
+void check_device_mesh(const Refiner::DeviceMesh &dm, hipStream_t stream, long long *viol,
+                       long long *exact_signs);
 
 }  // namespace gcdt
 

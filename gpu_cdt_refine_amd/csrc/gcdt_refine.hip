@@ -2532,6 +2532,20 @@ void Refiner::loop_graph() {
   st.cand_evals = (long long)h.acc[A_CANDEVALS];
 }
 
+Refiner::DeviceMesh Refiner::device_mesh() const {
+  DeviceMesh d;
+  d.P = m.P;
+  d.np = np;
+  d.T = o_T;
+  d.N = o_N;
+  d.nt = nt_out;
+  d.S = m.SV;
+  d.ns = ns;
+  d.P_in = inputs.empty() ? nullptr : inputs[0].P;
+  d.np_in = inputs.empty() ? 0 : inputs[0].np;
+  return d;
+}
+
 void Refiner::ensure_tmp(int n) {
   if (n <= tmp_list_cap) return;
   dfree(tmp_list);

@@ -230,6 +230,25 @@ int gcdt_session_kernel_times(gcdt_session *s, gcdt_kernel_time *rec,
                               int max_rec);
 void gcdt_session_destroy(gcdt_session *s);
 
+/* Invariant checks on the GPU with exact predicates (no reference
+ * counterpart: they replace the reference's host checks checkNeighbors
+ * main.cpp:79-113, checkIncircle main.cpp:153-242 and the range checks of
+ * checkResult main.cpp:336-362).  viol[10] receives counts per category:
+ * 0 index out of range, 1 neighbour links not symmetric, 2 triangle not
+ * counterclockwise, 3 boundary edge that is not a subsegment, 4 exact
+ * incircle > 0 across a non-subsegment edge, 5 subsegment that is not a mesh
+ * edge, 6 (unused here: segment coverage), 7 input point moved or missing,
+ * 8 (unused here: quality), 9 edge shared by three or more triangles.  A
+ * range violation skips the other checks.  gcdt_check_mesh takes host arrays
+ * in the triangulateio convention (neighbours as triangle indices, -1 on the
+ * boundary); gcdt_session_check checks the session's last result in HBM.
+ * exact_signs (may be NULL) receives how many incircle signs the filters
+ * left to the exact expansion stage. */
+int gcdt_check_mesh(const double *points, int np, const int *tris, const int *nbrs, int nt,
+                    const int *segs, int ns, const double *points_in, int np_in,
+                    long long *viol, long long *exact_signs);
+int gcdt_session_check(gcdt_session *s, long long *viol, long long *exact_signs);
+
 /* Batched refinement (no counterpart in the reference, which refines one
  * CDT per GPU_Refine_Quality call, refine.cu:6500-7139): n_inst independent
  * CDTs with their own theta (degrees), insertion policy (1 off-center) and
