@@ -262,6 +262,10 @@ def bench_refine(args):
         tune = {"best": best, "trials": len(trials)}
         log(f"# autotune: {best}")
     st, ms, dev = timed_runs(sc, args.steps, args.warmup, "chosen")
+    # exact invariants of the chosen form's mesh, checked in HBM (untimed)
+    t1 = time.perf_counter()
+    gcheck = sc.check()
+    gcheck["ms"] = (time.perf_counter() - t1) * 1e3
     sc.close()
     tris = int(st["n_triangles"])
     value = tris / (ms / 1e3)
@@ -317,6 +321,7 @@ def bench_refine(args):
         "roofline": roof,
         "whole_job": whole,
         "dropin_pcie_inclusive": dropin,
+        "gpu_check": gcheck,
         "cpu_baseline": cpu,
         "kernel_times_ms": {k: round(v["total_ms"], 3) for k, v in kt.items()} if kt else None,
     }
@@ -432,6 +437,18 @@ def bench_batch(args):
                                                   P, SS, float(theta[i]))
         checked.append({"instance": i, "violations": int(tot), "min_angle": mina})
     out["checked_instances"] = checked
+    # every instance on the GPU checker (index range, symmetry, orientation,
+    # boundary, exact Delaunay, subsegment edges, moved points, multiplicity)
+    t1 = time.time()
+    tot = {}
+    for i, (P, T, N, SS) in enumerate(ins):
+        r = res[i]
+        c = g.check_mesh(r.points, r.triangles, r.neighbors, r.segments, points_in=P)
+        for k, v in c.items():
+            tot[k] = tot.get(k, 0) + v
+    tot["instances"] = len(ins)
+    tot["seconds"] = time.time() - t1
+    out["gpu_check_all_instances"] = tot
     if not args.no_cpu_baseline:
         # the reference's Triangle -q on a sample of the instances, one process
         # per core (Triangle is sequential and keeps global state)
