@@ -148,6 +148,15 @@ def lib():
         L.gcdt_session_kernel_times.argtypes = [vp, C.POINTER(KernelTime), C.c_int]
         L.gcdt_session_kernel_times.restype = C.c_int
         L.gcdt_session_destroy.argtypes = [vp]
+        for name in ("gcdt_save_cdt_text", "gcdt_save_cdt_binary", "gcdt_save_triangle_files"):
+            getattr(L, name).argtypes = [C.c_char_p, vp, C.c_int, vp, vp, C.c_int, vp, C.c_int]
+            getattr(L, name).restype = C.c_int
+        L.gcdt_load_cdt.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.gcdt_load_cdt.restype = C.c_int
+        L.gcdt_file_mesh_counts.argtypes = [vp, ip, ip, ip]
+        L.gcdt_file_mesh_copy.argtypes = [vp, vp, vp, vp, vp]
+        L.gcdt_file_mesh_free.argtypes = [vp]
+        L.gcdt_io_last_error.restype = C.c_char_p
         L.gcdt_session_check.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.gcdt_session_check.restype = C.c_int
         L.gcdt_check_mesh.argtypes = [vp, C.c_int, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int,
@@ -375,6 +384,47 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def _io(rc, what):
+    if rc != 0:
+        raise OSError(f"{what}: {lib().gcdt_io_last_error().decode()}")
+
+
+def save_cdt(path, points, triangles, neighbors, segments, binary=False):
+    """Write a CDT in the reference's text layout (saveCDT, mesh.cpp:402-427;
+    coordinates with 17 significant digits) or the binary one."""
+    P, T, N, S = _as_inputs(points, triangles, neighbors, segments)
+    f = lib().gcdt_save_cdt_binary if binary else lib().gcdt_save_cdt_text
+    _io(f(os.fsencode(path), _ptr(P), len(P), _ptr(T), _ptr(N), len(T), _ptr(S), len(S)),
+        "save_cdt")
+
+
+def load_cdt(path):
+    """Read a CDT file (text -- also the reference's own saveCDT output -- or
+    binary); returns (points, triangles, neighbors, segments)."""
+    L = lib()
+    m = C.c_void_p()
+    _io(L.gcdt_load_cdt(os.fsencode(path), C.byref(m)), "load_cdt")
+    try:
+        np_, nt, ns = C.c_int(), C.c_int(), C.c_int()
+        L.gcdt_file_mesh_counts(m, C.byref(np_), C.byref(nt), C.byref(ns))
+        P = np.empty((np_.value, 2))
+        T = np.empty((nt.value, 3), np.int32)
+        N = np.empty((nt.value, 3), np.int32)
+        S = np.empty((ns.value, 2), np.int32)
+        L.gcdt_file_mesh_copy(m, _ptr(P), _ptr(T), _ptr(N), _ptr(S))
+    finally:
+        L.gcdt_file_mesh_free(m)
+    return P, T, N, S
+
+
+def save_triangle_files(basename, points, triangles, neighbors, segments):
+    """Write <basename>.node/.ele/.neigh/.poly in Triangle's formats
+    (zero-based, as with Triangle's -z)."""
+    P, T, N, S = _as_inputs(points, triangles, neighbors, segments)
+    _io(lib().gcdt_save_triangle_files(os.fsencode(basename), _ptr(P), len(P), _ptr(T), _ptr(N),
+                                        len(T), _ptr(S), len(S)), "save_triangle_files")
 
 
 CHECK_NAMES = ["range", "symmetry", "orientation", "boundary", "delaunay", "subsegment",

@@ -348,6 +348,30 @@ void gcdt_cdt_counts(const gcdt_cdt *c, int *np, int *nt, int *ns);
 void gcdt_cdt_copy(const gcdt_cdt *c, int *tris, int *nbrs, int *segs);
 void gcdt_cdt_free(gcdt_cdt *c);
 
+/* CDT files (reference GPU_Refine/mesh.cpp:402-532, saveCDT / readCDT).
+ * gcdt_save_cdt_text writes the reference's text layout (counts of points,
+ * triangles, segments; then points, triangles, neighbours, segments one per
+ * line) with 17 significant digits per coordinate, so the reference's readCDT
+ * reads it and nothing is lost; gcdt_save_cdt_binary writes "GCDTBIN1", three
+ * int64 counts and the raw arrays.  gcdt_load_cdt reads either (by the magic
+ * bytes), including files from the reference's saveCDT.
+ * gcdt_save_triangle_files writes <base>.node/.ele/.neigh/.poly in Triangle's
+ * formats with zero-based numbering (Triangle's -z).  Return 0 or -1 with
+ * gcdt_io_last_error(). */
+typedef struct gcdt_file_mesh gcdt_file_mesh;
+int gcdt_save_cdt_text(const char *path, const double *points, int np, const int *tris,
+                       const int *nbrs, int nt, const int *segs, int ns);
+int gcdt_save_cdt_binary(const char *path, const double *points, int np, const int *tris,
+                         const int *nbrs, int nt, const int *segs, int ns);
+int gcdt_load_cdt(const char *path, gcdt_file_mesh **out);
+void gcdt_file_mesh_counts(const gcdt_file_mesh *m, int *np, int *nt, int *ns);
+void gcdt_file_mesh_copy(const gcdt_file_mesh *m, double *points, int *tris, int *nbrs,
+                         int *segs);
+void gcdt_file_mesh_free(gcdt_file_mesh *m);
+int gcdt_save_triangle_files(const char *basename, const double *points, int np,
+                             const int *tris, const int *nbrs, int nt, const int *segs, int ns);
+const char *gcdt_io_last_error(void);
+
 /* The refinement's predicates evaluated on the host (same source as the
  * kernels): signs of orient2d(a,b,c) and incircle(a,b,c,d) for n queries
  * (ax, ay, bx, by, cx, cy, dx, dy).  No device use. */
