@@ -863,7 +863,7 @@ __device__ __forceinline__ unsigned long long claim_key(unsigned long long epoch
 // group reaches each ballot / shuffle with all of its lanes.
 constexpr int LIST_BLOCK = 256;
 constexpr int CLAIM_LANES = 16;
-constexpr int APPLY_LANES = 8;
+constexpr int APPLY_LANES = 4;
 
 template <int G>
 __device__ __forceinline__ unsigned long long group_bits(unsigned long long b) {
