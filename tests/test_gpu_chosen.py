@@ -1,0 +1,51 @@
+"""The chosen form as bench.py runs it -- FP32 filter, three claim
+iterations with shared rings, a bad pass evaluating up to 3x the previous
+winners, and the trained claim-priority network -- has no CPU restatement of
+the network's ordering, so it is held to the invariants instead: run-to-run
+identical meshes, no violation in the CPU or the GPU checker, and a Steiner
+count within 5% of the plain GPU path's."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CHOSEN = dict(pred_mode=1, claim_iters=3, shared_ring=1, cand_factor=3, priority=2,
+              learned_shift=1, graph=0)
+PLAIN = dict(pred_mode=0, claim_iters=1, priority=1, graph=0)
+
+
+@pytest.mark.parametrize("dist,theta", [(0, 25.0), (1, 28.0)])
+def test_chosen_form_invariants(gpu, oracle, dist, theta):
+    print(f"chosen form dist={dist} theta={theta}", flush=True)
+    model = os.path.join(os.path.dirname(gpu.__file__), "priority_mlp.npy")
+    P, S0 = gpu.generate_pslg(60000, 3000, L=1e5, maxlen=2e3, dist=dist, min_angle=5.0, seed=31)
+    T, N, S = gpu.build_cdt(P, S0)
+    meshes = []
+    for _ in range(2):
+        s = gpu.Session(P, T, N, S, theta=theta, **CHOSEN)
+        try:
+            s.set_priority_model(np.load(model))
+            st = s.run()
+            meshes.append((s.download(), s.check(), st))
+        finally:
+            s.close()
+    (a, ga, st), (b, gb, _) = meshes
+    assert st["status"] == 0
+    assert np.array_equal(a.points.view(np.uint64), b.points.view(np.uint64))
+    assert np.array_equal(a.triangles, b.triangles) and np.array_equal(a.neighbors, b.neighbors)
+    ga.pop("exact_signs")
+    assert all(v == 0 for v in ga.values()), ga
+    tot, viol, mina, ex = oracle.check_mesh(a.points, a.triangles, a.neighbors, a.segments,
+                                            P, S, theta)
+    assert tot == 0, viol
+    assert mina >= theta - 1e-6
+    p = gpu.Session(P, T, N, S, theta=theta, **PLAIN)
+    try:
+        sp = p.run()
+    finally:
+        p.close()
+    delta = (st["n_points"] - sp["n_points"]) / (sp["n_points"] - len(P))
+    assert abs(delta) < 0.05, delta
+    assert st["rounds"] < sp["rounds"]
