@@ -1929,7 +1929,8 @@ PassD Refiner::pass_base(int Kc) const {
   return ps;
 }
 
-// Claim iterations: claim -> win (-> block before the next iteration).  In
+// Claim iterations: claim -> win, repeated; from the second iteration on a
+// claim skips candidates blocked by an earlier iteration's winner stamps.  In
 // the host loop ps.epoch advances here; in the graph loop the kernels derive
 // it from the control block.
 void Refiner::enqueue_claims(PassD ps, int gc) {
