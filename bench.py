@@ -27,7 +27,8 @@ Prints ONE JSON line.  Also reports:
                 over its wall time
   cpu_baseline  the reference's Triangle "-q" path (compiled from its own
                 source into oracle/_ref) on a bounded sample, one host core
-Other workloads (--workload learned | batch | precision) print their own line.
+Other workloads (--workload learned | batch | precision | grid) print their own
+line; grid runs the 1M workload over theta {25, 30} x mode x insert.
 """
 import argparse
 import json
@@ -328,6 +329,71 @@ def bench_refine(args):
     print(json.dumps(out))
 
 
+def bench_grid(args):
+    """The 1M-point workload over theta {25, 30} x mode {Ruppert, Chew} x
+    insert {circumcenter, off-center}: chosen form vs the plain path per cell,
+    each chosen mesh checked in HBM; for the off-center cells Triangle -q (the
+    reference's CPU path, which always uses off-centers) on a bounded sample."""
+    import gpu_cdt_refine_amd as g
+    P, T, N, SS = make_input(args.points, args.segments, args.domain, 0.02 * args.domain, 0,
+                             args.seed)
+    log(f"# input: {len(P)} points, {len(T)} triangles, {len(SS)} segments")
+    n_cpu = args.cpu_sample_points // 2
+    Pc = Tc = Sc = None
+    if not args.no_cpu_baseline:
+        L = args.domain * (n_cpu / args.points) ** 0.5
+        Pc, Tc, _, Sc = make_input(n_cpu, int(args.segments * n_cpu / args.points), L,
+                                   0.02 * args.domain, 0, args.seed)
+    co = chosen_options(args)
+    cells = []
+    for theta in (25.0, 30.0):
+        for mode, mname in ((g.RUPPERT, "ruppert"), (g.CHEW, "chew")):
+            for insert, iname in ((g.Circumcenter, "circumcenter"), (g.Offcenter, "offcenter")):
+                kw = dict(theta=theta, insert=insert, mode=mode)
+                tag = f"{theta:g}/{mname}/{iname}"
+                cell = {"theta_deg": theta, "mode": mname, "insert": iname}
+                for name, o in (("plain", plain_options()), ("chosen", co)):
+                    s = g.Session(P, T, N, SS, **kw, **o)
+                    if o.get("priority") == 2:
+                        s.set_priority_model(np.load(MODEL))
+                    st, ms, dev = timed_runs(s, args.steps, args.warmup, f"{tag} {name}")
+                    r = {"ms_per_step": ms, "value": st["n_triangles"] / (ms / 1e3),
+                         "steiner": int(st["n_points"] - len(P)), "triangles": int(st["n_triangles"]),
+                         "rounds": int(st["rounds"]), "status": int(st["status"])}
+                    if name == "chosen":
+                        c = s.check()
+                        r["check_violations"] = int(sum(v for k, v in c.items() if k != "exact_signs"))
+                    s.close()
+                    cell[name] = r
+                cell["speedup"] = cell["plain"]["ms_per_step"] / cell["chosen"]["ms_per_step"]
+                cell["steiner_delta"] = (cell["chosen"]["steiner"] - cell["plain"]["steiner"]) / \
+                    max(1, cell["plain"]["steiner"])
+                if Pc is not None and iname == "offcenter":
+                    from oracle import pyoracle
+                    if os.path.exists(pyoracle.TRIREF_SO):
+                        t0 = time.time()
+                        _, To, _, _ = pyoracle.triangle_refine(Pc, Tc, Sc, theta,
+                                                               mode=1 if mode == g.RUPPERT else 0)
+                        dt = time.time() - t0
+                        cell["cpu_triangle"] = {"value": len(To) / dt, "seconds": dt,
+                                                "points_in": int(len(Pc)), "cores": 1}
+                log(f"# {tag}: chosen {cell['chosen']['ms_per_step']:.1f} ms, plain "
+                    f"{cell['plain']['ms_per_step']:.1f} ms, {cell['speedup']:.2f}x, Steiner "
+                    f"{100 * cell['steiner_delta']:+.1f}%")
+                cells.append(cell)
+    prim = [c for c in cells if c["theta_deg"] == 25.0 and c["mode"] == "ruppert"
+            and c["insert"] == "offcenter"][0]
+    out = {"metric": "output triangles/sec over the theta x mode x insert grid (1M uniform)",
+           "value": prim["chosen"]["value"], "unit": "triangles/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+           "dtype": "f64", "data": "synthetic (seeded uniform PSLG)",
+           "config": {"workload": refine_workload_name(args) + "-grid", "points": args.points,
+                      "segments": args.segments, "seed": args.seed, "chosen_options": co,
+                      "cpu_sample_points": None if Pc is None else int(len(Pc))},
+           "cells": cells}
+    print(json.dumps(out))
+
+
 def bench_learned(args):
     """Clustered 2M-point PSLG, 100k segments, theta 28: plain priority vs the
     learned priority, both with the chosen form's other settings."""
@@ -519,7 +585,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="refine", choices=["refine", "learned", "batch", "precision"])
+    ap.add_argument("--workload", default="refine", choices=["refine", "learned", "batch", "precision", "grid"])
     ap.add_argument("--points", type=int, default=1_000_000)
     ap.add_argument("--segments", type=int, default=50_000)
     ap.add_argument("--domain", type=float, default=1e6)
@@ -546,7 +612,7 @@ def main():
                           "(multi-GPU sharding is a design note only)"}))
         sys.exit(2)
     {"refine": bench_refine, "learned": bench_learned, "batch": bench_batch,
-     "precision": bench_precision}[args.workload](args)
+     "precision": bench_precision, "grid": bench_grid}[args.workload](args)
 
 
 if __name__ == "__main__":
