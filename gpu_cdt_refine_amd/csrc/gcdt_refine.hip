@@ -131,15 +131,13 @@ __global__ __launch_bounds__(1024) void k_scan(const int *blockcnt, int nb,
   }
 }
 
+// Scatter of one tile [base, base + CT): the selected indices in ascending
+// order from position run on; returns the position after the tile.
 template <class Pred>
-__global__ __launch_bounds__(CB) void k_scatter(Pred pred, int n,
-                                                const int *blockoff,
-                                                int *out_list, int *out_rank,
-                                                const int *src) {
+__device__ __forceinline__ int scatter_tile(const Pred &pred, int n, int base, int run,
+                                            int *out_list, int *out_rank, const int *src) {
   __shared__ int wcnt[CB / 64];
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int base = blockIdx.x * CT;
-  int run = blockoff[blockIdx.x];
   unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int k = 0; k < CI; k++) {
     int i = base + k * CB + threadIdx.x;
@@ -160,6 +158,33 @@ __global__ __launch_bounds__(CB) void k_scatter(Pred pred, int n,
     }
     run += tot;
     __syncthreads();
+  }
+  return run;
+}
+
+template <class Pred>
+__global__ __launch_bounds__(CB) void k_scatter(Pred pred, int n,
+                                                const int *blockoff,
+                                                int *out_list, int *out_rank,
+                                                const int *src) {
+  scatter_tile(pred, n, blockIdx.x * CT, blockoff[blockIdx.x], out_list, out_rank, src);
+}
+
+// The whole compaction in one launch when the input fits one tile (n <= CT):
+// the scatter from position 0, then the totals as k_scan writes them.
+template <class Pred>
+__global__ __launch_bounds__(CB) void k_compact1(Pred pred, int n, int *out_list,
+                                                 int *out_rank, const int *src, int *total,
+                                                 int *host_total,
+                                                 unsigned long long *total64) {
+  const int run = scatter_tile(pred, n, 0, 0, out_list, out_rank, src);
+  if (threadIdx.x == 0) {
+    *total = run;
+    if (total64) *total64 = (unsigned long long)run;
+    if (host_total) {
+      __hip_atomic_store(host_total, run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __threadfence_system();
+    }
   }
 }
 
@@ -1807,10 +1832,15 @@ int Refiner::compact(Pred pred, int n, int *out_list, int *out_rank,
     dalloc(&cb_off, (size_t)cb_cap);
   }
   tbegin(K_COMPACT);
-  k_count<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_cnt);
-  k_scan<<<1, 1024, 0, stream>>>(cb_cnt, nb, cb_off, d_total, total64 ? nullptr : d_host_total,
-                                  total64);
-  k_scatter<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_off, out_list, out_rank, src);
+  if (nb == 1) {
+    k_compact1<Pred><<<1, CB, 0, stream>>>(pred, n, out_list, out_rank, src, d_total,
+                                           total64 ? nullptr : d_host_total, total64);
+  } else {
+    k_count<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_cnt);
+    k_scan<<<1, 1024, 0, stream>>>(cb_cnt, nb, cb_off, d_total,
+                                    total64 ? nullptr : d_host_total, total64);
+    k_scatter<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_off, out_list, out_rank, src);
+  }
   tend(K_COMPACT, n);
   HC(hipGetLastError());
   if (total64) return -1;  // the count is read later with the pass counters
