@@ -93,7 +93,8 @@ class Refiner {
  private:
   template <class Pred>
   int compact(Pred pred, int n, int *out_list, int *out_rank, const int *src,
-              unsigned long long *total64 = nullptr);
+              unsigned long long *total64 = nullptr, bool counted = false);
+  void ensure_cb(int nb);
   int run_pass(const int *ids, int nc, int nseg, long long *progress, bool *overflowed,
                int *nwin_seg);
   PassD pass_base(int Kc) const;
@@ -154,7 +155,7 @@ class Refiner {
   int feat_cap = 0;
   int pass_index = 0;
   // compaction scratch
-  int *cb_cnt = nullptr, *cb_off = nullptr;
+  int *cb_cnt = nullptr;
   int cb_cap = 0;
   int *d_total = nullptr;
   int *h_total = nullptr;       // pinned, host-coherent: compaction totals

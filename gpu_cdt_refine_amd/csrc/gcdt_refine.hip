@@ -94,43 +94,6 @@ __global__ __launch_bounds__(CB) void k_count(Pred pred, int n, int *blockcnt) {
   }
 }
 
-// Exclusive scan of block counts (single block); total -> *total.
-// Exclusive scan of the per-block counts.  The total also goes to
-// host_total (pinned, host-coherent memory: the host reads it after a stream
-// synchronisation, no copy) and / or total64 (a pass counter) when given.
-__global__ __launch_bounds__(1024) void k_scan(const int *blockcnt, int nb,
-                                               int *blockoff, int *total, int *host_total,
-                                               unsigned long long *total64) {
-  __shared__ int sh[1024];
-  __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int base = 0; base < nb; base += 1024) {
-    int i = base + threadIdx.x;
-    int v = i < nb ? blockcnt[i] : 0;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < nb) blockoff[i] = carry + sh[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry += sh[1023];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    *total = carry;
-    if (total64) *total64 = (unsigned long long)carry;
-    if (host_total) {
-      __hip_atomic_store(host_total, carry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-      __threadfence_system();
-    }
-  }
-}
-
 // Scatter of one tile [base, base + CT): the selected indices in ascending
 // order from position run on; returns the position after the tile.
 template <class Pred>
@@ -162,16 +125,38 @@ __device__ __forceinline__ int scatter_tile(const Pred &pred, int n, int base, i
   return run;
 }
 
+// Scatter whose block offset is summed from the counts of the blocks before
+// it, so no scan launch sits between count and scatter; the last block
+// writes the totals: the device copy, and the pinned host-coherent copy
+// (read by the host after a stream synchronisation, no copy) or the pass
+// counter total64.
 template <class Pred>
-__global__ __launch_bounds__(CB) void k_scatter(Pred pred, int n,
-                                                const int *blockoff,
-                                                int *out_list, int *out_rank,
-                                                const int *src) {
-  scatter_tile(pred, n, blockIdx.x * CT, blockoff[blockIdx.x], out_list, out_rank, src);
+__global__ __launch_bounds__(CB) void k_scatter_sum(Pred pred, int n, const int *blockcnt,
+                                                    int *out_list, int *out_rank,
+                                                    const int *src, int *total,
+                                                    int *host_total,
+                                                    unsigned long long *total64) {
+  __shared__ int wred[CB / 64];
+  int s = 0;
+  for (int b = threadIdx.x; b < (int)blockIdx.x; b += CB) s += blockcnt[b];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) wred[threadIdx.x >> 6] = s;
+  __syncthreads();
+  int run = 0;
+  for (int w = 0; w < CB / 64; w++) run += wred[w];
+  run = scatter_tile(pred, n, blockIdx.x * CT, run, out_list, out_rank, src);
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    *total = run;
+    if (total64) *total64 = (unsigned long long)run;
+    if (host_total) {
+      __hip_atomic_store(host_total, run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __threadfence_system();
+    }
+  }
 }
 
 // The whole compaction in one launch when the input fits one tile (n <= CT):
-// the scatter from position 0, then the totals as k_scan writes them.
+// the scatter from position 0, then the totals as k_scatter_sum writes them.
 template <class Pred>
 __global__ __launch_bounds__(CB) void k_compact1(Pred pred, int n, int *out_list,
                                                  int *out_rank, const int *src, int *total,
@@ -314,17 +299,47 @@ __global__ void k_check_segs(const int *ST, int ns, unsigned long long *cnt) {
 // Bad-triangle marking (refine.cu:376-497 kernelMarkBadTriangles)
 // ---------------------------------------------------------------------------
 // nt < 0: the live count and the go flag come from ctl (graph-driven loop).
+// Quality test of triangle t if its check flag is set; returns its flags.
+__device__ __forceinline__ uint8_t mark_bad_one(const MeshD &m, const ParamsD &pr, int t) {
+  uint8_t f = m.tflag[t];
+  if (!(f & TF_ALIVE) || !(f & TF_CHECK)) return f;
+  f &= (uint8_t)~TF_CHECK;
+  if (tri_is_bad(m, params_at(pr, m, t), m.T[t])) f |= TF_BAD;
+  else f &= (uint8_t)~TF_BAD;
+  m.tflag[t] = f;
+  return f;
+}
+
 __global__ __launch_bounds__(256) void k_mark_bad(MeshD m, ParamsD pr, int nt,
                                                   const Ctl *ctl) {
   if (ctl) nt = ctl->i[C_GO] ? ctl->i[C_NT] : 0;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt;
-       t += gridDim.x * blockDim.x) {
-    uint8_t f = m.tflag[t];
-    if (!(f & TF_ALIVE) || !(f & TF_CHECK)) continue;
-    f &= (uint8_t)~TF_CHECK;
-    if (tri_is_bad(m, params_at(pr, m, t), m.T[t])) f |= TF_BAD;
-    else f &= (uint8_t)~TF_BAD;
-    m.tflag[t] = f;
+       t += gridDim.x * blockDim.x)
+    mark_bad_one(m, pr, t);
+}
+
+// k_mark_bad over the compaction's tiles, also counting each tile's bad
+// triangles (PredBad) for the scatter that follows: the count pass of the
+// bad-triangle compaction rides on the marking pass.
+__global__ __launch_bounds__(CB) void k_mark_bad_count(MeshD m, ParamsD pr, int nt,
+                                                       int *blockcnt) {
+  __shared__ int wsum[CB / 64];
+  const int base = blockIdx.x * CT;
+  int c = 0;
+  for (int k = 0; k < CI; k++) {
+    int t = base + k * CB + threadIdx.x;
+    if (t < nt) {
+      const uint8_t f = mark_bad_one(m, pr, t);
+      c += (f & (TF_ALIVE | TF_BAD | TF_GIVEUP)) == (TF_ALIVE | TF_BAD);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int sum = 0;
+    for (int w = 0; w < CB / 64; w++) sum += wsum[w];
+    blockcnt[blockIdx.x] = sum;
   }
 }
 
@@ -1652,7 +1667,7 @@ Refiner::~Refiner() {
   dfree(c_act); dfree(c_slow); dfree(tmp_list); dfree(mix_list);
   dfree(sel_prio); dfree(sel_tmp); dfree(sel_list); dfree(d_hist);
   dfree(c_feat); dfree(c_score); dfree(c_deg); dfree(t_cnt); dfree(mlp_params);
-  dfree(cb_cnt); dfree(cb_off); dfree(d_total); dfree(d_cnt);
+  dfree(cb_cnt); dfree(d_total); dfree(d_cnt);
   dfree(o_T); dfree(o_N); dfree(o_tstat); dfree(o_pstat); dfree(newid);
   dfree(h_key); dfree(h_val);
   dfree(d_ctl); dfree(g_cnt); dfree(g_off);
@@ -1816,36 +1831,44 @@ const char *Refiner::kname(int id) {
   return names[id];
 }
 
+// Stream compaction of [0, n) by pred, in ascending index order: one launch
+// when n fits one tile, else per-tile counts then a scatter that sums the
+// counts before its tile (counted: the counts are already in cb_cnt).  With
+// total64 the count stays on the device and -1 is returned; otherwise the
+// call synchronises and returns it.
 template <class Pred>
 int Refiner::compact(Pred pred, int n, int *out_list, int *out_rank,
-                     const int *src, unsigned long long *total64) {
+                     const int *src, unsigned long long *total64, bool counted) {
   if (n <= 0) {
     if (total64) HC(hipMemsetAsync(total64, 0, sizeof(unsigned long long), stream));
     return 0;
   }
   int nb = (n + CT - 1) / CT;
-  if (nb > cb_cap) {
-    dfree(cb_cnt);
-    dfree(cb_off);
-    cb_cap = std::max(nb, 2 * cb_cap);
-    dalloc(&cb_cnt, (size_t)cb_cap);
-    dalloc(&cb_off, (size_t)cb_cap);
-  }
+  ensure_cb(nb);
   tbegin(K_COMPACT);
   if (nb == 1) {
     k_compact1<Pred><<<1, CB, 0, stream>>>(pred, n, out_list, out_rank, src, d_total,
                                            total64 ? nullptr : d_host_total, total64);
   } else {
-    k_count<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_cnt);
-    k_scan<<<1, 1024, 0, stream>>>(cb_cnt, nb, cb_off, d_total,
-                                    total64 ? nullptr : d_host_total, total64);
-    k_scatter<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_off, out_list, out_rank, src);
+    if (!counted) k_count<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_cnt);
+    k_scatter_sum<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_cnt, out_list, out_rank, src,
+                                               d_total, total64 ? nullptr : d_host_total,
+                                               total64);
   }
   tend(K_COMPACT, n);
   HC(hipGetLastError());
   if (total64) return -1;  // the count is read later with the pass counters
   HC(hipStreamSynchronize(stream));
   return __atomic_load_n(h_total, __ATOMIC_ACQUIRE);
+}
+
+// Per-tile counts of the compaction (kept across calls: a counting pass may
+// fill them before compact() runs).
+void Refiner::ensure_cb(int nb) {
+  if (nb <= cb_cap) return;
+  dfree(cb_cnt);
+  cb_cap = std::max(nb, 2 * cb_cap);
+  dalloc(&cb_cnt, (size_t)cb_cap);
 }
 
 void Refiner::read_counters() {
@@ -2251,11 +2274,17 @@ bool Refiner::bad_pass(int ne, int *prev_win) {
   ensure_mix(std::min<long long>((long long)ns + nt, (long long)cap + 1));
   if (ne > 0)
     HC(hipMemcpyAsync(mix_list, tmp_list, sizeof(int) * (size_t)ne, hipMemcpyDeviceToDevice, stream));
-  tbegin(K_MARKBAD);
-  k_mark_bad<<<std::min(nblk(nt), 65536), 256, 0, stream>>>(m, pr, nt, nullptr);
-  tend(K_MARKBAD, nt);
   ensure_tmp(nt);
-  int nb = compact(PredBad{m.tflag}, nt, tmp_list, nullptr, nullptr);
+  const int ntile = (nt + CT - 1) / CT;
+  tbegin(K_MARKBAD);
+  if (ntile > 1) {
+    ensure_cb(ntile);
+    k_mark_bad_count<<<ntile, CB, 0, stream>>>(m, pr, nt, cb_cnt);
+  } else {
+    k_mark_bad<<<std::min(nblk(nt), 65536), 256, 0, stream>>>(m, pr, nt, nullptr);
+  }
+  tend(K_MARKBAD, nt);
+  int nb = compact(PredBad{m.tflag}, nt, tmp_list, nullptr, nullptr, nullptr, ntile > 1);
   if (ne == 0 && nb == 0) {
     st.status = 0;
     return true;
