@@ -155,7 +155,7 @@ class Refiner {
   int feat_cap = 0;
   int pass_index = 0;
   // compaction scratch
-  int *cb_cnt = nullptr;
+  int *cb_cnt = nullptr, *cb_off = nullptr;
   int cb_cap = 0;
   int *d_total = nullptr;
   int *h_total = nullptr;       // pinned, host-coherent: compaction totals

@@ -41,6 +41,7 @@ constexpr int CB = 256;       // threads per block
 constexpr int CI = 16;        // items per thread
 constexpr int CT = CB * CI;   // items per block tile
 constexpr int NBUCKET = 8192;  // priority buckets (edge_bucket range)
+constexpr int SUM_TILES = 4096;  // compactions of more tiles use a scan launch
 
 struct PredSenc {
   const int *senc;
@@ -123,6 +124,55 @@ __device__ __forceinline__ int scatter_tile(const Pred &pred, int n, int base, i
     __syncthreads();
   }
   return run;
+}
+
+// Exclusive scan of the per-tile counts (one block: wave64 shuffle scans,
+// then a scan of the 16 wave totals), for compactions of many tiles; the
+// totals go where k_scatter_sum writes them.
+__global__ __launch_bounds__(1024) void k_scan(const int *blockcnt, int nb,
+                                               int *blockoff, int *total, int *host_total,
+                                               unsigned long long *total64) {
+  __shared__ int wsum[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int carry = 0;
+  for (int base = 0; base < nb; base += 1024) {
+    int i = base + threadIdx.x;
+    int v = i < nb ? blockcnt[i] : 0;
+    int x = v;  // inclusive scan within the wave
+    for (int off = 1; off < 64; off <<= 1) {
+      int t = __shfl_up(x, off, 64);
+      if (lane >= off) x += t;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (wave == 0) {
+      int w = lane < 16 ? wsum[lane] : 0;
+      for (int off = 1; off < 16; off <<= 1) {
+        int t = __shfl_up(w, off, 64);
+        if (lane >= off) w += t;
+      }
+      if (lane < 16) wsum[lane] = w;
+    }
+    __syncthreads();
+    if (i < nb) blockoff[i] = carry + (wave ? wsum[wave - 1] : 0) + x - v;
+    carry += wsum[15];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *total = carry;
+    if (total64) *total64 = (unsigned long long)carry;
+    if (host_total) {
+      __hip_atomic_store(host_total, carry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __threadfence_system();
+    }
+  }
+}
+
+template <class Pred>
+__global__ __launch_bounds__(CB) void k_scatter(Pred pred, int n, const int *blockoff,
+                                                int *out_list, int *out_rank,
+                                                const int *src) {
+  scatter_tile(pred, n, blockIdx.x * CT, blockoff[blockIdx.x], out_list, out_rank, src);
 }
 
 // Scatter whose block offset is summed from the counts of the blocks before
@@ -1667,7 +1717,7 @@ Refiner::~Refiner() {
   dfree(c_act); dfree(c_slow); dfree(tmp_list); dfree(mix_list);
   dfree(sel_prio); dfree(sel_tmp); dfree(sel_list); dfree(d_hist);
   dfree(c_feat); dfree(c_score); dfree(c_deg); dfree(t_cnt); dfree(mlp_params);
-  dfree(cb_cnt); dfree(d_total); dfree(d_cnt);
+  dfree(cb_cnt); dfree(cb_off); dfree(d_total); dfree(d_cnt);
   dfree(o_T); dfree(o_N); dfree(o_tstat); dfree(o_pstat); dfree(newid);
   dfree(h_key); dfree(h_val);
   dfree(d_ctl); dfree(g_cnt); dfree(g_off);
@@ -1833,7 +1883,9 @@ const char *Refiner::kname(int id) {
 
 // Stream compaction of [0, n) by pred, in ascending index order: one launch
 // when n fits one tile, else per-tile counts then a scatter that sums the
-// counts before its tile (counted: the counts are already in cb_cnt).  With
+// counts before its tile (counted: the counts are already in cb_cnt).  The
+// summing costs O(tiles^2) reads in all, so beyond SUM_TILES tiles a scan
+// launch computes the tile offsets instead.  With
 // total64 the count stays on the device and -1 is returned; otherwise the
 // call synchronises and returns it.
 template <class Pred>
@@ -1851,9 +1903,15 @@ int Refiner::compact(Pred pred, int n, int *out_list, int *out_rank,
                                            total64 ? nullptr : d_host_total, total64);
   } else {
     if (!counted) k_count<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_cnt);
-    k_scatter_sum<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_cnt, out_list, out_rank, src,
-                                               d_total, total64 ? nullptr : d_host_total,
-                                               total64);
+    if (nb <= SUM_TILES) {
+      k_scatter_sum<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_cnt, out_list, out_rank, src,
+                                                 d_total, total64 ? nullptr : d_host_total,
+                                                 total64);
+    } else {
+      k_scan<<<1, 1024, 0, stream>>>(cb_cnt, nb, cb_off, d_total,
+                                      total64 ? nullptr : d_host_total, total64);
+      k_scatter<Pred><<<nb, CB, 0, stream>>>(pred, n, cb_off, out_list, out_rank, src);
+    }
   }
   tend(K_COMPACT, n);
   HC(hipGetLastError());
@@ -1867,8 +1925,10 @@ int Refiner::compact(Pred pred, int n, int *out_list, int *out_rank,
 void Refiner::ensure_cb(int nb) {
   if (nb <= cb_cap) return;
   dfree(cb_cnt);
+  dfree(cb_off);
   cb_cap = std::max(nb, 2 * cb_cap);
   dalloc(&cb_cnt, (size_t)cb_cap);
+  dalloc(&cb_off, (size_t)cb_cap);
 }
 
 void Refiner::read_counters() {
