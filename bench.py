@@ -27,6 +27,9 @@ Prints ONE JSON line.  Also reports:
                 over its wall time
   cpu_baseline  the reference's Triangle "-q" path (compiled from its own
                 source into oracle/_ref) on a bounded sample, one host core
+  median_step_ms, host
+                median step times of each timed form; the host's CPU count,
+                model and clock
 Other workloads (--workload learned | batch | precision | grid) print their own
 line; grid runs the 1M workload over theta {25, 30} x mode x insert.
 """
@@ -114,7 +117,28 @@ def timed_runs(sess, steps, warmup, tag):
         walls.append(time.perf_counter() - t0)
         devs.append(st["device_ms"])
         log(f"# {tag} step {i}: wall {walls[-1] * 1e3:.1f} ms, device {st['device_ms']:.1f} ms")
+    MEDIANS[tag] = {"wall_ms": float(np.median(walls)) * 1e3, "device_ms": float(np.median(devs))}
     return st, float(np.mean(walls)) * 1e3, float(np.mean(devs))
+
+
+MEDIANS = {}  # per timed form: median step time (wall and HIP-event device time)
+
+
+def host_info():
+    """The host the run used: logical CPUs, CPU model and clock."""
+    info = {"nproc": os.cpu_count(), "cpu_model": None, "cpu_mhz": None}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "model name" and info["cpu_model"] is None:
+                    info["cpu_model"] = v.strip()
+                elif k == "cpu MHz" and info["cpu_mhz"] is None:
+                    info["cpu_mhz"] = float(v)
+    except (OSError, ValueError):
+        pass
+    return info
 
 
 def summary(st):
@@ -325,6 +349,8 @@ def bench_refine(args):
         "gpu_check": gcheck,
         "cpu_baseline": cpu,
         "kernel_times_ms": {k: round(v["total_ms"], 3) for k, v in kt.items()} if kt else None,
+        "median_step_ms": dict(MEDIANS),
+        "host": host_info(),
     }
     print(json.dumps(out))
 
