@@ -74,6 +74,7 @@ struct MeshD {
   int *ppar;
   struct TriRec *T;
   uint8_t *tflag;
+  uint16_t *tbucket;  // shortest-edge bucket, written when a triangle is marked bad
   int2 *SV;
   int *SP, *ST, *senc;
   const int2 *SEG;

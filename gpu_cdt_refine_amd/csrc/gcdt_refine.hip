@@ -354,8 +354,14 @@ __device__ __forceinline__ uint8_t mark_bad_one(const MeshD &m, const ParamsD &p
   uint8_t f = m.tflag[t];
   if (!(f & TF_ALIVE) || !(f & TF_CHECK)) return f;
   f &= (uint8_t)~TF_CHECK;
-  if (tri_is_bad(m, params_at(pr, m, t), m.T[t])) f |= TF_BAD;
-  else f &= (uint8_t)~TF_BAD;
+  const TriRec &r = m.T[t];
+  if (tri_is_bad(m, params_at(pr, m, t), r)) {
+    f |= TF_BAD;
+    // the candidate selection's priority, read later without the record
+    m.tbucket[t] = (uint16_t)shortest_edge_bucket(r);
+  } else {
+    f &= (uint8_t)~TF_BAD;
+  }
   m.tflag[t] = f;
   return f;
 }
@@ -1588,7 +1594,7 @@ __global__ __launch_bounds__(256) void k_dprio_hist(MeshD m, const int *ids,
   __syncthreads();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
-    int p = shortest_edge_bucket(m.T[ids[i]]);
+    int p = m.tbucket[ids[i]];
     prio[i] = p;
     atomicAdd(&h[p], 1u);
   }
@@ -1738,7 +1744,7 @@ Refiner::~Refiner() {
 
 void Refiner::free_mesh() {
   dfree(m.P); dfree(m.pflag); dfree(m.ppar);
-  dfree(m.T); dfree(m.tflag); dfree(m.claim);
+  dfree(m.T); dfree(m.tflag); dfree(m.tbucket); dfree(m.claim);
   dfree(m.SV); dfree(m.SP); dfree(m.ST); dfree(m.senc);
   capP = capT = capS = 0;
 }
@@ -1825,6 +1831,7 @@ void Refiner::ensure_capacity(long long needP, long long needT, long long needS)
     long long c = std::min(std::max<long long>(needT, capT + capT / 2), MAX_TRIANGLES);
     dgrow(&m.T, (size_t)nt, (size_t)c, stream);
     dgrow(&m.tflag, (size_t)nt, (size_t)c, stream);
+    dgrow(&m.tbucket, (size_t)nt, (size_t)c, stream);
     unsigned long long *nc;
     dalloc(&nc, 2 * (size_t)c);
     HC(hipMemsetAsync(nc, 0, sizeof(unsigned long long) * 2 * c, stream));
@@ -1974,7 +1981,7 @@ __global__ __launch_bounds__(256) void k_prio_hist(MeshD m, const int *ids,
   __syncthreads();
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
-    int p = shortest_edge_bucket(m.T[ids[i]]);
+    int p = m.tbucket[ids[i]];
     prio[i] = p;
     atomicAdd(&h[p], 1u);
   }
@@ -2192,7 +2199,7 @@ void Refiner::run() {
     capP = wantP; capT = wantT; capS = wantS;
     dalloc(&m.P, capP); dalloc(&m.pflag, capP); dalloc(&m.ppar, capP);
     dalloc(&m.T, capT);
-    dalloc(&m.tflag, capT); dalloc(&m.claim, 2 * (size_t)capT);
+    dalloc(&m.tflag, capT); dalloc(&m.tbucket, capT); dalloc(&m.claim, 2 * (size_t)capT);
     dalloc(&m.SV, capS); dalloc(&m.SP, capS); dalloc(&m.ST, capS); dalloc(&m.senc, capS);
   }
   if (ns0 > seg_in_cap || !seg_in) {
