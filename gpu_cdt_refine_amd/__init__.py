@@ -108,6 +108,8 @@ class Stats(C.Structure):
         ("tri_examined_bad", C.c_longlong),
         ("ws_ints_seg", C.c_longlong),
         ("ws_ints_bad", C.c_longlong),
+        ("giveup_tris_left", C.c_longlong),
+        ("giveup_segs_left", C.c_longlong),
     ]
 
     def as_dict(self):
@@ -564,6 +566,33 @@ def host_predicates(queries, pred_mode=0):
     L.gcdt_host_predicates.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
     L.gcdt_host_predicates(_ptr(q), len(q), pred_mode, _ptr(o), _ptr(ic))
     return o, ic
+
+
+def _constructions(fn, tris, theta, offcenter):
+    q = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 6)
+    ccw = np.empty(len(q), np.float64)
+    stp = np.empty((len(q), 2), np.float64)
+    ok = np.empty(len(q), np.int32)
+    f = getattr(lib(), fn)
+    f.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    f.restype = C.c_int
+    rc = f(_ptr(q), len(q), float(theta), int(offcenter), _ptr(ccw), _ptr(stp), _ptr(ok))
+    if rc != 0:
+        _err(rc, fn)
+    return ccw, stp, ok
+
+
+def host_constructions(tris, theta=25.0, offcenter=1):
+    """The kernels' constructions run on the host for an (n, 6) array of
+    triangles (p0, p1, p2): the adaptive orient2d value ccw(p0, p1, p2) and the
+    Steiner point of the triangle with apex p0, org p1, dest p2 (ok = 0 when
+    degenerate)."""
+    return _constructions("gcdt_host_constructions", tris, theta, offcenter)
+
+
+def device_constructions(tris, theta=25.0, offcenter=1):
+    """host_constructions evaluated on the GPU."""
+    return _constructions("gcdt_device_constructions", tris, theta, offcenter)
 
 
 def generate_pslg(n, nseg, L=1000.0, maxlen=0.0, dist=0, min_angle=5.0, seed=1):

@@ -262,8 +262,12 @@ double gcdt_batch_run_ms(const gcdt_batch *b) {
   return t;
 }
 
+// Severity order of the status codes: converged, converged with given-up
+// elements, round cap, stalled.
+static int status_rank(int s) { return s == 0 ? 0 : s == 3 ? 1 : s == 1 ? 2 : 3; }
+
 // Refine every wave.  stats: sums over waves (rounds and final_cav_cap: the
-// largest; status: the largest code).
+// largest; status: the most severe code).
 int gcdt_batch_run(gcdt_batch *b, gcdt_stats *stats) {
   g_batch_err.clear();
   try {
@@ -288,7 +292,9 @@ int gcdt_batch_run(gcdt_batch *b, gcdt_stats *stats) {
       tot.lost += s.lost;
       tot.giveups += s.giveups;
       tot.final_cav_cap = std::max(tot.final_cav_cap, s.final_cav_cap);
-      tot.status = std::max(tot.status, s.status);
+      if (status_rank(s.status) > status_rank(tot.status)) tot.status = s.status;
+      tot.giveup_tris_left += s.giveup_tris_left;
+      tot.giveup_segs_left += s.giveup_segs_left;
       tot.device_ms += s.device_ms;
       tot.pred_fp32 += s.pred_fp32;
       tot.pred_fp64 += s.pred_fp64;

@@ -97,6 +97,14 @@ bool options_ok(const gcdt_options *o, std::string &g_err) {
     g_err = "cav_lds_cap must be in [4, 2048]";
     return false;
   }
+  // The graph loop resets the 11-bit claim epoch only between graph
+  // launches; one launch uses claim_iters + 1 epochs per pass.
+  if (o->graph && (long long)o->graph_rounds * (o->graph_seg_passes + 1) * (o->claim_iters + 1) + 2 >
+                      2047) {
+    g_err = "graph_rounds * (graph_seg_passes + 1) * (claim_iters + 1) + 2 must not exceed 2047 "
+            "(claim epochs of one graph launch)";
+    return false;
+  }
   if (o->graph && o->log_training) { g_err = "log_training needs the host-driven loop (graph = 0)"; return false; }
   if (o->mixed != 0 && o->mixed != 1) { g_err = "mixed must be 0 or 1"; return false; }
   if (o->shared_ring != 0 && o->shared_ring != 1) { g_err = "shared_ring must be 0 or 1"; return false; }
@@ -300,6 +308,33 @@ int gcdt_device_predicates(const double *pts8, int n, int pred_mode,
   try {
     gcdt::device_predicates(pts8, n, pred_mode, orient_out, incircle_out,
                             stage_counts);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -2;
+  }
+}
+
+int gcdt_host_constructions(const double *tri6, int n, double theta_deg, int offcenter,
+                            double *ccw_out, double *steiner_out, int *ok_out) {
+  g_err.clear();
+  if (n < 0 || (n > 0 && (!tri6 || !ccw_out || !steiner_out || !ok_out))) {
+    g_err = "bad arguments";
+    return -1;
+  }
+  gcdt::host_constructions(tri6, n, theta_deg, offcenter, ccw_out, steiner_out, ok_out);
+  return 0;
+}
+
+int gcdt_device_constructions(const double *tri6, int n, double theta_deg, int offcenter,
+                              double *ccw_out, double *steiner_out, int *ok_out) {
+  g_err.clear();
+  if (n < 0 || (n > 0 && (!tri6 || !ccw_out || !steiner_out || !ok_out))) {
+    g_err = "bad arguments";
+    return -1;
+  }
+  try {
+    gcdt::device_constructions(tri6, n, theta_deg, offcenter, ccw_out, steiner_out, ok_out);
     return 0;
   } catch (const std::exception &e) {
     g_err = e.what();

@@ -63,6 +63,8 @@ enum : int {
   CNT_WS_SEG,    // ints written to candidate workspaces (segment kernels)
   CNT_WS_BAD,    // ... (bad-triangle kernels)
   CNT_NWIN,      // winners of the pass
+  CNT_LEFT_TRI,  // end of the run: alive triangles flagged TF_GIVEUP
+  CNT_LEFT_SEG,  // end of the run: subsegments with senc == 2
   CNT_N = 24
 };
 
@@ -205,7 +207,7 @@ __device__ __forceinline__ unsigned long long pass_epoch(const PassD &ps) {
 // Winners' regions are stamped with the epoch after the pass's last claim
 // epoch: no claim of the pass can displace them.
 __device__ __forceinline__ unsigned long long pass_sticky(const PassD &ps) {
-  return pass_epoch0(ps) + (unsigned long long)ps.iters;
+  return (pass_epoch0(ps) + (unsigned long long)ps.iters) & 2047ull;
 }
 __device__ __forceinline__ int pass_index_of(const PassD &ps) {
   return ps.ctl ? ps.ctl->i[C_PASS_INDEX] : ps.pass_index;
@@ -289,18 +291,29 @@ __device__ __forceinline__ bool tri_is_bad(const MeshD &m, const ParamsD &pr,
 }
 
 // Circumcenter / Ungor off-center (refine.cu:3780-3868,
-// triangle.cpp:6523-6618).  false if numerically degenerate.
-__device__ __forceinline__ bool steiner_point(const ParamsD &pr,
-                                              const TriRec &r, double &cx,
-                                              double &cy) {
+// triangle.cpp:6523-6618).  The denominator is Triangle's adaptive
+// counterclockwise(dest, apex, org) value (triangle.cpp:6556-6563): the plain
+// FP64 determinant when its filter certifies it, else the adaptive stages
+// (orient2d_adapt_value), whose value stays accurate for near-collinear
+// triangles.  Returns 1, 0 if the triangle is degenerate (collinear or the
+// point coincides with a vertex), or STEINER_DEFER (FAST only) when the
+// adaptive stages are needed: the candidate then goes to the exact kernel.
+constexpr int STEINER_DEFER = 2;
+template <bool FAST>
+__host__ __device__ __forceinline__ int steiner_point(const ParamsD &pr, const TriRec &r, double &cx,
+                                             double &cy) {
   double2 A = r.X[0], O = r.X[1], D = r.X[2];
   double xdo = D.x - O.x, ydo = D.y - O.y;
   double xao = A.x - O.x, yao = A.y - O.y;
   double dodist = xdo * xdo + ydo * ydo;
   double aodist = xao * xao + yao * yao;
   double dadist = (D.x - A.x) * (D.x - A.x) + (D.y - A.y) * (D.y - A.y);
-  double det = xdo * yao - xao * ydo;
-  if (det == 0.0) return false;
+  double det, detsum = 0.0;
+  if (!orient2d_value_filter(D.x, D.y, A.x, A.y, O.x, O.y, &det, &detsum)) {
+    if (FAST) return STEINER_DEFER;
+    det = orient2d_adapt_value(D.x, D.y, A.x, A.y, O.x, O.y, detsum);
+  }
+  if (det == 0.0) return 0;
   double denominator = 0.5 / det;
   double dx = (yao * dodist - ydo * aodist) * denominator;
   double dy = (xdo * aodist - xao * dodist) * denominator;
@@ -328,9 +341,9 @@ __device__ __forceinline__ bool steiner_point(const ParamsD &pr,
   cy = O.y + dy;
   if ((cx == A.x && cy == A.y) || (cx == O.x && cy == O.y) ||
       (cx == D.x && cy == D.y))
-    return false;
-  if (!(cx == cx) || !(cy == cy)) return false;
-  return true;
+    return 0;
+  if (!(cx == cx) || !(cy == cy)) return 0;
+  return 1;
 }
 
 // Half-octave bucket of a squared edge length (Triangle's queue order,

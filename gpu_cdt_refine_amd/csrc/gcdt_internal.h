@@ -198,6 +198,10 @@ class Refiner {
 
 void device_predicates(const double *pts8, int n, int pred_mode, int *o,
                        int *ic, long long *stage_counts);
+void device_constructions(const double *tri6, int n, double theta_deg, int offcenter,
+                          double *ccw, double *stp, int *ok);
+void host_constructions(const double *tri6, int n, double theta_deg, int offcenter, double *ccw,
+                        double *stp, int *ok);
 void mlp_forward_device(const float *dX, int n, const float *dP, float *dy,
                         hipStream_t stream);
 void mlp_forward_device_n(const float *dX, const int *dn, const int *dgo,

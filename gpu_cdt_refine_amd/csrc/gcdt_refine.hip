@@ -603,7 +603,10 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
         const double sh2 = shortest_edge_sq(r0);
         near2 = 1e-12 * sh2;
         if (pr.priority == 1 || pr.priority == 2) prio = edge_bucket(sh2);
-        if (!steiner_point(lp, r0, cx, cy)) {
+        const int sp = steiner_point<FAST>(lp, r0, cx, cy);
+        if (sp == STEINER_DEFER) {
+          status = ST_SLOW;
+        } else if (!sp) {
           status = ST_GIVEUP;
         } else {
           int tl = -1, jl = -1, sg = -1;
@@ -1240,6 +1243,24 @@ __global__ void k_export(MeshD m, int nt, const int *newid, int *otri,
   if (oinst) oinst[q] = tv.w;
 }
 
+// Given-up elements left at the end of a run: alive triangles still flagged
+// TF_GIVEUP (bad, never split) and subsegments whose split was given up.
+__global__ __launch_bounds__(256) void k_count_left(const uint8_t *tflag, int nt, const int *senc,
+                                                    int ns, unsigned long long *cnt) {
+  unsigned long long a = 0, b = 0;
+  const int n = nt > ns ? nt : ns;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (i < nt && (tflag[i] & (TF_ALIVE | TF_GIVEUP)) == (TF_ALIVE | TF_GIVEUP)) a++;
+    if (i < ns && senc[i] == 2) b++;
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if ((threadIdx.x & 63) == 0) {
+    if (a) atomicAdd(&cnt[CNT_LEFT_TRI], a);
+    if (b) atomicAdd(&cnt[CNT_LEFT_SEG], b);
+  }
+}
+
 __global__ void k_pstatus(const uint8_t *pflag, int np, uint8_t *out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
@@ -1261,6 +1282,25 @@ __global__ void k_predicates(const double *q, int n, int pred_mode, int *o,
                           pred_mode == 1, &st);
   }
   flush_stats(cnt, st);
+}
+
+// Constructions on the device, for tests: q = (p0, p1, p2) per query;
+// ccw = Triangle's adaptive counterclockwise(p0, p1, p2) value, and the
+// refinement's Steiner point of the triangle with apex p0, org p1, dest p2.
+__global__ void k_constructions(const double *q, int n, ParamsD pr, double *ccw, double *stp,
+                                int *ok) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double *p = q + 6 * (size_t)i;
+    ccw[i] = orient2d_value(p[0], p[1], p[2], p[3], p[4], p[5]);
+    TriRec r;
+    r.X[0] = make_double2(p[0], p[1]);
+    r.X[1] = make_double2(p[2], p[3]);
+    r.X[2] = make_double2(p[4], p[5]);
+    double cx = 0.0, cy = 0.0;
+    ok[i] = steiner_point<false>(pr, r, cx, cy);
+    stp[2 * (size_t)i] = cx;
+    stp[2 * (size_t)i + 1] = cy;
+  }
 }
 
 // Learned priority: cprio = (bucket >> shift) * 256 + (255 - quantised
@@ -2263,6 +2303,16 @@ void Refiner::run() {
   else if (opt.mixed) loop_mixed();
   else loop_host();
   st.final_cav_cap = K;
+  // given-up elements still in the mesh: a run that converged otherwise
+  // reports them as status 3, not as converged
+  HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
+  k_count_left<<<std::min(nblk(std::max(nt, ns)), 2048), 256, 0, stream>>>(m.tflag, nt, m.senc, ns,
+                                                                           d_cnt);
+  HC(hipGetLastError());
+  read_counters();
+  st.giveup_tris_left = (long long)h_cnt[CNT_LEFT_TRI];
+  st.giveup_segs_left = (long long)h_cnt[CNT_LEFT_SEG];
+  if (st.status == 0 && (st.giveup_tris_left > 0 || st.giveup_segs_left > 0)) st.status = 3;
   // --- export --------------------------------------------------------------
   if (nt > newid_cap) {
     dfree(newid);
@@ -2728,6 +2778,54 @@ int Refiner::kernel_times(gcdt_kernel_time *rec, int max_rec) const {
     k++;
   }
   return k;
+}
+
+ParamsD construction_params(double theta_deg, int offcenter) {
+  ParamsD pr{};
+  double ga = std::cos(theta_deg * 3.141592653589793238462643383279502884197169399375105820974944592308 / 180.0);
+  pr.offconst = (ga == 1.0) ? 0.0 : 0.475 * std::sqrt((1.0 + ga) / (1.0 - ga));
+  pr.goodangle = ga * ga;
+  pr.lens = (2.0 * pr.goodangle - 1.0) * (2.0 * pr.goodangle - 1.0);
+  pr.offcenter = offcenter;
+  pr.ruppert = 1;
+  return pr;
+}
+
+void device_constructions(const double *tri6, int n, double theta_deg, int offcenter,
+                          double *ccw, double *stp, int *ok) {
+  if (n <= 0) return;
+  double *dq, *dccw, *dstp;
+  int *dok;
+  dalloc(&dq, 6 * (size_t)n);
+  dalloc(&dccw, (size_t)n);
+  dalloc(&dstp, 2 * (size_t)n);
+  dalloc(&dok, (size_t)n);
+  HC(hipMemcpy(dq, tri6, sizeof(double) * 6 * n, hipMemcpyHostToDevice));
+  k_constructions<<<std::min(nblk(n), 256), 256>>>(dq, n, construction_params(theta_deg, offcenter),
+                                                   dccw, dstp, dok);
+  HC(hipGetLastError());
+  HC(hipDeviceSynchronize());
+  HC(hipMemcpy(ccw, dccw, sizeof(double) * n, hipMemcpyDeviceToHost));
+  HC(hipMemcpy(stp, dstp, sizeof(double) * 2 * n, hipMemcpyDeviceToHost));
+  HC(hipMemcpy(ok, dok, sizeof(int) * n, hipMemcpyDeviceToHost));
+  dfree(dq); dfree(dccw); dfree(dstp); dfree(dok);
+}
+
+void host_constructions(const double *tri6, int n, double theta_deg, int offcenter, double *ccw,
+                        double *stp, int *ok) {
+  const ParamsD pr = construction_params(theta_deg, offcenter);
+  for (int i = 0; i < n; i++) {
+    const double *p = tri6 + 6 * (size_t)i;
+    ccw[i] = orient2d_value(p[0], p[1], p[2], p[3], p[4], p[5]);
+    TriRec r;
+    r.X[0] = make_double2(p[0], p[1]);
+    r.X[1] = make_double2(p[2], p[3]);
+    r.X[2] = make_double2(p[4], p[5]);
+    double cx = 0.0, cy = 0.0;
+    ok[i] = steiner_point<false>(pr, r, cx, cy);
+    stp[2 * (size_t)i] = cx;
+    stp[2 * (size_t)i + 1] = cy;
+  }
 }
 
 void device_predicates(const double *pts8, int n, int pred_mode, int *o,

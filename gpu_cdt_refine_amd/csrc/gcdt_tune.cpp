@@ -71,6 +71,8 @@ int autotune(gcdt_options &opt, int reps,
       if (v == keep) continue;
       gcdt_options o = best;
       o.*(k.field) = v;
+      std::string why;
+      if (!options_ok(&o, why)) continue;  // e.g. too many claim epochs per graph launch
       double ms = median_ms(o, reps, run);
       log(o, ms);
       if (ms < best_ms) {

@@ -231,15 +231,108 @@ GCDT_HD double orient2d(double ax, double ay, double bx, double by, double cx,
   return orient2d_exact(ax, ay, bx, by, cx, cy);
 }
 
+// The value (not only the sign) of an adaptive orient2d, stage by stage as
+// in Shewchuk's orient2d / orient2dadapt (CMU-CS-96-140, section 4), which is
+// what Triangle's counterclockwise() returns (triangle.cpp:5155-5291) and
+// what its findcircumcenter divides by (triangle.cpp:6556-6563).  Stage A:
+// the FP64 determinant when its filter certifies it; B: the estimate of the
+// exact expansion of the rounded differences' products; C: that estimate
+// corrected by the differences' tails; D: the largest component of the exact
+// expansion.  The filter part is inline; the rest is out of line (its 40
+// doubles of expansions stay off the hot kernels' register budget).
+#define GCDT_CCW_BOUND_B ((2.0 + 12.0 * GCDT_EPS) * GCDT_EPS)
+#define GCDT_CCW_BOUND_C ((9.0 + 64.0 * GCDT_EPS) * GCDT_EPS * GCDT_EPS)
+#define GCDT_RESULT_ERR ((3.0 + 8.0 * GCDT_EPS) * GCDT_EPS)
+
+// (a1*b1 - a2*b2) as a 4-component expansion u[0..3] in increasing magnitude:
+// the two exact products (x1 + x0) - (y1 + y0), subtracting y0 then y1, each
+// by a difference and a two-sum (Shewchuk's Two-Two-Diff).
+GCDT_HD void prod_diff_expansion(double a1, double b1, double a2, double b2, double *u) {
+  double x1, x0, y1, y0, j, z, d, d2, top;
+  two_product(a1, b1, x1, x0);
+  two_product(a2, b2, y1, y0);
+  two_diff(x0, y0, d, u[0]);
+  two_sum(x1, d, j, z);
+  two_diff(z, y1, d2, u[1]);
+  two_sum(j, d2, top, u[2]);
+  u[3] = top;
+}
+
+GCDT_HD_NOINLINE double orient2d_adapt_value(double ax, double ay, double bx, double by,
+                                             double cx, double cy, double detsum) {
+  const double acx = ax - cx, bcx = bx - cx, acy = ay - cy, bcy = by - cy;
+  double B[4], C1[8], C2[12], D[16], u[4];
+  prod_diff_expansion(acx, bcy, acy, bcx, B);
+  double det = ((B[0] + B[1]) + B[2]) + B[3];
+  double err = GCDT_CCW_BOUND_B * detsum;
+  if (det >= err || -det >= err) return det;
+  double t, acxt, bcxt, acyt, bcyt;
+  two_diff(ax, cx, t, acxt);
+  two_diff(bx, cx, t, bcxt);
+  two_diff(ay, cy, t, acyt);
+  two_diff(by, cy, t, bcyt);
+  if (acxt == 0.0 && acyt == 0.0 && bcxt == 0.0 && bcyt == 0.0) return det;
+  err = GCDT_CCW_BOUND_C * detsum + GCDT_RESULT_ERR * fabs(det);
+  det += (acx * bcyt + bcy * acxt) - (acy * bcxt + bcx * acyt);
+  if (det >= err || -det >= err) return det;
+  prod_diff_expansion(acxt, bcy, acyt, bcx, u);
+  int n1 = fast_expansion_sum_zeroelim(4, B, 4, u, C1);
+  prod_diff_expansion(acx, bcyt, acy, bcxt, u);
+  int n2 = fast_expansion_sum_zeroelim(n1, C1, 4, u, C2);
+  prod_diff_expansion(acxt, bcyt, acyt, bcxt, u);
+  int n3 = fast_expansion_sum_zeroelim(n2, C2, 4, u, D);
+  return D[n3 - 1];
+}
+
+// Stage A of the value: true when the FP64 determinant is returned as is
+// (*det), false when the adaptive stages must run (*detsum is their input).
+GCDT_HD bool orient2d_value_filter(double ax, double ay, double bx, double by, double cx,
+                                   double cy, double *det, double *detsum) {
+  const double l = (ax - cx) * (by - cy), r = (ay - cy) * (bx - cx);
+  *det = l - r;
+  if (l > 0.0) {
+    if (r <= 0.0) return true;
+    *detsum = l + r;
+  } else if (l < 0.0) {
+    if (r >= 0.0) return true;
+    *detsum = -l - r;
+  } else {
+    return true;
+  }
+  const double err = GCDT_CCW_BOUND_A * *detsum;
+  return *det >= err || -*det >= err;
+}
+
+GCDT_HD double orient2d_value(double ax, double ay, double bx, double by, double cx,
+                              double cy) {
+  double det, detsum = 0.0;
+  if (orient2d_value_filter(ax, ay, bx, by, cx, cy, &det, &detsum)) return det;
+  return orient2d_adapt_value(ax, ay, bx, by, cx, cy, detsum);
+}
+
+// Range of the FP32 stages: every coordinate difference must be 0 or have a
+// magnitude in [2^-lo, 2^hi], so that no FP32 difference, product or degree-4
+// term is subnormal or infinite and the relative error bounds below hold.
+// (An FP32 sum that cancels into the subnormal range is exact.)
+GCDT_HD bool fp32_in_range(float x, float lo, float hi) {
+  const float a = fabsf(x);
+  return a == 0.0f || (a >= lo && a <= hi);
+}
+
 // FP32 semi-static stage for orient2d.  Returns +1/-1 when certified, 0 when
 // undecided.  The differences are formed in FP64 (relative error <= u64) and
 // rounded to FP32 (relative error <= u32 = 2^-24 each); products and the final
 // difference add three FP32 roundings.  The bound below covers all of them
 // with margin: |det - det_f32| <= (4*u32 + 8*u64 + O(u32^2)) * (|l| + |r|).
+// Differences outside [2^-60, 2^60] defer (products stay normal).
 GCDT_HD int orient2d_fp32_sign(double ax, double ay, double bx, double by,
                                double cx, double cy) {
   float acx = (float)(ax - cx), bcy = (float)(by - cy);
   float acy = (float)(ay - cy), bcx = (float)(bx - cx);
+  const float lo = 0x1p-60f, hi = 0x1p60f;
+  if (!(fp32_in_range(acx, lo, hi) && fp32_in_range(bcy, lo, hi) &&
+        fp32_in_range(acy, lo, hi) && fp32_in_range(bcx, lo, hi)))
+    return 0;
   float l = acx * bcy, r = acy * bcx;
   float det = l - r;
   float bound = 2.4e-7f * (fabsf(l) + fabsf(r));  // 4.03 * 2^-24 rounded up
@@ -387,10 +480,17 @@ GCDT_HD double incircle_fp64(double ax, double ay, double bx, double by,
 // the six differences carries relative error <= u32 + u64 after rounding to
 // FP32; the degree-4 evaluation adds at most 7 FP32 roundings per term.  The
 // bound 1.2e-6 * permanent (about 20 * 2^-24) covers both with margin.
+// Differences outside [2^-30, 2^30] defer: degree-4 terms then stay within
+// [2^-120, 2^120], inside FP32's normal range.
 GCDT_HD int incircle_fp32_sign(double ax, double ay, double bx, double by,
                                double cx, double cy, double dx, double dy) {
   float adx = (float)(ax - dx), bdx = (float)(bx - dx), cdx = (float)(cx - dx);
   float ady = (float)(ay - dy), bdy = (float)(by - dy), cdy = (float)(cy - dy);
+  const float lo = 0x1p-30f, hi = 0x1p30f;
+  if (!(fp32_in_range(adx, lo, hi) && fp32_in_range(bdx, lo, hi) &&
+        fp32_in_range(cdx, lo, hi) && fp32_in_range(ady, lo, hi) &&
+        fp32_in_range(bdy, lo, hi) && fp32_in_range(cdy, lo, hi)))
+    return 0;
   float bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
   float alift = adx * adx + ady * ady;
   float cdxady = cdx * ady, adxcdy = adx * cdy;

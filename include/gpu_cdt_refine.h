@@ -176,7 +176,13 @@ typedef struct gcdt_stats {
   long long inserted, seg_splits, rej_encroach, rej_blocked, rej_overflow,
       lost, giveups;
   int final_cav_cap;
-  int status;            /* 0 converged, 1 round cap, 2 stalled */
+  int status;            /* 0 converged: no bad triangle and no encroached
+                            subsegment is left; 1 round cap; 2 stalled (no
+                            progress possible); 3 converged except for
+                            given-up elements (giveup_tris_left triangles
+                            below theta, giveup_segs_left encroached
+                            subsegments whose split point or cavity could not
+                            be formed in FP64) */
   double device_ms;      /* HIP-event time of the refinement on the device */
   long long pred_fp32, pred_fp64, pred_exact; /* predicate stage hits */
   long long n_points, n_triangles, n_subsegs;
@@ -188,6 +194,8 @@ typedef struct gcdt_stats {
                             tests), segment / bad-triangle candidates */
   long long ws_ints_seg, ws_ints_bad; /* ints written to candidate workspaces
                             (cavity + ring lists, ordered star boundary) */
+  long long giveup_tris_left, giveup_segs_left; /* given-up elements still in
+                            the refined mesh (see status 3) */
 } gcdt_stats;
 
 typedef struct gcdt_mesh gcdt_mesh;       /* host-side result */
@@ -384,6 +392,20 @@ int gcdt_host_predicates(const double *pts8, int n, int pred_mode,
 int gcdt_device_predicates(const double *pts8, int n, int pred_mode,
                            int *orient_out, int *incircle_out,
                            long long *stage_counts);
+
+/* The refinement's constructions, for tests (no reference counterpart as
+ * entry points): for n triangles tri6 = (p0, p1, p2), ccw_out[i] receives the
+ * value of Triangle's adaptive counterclockwise(p0, p1, p2)
+ * (triangle.cpp:5155-5291, what findcircumcenter divides by) and
+ * steiner_out[2i..2i+1] the circumcenter / off-center of the triangle with
+ * apex p0, org p1, dest p2 as the refinement computes it
+ * (triangle.cpp:6523-6618, refine.cu:3780-3868); ok_out[i] is 0 where the
+ * triangle is degenerate.  The host version runs the kernels' source on the
+ * CPU; the device version on HIP device 0. */
+int gcdt_host_constructions(const double *tri6, int n, double theta_deg, int offcenter,
+                            double *ccw_out, double *steiner_out, int *ok_out);
+int gcdt_device_constructions(const double *tri6, int n, double theta_deg, int offcenter,
+                              double *ccw_out, double *steiner_out, int *ok_out);
 
 #ifdef __cplusplus
 }

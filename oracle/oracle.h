@@ -72,8 +72,11 @@ typedef struct {
   long long inserted, seg_splits, rej_encroach, rej_blocked, rej_overflow,
       lost, giveups;
   int final_cav_cap;
-  int status; /* 0 ok, 1 round cap hit, 2 stalled */
+  int status; /* 0 ok, 1 round cap hit, 2 stalled, 3 converged except for
+                 given-up elements (counted below) */
   long long cand_evals; /* bad-pass candidates evaluated */
+  long long giveup_tris_left, giveup_segs_left; /* alive triangles still
+                 flagged given-up; subsegments whose split was given up */
 } orc_stats;
 
 typedef struct orc_result orc_result;
@@ -100,6 +103,11 @@ void orc_result_free(orc_result *r);
  * computes it; returns 0 if degenerate. */
 int orc_steiner_point(const double *apex, const double *org, const double *dest,
                       double theta_deg, int offcenter, double *out);
+
+/* Value of Triangle's adaptive orient2d counterclockwise(pa, pb, pc)
+ * (triangle.cpp:5155-5291): the sign is exact, the value is the first
+ * adaptive stage's that certifies it. */
+double orc_ccw_value(const double *pa, const double *pb, const double *pc);
 
 /* Invariant checker.  Returns 0 when every check passes; otherwise the
  * number of violations, with per-check counts in viol[] (see check_oracle.c

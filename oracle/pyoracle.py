@@ -57,7 +57,8 @@ class OrcStats(C.Structure):
                 ("rej_encroach", C.c_longlong), ("rej_blocked", C.c_longlong),
                 ("rej_overflow", C.c_longlong), ("lost", C.c_longlong),
                 ("giveups", C.c_longlong), ("final_cav_cap", C.c_int), ("status", C.c_int),
-                ("cand_evals", C.c_longlong)]
+                ("cand_evals", C.c_longlong), ("giveup_tris_left", C.c_longlong),
+                ("giveup_segs_left", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -94,6 +95,8 @@ def orc():
         L.orc_orient2d.argtypes = [vp, vp, vp]
         L.orc_incircle.argtypes = [vp, vp, vp, vp]
         L.orc_steiner_point.argtypes = [vp, vp, vp, C.c_double, C.c_int, vp]
+        L.orc_ccw_value.restype = C.c_double
+        L.orc_ccw_value.argtypes = [vp, vp, vp]
         _orc = L
     return _orc
 

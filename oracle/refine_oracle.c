@@ -194,8 +194,154 @@ static int tri_is_bad(const Mesh *m, int t) {
   return 1;
 }
 
+/* ---- the VALUE of Triangle's adaptive orient2d ----------------------------
+ * Triangle divides by counterclockwise(dest, apex, org) in findcircumcenter
+ * (triangle.cpp:6556-6563); the reference's computeCircumcenter falls back to
+ * cuda_ccw only when the raw determinant is 0 (refine.cu:3799-3804).  The
+ * value (not only its sign) depends on which adaptive stage returns, so this
+ * follows Shewchuk's orient2d / orient2dadapt stage by stage (CMU-CS-96-140,
+ * section 4; Triangle's copy at triangle.cpp:5155-5291): filter A on the FP64
+ * determinant, then B = the exact product expansion of the rounded
+ * differences (returns its estimate), then C = that estimate corrected by the
+ * difference tails, then D = the exact expansion, whose largest component is
+ * returned.  Products are split with Dekker's splitter here (the device code
+ * uses FMA; both give the exact error term). */
+static const double ORC_EPS = 1.1102230246251565e-16; /* 2^-53 */
+#define ORC_CCW_A ((3.0 + 16.0 * ORC_EPS) * ORC_EPS)
+#define ORC_CCW_B ((2.0 + 12.0 * ORC_EPS) * ORC_EPS)
+#define ORC_CCW_C ((9.0 + 64.0 * ORC_EPS) * ORC_EPS * ORC_EPS)
+#define ORC_RESULT_ERR ((3.0 + 8.0 * ORC_EPS) * ORC_EPS)
+
+static void orc_two_sum(double a, double b, double *x, double *y) {
+  double s = a + b, bv = s - a, av = s - bv;
+  *x = s;
+  *y = (a - av) + (b - bv);
+}
+/* tail of x = fl(a - b) */
+static double orc_diff_tail(double a, double b, double x) {
+  double bv = a - x, av = x + bv;
+  return (a - av) + (bv - b);
+}
+static void orc_split(double a, double *hi, double *lo) {
+  double c = 134217729.0 * a; /* 2^27 + 1 */
+  double big = c - a;
+  *hi = c - big;
+  *lo = a - *hi;
+}
+static void orc_two_product(double a, double b, double *x, double *y) {
+  double p = a * b, ah, al, bh, bl;
+  orc_split(a, &ah, &al);
+  orc_split(b, &bh, &bl);
+  double e1 = p - ah * bh, e2 = e1 - al * bh, e3 = e2 - ah * bl;
+  *x = p;
+  *y = al * bl - e3;
+}
+/* (a1 + a0) - (b1 + b0) as a 4-component expansion x[0..3] (increasing):
+ * subtract b0 from (a1, a0), then b1 from the result, each by a
+ * one-component difference followed by a two-sum. */
+static void orc_two_two_diff(double a1, double a0, double b1, double b0, double *x) {
+  double i, j, z;
+  double d = a0 - b0;
+  x[0] = orc_diff_tail(a0, b0, d);
+  orc_two_sum(a1, d, &j, &z);
+  double d2 = z - b1;
+  x[1] = orc_diff_tail(z, b1, d2);
+  orc_two_sum(j, d2, &i, &x[2]);
+  x[3] = i;
+}
+/* Fast-Expansion-Sum with zero elimination (Shewchuk's Figure 6 variant). */
+static int orc_fast_expansion_sum(int elen, const double *e, int flen, const double *f,
+                                  double *h) {
+  double Q, Qnew, hh, enow = e[0], fnow = f[0];
+  int ei = 0, fi = 0, hi = 0;
+  if ((fnow > enow) == (fnow > -enow)) { Q = enow; enow = ++ei < elen ? e[ei] : 0.0; }
+  else { Q = fnow; fnow = ++fi < flen ? f[fi] : 0.0; }
+  if (ei < elen && fi < flen) {
+    if ((fnow > enow) == (fnow > -enow)) {
+      Qnew = enow + Q; hh = Q - (Qnew - enow); enow = ++ei < elen ? e[ei] : 0.0;
+    } else {
+      Qnew = fnow + Q; hh = Q - (Qnew - fnow); fnow = ++fi < flen ? f[fi] : 0.0;
+    }
+    Q = Qnew;
+    if (hh != 0.0) h[hi++] = hh;
+    while (ei < elen && fi < flen) {
+      if ((fnow > enow) == (fnow > -enow)) {
+        orc_two_sum(Q, enow, &Qnew, &hh); enow = ++ei < elen ? e[ei] : 0.0;
+      } else {
+        orc_two_sum(Q, fnow, &Qnew, &hh); fnow = ++fi < flen ? f[fi] : 0.0;
+      }
+      Q = Qnew;
+      if (hh != 0.0) h[hi++] = hh;
+    }
+  }
+  while (ei < elen) {
+    orc_two_sum(Q, enow, &Qnew, &hh); enow = ++ei < elen ? e[ei] : 0.0;
+    Q = Qnew;
+    if (hh != 0.0) h[hi++] = hh;
+  }
+  while (fi < flen) {
+    orc_two_sum(Q, fnow, &Qnew, &hh); fnow = ++fi < flen ? f[fi] : 0.0;
+    Q = Qnew;
+    if (hh != 0.0) h[hi++] = hh;
+  }
+  if (Q != 0.0 || hi == 0) h[hi++] = Q;
+  return hi;
+}
+/* (a1*b1 - a2*b2) of doubles as the 4-component expansion of the two exact
+ * products' difference */
+static void orc_prod_diff(double a1, double b1, double a2, double b2, double *u) {
+  double s1, s0, t1, t0;
+  orc_two_product(a1, b1, &s1, &s0);
+  orc_two_product(a2, b2, &t1, &t0);
+  orc_two_two_diff(s1, s0, t1, t0, u);
+}
+static double orc_ccw_adapt(const double *pa, const double *pb, const double *pc,
+                            double detsum) {
+  double acx = pa[0] - pc[0], bcx = pb[0] - pc[0];
+  double acy = pa[1] - pc[1], bcy = pb[1] - pc[1];
+  double B[4], C1[8], C2[12], D[16], u[4];
+  orc_prod_diff(acx, bcy, acy, bcx, B);
+  double det = B[0] + B[1] + B[2] + B[3];
+  double err = ORC_CCW_B * detsum;
+  if (det >= err || -det >= err) return det;
+  double acxt = orc_diff_tail(pa[0], pc[0], acx), bcxt = orc_diff_tail(pb[0], pc[0], bcx);
+  double acyt = orc_diff_tail(pa[1], pc[1], acy), bcyt = orc_diff_tail(pb[1], pc[1], bcy);
+  if (acxt == 0.0 && acyt == 0.0 && bcxt == 0.0 && bcyt == 0.0) return det;
+  err = ORC_CCW_C * detsum + ORC_RESULT_ERR * fabs(det);
+  det += (acx * bcyt + bcy * acxt) - (acy * bcxt + bcx * acyt);
+  if (det >= err || -det >= err) return det;
+  orc_prod_diff(acxt, bcy, acyt, bcx, u);
+  int n1 = orc_fast_expansion_sum(4, B, 4, u, C1);
+  orc_prod_diff(acx, bcyt, acy, bcxt, u);
+  int n2 = orc_fast_expansion_sum(n1, C1, 4, u, C2);
+  orc_prod_diff(acxt, bcyt, acyt, bcxt, u);
+  int n3 = orc_fast_expansion_sum(n2, C2, 4, u, D);
+  return D[n3 - 1];
+}
+/* counterclockwise(pa, pb, pc) of Triangle, value included. */
+double orc_ccw_value(const double *pa, const double *pb, const double *pc) {
+  double dl = (pa[0] - pc[0]) * (pb[1] - pc[1]);
+  double dr = (pa[1] - pc[1]) * (pb[0] - pc[0]);
+  double det = dl - dr, detsum;
+  if (dl > 0.0) {
+    if (dr <= 0.0) return det;
+    detsum = dl + dr;
+  } else if (dl < 0.0) {
+    if (dr >= 0.0) return det;
+    detsum = -dl - dr;
+  } else {
+    return det;
+  }
+  double err = ORC_CCW_A * detsum;
+  if (det >= err || -det >= err) return det;
+  return orc_ccw_adapt(pa, pb, pc, detsum);
+}
+
 /* findcircumcenter / computeCircumcenter: triangle.cpp:6523-6618,
- * refine.cu:3780-3868.  Returns 0 if the triangle is numerically degenerate. */
+ * refine.cu:3780-3868.  The denominator is Triangle's adaptive
+ * counterclockwise(dest, apex, org) (its sign is exact and its value
+ * accurate for near-collinear triangles, where the raw determinant loses
+ * every digit).  Returns 0 if the triangle is degenerate (collinear). */
 static int steiner_point(const Mesh *m, int t, double *c) {
   const int *v = m->TV + 3 * (size_t)t;
   const double *A = PT(m, v[0]), *O = PT(m, v[1]), *D = PT(m, v[2]);
@@ -204,7 +350,7 @@ static int steiner_point(const Mesh *m, int t, double *c) {
   double dodist = xdo * xdo + ydo * ydo;
   double aodist = xao * xao + yao * yao;
   double dadist = (D[0] - A[0]) * (D[0] - A[0]) + (D[1] - A[1]) * (D[1] - A[1]);
-  double det = xdo * yao - xao * ydo;
+  double det = orc_ccw_value(D, A, O);
   if (det == 0.0) return 0;
   double denominator = 0.5 / det;
   double dx = (yao * dodist - ydo * aodist) * denominator;
@@ -1038,6 +1184,11 @@ static orc_result *refine_impl(int np, const double *points, int nt,
   }
   st->final_cav_cap = K;
   free(ids);
+  /* given-up elements still in the mesh: status 3 instead of converged */
+  for (int t = 0; t < m->nt; t++)
+    st->giveup_tris_left += (m->tflag[t] & (TF_ALIVE | TF_GIVEUP)) == (TF_ALIVE | TF_GIVEUP);
+  for (int s = 0; s < m->ns; s++) st->giveup_segs_left += m->senc[s] == 2;
+  if (st->status == 0 && (st->giveup_tris_left > 0 || st->giveup_segs_left > 0)) st->status = 3;
 
   /* export: live triangles in slot order */
   orc_result *r = calloc(1, sizeof(orc_result));

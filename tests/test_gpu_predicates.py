@@ -32,3 +32,30 @@ def test_fp32_filter_gives_identical_mesh(gpu, oracle):
     assert np.array_equal(a.points.view(np.uint64), b.points.view(np.uint64))
     assert np.array_equal(a.triangles, b.triangles)
     assert b.stats["pred_fp32"] > 0
+
+
+def test_device_fp32_stage_exact_at_tiny_scales(gpu):
+    from test_predicates_cpu import tiny_scale_queries
+    rng = np.random.default_rng(99)
+    q = tiny_scale_queries(rng, 600)
+    o, ic, stages = gpu.device_predicates(q, pred_mode=1)
+    exp_o = np.array([exact_orient(r[0:2], r[2:4], r[4:6]) for r in q])
+    exp_i = np.array([exact_incircle(r[0:2], r[2:4], r[4:6], r[6:8]) for r in q])
+    assert np.array_equal(o, exp_o)
+    assert np.array_equal(ic, exp_i)
+
+
+def test_device_constructions_equal_host(gpu):
+    """The adaptive orient2d value and the Steiner point on the device equal
+    the same source on the host bit for bit (which the CPU tests pin against
+    the reference's Triangle)."""
+    from test_predicates_cpu import near_collinear_triangles
+    rng = np.random.default_rng(5)
+    tris = near_collinear_triangles(rng, 3000)
+    for theta, off in ((25.0, 1), (25.0, 0), (30.0, 1)):
+        hc, hs, hok = gpu.host_constructions(tris, theta, off)
+        dc, ds, dok = gpu.device_constructions(tris, theta, off)
+        assert np.array_equal(hc.view(np.uint64), dc.view(np.uint64))
+        assert np.array_equal(hok, dok)
+        m = hok != 0
+        assert np.array_equal(hs[m].view(np.uint64), ds[m].view(np.uint64))

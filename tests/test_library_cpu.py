@@ -31,3 +31,19 @@ def test_library_loads_without_device():
     o = g.make_options(theta=30.0)
     assert o.theta_deg == 30.0 and o.cand_factor == 4 and o.claim_iters == 1
     assert L.gcdt_last_error() == b""
+
+
+def test_options_check_refuses_claim_epoch_overflow():
+    """One graph launch may use at most 2047 claim epochs (11 bits of the
+    claim key); the options check refuses settings beyond that before any
+    device work."""
+    import ctypes as C
+    L = g.lib()
+    out = C.c_void_p()
+    bad = g.make_options(graph=1, graph_rounds=64, graph_seg_passes=16, claim_iters=2)
+    rc = L.gcdt_refine_host(None, 0, None, None, 0, None, 0, C.byref(bad), C.byref(out), None)
+    assert rc == -1 and b"claim epochs" in L.gcdt_last_error()
+    # 64 * (15 + 1) * (1 + 1) + 2 = 2050 epochs: just over the limit
+    edge = g.make_options(graph=1, graph_rounds=64, graph_seg_passes=15, claim_iters=1)
+    rc = L.gcdt_refine_host(None, 0, None, None, 0, None, 0, C.byref(edge), C.byref(out), None)
+    assert rc == -1 and b"claim epochs" in L.gcdt_last_error()

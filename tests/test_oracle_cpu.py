@@ -106,6 +106,99 @@ def test_steiner_point_matches_triangle_findcircumcenter(oracle):
     assert n_off > 0
 
 
+def _near_collinear(rng, k):
+    """Triangles whose apex sits within 1e-6 .. 1e-15 (relative) of the
+    line through org and dest, at several coordinate scales; some exactly
+    collinear, some on integer coordinates (exact differences)."""
+    scale = 10.0 ** rng.integers(-6, 7)
+    a = rng.uniform(0, 1, 2) * scale
+    b = a + rng.uniform(-1, 1, 2) * scale
+    f = rng.uniform(0.05, 0.95)
+    c = a + f * (b - a)
+    if k % 5 == 0:
+        pass  # the FP64 point nearest the line: often exactly collinear
+    elif k % 5 == 1:
+        a, b = np.round(a), np.round(b)
+        c = np.round(a + f * (b - a))
+    else:
+        d = np.array([-(b - a)[1], (b - a)[0]])
+        c = c + d * 10.0 ** rng.uniform(-15, -6)
+    return np.array([c, a, b])
+
+
+def test_ccw_value_matches_triangle_counterclockwise(oracle):
+    """The adaptive orient2d value the circumcenter divides by, bit-exact
+    against the reference's Triangle counterclockwise() on near-collinear
+    triples (every adaptive stage reached), and sign-exact against rationals."""
+    if not os.path.exists(oracle.TRIREF_SO):
+        pytest.skip("reference Triangle not built")
+    from fractions import Fraction as F
+    L = oracle.orc()
+    R = oracle.triref()
+    rng = np.random.default_rng(17)
+    nonfilter = 0
+    for k in range(4000):
+        p = _near_collinear(rng, k)
+        pa, pb, pc = (np.ascontiguousarray(x) for x in p)
+        v = L.orc_ccw_value(oracle._p(pa), oracle._p(pb), oracle._p(pc))
+        r = R.triref_orient2d(oracle._p(pa), oracle._p(pb), oracle._p(pc))
+        assert np.float64(v).view(np.uint64) == np.float64(r).view(np.uint64), (p, v, r)
+        ex = ((F(pa[0]) - F(pc[0])) * (F(pb[1]) - F(pc[1])) -
+              (F(pa[1]) - F(pc[1])) * (F(pb[0]) - F(pc[0])))
+        assert (v > 0) == (ex > 0) and (v < 0) == (ex < 0), (p, v, ex)
+        det = (pa[0] - pc[0]) * (pb[1] - pc[1]) - (pa[1] - pc[1]) * (pb[0] - pc[0])
+        nonfilter += v != det
+    assert nonfilter > 100  # the adaptive stages were exercised
+
+
+def test_steiner_point_near_collinear_matches_triangle(oracle):
+    """findcircumcenter on near-collinear triangles (where the raw FP64
+    determinant loses its digits and Triangle's adaptive denominator takes
+    over): bit-exact against the reference's Triangle."""
+    if not os.path.exists(oracle.TRIREF_SO):
+        pytest.skip("reference Triangle not built")
+    import ctypes as C
+    L = oracle.orc()
+    R = oracle.triref()
+    rng = np.random.default_rng(23)
+    checked = 0
+    for k in range(6000):
+        p = _near_collinear(rng, k)
+        if oracle.orient2d(p[0], p[1], p[2]) < 0:
+            p[[1, 2]] = p[[2, 1]]
+        apex, org, dest = (np.ascontiguousarray(x) for x in p)
+        for theta, off in ((25.0, 1), (25.0, 0)):
+            out = np.zeros(2)
+            ok = L.orc_steiner_point(oracle._p(apex), oracle._p(org), oracle._p(dest),
+                                     C.c_double(theta), off, oracle._p(out))
+            if not ok:
+                continue
+            ref = np.zeros(2)
+            R.triref_findcircumcenter(oracle._p(org), oracle._p(dest), oracle._p(apex),
+                                      C.c_double(theta), off, oracle._p(ref))
+            assert np.array_equal(out.view(np.uint64), ref.view(np.uint64)), (p, theta, off, out, ref)
+            checked += 1
+    assert checked > 4000
+
+
+def test_oracle_give_ups_are_reported(oracle):
+    """The precision-stress family (jittered lattice, cocircular groups,
+    lattice-line segments) makes the refinement give up on a few elements;
+    the run then ends with status 3 and counts them, and every quality
+    violation the checker finds is one of those given-up triangles."""
+    import gpu_cdt_refine_amd as g
+    P, S = g.generate_pslg(20000, 400, L=1000.0 * 0.02 ** 0.5, maxlen=20.0, dist=2,
+                           min_angle=5.0, seed=2)
+    T, N, SS = g.build_cdt(P, S)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, SS, theta=25.0)
+    assert st["giveups"] > 0
+    left = st["giveup_tris_left"] + st["giveup_segs_left"]
+    assert st["status"] == (3 if left else 0), st
+    tot, viol, mina, ex = oracle.check_mesh(Po, To, No, So, P, SS, 25.0)
+    assert viol["quality"] <= st["giveup_tris_left"], (viol, st)
+    assert tot == viol["quality"], viol
+
+
 def test_small_input_angles_terminate(oracle):
     """Two segments meeting at 10 degrees: concentric shells + MPW keep the
     refinement finite; the only sub-theta triangles are MPW-exempt."""
