@@ -388,7 +388,8 @@ def bench_grid(args):
                          "rounds": int(st["rounds"]), "status": int(st["status"])}
                     if name == "chosen":
                         c = s.check()
-                        r["check_violations"] = int(sum(v for k, v in c.items() if k != "exact_signs"))
+                        r["check_violations"] = int(g.violations(c))
+                        r["min_angle"] = c["min_angle"]
                     s.close()
                     cell[name] = r
                 cell["speedup"] = cell["plain"]["ms_per_step"] / cell["chosen"]["ms_per_step"]
@@ -530,14 +531,21 @@ def bench_batch(args):
         checked.append({"instance": i, "violations": int(tot), "min_angle": mina})
     out["checked_instances"] = checked
     # every instance on the GPU checker (index range, symmetry, orientation,
-    # boundary, exact Delaunay, subsegment edges, moved points, multiplicity)
+    # boundary, exact Delaunay, subsegment edges, segment coverage, moved
+    # points, quality with the MPW exemption at the instance's theta,
+    # multiplicity)
     t1 = time.time()
     tot = {}
+    min_margin = None
     for i, (P, T, N, SS) in enumerate(ins):
         r = res[i]
-        c = g.check_mesh(r.points, r.triangles, r.neighbors, r.segments, points_in=P)
+        c = g.check_mesh(r.points, r.triangles, r.neighbors, r.segments, points_in=P,
+                         segments_in=SS, theta=float(theta[i]))
+        margin = c.pop("min_angle") - float(theta[i])
+        min_margin = margin if min_margin is None else min(min_margin, margin)
         for k, v in c.items():
             tot[k] = tot.get(k, 0) + v
+    tot["min_angle_minus_theta"] = min_margin
     tot["instances"] = len(ins)
     tot["seconds"] = time.time() - t1
     out["gpu_check_all_instances"] = tot

@@ -159,10 +159,13 @@ def lib():
         L.gcdt_file_mesh_copy.argtypes = [vp, vp, vp, vp, vp]
         L.gcdt_file_mesh_free.argtypes = [vp]
         L.gcdt_io_last_error.restype = C.c_char_p
-        L.gcdt_session_check.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.gcdt_session_check.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                         C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
         L.gcdt_session_check.restype = C.c_int
         L.gcdt_check_mesh.argtypes = [vp, C.c_int, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int,
-                                      C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+                                      vp, C.c_int, C.c_double, C.POINTER(C.c_longlong),
+                                      C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                      C.POINTER(C.c_double)]
         L.gcdt_check_mesh.restype = C.c_int
         L.gcdt_last_error.restype = C.c_char_p
         L.gcdt_device_predicates.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
@@ -431,32 +434,57 @@ def save_triangle_files(basename, points, triangles, neighbors, segments):
 
 CHECK_NAMES = ["range", "symmetry", "orientation", "boundary", "delaunay", "subsegment",
                "coverage", "moved", "quality", "manifold"]
-DEVICE_CHECKS = ["range", "symmetry", "orientation", "boundary", "delaunay", "subsegment",
-                 "moved", "manifold"]
+# categories that need the input segments (and theta)
+SEGMENT_CHECKS = ["coverage", "quality"]
 
 
-def _check_dict(v):
-    return {name: int(v[i]) for i, name in enumerate(CHECK_NAMES) if name in DEVICE_CHECKS}
+def _check_dict(v, with_segments):
+    return {name: int(v[i]) for i, name in enumerate(CHECK_NAMES)
+            if with_segments or name not in SEGMENT_CHECKS}
 
 
-def check_mesh(points, triangles, neighbors, segments, points_in=None):
+def check_mesh(points, triangles, neighbors, segments, points_in=None, segments_in=None,
+               theta=None):
     """Invariant checks on the GPU with exact predicates (gcdt_check_mesh):
     violation counts per category -- index range, neighbour symmetry,
     orientation, boundary edges that are not subsegments, Delaunay across
     non-subsegment edges, subsegments that are not edges, input points moved
-    or missing, edges in three or more triangles -- plus "exact_signs", the
-    number of incircle signs left to the exact stage (not a violation)."""
+    or missing, edges in three or more triangles, and, when the input
+    segments and theta are given, input segments not covered by a chain of
+    subsegments and triangles below theta outside the Miller-Pav-Walkington
+    exemption.  Also "exact_signs" (incircle signs left to the exact stage),
+    "exempt" and "min_angle" (none of them violations)."""
     P, T, N, S = _as_inputs(points, triangles, neighbors, segments)
     Pin = np.zeros((0, 2)) if points_in is None else np.ascontiguousarray(points_in, np.float64).reshape(-1, 2)
+    with_seg = segments_in is not None
+    if with_seg and theta is None:
+        raise ValueError("the quality check needs theta")
+    Sin = (np.ascontiguousarray(segments_in, np.int32).reshape(-1, 2) if with_seg
+           else np.zeros((0, 2), np.int32))
     v = (C.c_longlong * 10)()
     ex = C.c_longlong(0)
+    nex = C.c_longlong(0)
+    mina = C.c_double(0.0)
     rc = lib().gcdt_check_mesh(_ptr(P), len(P), _ptr(T), _ptr(N), len(T), _ptr(S), len(S),
-                               _ptr(Pin), len(Pin), v, C.byref(ex))
+                               _ptr(Pin), len(Pin), _ptr(Sin) if with_seg else None, len(Sin),
+                               float(theta or 0.0), v, C.byref(ex), C.byref(nex), C.byref(mina))
     if rc != 0:
         _err(rc, "gcdt_check_mesh")
-    d = _check_dict(v)
+    d = _check_dict(v, with_seg)
     d["exact_signs"] = ex.value  # signs the filters left to the exact stage
+    if with_seg:
+        d["exempt"] = nex.value
+        d["min_angle"] = mina.value
     return d
+
+
+NON_VIOLATIONS = ("exact_signs", "exempt", "min_angle")
+
+
+def violations(d):
+    """Total of the violation categories of a check_mesh / Session.check
+    result (exact_signs, exempt and min_angle are not violations)."""
+    return sum(v for k, v in d.items() if k not in NON_VIOLATIONS)
 
 
 class Session:
@@ -515,14 +543,19 @@ class Session:
 
     def check(self):
         """Invariant counts of the last refined mesh, checked in HBM
-        (gcdt_session_check); see check_mesh."""
+        (gcdt_session_check) against the session's input segments and theta;
+        see check_mesh."""
         v = (C.c_longlong * 10)()
         ex = C.c_longlong(0)
-        rc = lib().gcdt_session_check(self.h, v, C.byref(ex))
+        nex = C.c_longlong(0)
+        mina = C.c_double(0.0)
+        rc = lib().gcdt_session_check(self.h, v, C.byref(ex), C.byref(nex), C.byref(mina))
         if rc != 0:
             _err(rc, "gcdt_session_check")
-        d = _check_dict(v)
+        d = _check_dict(v, True)
         d["exact_signs"] = ex.value
+        d["exempt"] = nex.value
+        d["min_angle"] = mina.value
         return d
 
     def kernel_times(self):

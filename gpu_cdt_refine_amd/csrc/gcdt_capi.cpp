@@ -189,12 +189,17 @@ int gcdt_session_run(gcdt_session *s, gcdt_stats *stats) {
   }
 }
 
-int gcdt_session_check(gcdt_session *s, long long *viol, long long *exact_signs) {
+int gcdt_session_check(gcdt_session *s, long long *viol, long long *exact_signs,
+                       long long *n_exempt, double *min_angle_deg) {
   g_err.clear();
   try {
     if (!s || !viol) throw std::runtime_error("session and viol must be non-NULL");
     if (s->r->input_count() != 1) throw std::runtime_error("session holds no single refined mesh");
-    gcdt::check_device_mesh(s->r->device_mesh(), s->r->stream, viol, exact_signs);
+    gcdt::CheckExtra ex;
+    gcdt::check_device_mesh(s->r->device_mesh(), s->r->stream, s->r->opt.theta_deg, viol,
+                            exact_signs, &ex);
+    if (n_exempt) *n_exempt = ex.n_exempt;
+    if (min_angle_deg) *min_angle_deg = ex.min_angle_deg;
     return 0;
   } catch (const std::exception &e) {
     g_err = e.what();
@@ -204,14 +209,17 @@ int gcdt_session_check(gcdt_session *s, long long *viol, long long *exact_signs)
 
 int gcdt_check_mesh(const double *points, int np, const int *tris, const int *nbrs, int nt,
                     const int *segs, int ns, const double *points_in, int np_in,
-                    long long *viol, long long *exact_signs) {
+                    const int *segs_in, int ns_in, double theta_deg, long long *viol,
+                    long long *exact_signs, long long *n_exempt, double *min_angle_deg) {
   g_err.clear();
-  void *d[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  void *d[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   hipStream_t stream = nullptr;
   try {
-    if (!viol || np < 0 || nt < 0 || ns < 0 || np_in < 0 || (np > 0 && !points) ||
+    if (!viol || np < 0 || nt < 0 || ns < 0 || np_in < 0 || ns_in < 0 || (np > 0 && !points) ||
         (nt > 0 && (!tris || !nbrs)) || (ns > 0 && !segs) || (np_in > 0 && !points_in))
       throw std::runtime_error("invalid mesh arrays");
+    if (segs_in && !(theta_deg > 0.0 && theta_deg < 60.0))
+      throw std::runtime_error("theta must lie in (0, 60) degrees for the quality check");
     auto up = [&](int k, const void *src, size_t bytes) {
       if (hipMalloc(&d[k], bytes ? bytes : 1) != hipSuccess) throw std::runtime_error("hipMalloc failed");
       if (bytes && hipMemcpy(d[k], src, bytes, hipMemcpyHostToDevice) != hipSuccess)
@@ -222,11 +230,16 @@ int gcdt_check_mesh(const double *points, int np, const int *tris, const int *nb
     up(2, nbrs, sizeof(int) * 3 * (size_t)nt);
     up(3, segs, sizeof(int) * 2 * (size_t)ns);
     up(4, points_in, sizeof(double) * 2 * (size_t)np_in);
+    if (segs_in) up(5, segs_in, sizeof(int) * 2 * (size_t)ns_in);
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
       throw std::runtime_error("stream creation failed");
     gcdt::Refiner::DeviceMesh dm{(const double2 *)d[0], np, (const int *)d[1], (const int *)d[2], nt,
-                                 (const int2 *)d[3], ns, (const double2 *)d[4], np_in};
-    gcdt::check_device_mesh(dm, stream, viol, exact_signs);
+                                 (const int2 *)d[3], ns, (const double2 *)d[4], np_in,
+                                 segs_in ? (const int2 *)d[5] : nullptr, segs_in ? ns_in : 0};
+    gcdt::CheckExtra ex;
+    gcdt::check_device_mesh(dm, stream, theta_deg, viol, exact_signs, &ex);
+    if (n_exempt) *n_exempt = ex.n_exempt;
+    if (min_angle_deg) *min_angle_deg = ex.min_angle_deg;
   } catch (const std::exception &e) {
     g_err = e.what();
     for (void *p : d) (void)hipFree(p);

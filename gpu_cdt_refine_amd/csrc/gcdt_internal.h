@@ -72,6 +72,8 @@ class Refiner {
     int ns;
     const double2 *P_in;
     int np_in;
+    const int2 *S_in;  // input segments (coverage and quality); may be null
+    int ns_in;
   };
   DeviceMesh device_mesh() const;
   void run();
@@ -213,9 +215,15 @@ bool options_ok(const gcdt_options *o, std::string &err);
 
 // Device invariant checks of a mesh in device memory (gcdt_check.hip);
 // viol[10] receives the per-category counts of gcdt_check_mesh, exact_signs
-// (if non-null) the number of signs left to the exact stage.
-void check_device_mesh(const Refiner::DeviceMesh &dm, hipStream_t stream, long long *viol,
-                       long long *exact_signs);
+// (if non-null) the number of signs left to the exact stage, extra (if
+// non-null) the MPW-exempt count and the smallest angle.  Coverage and
+// quality (categories 6 and 8) are checked when dm.S_in is set.
+struct CheckExtra {
+  long long n_exempt = 0;
+  double min_angle_deg = 180.0;
+};
+void check_device_mesh(const Refiner::DeviceMesh &dm, hipStream_t stream, double theta_deg,
+                       long long *viol, long long *exact_signs, CheckExtra *extra);
 
 }  // namespace gcdt
 

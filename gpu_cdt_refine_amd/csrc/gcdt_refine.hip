@@ -2720,6 +2720,8 @@ Refiner::DeviceMesh Refiner::device_mesh() const {
   d.ns = ns;
   d.P_in = inputs.empty() ? nullptr : inputs[0].P;
   d.np_in = inputs.empty() ? 0 : inputs[0].np;
+  d.S_in = inputs.empty() ? nullptr : (const int2 *)inputs[0].S;
+  d.ns_in = inputs.empty() ? 0 : inputs[0].ns;
   return d;
 }
 

@@ -240,22 +240,31 @@ void gcdt_session_destroy(gcdt_session *s);
 
 /* Invariant checks on the GPU with exact predicates (no reference
  * counterpart: they replace the reference's host checks checkNeighbors
- * main.cpp:79-113, checkIncircle main.cpp:153-242 and the range checks of
- * checkResult main.cpp:336-362).  viol[10] receives counts per category:
- * 0 index out of range, 1 neighbour links not symmetric, 2 triangle not
- * counterclockwise, 3 boundary edge that is not a subsegment, 4 exact
- * incircle > 0 across a non-subsegment edge, 5 subsegment that is not a mesh
- * edge, 6 (unused here: segment coverage), 7 input point moved or missing,
- * 8 (unused here: quality), 9 edge shared by three or more triangles.  A
- * range violation skips the other checks.  gcdt_check_mesh takes host arrays
+ * main.cpp:79-113, checkIncircle main.cpp:153-242, checkQuality
+ * main.cpp:280-328 and the range checks of checkResult main.cpp:336-362).
+ * viol[10] receives counts per category: 0 index out of range, 1 neighbour
+ * links not symmetric, 2 triangle not counterclockwise, 3 boundary edge that
+ * is not a subsegment, 4 exact incircle > 0 across a non-subsegment edge,
+ * 5 subsegment that is not a mesh edge, 6 input segment not covered end to
+ * end by a chain of output subsegments on it (plus the number of output
+ * subsegments no chain used), 7 input point moved or missing, 8 smallest
+ * angle below theta_deg outside the Miller-Pav-Walkington exemption
+ * (triangle.cpp:7331-7378), 9 edge shared by three or more triangles.  A
+ * range violation skips the other checks.  Categories 6 and 8 are checked
+ * only when segs_in (the input segments, 2 ints each) is given; gcdt_session_check
+ * takes them and theta from the session.  gcdt_check_mesh takes host arrays
  * in the triangulateio convention (neighbours as triangle indices, -1 on the
  * boundary); gcdt_session_check checks the session's last result in HBM.
- * exact_signs (may be NULL) receives how many incircle signs the filters
- * left to the exact expansion stage. */
+ * exact_signs receives how many incircle signs the filters left to the
+ * exact expansion stage, n_exempt the MPW-exempt triangles and min_angle_deg
+ * the smallest angle of the triangles that are not exempt; any of the three
+ * may be NULL. */
 int gcdt_check_mesh(const double *points, int np, const int *tris, const int *nbrs, int nt,
                     const int *segs, int ns, const double *points_in, int np_in,
-                    long long *viol, long long *exact_signs);
-int gcdt_session_check(gcdt_session *s, long long *viol, long long *exact_signs);
+                    const int *segs_in, int ns_in, double theta_deg, long long *viol,
+                    long long *exact_signs, long long *n_exempt, double *min_angle_deg);
+int gcdt_session_check(gcdt_session *s, long long *viol, long long *exact_signs,
+                       long long *n_exempt, double *min_angle_deg);
 
 /* Batched refinement (no counterpart in the reference, which refines one
  * CDT per GPU_Refine_Quality call, refine.cu:6500-7139): n_inst independent

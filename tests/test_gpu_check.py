@@ -2,9 +2,10 @@
 CPU checker of the oracle: the same violation counts, category by category,
 on refined meshes (all zero) and on meshes with faults planted on the CPU --
 a flipped edge (Delaunay), a broken neighbour link (symmetry), a reversed
-triangle (orientation), a dropped subsegment (boundary / subsegment edge), a
-moved input point, an index out of range and a duplicated triangle (edge
-multiplicity)."""
+triangle (orientation), a dropped subsegment (boundary / subsegment edge,
+segment coverage), a moved input point, a moved Steiner point (quality), an
+index out of range and a duplicated triangle (edge multiplicity) -- and the
+same Miller-Pav-Walkington exemptions and smallest angle."""
 import numpy as np
 import pytest
 
@@ -14,7 +15,7 @@ pytestmark = pytest.mark.gpu
 PAIRS = {"range": "range", "symmetry": "neighbors", "orientation": "orientation",
          "boundary": "unsegmented_boundary", "delaunay": "delaunay",
          "subsegment": "subseg_not_edge", "moved": "input_point_moved",
-         "manifold": "nonmanifold"}
+         "manifold": "nonmanifold", "coverage": "segment_not_covered", "quality": "quality"}
 
 
 def _refined(gpu, oracle, n=3000, ns=150, seed=101, **kw):
@@ -24,15 +25,19 @@ def _refined(gpu, oracle, n=3000, ns=150, seed=101, **kw):
     return P, S, r
 
 
-def _same(gpu, oracle, Po, To, No, So, P, S):
-    g = gpu.check_mesh(Po, To, No, So, points_in=P)
+def _same(gpu, oracle, Po, To, No, So, P, S, theta=25.0):
+    g = gpu.check_mesh(Po, To, No, So, points_in=P, segments_in=S, theta=theta)
     g.pop("exact_signs")
-    tot, cpu, _, _ = oracle.check_mesh(Po, To, No, So, P, S, 25.0)
+    tot, cpu, mina, nex = oracle.check_mesh(Po, To, No, So, P, S, theta)
     if cpu["range"] == 0:
         for k, c in PAIRS.items():
             assert g[k] == cpu[c], (k, g[k], cpu[c])
+        assert g.pop("exempt") == nex
+        assert abs(g.pop("min_angle") - mina) < 1e-6, (mina, g)
     else:
         assert g["range"] == cpu["range"]
+        g.pop("exempt")
+        g.pop("min_angle")
     return g
 
 
@@ -52,10 +57,10 @@ def test_session_check_in_hbm(gpu, oracle):
         r = s.download()
     finally:
         s.close()
-    g.pop("exact_signs")
-    assert all(v == 0 for v in g.values()), g
-    h = gpu.check_mesh(r.points, r.triangles, r.neighbors, r.segments, points_in=P)
-    h.pop("exact_signs")
+    assert gpu.violations(g) == 0, g
+    assert g["min_angle"] >= 28.0 - 1e-9
+    h = gpu.check_mesh(r.points, r.triangles, r.neighbors, r.segments, points_in=P,
+                       segments_in=S, theta=28.0)
     assert h == g
 
 
@@ -99,6 +104,20 @@ def test_planted_faults(gpu, oracle):
     S4 = So.copy()
     S4[0] = [0, len(P) // 2]
     _same(gpu, oracle, Po, To, No, S4, P, S)
+    # drop a subsegment from the middle of an input segment's chain: the chain
+    # breaks (coverage) and the edge becomes an unsegmented interior edge
+    inner = [q for q in range(len(So)) if So[q, 0] >= len(P) and So[q, 1] >= len(P)]
+    S5 = np.delete(So, inner[0], axis=0)
+    g = _same(gpu, oracle, Po, To, No, S5, P, S)
+    assert g["coverage"] >= 1
+    # move a Steiner point halfway towards a neighbour: skinny triangles
+    t = int(np.nonzero((To >= len(P)).any(1))[0][0])
+    v = int(To[t][To[t] >= len(P)][0])
+    w = int(To[t][(list(To[t]).index(v) + 1) % 3])
+    P6 = Po.copy()
+    P6[v] = 0.5 * (Po[v] + Po[w])
+    g = _same(gpu, oracle, P6, To, No, So, P, S)
+    assert g["quality"] >= 1
     # move an input point by one ulp
     P5 = Po.copy()
     P5[0, 0] = np.nextafter(P5[0, 0], 1e9)
@@ -112,8 +131,25 @@ def test_planted_faults(gpu, oracle):
     # an index out of range stops the other checks
     T7 = To.copy()
     T7[0, 0] = len(Po) + 5
-    g = gpu.check_mesh(Po, T7, No, So, points_in=P)
+    g = gpu.check_mesh(Po, T7, No, So, points_in=P, segments_in=S, theta=25.0)
     assert g["range"] == 1
+
+
+def test_mpw_exemptions_match(gpu, oracle):
+    """Small input angles: the refinement leaves Miller-Pav-Walkington-exempt
+    triangles near them; both checkers find the same exemptions and no
+    violation."""
+    pts = np.array([[0.0, 0.0], [100.0, 0.0], [100.0, 100.0 * np.tan(np.radians(10))],
+                    [100.0, -100.0 * np.tan(np.radians(7))], [-40.0, 30.0], [-30.0, -50.0]])
+    segs = np.array([[0, 1], [0, 2], [0, 3]], np.int32)
+    P, T, N, S = oracle.triangle_cdt(pts, segs)
+    r = gpu.refine(P, T, N, S, theta=25.0)
+    g = gpu.check_mesh(r.points, r.triangles, r.neighbors, r.segments, points_in=P,
+                       segments_in=S, theta=25.0)
+    tot, cpu, mina, nex = oracle.check_mesh(r.points, r.triangles, r.neighbors, r.segments,
+                                            P, S, 25.0)
+    assert nex > 0 and g["exempt"] == nex
+    assert tot == 0 and gpu.violations(g) == 0, (cpu, g)
 
 
 def test_cocircular_lattice_exact_stage(gpu, oracle):
@@ -124,6 +160,8 @@ def test_cocircular_lattice_exact_stage(gpu, oracle):
                               min_angle=5.0, seed=2)
     T, N, SS = gpu.build_cdt(P, S2)
     g = _same(gpu, oracle, P, T, N, SS, P, SS)
+    # the unrefined CDT: only the quality category may count
+    g.pop("quality")
     assert all(v == 0 for v in g.values()), g
 
 
@@ -140,5 +178,6 @@ def test_exact_stage_on_cocircular_grid(gpu, oracle):
     assert c.pop("exact_signs") >= (g - 1) ** 2 // 2
     tot, cpu, _, _ = oracle.check_mesh(P, T, N, S, P, S, 25.0)
     for k, cc in PAIRS.items():
-        assert c[k] == cpu[cc], (k, c[k], cpu[cc])
+        if k not in gpu.SEGMENT_CHECKS:
+            assert c[k] == cpu[cc], (k, c[k], cpu[cc])
     assert all(v == 0 for v in c.values()), c

@@ -35,8 +35,8 @@ def test_chosen_form_invariants(gpu, oracle, dist, theta):
     assert st["status"] == 0
     assert np.array_equal(a.points.view(np.uint64), b.points.view(np.uint64))
     assert np.array_equal(a.triangles, b.triangles) and np.array_equal(a.neighbors, b.neighbors)
-    ga.pop("exact_signs")
-    assert all(v == 0 for v in ga.values()), ga
+    assert gpu.violations(ga) == 0, ga
+    assert ga["min_angle"] >= theta - 1e-6
     tot, viol, mina, ex = oracle.check_mesh(a.points, a.triangles, a.neighbors, a.segments,
                                             P, S, theta)
     assert tot == 0, viol

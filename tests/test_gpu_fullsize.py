@@ -11,6 +11,8 @@ import os
 import numpy as np
 import pytest
 
+from gpu_cdt_refine_amd import violations as gpu_violations
+
 pytestmark = pytest.mark.gpu
 
 CHOSEN = dict(pred_mode=1, claim_iters=3, shared_ring=1, cand_factor=3, priority=2,
@@ -42,9 +44,8 @@ def _run(gpu, P, T, N, S, theta, insert, mode, opts):
 
 def _clean(oracle, st, c, r, P, S, theta):
     assert st["status"] == 0
-    c = dict(c)
-    c.pop("exact_signs")
-    assert all(v == 0 for v in c.values()), c
+    assert gpu_violations(c) == 0, c
+    assert c["min_angle"] >= theta - 1e-6
     tot, viol, mina, _ = oracle.check_mesh(r.points, r.triangles, r.neighbors, r.segments,
                                            P, S, theta)
     assert tot == 0, viol
