@@ -167,6 +167,10 @@ def lib():
                                       C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                       C.POINTER(C.c_double)]
         L.gcdt_check_mesh.restype = C.c_int
+        L.gcdt_check_mesh_device_noalloc.argtypes = [vp, C.c_int, vp, vp, C.c_int, vp, C.c_int,
+                                                     vp, C.c_int, vp, C.c_int, C.c_double, vp,
+                                                     vp]
+        L.gcdt_check_mesh_device_noalloc.restype = C.c_int
         L.gcdt_last_error.restype = C.c_char_p
         L.gcdt_device_predicates.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
         L.gcdt_device_predicates.restype = C.c_int
@@ -476,6 +480,20 @@ def check_mesh(points, triangles, neighbors, segments, points_in=None, segments_
         d["exempt"] = nex.value
         d["min_angle"] = mina.value
     return d
+
+
+def check_mesh_device_noalloc(d_points, np_, d_tris, d_nbrs, nt, d_segs, ns, d_points_in,
+                              np_in, d_segs_in, ns_in, theta, d_out, stream=None):
+    """gcdt_check_mesh_device_noalloc: the checks of check_mesh on a mesh in
+    device memory (integer device addresses), enqueued on `stream` (a
+    hipStream_t handle or None) without allocating; results go to the device
+    array d_out (13 int64, see include/gpu_cdt_refine.h).  Raises on
+    rejected input (e.g. a mesh larger than the static scratch holds)."""
+    rc = lib().gcdt_check_mesh_device_noalloc(d_points, int(np_), d_tris, d_nbrs, int(nt), d_segs,
+                                              int(ns), d_points_in, int(np_in), d_segs_in,
+                                              int(ns_in), float(theta), d_out, stream)
+    if rc != 0:
+        _err(rc, "gcdt_check_mesh_device_noalloc")
 
 
 NON_VIOLATIONS = ("exact_signs", "exempt", "min_angle")

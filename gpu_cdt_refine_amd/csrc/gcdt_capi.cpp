@@ -207,6 +207,31 @@ int gcdt_session_check(gcdt_session *s, long long *viol, long long *exact_signs,
   }
 }
 
+int gcdt_check_mesh_device_noalloc(const double *d_points, int np, const int *d_tris,
+                                   const int *d_nbrs, int nt, const int *d_segs, int ns,
+                                   const double *d_points_in, int np_in, const int *d_segs_in,
+                                   int ns_in, double theta_deg, long long *d_out,
+                                   void *stream) {
+  g_err.clear();
+  try {
+    if (!d_out || np < 0 || nt < 0 || ns < 0 || np_in < 0 || ns_in < 0 ||
+        (np > 0 && !d_points) || (nt > 0 && (!d_tris || !d_nbrs)) || (ns > 0 && !d_segs) ||
+        (np_in > 0 && !d_points_in))
+      throw std::runtime_error("invalid mesh arrays");
+    if (d_segs_in && !(theta_deg > 0.0 && theta_deg < 60.0))
+      throw std::runtime_error("theta must lie in (0, 60) degrees for the quality check");
+    gcdt::Refiner::DeviceMesh dm{(const double2 *)d_points, np, d_tris, d_nbrs, nt,
+                                 (const int2 *)d_segs, ns, (const double2 *)d_points_in, np_in,
+                                 d_segs_in ? (const int2 *)d_segs_in : nullptr,
+                                 d_segs_in ? ns_in : 0};
+    gcdt::check_device_mesh_noalloc(dm, (hipStream_t)stream, theta_deg, d_out);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -2;
+  }
+}
+
 int gcdt_check_mesh(const double *points, int np, const int *tris, const int *nbrs, int nt,
                     const int *segs, int ns, const double *points_in, int np_in,
                     const int *segs_in, int ns_in, double theta_deg, long long *viol,

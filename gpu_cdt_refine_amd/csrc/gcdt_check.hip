@@ -34,6 +34,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -570,6 +571,40 @@ constexpr int CK_EXACT_LANES = 4096;  // lanes of the exact-stage grid
 
 }  // namespace
 
+// ---- scratch of the non-allocating entry point ----
+// gcdt_check_mesh_device_noalloc carves every buffer of a check out of this
+// array (64 MiB per device, held until the process ends).  In host code its
+// name is only a stand-in: the device address is taken once per device with
+// hipGetSymbolAddress (noalloc_scratch) and used for every kernel argument.
+// Calls that overlap on one device share the array and race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+constexpr int NOALLOC_EXACT_LANES = 512;  // exact-stage lanes (8 KiB of arena each)
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[MAX_DEVICES] = {nullptr};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call per device).
+char *noalloc_scratch() {
+  int dev = 0;
+  CK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) throw std::runtime_error("device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") +
+                               hipGetErrorString(e));
+    g_scratch_ptr[dev] = (char *)p;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
 // Runs every device check on a mesh already in device memory, with scratch
 // from hipMalloc (the allocating entry points).
 void check_device_mesh(const Refiner::DeviceMesh &dm, hipStream_t stream, double theta_deg,
@@ -595,6 +630,28 @@ void check_device_mesh(const Refiner::DeviceMesh &dm, hipStream_t stream, double
   (void)hipFree(base);
   (void)hipFree(dout);
   unpack(h, viol, exact_signs, extra);
+}
+
+size_t check_noalloc_bytes(int np, int nt, int ns) {
+  return ck_layout(np, nt, ns, NOALLOC_EXACT_LANES, nullptr, nullptr);
+}
+size_t check_noalloc_limit() { return NOALLOC_SCRATCH_BYTES; }
+
+// The same checks in the static scratch array, results to the device array
+// out[13]; no allocation and no host synchronisation.  Throws before
+// enqueuing anything when the mesh needs more scratch than the array holds.
+void check_device_mesh_noalloc(const Refiner::DeviceMesh &dm, hipStream_t stream,
+                               double theta_deg, long long *out) {
+  const size_t bytes = check_noalloc_bytes(dm.np, dm.nt, dm.ns);
+  if (bytes > NOALLOC_SCRATCH_BYTES)
+    throw std::runtime_error("mesh needs " + std::to_string(bytes) +
+                             " bytes of check scratch; the static array holds " +
+                             std::to_string(NOALLOC_SCRATCH_BYTES) +
+                             " (use gcdt_check_mesh for larger meshes)");
+  char *base = noalloc_scratch();
+  Scratch s;
+  ck_layout(dm.np, dm.nt, dm.ns, NOALLOC_EXACT_LANES, base, &s);
+  ck_enqueue(make_view(dm), s, theta_deg, 1e-9, out, stream);
 }
 
 }  // namespace gcdt

@@ -224,6 +224,13 @@ struct CheckExtra {
 };
 void check_device_mesh(const Refiner::DeviceMesh &dm, hipStream_t stream, double theta_deg,
                        long long *viol, long long *exact_signs, CheckExtra *extra);
+// The same checks in the library's static scratch array (no allocation, no
+// host synchronisation): results to the device array out[13] (viol[10],
+// exact signs, exempt count, smallest angle as the bits of a double).
+void check_device_mesh_noalloc(const Refiner::DeviceMesh &dm, hipStream_t stream,
+                               double theta_deg, long long *out);
+size_t check_noalloc_bytes(int np, int nt, int ns);  // scratch a mesh needs
+size_t check_noalloc_limit();                        // scratch the array holds
 
 }  // namespace gcdt
 

@@ -266,6 +266,30 @@ int gcdt_check_mesh(const double *points, int np, const int *tris, const int *nb
 int gcdt_session_check(gcdt_session *s, long long *viol, long long *exact_signs,
                        long long *n_exempt, double *min_angle_deg);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The checks of gcdt_check_mesh on a mesh already in
+ * device memory (device pointers, same layouts and meaning), enqueued on
+ * `stream` (a hipStream_t; NULL = the null stream) with no allocation, no
+ * memset node and no host synchronisation.  Scratch comes from a static
+ * 64 MiB device array per GPU, held until the process ends; a mesh whose
+ * scratch would not fit (a subsegment hash table of 8 B x the power of two
+ * >= 2 ns + 16, an edge hash table of 12 B x the power of two >= 6 nt + 16,
+ * 12 B per triangle, 12 B per point, 8 B per subsegment and a 4 MiB
+ * exact-stage arena: meshes of up to about 350,000 triangles fit) is
+ * rejected with -2 and a message in gcdt_last_error before anything is
+ * enqueued.  Results go to
+ * the device array d_out[13]: viol[0..9] as in gcdt_check_mesh, [10] signs
+ * left to the exact stage, [11] MPW-exempt triangles, [12] the smallest
+ * non-exempt angle in degrees as the bits of a double (180 if none).  Calls
+ * share the array: calls that overlap on one GPU race; each call clears what
+ * it uses. */
+int gcdt_check_mesh_device_noalloc(const double *d_points, int np, const int *d_tris,
+                                   const int *d_nbrs, int nt, const int *d_segs, int ns,
+                                   const double *d_points_in, int np_in, const int *d_segs_in,
+                                   int ns_in, double theta_deg, long long *d_out,
+                                   void *stream);
+
 /* Batched refinement (no counterpart in the reference, which refines one
  * CDT per GPU_Refine_Quality call, refine.cu:6500-7139): n_inst independent
  * CDTs with their own theta (degrees), insertion policy (1 off-center) and
