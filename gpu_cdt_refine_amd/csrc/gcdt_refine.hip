@@ -621,18 +621,28 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
             cst.dd_hits += wst.dd_hits;
             cst.exact_calls += wst.exact_calls;
           }
+          // Blocked by a subsegment: mark it, or give the triangle up when
+          // the subsegment was given up (it is never split, so neither is
+          // the triangle).  senc only turns 2 in segment launches, so the
+          // group's lanes read the same value.
           if (w == 5) {
             status = ST_SLOW;
           } else if (w == 3) {
-            status = ST_REJ_BLOCKED;
-            if (gl == 0 && mark_enc_local(m, sg)) cc.c[CC_MARKS]++;
+            if (m.senc[sg] == 2) {
+              status = ST_GIVEUP;
+            } else {
+              status = ST_REJ_BLOCKED;
+              if (gl == 0 && mark_enc_local(m, sg)) cc.c[CC_MARKS]++;
+            }
           } else if (w == 2 || w == 4) {
             status = ST_GIVEUP;
           } else {
             seed0 = tl;
             if (w == 1) {
               int s = comp(m.T[tl].S, jl);
-              if (s >= 0) {
+              if (s >= 0 && m.senc[s] == 2) {
+                status = ST_GIVEUP;
+              } else if (s >= 0) {
                 status = ST_REJ_BLOCKED;
                 if (gl == 0 && mark_enc_local(m, s)) cc.c[CC_MARKS]++;
               } else {
@@ -803,7 +813,10 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
         status = ST_GIVEUP;
       } else if (!IS_SEG && sc[SC_ENC]) {
         // Ruppert/Chew rule: a Steiner point that encroaches a subsegment on
-        // its cavity boundary is not inserted; every such subsegment is marked.
+        // its cavity boundary is not inserted; every such subsegment is
+        // marked.  When every subsegment it encroaches was given up, the
+        // triangle is given up too.
+        bool live = false;
         for (int it = gl; it < 3 * n; it += LG) {
           int t = L[it / 3], j = it % 3;
           const TriRec &rt = m.T[t];
@@ -816,9 +829,15 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
             const int2 sv = m.SV[s];
             enc = encroaches(m, lp, sv.x, sv.y, cx, cy);
           }
-          if (enc && mark_enc_local(m, s)) cc.c[CC_MARKS]++;
+          if (enc) {
+            live |= m.senc[s] != 2;
+            if (mark_enc_local(m, s)) cc.c[CC_MARKS]++;
+          }
         }
-        status = ST_REJ_ENC;
+        // every lane of this group is here (the branch is group-uniform)
+        const unsigned long long gmask = LG == 64 ? ~0ull : ((1ull << (LG % 64)) - 1ull);
+        const bool any_live = ((__ballot(live) >> (grp * LG)) & gmask) != 0ull;
+        status = any_live ? ST_REJ_ENC : ST_GIVEUP;
       }
     }
     // ---- star order of the boundary (valid candidates) ----

@@ -735,6 +735,14 @@ static void mark_enc(Mesh *m, int s, long long *marks) {
   }
 }
 
+/* A bad triangle's Steiner point blocked by subsegment s (walk): mark s and
+ * reject the candidate, or give the triangle up when s was given up. */
+static int blocked_by(Mesh *m, int s, long long *marks) {
+  if (m->senc[s] == 2) return ST_GIVEUP;
+  mark_enc(m, s, marks);
+  return ST_REJ_BLOCKED;
+}
+
 static int cmp_ll(const void *a, const void *b) {
   long long x = *(const long long *)a, y = *(const long long *)b;
   return x < y ? -1 : (x > y);
@@ -785,12 +793,14 @@ static int run_pass(Mesh *m, const int *ids, int nc, int nseg, int K,
       if (!steiner_point(m, t0, c->c)) { c->status = ST_GIVEUP; goto done; }
       int tl = -1, jl = -1, sg = -1;
       int w = walk(m, t0, c->c, &tl, &jl, &sg);
-      if (w == 3) { c->status = ST_REJ_BLOCKED; mark_enc(m, sg, &marks); goto done; }
+      /* blocked by a subsegment: mark it, or give up when it was given up
+       * (it will never be split, so the triangle could never be) */
+      if (w == 3) { c->status = blocked_by(m, sg, &marks); goto done; }
       if (w == 2 || w == 4) { c->status = ST_GIVEUP; goto done; }
       seed0 = tl;
       if (w == 1) {
         int s = m->TS[3 * tl + jl];
-        if (s >= 0) { c->status = ST_REJ_BLOCKED; mark_enc(m, s, &marks); goto done; }
+        if (s >= 0) { c->status = blocked_by(m, s, &marks); goto done; }
         int nb = m->TN[3 * tl + jl];
         if (nb < 0) { c->status = ST_GIVEUP; goto done; }
         seed1 = nb >> 2;
@@ -816,20 +826,23 @@ static int run_pass(Mesh *m, const int *ids, int nc, int nseg, int K,
     }
     if (c->status == ST_VALID && !is_seg) {
       /* Ruppert/Chew rule: a Steiner point that encroaches a subsegment on
-       * its cavity boundary is not inserted; every such subsegment is marked. */
-      int enc = 0;
+       * its cavity boundary is not inserted; every such subsegment is marked.
+       * When every subsegment it encroaches was given up, the triangle is
+       * given up too. */
+      int enc = 0, live = 0;
       for (int idx = 0; idx < c->n; idx++) {
         int t = c->L[idx];
         for (int j = 0; j < 3; j++) {
           int s = m->TS[3 * t + j];
           if (s < 0) continue;
           if (encroaches(m, m->SV[2 * s], m->SV[2 * s + 1], c->c)) {
+            live |= m->senc[s] != 2;
             mark_enc(m, s, &marks);
             enc = 1;
           }
         }
       }
-      if (enc) c->status = ST_REJ_ENC;
+      if (enc) c->status = live ? ST_REJ_ENC : ST_GIVEUP;
     }
   done:
     if (c->status == ST_VALID) {

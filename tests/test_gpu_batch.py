@@ -78,6 +78,54 @@ def test_batch_instances_equal_solo(gpu, oracle, wave_triangles, graph):
         assert np.array_equal(res[i].tstatus, r.tstatus)
 
 
+def _family(gpu, specs):
+    """Instances of the batched family's kinds: uniform (dist 0) or
+    Gaussian-clustered (dist 1, sigma 5% of the domain), 5% segments no
+    longer than 5% of the domain, the domain scaled with the point count."""
+    out = []
+    for n, dist, seed in specs:
+        L = 1000.0 * (n / 10000) ** 0.5
+        P, S = gpu.generate_pslg(n, max(1, n // 20), L=L, maxlen=0.05 * L, dist=dist,
+                                 min_angle=5.0, seed=seed)
+        T, N, SS = gpu.build_cdt(P, S)
+        out.append((P, T, N, SS))
+    return out
+
+
+FAMILY = [(3000, 1, 41), (1200, 0, 42), (6000, 1, 43), (800, 1, 44), (4500, 0, 45),
+          (2500, 1, 46)]
+FTHETA = [21.3, 29.5, 24.0, 27.7, 20.9, 25.5]
+FMODE = [1, 0, 1, 0, 1, 1]
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_batch_family_equals_oracle_batch(gpu, oracle, graph):
+    """Uniform and Gaussian-clustered instances of the batched family with
+    per-instance theta and mode, under the chosen form's claim rule (three
+    claim iterations, shared rings): bit-identical to the oracle's batched
+    refinement, and every instance clean on the GPU checker (including
+    segment coverage and every angle at least its own theta)."""
+    print(f"batch family vs oracle graph={graph}", flush=True)
+    ins = _family(gpu, FAMILY)
+    off = [1] * len(ins)
+    kw = dict(cand_min=512, cand_factor=3, claim_iters=3, shared_ring=1)
+    b = gpu.Batch(ins, theta=FTHETA, insert=off, mode=FMODE, graph=graph, **kw)
+    try:
+        st = b.run()
+        res = b.results()
+    finally:
+        b.close()
+    ores, ost = oracle.oracle_refine_batch(ins, FTHETA, off, FMODE, **kw)
+    assert st["status"] == ost["status"] == 0
+    assert st["inserted"] == ost["inserted"] and st["seg_splits"] == ost["seg_splits"]
+    for i, (P, T, N, S) in enumerate(ins):
+        _same(res[i], ores[i])
+        c = gpu.check_mesh(res[i].points, res[i].triangles, res[i].neighbors, res[i].segments,
+                           points_in=P, segments_in=S, theta=FTHETA[i])
+        assert gpu.violations(c) == 0, (i, c)
+        assert c["min_angle"] >= FTHETA[i] - 1e-9 or c["exempt"] > 0, (i, c)
+
+
 def test_batch_rerun_is_repeatable(gpu, oracle):
     ins = _instances(oracle, SPECS[:3])
     b = gpu.Batch(ins, theta=THETA[:3], insert=OFF[:3], mode=MODE[:3], graph=1)

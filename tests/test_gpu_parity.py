@@ -138,3 +138,40 @@ def test_gpu_equals_oracle_precision_stress(gpu, oracle, pred_mode, graph):
     assert np.array_equal(r.neighbors, No)
     assert np.array_equal(r.segments, So)
     assert r.stats["pred_exact"] + r.stats["pred_dd"] > 0
+
+
+def _near_degenerate_family_instance(gpu):
+    """A Gaussian-clustered instance of the batched family (69,161 points,
+    3,458 segments) in which two segment chains pass within ~1e-7 of each
+    other near coordinate 2000: the refinement must resolve that feature, and
+    four segment splits there meet a nearly collinear neighbour vertex on the
+    wrong side of FP64 rounding (their star would be inverted)."""
+    n = 69161
+    L = 1000.0 * (n / 10000) ** 0.5
+    P, S = gpu.generate_pslg(n, n // 20, L=L, maxlen=0.05 * L, dist=1, min_angle=5.0, seed=7000 + 4009)
+    T, N, SS = gpu.build_cdt(P, S)
+    return P, T, N, SS
+
+
+def test_gpu_equals_oracle_given_up_subsegments(gpu, oracle):
+    """Given-up subsegments: bad triangles that encroach (or walk into) only
+    given-up subsegments are given up too, so the run ends with status 3 and
+    counts them; every quality violation left is one of them.  Bit-identical
+    to the oracle."""
+    print("given-up subsegments: family instance 4009", flush=True)
+    P, T, N, S = _near_degenerate_family_instance(gpu)
+    kw = dict(claim_iters=3, shared_ring=1, priority=1)
+    theta = 21.33356473295036
+    r = gpu.refine(P, T, N, S, theta=theta, **kw)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, theta=theta, **kw)
+    for k in ("status", "rounds", "inserted", "seg_splits", "giveups", "giveup_tris_left",
+              "giveup_segs_left"):
+        assert r.stats[k] == st[k], k
+    assert st["status"] == 3 and st["giveup_segs_left"] > 0, st
+    assert np.array_equal(r.points.view(np.uint64), Po.view(np.uint64))
+    assert np.array_equal(r.triangles, To)
+    assert np.array_equal(r.neighbors, No)
+    c = gpu.check_mesh(r.points, r.triangles, r.neighbors, r.segments, points_in=P,
+                       segments_in=S, theta=theta)
+    assert c["quality"] <= st["giveup_tris_left"], (c, st)
+    assert gpu.violations(c) == c["quality"], c

@@ -322,3 +322,26 @@ def test_mix_below_invariants(oracle, mix_below):
     assert tot == 0, viol
     assert mina >= 25.0 - 1e-7
     assert abs((len(Po) - len(Pc)) / (len(Pc) - len(P))) < 0.05
+
+
+def test_oracle_gives_up_triangles_stuck_on_given_up_subsegments(oracle):
+    """A batched-family instance (Gaussian clusters, 69,161 points) whose
+    input has two segment chains ~1e-7 apart near coordinate 2000: four
+    segment splits there cannot form a positively oriented star in FP64 and
+    are given up; the bad triangles that then encroach (or walk into) only
+    those subsegments are given up as well, so the run converges with status
+    3 instead of stalling, and every quality violation left is a given-up
+    triangle."""
+    import gpu_cdt_refine_amd as g
+    n = 69161
+    L = 1000.0 * (n / 10000) ** 0.5
+    P, S = g.generate_pslg(n, n // 20, L=L, maxlen=0.05 * L, dist=1, min_angle=5.0,
+                           seed=7000 + 4009)
+    T, N, SS = g.build_cdt(P, S)
+    theta = 21.33356473295036
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, SS, theta=theta, claim_iters=3,
+                                                  shared_ring=1, priority=1)
+    assert st["status"] == 3 and st["giveup_segs_left"] == 4, st
+    tot, viol, mina, ex = oracle.check_mesh(Po, To, No, So, P, SS, theta)
+    assert 0 < viol["quality"] <= st["giveup_tris_left"], (viol, st)
+    assert tot == viol["quality"], viol
