@@ -8,6 +8,8 @@ calls raise.
 Main entry points
   refine(points, triangles, neighbors, segments, theta=25, ...) -> RefineResult
       the reference's GPU_Refine_Quality (refine.h:385-386) on numpy arrays
+  dropin_refine(...)  the C++ drop-in GPU_Refine_Quality itself (host arrays in
+                      and out, triangulateio records), with its wall time
   Session(...)        device-resident input; run() repeatedly (benchmarks)
   generate_pslg(...)  seeded synthetic PSLG (points + non-crossing segments)
   build_cdt(...)      constrained Delaunay triangulation with hull segments
@@ -22,6 +24,14 @@ LIB_PATH = os.path.join(_HERE, "libgcdt.so")
 
 Circumcenter, Offcenter = 0, 1
 RUPPERT, CHEW = 1, 0
+
+# The chosen form (what the drop-in runs by default): FP32-filtered exact
+# predicates, three claim iterations with shared rings, a bad pass of up to
+# 3x the previous winners, the learned priority coarsened by one half-octave,
+# and the launch shapes the autotuner picked on the 10^6-point workload.
+CHOSEN_FORM = dict(pred_mode=1, claim_iters=3, shared_ring=1, cand_factor=3, priority=2,
+                   learned_shift=1, grid_cavity=4096, grid_claim=1024, grid_apply=8192,
+                   cavity_lanes=8, cav_lds_cap=16)
 
 
 class Options(C.Structure):
@@ -171,6 +181,17 @@ def lib():
                                                      vp, C.c_int, vp, C.c_int, C.c_double, vp,
                                                      vp]
         L.gcdt_check_mesh_device_noalloc.restype = C.c_int
+        L.gcdt_dropin_set_options.argtypes = [vp]
+        L.gcdt_dropin_set_options.restype = C.c_int
+        L.gcdt_dropin_set_priority_model.argtypes = [vp]
+        L.gcdt_dropin_set_priority_model.restype = C.c_int
+        L.gcdt_dropin_last_stats.argtypes = [vp]
+        L.gcdt_dropin_last_stats.restype = C.c_int
+        L.gcdt_dropin_release.argtypes = []
+        L.gcdt_dropin_release.restype = None
+        L.gcdt_dropin_free.argtypes = [vp, vp, vp]
+        L.gcdt_dropin_free.restype = None
+        getattr(L, DROPIN_SYMBOL).restype = None
         L.gcdt_last_error.restype = C.c_char_p
         L.gcdt_device_predicates.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
         L.gcdt_device_predicates.restype = C.c_int
@@ -296,6 +317,95 @@ def refine(points, triangles, neighbors, segments, theta=25.0,
     if rc != 0:
         _err(rc, "gcdt_refine_host")
     return RefineResult(*_fetch_mesh(m), st.as_dict())
+
+
+class TriangulateIO(C.Structure):
+    """The reference's triangulateio record (GPU_Refine/triangle.h:253-284),
+    as declared in include/gpu_cdt_refine.h."""
+    _fields_ = [
+        ("pointlist", C.POINTER(C.c_double)), ("pointattributelist", C.POINTER(C.c_double)),
+        ("pointmarkerlist", C.POINTER(C.c_int)), ("numberofpoints", C.c_int),
+        ("numberofpointattributes", C.c_int), ("trianglelist", C.POINTER(C.c_int)),
+        ("triangleattributelist", C.POINTER(C.c_double)),
+        ("trianglearealist", C.POINTER(C.c_double)), ("neighborlist", C.POINTER(C.c_int)),
+        ("numberoftriangles", C.c_int), ("numberofcorners", C.c_int),
+        ("numberoftriangleattributes", C.c_int), ("segmentlist", C.POINTER(C.c_int)),
+        ("segmentmarkerlist", C.POINTER(C.c_int)), ("numberofsegments", C.c_int),
+        ("holelist", C.POINTER(C.c_double)), ("numberofholes", C.c_int),
+        ("regionlist", C.POINTER(C.c_double)), ("numberofregions", C.c_int),
+        ("edgelist", C.POINTER(C.c_int)), ("edgemarkerlist", C.POINTER(C.c_int)),
+        ("normlist", C.POINTER(C.c_double)), ("numberofedges", C.c_int),
+        ("numberofflips", C.c_int)]
+
+
+# GPU_Refine_Quality keeps the reference's C++ linkage (refine.h:385-386)
+DROPIN_SYMBOL = ("_Z18GPU_Refine_QualityP13triangulateioS0_d12InsertPolicy12DeletePolicyiiPP7"
+                 "PStatusPP7TStatus")
+
+
+def dropin_set_options(theta=25.0, insert=Offcenter, mode=RUPPERT, plain=False, **options):
+    """Options of the drop-in (gcdt_dropin_set_options).  plain=True selects
+    the library defaults (the plain GPU path); otherwise the chosen form with
+    the given overrides.  theta / insert / mode come from each call."""
+    if plain:
+        o = make_options(theta, insert, mode, **options)
+    else:
+        o = make_options(theta, insert, mode, **dict(CHOSEN_FORM, **options))
+    if lib().gcdt_dropin_set_options(C.byref(o)) != 0:
+        _err(-1, "gcdt_dropin_set_options")
+
+
+def dropin_reset():
+    """Back to the chosen form with the compiled-in priority network; frees
+    the drop-in's cached device buffers."""
+    L = lib()
+    if L.gcdt_dropin_set_options(None) != 0 or L.gcdt_dropin_set_priority_model(None) != 0:
+        _err(-1, "gcdt_dropin_set_options")
+    L.gcdt_dropin_release()
+
+
+def dropin_refine(points, triangles, neighbors, segments, theta=25.0, insert=Offcenter,
+                  mode=RUPPERT, debug=False):
+    """Calls the C++ drop-in GPU_Refine_Quality on triangulateio records built
+    from the arrays and returns (RefineResult, seconds): the wall time of the
+    call alone (host arrays in, host arrays out, the reference's timing,
+    main.cpp:1860-1868).  The result arrays are copied out and the call's
+    new[] arrays freed."""
+    P, T, N, S = _as_inputs(points, triangles, neighbors, segments)
+    L = lib()
+    f = getattr(L, DROPIN_SYMBOL)
+    inp, res = TriangulateIO(), TriangulateIO()
+    inp.pointlist = P.ctypes.data_as(C.POINTER(C.c_double))
+    inp.numberofpoints = len(P)
+    inp.trianglelist = T.ctypes.data_as(C.POINTER(C.c_int))
+    inp.neighborlist = N.ctypes.data_as(C.POINTER(C.c_int))
+    inp.numberoftriangles = len(T)
+    inp.numberofcorners = 3
+    inp.segmentlist = S.ctypes.data_as(C.POINTER(C.c_int))
+    inp.numberofsegments = len(S)
+    ps, ts = C.c_void_p(), C.c_void_p()
+    import time
+    t0 = time.perf_counter()
+    f(C.byref(inp), C.byref(res), C.c_double(theta), C.c_int(int(insert)), C.c_int(1),
+      C.c_int(int(mode)), C.c_int(-1), C.byref(ps) if debug else None,
+      C.byref(ts) if debug else None)
+    dt = time.perf_counter() - t0
+    np_, nt, ns = res.numberofpoints, res.numberoftriangles, res.numberofsegments
+    Po = np.ctypeslib.as_array(res.pointlist, shape=(np_, 2)).copy()
+    To = np.ctypeslib.as_array(res.trianglelist, shape=(nt, 3)).copy()
+    No = np.ctypeslib.as_array(res.neighborlist, shape=(nt, 3)).copy()
+    So = (np.ctypeslib.as_array(res.segmentlist, shape=(ns, 2)).copy() if ns
+          else np.zeros((0, 2), np.int32))
+    pst = tst = None
+    if debug:
+        pst = np.ctypeslib.as_array(C.cast(ps, C.POINTER(C.c_uint8)), shape=(np_,)).copy()
+        tst = np.ctypeslib.as_array(C.cast(ts, C.POINTER(C.c_uint8)), shape=(nt,)).copy()
+    L.gcdt_dropin_free(C.byref(res), ps if debug else None, ts if debug else None)
+    st = Stats()
+    L.gcdt_dropin_last_stats(C.byref(st))
+    d = st.as_dict()
+    d["numberofflips"] = res.numberofflips
+    return RefineResult(Po, To, No, So, pst, tst, d), dt
 
 
 class Batch:

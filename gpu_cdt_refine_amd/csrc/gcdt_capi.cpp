@@ -3,11 +3,15 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <vector>
 #include <algorithm>
 
 #include "gcdt_internal.h"
 #include "gpu_cdt_refine.h"
+#include "priority_mlp_weights.h"
 
 using gcdt::MeshHost;
 using gcdt::Refiner;
@@ -383,6 +387,119 @@ int gcdt_device_constructions(const double *tri6, int n, double theta_deg, int o
 }  // extern "C"
 
 // Drop-in entry point with the reference's signature (refine.h:385-386).
+// ---------------------------------------------------------------------------
+// The drop-in's process-wide state: its options (the chosen form unless a
+// caller sets others), the priority network, and one cached refiner whose
+// device buffers serve every call with the same options.
+// ---------------------------------------------------------------------------
+namespace {
+struct DropIn {
+  std::mutex mu;
+  bool configured = false;
+  gcdt_options opt{};
+  std::vector<float> model;
+  std::unique_ptr<Refiner> r;
+  gcdt_stats last{};
+};
+// Never destroyed: freeing device memory from a static destructor could run
+// after the HIP runtime has shut down (gcdt_dropin_release frees it early).
+DropIn &dropin() {
+  static DropIn *d = new DropIn;
+  return *d;
+}
+// The chosen form: FP32 filter, three claim iterations with shared rings,
+// a bad pass taking up to 3x the previous pass's winners, the learned
+// priority (coarsened by one half-octave); launch shapes the autotuner picked
+// on the 10^6-point workload (they change timing only, never the mesh).
+void chosen_defaults(gcdt_options *o) {
+  gcdt_default_options(o);
+  o->pred_mode = 1;
+  o->claim_iters = 3;
+  o->shared_ring = 1;
+  o->cand_factor = 3;
+  o->priority = 2;
+  o->learned_shift = 1;
+  o->grid_cavity = 4096;
+  o->grid_claim = 1024;
+  o->grid_apply = 8192;
+  o->cavity_lanes = 8;
+  o->cav_lds_cap = 16;
+}
+void dropin_configure(DropIn &d) {
+  if (d.configured) return;
+  chosen_defaults(&d.opt);
+  d.model.assign(gcdt::kEmbeddedMlp, gcdt::kEmbeddedMlp + gcdt::kEmbeddedMlpCount);
+  d.configured = true;
+}
+// same options apart from the per-call angle, policy and mode
+bool same_run_options(const gcdt_options &a, const gcdt_options &b) {
+  gcdt_options x = a, y = b;
+  x.theta_deg = y.theta_deg = 0.0;
+  x.offcenter = y.offcenter = 0;
+  x.mode = y.mode = 0;
+  return std::memcmp(&x, &y, sizeof(x)) == 0;
+}
+}  // namespace
+
+extern "C" {
+
+int gcdt_dropin_set_options(const gcdt_options *opt) {
+  g_err.clear();
+  DropIn &d = dropin();
+  std::lock_guard<std::mutex> lk(d.mu);
+  dropin_configure(d);
+  gcdt_options o;
+  if (opt) o = *opt;
+  else chosen_defaults(&o);
+  o.theta_deg = 25.0;  // validated with a representative angle; calls set their own
+  if (!valid_options(&o)) return -1;
+  d.opt = o;
+  d.r.reset();
+  return 0;
+}
+
+int gcdt_dropin_set_priority_model(const float *params) {
+  g_err.clear();
+  DropIn &d = dropin();
+  std::lock_guard<std::mutex> lk(d.mu);
+  dropin_configure(d);
+  if (params) d.model.assign(params, params + gcdt::kEmbeddedMlpCount);
+  else d.model.assign(gcdt::kEmbeddedMlp, gcdt::kEmbeddedMlp + gcdt::kEmbeddedMlpCount);
+  if (d.r && d.r->opt.priority == 2) d.r->set_priority_model(d.model.data(), (int)d.model.size());
+  return 0;
+}
+
+int gcdt_dropin_last_stats(gcdt_stats *stats) {
+  if (!stats) return -1;
+  DropIn &d = dropin();
+  std::lock_guard<std::mutex> lk(d.mu);
+  *stats = d.last;
+  return 0;
+}
+
+void gcdt_dropin_release(void) {
+  DropIn &d = dropin();
+  std::lock_guard<std::mutex> lk(d.mu);
+  d.r.reset();
+}
+
+void gcdt_dropin_free(struct triangulateio *result, PStatus *ps_debug, TStatus *ts_debug) {
+  if (result) {
+    delete[] result->pointlist;
+    delete[] result->trianglelist;
+    delete[] result->neighborlist;
+    delete[] result->segmentlist;
+    result->pointlist = nullptr;
+    result->trianglelist = nullptr;
+    result->neighborlist = nullptr;
+    result->segmentlist = nullptr;
+  }
+  delete[] ps_debug;
+  delete[] ts_debug;
+}
+
+}  // extern "C"
+
 void GPU_Refine_Quality(struct triangulateio *input,
                         struct triangulateio *result, double theta,
                         InsertPolicy insertpolicy, DeletePolicy deletepolicy,
@@ -390,40 +507,61 @@ void GPU_Refine_Quality(struct triangulateio *input,
                         TStatus **ts_debug) {
   (void)deletepolicy;
   (void)debug_iter;
-  gcdt_options opt;
-  gcdt_default_options(&opt);
+  DropIn &d = dropin();
+  std::lock_guard<std::mutex> lk(d.mu);
+  dropin_configure(d);
+  gcdt_options opt = d.opt;
   opt.theta_deg = theta;
   opt.offcenter = insertpolicy == Offcenter ? 1 : 0;
   opt.mode = mode;
-  gcdt_mesh *m = nullptr;
-  int rc = gcdt_refine_host(input->pointlist, input->numberofpoints,
-                            input->trianglelist, input->neighborlist,
-                            input->numberoftriangles, input->segmentlist,
-                            input->numberofsegments, &opt, &m, nullptr);
-  if (rc != 0) {
-    std::fprintf(stderr, "GPU_Refine_Quality: %s\n", gcdt_last_error());
+  g_err.clear();
+  if (!input || !result || !valid_options(&opt)) {
+    std::fprintf(stderr, "GPU_Refine_Quality: %s\n",
+                 (!input || !result) ? "input and result must be non-NULL" : g_err.c_str());
     std::exit(EXIT_FAILURE);
   }
-  const MeshHost &h = m->h;
-  result->numberofpoints = h.np;
-  result->pointlist = new double[2 * (size_t)h.np];
-  std::memcpy(result->pointlist, h.P.data(), sizeof(double) * 2 * (size_t)h.np);
-  result->numberoftriangles = h.nt;
-  result->numberofcorners = 3;
-  result->trianglelist = new int[3 * (size_t)h.nt];
-  std::memcpy(result->trianglelist, h.T.data(), sizeof(int) * 3 * (size_t)h.nt);
-  result->neighborlist = new int[3 * (size_t)h.nt];
-  std::memcpy(result->neighborlist, h.N.data(), sizeof(int) * 3 * (size_t)h.nt);
-  result->numberofsegments = h.ns;
-  result->segmentlist = new int[2 * (size_t)h.ns];
-  std::memcpy(result->segmentlist, h.S.data(), sizeof(int) * 2 * (size_t)h.ns);
-  if (ps_debug) {
-    *ps_debug = new PStatus[(size_t)h.np];
-    for (int i = 0; i < h.np; i++) (*ps_debug)[i]._status = h.pstat[i];
+  try {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("no HIP device");
+    if (d.r && (!same_run_options(d.r->opt, opt) || d.r->opt.device != dev)) d.r.reset();
+    if (!d.r) {
+      opt.device = dev;
+      d.r.reset(new Refiner(opt));
+      if (opt.priority == 2) d.r->set_priority_model(d.model.data(), (int)d.model.size());
+    }
+    Refiner &r = *d.r;
+    r.set_angle(opt.theta_deg, opt.offcenter, opt.mode);
+    r.upload(input->pointlist, input->numberofpoints, input->trianglelist, input->neighborlist,
+             input->numberoftriangles, input->segmentlist, input->numberofsegments);
+    r.run();
+    d.last = r.stats();
+    const int np = r.out_np(), nt = r.out_nt(), ns = r.out_ns();
+    result->numberofpoints = np;
+    result->numberoftriangles = nt;
+    result->numberofcorners = 3;
+    result->numberofsegments = ns;
+    result->pointlist = new double[2 * (size_t)np];
+    result->trianglelist = new int[3 * (size_t)nt];
+    result->neighborlist = new int[3 * (size_t)nt];
+    result->segmentlist = new int[2 * (size_t)ns];
+    Refiner::HostDst dst;
+    dst.P = result->pointlist;
+    dst.T = result->trianglelist;
+    dst.N = result->neighborlist;
+    dst.S = result->segmentlist;
+    dst.P_in_host = input->pointlist;
+    dst.np_in_host = input->numberofpoints;
+    if (ps_debug) {
+      *ps_debug = new PStatus[(size_t)np];
+      dst.pstat = reinterpret_cast<unsigned char *>(*ps_debug);
+    }
+    if (ts_debug) {
+      *ts_debug = new TStatus[(size_t)nt];
+      dst.tstat = reinterpret_cast<unsigned char *>(*ts_debug);
+    }
+    r.download_into(dst);
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "GPU_Refine_Quality: %s\n", e.what());
+    std::exit(EXIT_FAILURE);
   }
-  if (ts_debug) {
-    *ts_debug = new TStatus[(size_t)h.nt];
-    for (int i = 0; i < h.nt; i++) (*ts_debug)[i]._status = h.tstat[i];
-  }
-  gcdt_mesh_free(m);
 }

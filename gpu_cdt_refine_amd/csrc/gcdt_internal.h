@@ -10,6 +10,7 @@
 
 #include "gcdt_device.h"
 #include "gpu_cdt_refine.h"
+#include "host_copy.h"
 
 namespace gcdt {
 
@@ -24,6 +25,7 @@ struct MeshHost {
 // One device-resident input CDT (a single mesh, or the union of a batch).
 struct InputSet {
   int np = 0, nt = 0, ns = 0, n_inst = 0;
+  int capP = 0, capT = 0, capS = 0;  // allocated sizes (reuse on upload)
   double2 *P = nullptr;
   int *T = nullptr, *N = nullptr, *S = nullptr;
   int *tinst = nullptr;     // instance of every input triangle (batches)
@@ -57,6 +59,9 @@ class Refiner {
                 int nt, const int *segs, int ns, const int *tri_inst,
                 const InstPar *ipar, int n_inst);
   void select_input(int k);
+  // theta / insertion policy / mode of the next run (a cached refiner serves
+  // calls with different angles)
+  void set_angle(double theta_deg, int offcenter, int mode);
   int input_count() const { return (int)inputs.size(); }
   // device copies of the last run's output (for batches kept across waves)
   const double2 *out_points() const { return m.P; }
@@ -78,6 +83,22 @@ class Refiner {
   DeviceMesh device_mesh() const;
   void run();
   void download(MeshHost &out);
+  // The last run's mesh straight into caller arrays (any may be null):
+  // points 2*np doubles, triangles / neighbours 3*nt ints, subsegments 2*ns
+  // ints, point / triangle status bytes, triangle instances.  The first
+  // np_in_host points are the input points and are copied from the host
+  // array P_in_host when it is given.
+  struct HostDst {
+    double *P = nullptr;
+    int *T = nullptr, *N = nullptr, *S = nullptr;
+    unsigned char *pstat = nullptr, *tstat = nullptr;
+    int *tinst = nullptr;
+    const double *P_in_host = nullptr;
+    int np_in_host = 0;
+  };
+  void download_into(const HostDst &d);
+  int out_nt() const { return nt_out; }
+  int out_ns() const { return ns; }
   int kernel_times(gcdt_kernel_time *rec, int max_rec) const;
   const gcdt_stats &stats() const { return st; }
 
@@ -93,6 +114,9 @@ class Refiner {
   std::vector<float> log_X, log_t;
 
  private:
+  void copy_input(InputSet &in, const double *points, int np, const int *tris, const int *nbrs,
+                  int nt, const int *segs, int ns, const int *tri_inst, const InstPar *ipar,
+                  int n_inst);
   template <class Pred>
   int compact(Pred pred, int n, int *out_list, int *out_rank, const int *src,
               unsigned long long *total64 = nullptr, bool counted = false);
@@ -116,6 +140,7 @@ class Refiner {
   int select_by_priority(int *ids, int nb, int cap, int **out_ids);
   void ensure_capacity(long long needP, long long needT, long long needS);
   void ensure_cand(int nc);
+  void ensure_ws(long long ints);
   void ensure_tmp(int n);
   int cands_per_pass(int K) const;
   void read_counters();
@@ -189,6 +214,14 @@ class Refiner {
   int n_inst = 0;
   int *o_inst = nullptr;
   hipEvent_t ev0, ev1;
+  // input validation counters
+  unsigned long long *d_val = nullptr, *h_val4 = nullptr;
+  // download staging: two pinned buffers, filled by the copy engine while
+  // host threads drain the other one into the caller's arrays
+  char *h_stage[2] = {nullptr, nullptr};
+  size_t stage_bytes = 0;
+  hipEvent_t ev_stage[2] = {nullptr, nullptr};
+  CopyPool *copier = nullptr;
   struct KT {
     long long launches = 0;
     double total_ms = 0;

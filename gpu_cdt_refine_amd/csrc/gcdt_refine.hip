@@ -451,6 +451,9 @@ struct CavCounts {
 };
 
 constexpr int BIG_LDS_BYTES = 160 * 1024;
+// Largest grid of the exact-predicate cavity kernel (wave64 blocks); its
+// arena holds GCDT_INCIRCLE_WS doubles per lane: 512 x 64 x 8 KiB = 256 MiB.
+constexpr int EXACT_GRID = 512;
 
 // ---------------------------------------------------------------------------
 // Candidate evaluation: a wave64 evaluates 64/LG candidates at once, one per
@@ -545,7 +548,9 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
   const bool fp32 = pr.use_fp32 != 0;
   CavCounts cc;
   for (int k = 0; k < CC_N; k++) cc.c[k] = 0u;
-  double *pws = ps.pws + ((size_t)blockIdx.x * 64 + lane) * GCDT_INCIRCLE_WS;
+  // the exact stages' arena: only the exact kernel (whose grid the arena
+  // covers) uses it
+  double *pws = FAST ? nullptr : ps.pws + ((size_t)blockIdx.x * 64 + lane) * GCDT_INCIRCLE_WS;
   const int nc = pass_nc(ps);
   const int nseg = pass_nseg(ps);
   const int *ids = pass_ids(ps);
@@ -1760,15 +1765,28 @@ Refiner::Refiner(const gcdt_options &o) : opt(o) {
   dalloc(&g_cnt, GC);
   dalloc(&g_off, GC);
   HC(hipHostMalloc((void **)&h_ctl, sizeof(Ctl), hipHostMallocDefault));
-  double ga = std::cos(opt.theta_deg * 3.141592653589793238462643383279502884197169399375105820974944592308 / 180.0);
-  pr.offconst = (ga == 1.0) ? 0.0 : 0.475 * std::sqrt((1.0 + ga) / (1.0 - ga));
-  pr.goodangle = ga * ga;
-  pr.lens = (2.0 * pr.goodangle - 1.0) * (2.0 * pr.goodangle - 1.0);
-  pr.offcenter = opt.offcenter;
-  pr.ruppert = opt.mode == 1;
+  dalloc(&d_val, 4);
+  HC(hipHostMalloc((void **)&h_val4, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+  for (int k = 0; k < 2; k++) HC(hipEventCreateWithFlags(&ev_stage[k], hipEventDisableTiming));
+  set_angle(opt.theta_deg, opt.offcenter, opt.mode);
   pr.use_fp32 = opt.pred_mode == 1;
   pr.priority = opt.priority;
   pr.ipar = nullptr;
+}
+
+// Quality threshold, off-center constant and encroachment rule of theta
+// (refine.cu:6557-6567 with the constant kept in double, as Triangle does,
+// triangle.cpp:3527-3533).
+void Refiner::set_angle(double theta_deg, int offcenter, int mode) {
+  opt.theta_deg = theta_deg;
+  opt.offcenter = offcenter;
+  opt.mode = mode;
+  double ga = std::cos(theta_deg * 3.141592653589793238462643383279502884197169399375105820974944592308 / 180.0);
+  pr.offconst = (ga == 1.0) ? 0.0 : 0.475 * std::sqrt((1.0 + ga) / (1.0 - ga));
+  pr.goodangle = ga * ga;
+  pr.lens = (2.0 * pr.goodangle - 1.0) * (2.0 * pr.goodangle - 1.0);
+  pr.offcenter = offcenter;
+  pr.ruppert = mode == 1;
 }
 
 Refiner::~Refiner() {
@@ -1793,6 +1811,13 @@ Refiner::~Refiner() {
   if (h_scalar) (void)hipHostFree(h_scalar);
   if (h_total) (void)hipHostFree(h_total);
   if (h_cnt) (void)hipHostFree(h_cnt);
+  dfree(d_val);
+  if (h_val4) (void)hipHostFree(h_val4);
+  delete copier;
+  for (int k = 0; k < 2; k++) {
+    if (h_stage[k]) (void)hipHostFree(h_stage[k]);
+    (void)hipEventDestroy(ev_stage[k]);
+  }
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
   for (auto &kt : ktimes) {
@@ -1810,6 +1835,15 @@ void Refiner::free_mesh() {
 
 void Refiner::upload(const double *points, int np, const int *tris,
                      const int *nbrs, int nt, const int *segs, int ns) {
+  // a single resident input whose buffers are large enough is overwritten
+  // in place (a process that refines many CDTs allocates once)
+  if (inputs.size() == 1 && !inputs[0].tinst && inputs[0].capP >= np && inputs[0].capT >= nt &&
+      inputs[0].capS >= ns) {
+    InputSet &in = inputs[0];
+    copy_input(in, points, np, tris, nbrs, nt, segs, ns, nullptr, nullptr, 0);
+    select_input(0);
+    return;
+  }
   for (auto &in : inputs) {
     dfree(in.P); dfree(in.T); dfree(in.N); dfree(in.S); dfree(in.tinst); dfree(in.ipar);
   }
@@ -1818,44 +1852,78 @@ void Refiner::upload(const double *points, int np, const int *tris,
   select_input(0);
 }
 
+// Every index the init kernels dereference, counted on the device: vertex
+// ids of triangles, neighbour ids, segment vertex ids, instance ids.
+__global__ void k_validate_input(const int *T, const int *N, int nt, const int *S, int ns, int np,
+                                 const int *tinst, int n_inst, unsigned long long *bad) {
+  unsigned long long b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+  const long long n = max(3LL * nt, 2LL * ns);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (i < 3LL * nt) {
+      const int a = T[i], u = N[i];
+      b0 += a < 0 || a >= np;
+      b1 += u < -1 || u >= nt;
+      if (tinst && i < nt) b3 += tinst[i] < 0 || tinst[i] >= n_inst;
+    }
+    if (i < 2LL * ns) b2 += S[i] < 0 || S[i] >= np;
+  }
+  if (b0) atomicAdd(&bad[0], b0);
+  if (b1) atomicAdd(&bad[1], b1);
+  if (b2) atomicAdd(&bad[2], b2);
+  if (b3) atomicAdd(&bad[3], b3);
+}
+
+// Copies one input CDT into `in` (buffers large enough) and validates its
+// indices on the device before any kernel gathers through them.
+void Refiner::copy_input(InputSet &in, const double *points, int np, const int *tris,
+                         const int *nbrs, int nt, const int *segs, int ns, const int *tri_inst,
+                         const InstPar *ipar, int n_inst) {
+  in.np = np; in.nt = nt; in.ns = ns;
+  in.n_inst = tri_inst ? n_inst : 0;
+  HC(hipMemcpyAsync(in.P, points, sizeof(double2) * np, hipMemcpyHostToDevice, stream));
+  HC(hipMemcpyAsync(in.T, tris, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, stream));
+  HC(hipMemcpyAsync(in.N, nbrs, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, stream));
+  if (ns) HC(hipMemcpyAsync(in.S, segs, sizeof(int) * 2 * ns, hipMemcpyHostToDevice, stream));
+  if (tri_inst) {
+    HC(hipMemcpyAsync(in.tinst, tri_inst, sizeof(int) * nt, hipMemcpyHostToDevice, stream));
+    HC(hipMemcpyAsync(in.ipar, ipar, sizeof(InstPar) * n_inst, hipMemcpyHostToDevice, stream));
+  }
+  HC(hipMemsetAsync(d_val, 0, 4 * sizeof(unsigned long long), stream));
+  k_validate_input<<<std::min(nblk(std::max(3LL * nt, 2LL * ns)), 4096), 256, 0, stream>>>(
+      in.T, in.N, nt, in.S, ns, np, tri_inst ? in.tinst : nullptr, n_inst, d_val);
+  HC(hipGetLastError());
+  HC(hipMemcpyAsync(h_val4, d_val, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+  HC(hipStreamSynchronize(stream));
+  if (h_val4[0]) throw std::runtime_error("triangle vertex index out of range");
+  if (h_val4[1]) throw std::runtime_error("neighbour index out of range");
+  if (h_val4[2]) throw std::runtime_error("segment vertex index out of range");
+  if (h_val4[3]) throw std::runtime_error("triangle instance index out of range");
+}
+
 int Refiner::add_input(const double *points, int np, const int *tris,
                        const int *nbrs, int nt, const int *segs, int ns,
                        const int *tri_inst, const InstPar *ipar, int n_inst) {
   if (np < 3 || nt < 1) throw std::runtime_error("input mesh is empty");
   if ((long long)nt * 4 >= (1LL << 31) / 2)
     throw std::runtime_error("too many triangles for 32-bit otri encoding");
-  // Host-side validation: every index the init kernels dereference.
-  for (long long i = 0; i < 3LL * nt; i++) {
-    if (tris[i] < 0 || tris[i] >= np)
-      throw std::runtime_error("triangle vertex index out of range");
-    if (nbrs[i] < -1 || nbrs[i] >= nt)
-      throw std::runtime_error("neighbour index out of range");
-  }
-  for (long long i = 0; i < 2LL * ns; i++)
-    if (segs[i] < 0 || segs[i] >= np)
-      throw std::runtime_error("segment vertex index out of range");
-  if (tri_inst)
-    for (long long i = 0; i < nt; i++)
-      if (tri_inst[i] < 0 || tri_inst[i] >= n_inst)
-        throw std::runtime_error("triangle instance index out of range");
+  if (ns < 0 || (tri_inst && n_inst < 1)) throw std::runtime_error("invalid input counts");
   InputSet in;
-  in.np = np; in.nt = nt; in.ns = ns;
-  in.n_inst = tri_inst ? n_inst : 0;
+  in.capP = np; in.capT = nt; in.capS = ns;
   dalloc(&in.P, (size_t)np);
   dalloc(&in.T, 3 * (size_t)nt);
   dalloc(&in.N, 3 * (size_t)nt);
   dalloc(&in.S, 2 * (size_t)(ns > 0 ? ns : 1));
-  HC(hipMemcpyAsync(in.P, points, sizeof(double2) * np, hipMemcpyHostToDevice, stream));
-  HC(hipMemcpyAsync(in.T, tris, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, stream));
-  HC(hipMemcpyAsync(in.N, nbrs, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, stream));
-  if (ns) HC(hipMemcpyAsync(in.S, segs, sizeof(int) * 2 * ns, hipMemcpyHostToDevice, stream));
   if (tri_inst) {
     dalloc(&in.tinst, (size_t)nt);
     dalloc(&in.ipar, (size_t)n_inst);
-    HC(hipMemcpyAsync(in.tinst, tri_inst, sizeof(int) * nt, hipMemcpyHostToDevice, stream));
-    HC(hipMemcpyAsync(in.ipar, ipar, sizeof(InstPar) * n_inst, hipMemcpyHostToDevice, stream));
   }
-  HC(hipStreamSynchronize(stream));
+  try {
+    copy_input(in, points, np, tris, nbrs, nt, segs, ns, tri_inst, ipar, n_inst);
+  } catch (...) {
+    dfree(in.P); dfree(in.T); dfree(in.N); dfree(in.S); dfree(in.tinst); dfree(in.ipar);
+    throw;
+  }
   inputs.push_back(in);
   return (int)inputs.size() - 1;
 }
@@ -2002,6 +2070,17 @@ void Refiner::read_counters() {
   HC(hipStreamSynchronize(stream));
 }
 
+// Candidate workspace of at least `ints` ints (grown by half again at least,
+// capped by the budget; passes never need more than the budget).
+void Refiner::ensure_ws(long long ints) {
+  ints = std::min(ints, opt.cand_budget);
+  if (c_ws && ws_ints >= ints) return;
+  long long c = std::min(opt.cand_budget, std::max(ints, ws_ints + ws_ints / 2));
+  dfree(c_ws);
+  dalloc(&c_ws, (size_t)std::max(c, 1LL));
+  ws_ints = c;
+}
+
 void Refiner::ensure_cand(int nc) {
   if (nc <= c_cap) return;
   int c = std::max(nc, c_cap * 2);
@@ -2138,6 +2217,7 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
   // room for every candidate winning: the winners' count is only read back
   // with the pass counters, after the apply kernel
   ensure_capacity(np + (long long)nc, nt + 2LL * nc, ns + (long long)nseg);
+  ensure_ws((long long)nc * (6LL * K + 16));
   PassD ps = pass_base(K);
   ps.ids = ids;
   ps.nc = nc;
@@ -2161,11 +2241,11 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
   ps.feat = want_feat ? c_feat : nullptr;
   ps.pass_index = pass_index;
   HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
-  // 64/lg candidates per wave64 block; the exact-predicate arena holds
-  // pws_blocks*256 lanes, i.e. pws_blocks*4 blocks of 64
+  // 64/lg candidates per wave64 block (the exact kernel's grid is capped by
+  // its arena in launch_cavity)
   int lg = cavity_lanes(std::min(K, opt.cav_lds_cap));
   auto cav_grid = [&](int n) {
-    return std::min((n + 64 / lg - 1) / (64 / lg), std::min(opt.grid_cavity, pws_blocks * 4));
+    return std::min((n + 64 / lg - 1) / (64 / lg), opt.grid_cavity);
   };
   if (nseg > 0) {
     tbegin(K_SEGCAV);
@@ -2306,15 +2386,14 @@ void Refiner::run() {
   log_X.clear();
   log_t.clear();
   HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * 2 * capT, stream));
-  if (!c_ws || ws_ints < opt.cand_budget) {
-    dfree(c_ws);
-    dalloc(&c_ws, (size_t)opt.cand_budget);
-    ws_ints = opt.cand_budget;
-  }
-  int want_pws = (opt.grid_cavity + 3) / 4;
-  if (!c_pws || pws_blocks < want_pws) {
-    dfree(c_pws);
-    pws_blocks = std::max(want_pws, 1);
+  // Candidate workspace: the graph loop sizes its passes on the device and
+  // takes the whole budget; the host loop grows it to each pass's need.
+  if (opt.graph) ensure_ws(opt.cand_budget);
+  // Exact-stage arena: GCDT_INCIRCLE_WS doubles for each lane of the exact
+  // kernel's grid (at most EXACT_GRID blocks of 64; the filtered kernel does
+  // not use it).
+  if (!c_pws) {
+    pws_blocks = EXACT_GRID / 4;
     dalloc(&c_pws, (size_t)pws_blocks * 256 * GCDT_INCIRCLE_WS);
   }
   st.status = 1;
@@ -2530,7 +2609,7 @@ void Refiner::launch_cavity(bool is_seg, int grid, const PassD &ps) {
   if (lds_slow > (size_t)BIG_LDS_BYTES)
     throw std::runtime_error("cavity capacity " + std::to_string(ps.K) + " needs more LDS than a block has");
   if (grid < 1) grid = 1;
-  int slow_grid = std::min(grid, std::min(1024, pws_blocks * 4));
+  int slow_grid = std::min(grid, pws_blocks * 4);
   if (is_seg) {
     launch_cavity_t<true, true>(lg, grid, lds, stream, m, pr, ps, opt.cav_cap_max);
     launch_cavity_t<true, false>(lgs, slow_grid, lds_slow, stream, m, pr, ps, opt.cav_cap_max);
@@ -2761,25 +2840,75 @@ void Refiner::download(MeshHost &out) {
   out.S.resize(2 * (size_t)ns);
   out.pstat.resize((size_t)np);
   out.tstat.resize((size_t)nt_out);
-  if (np > o_pcap) {
-    dfree(o_pstat);
-    o_pcap = np;
-    dalloc(&o_pstat, (size_t)o_pcap);
-  }
-  k_pstatus<<<nblk(np), 256, 0, stream>>>(m.pflag, np, o_pstat);
-  HC(hipGetLastError());
-  HC(hipMemcpyAsync(out.P.data(), m.P, sizeof(double2) * np, hipMemcpyDeviceToHost, stream));
-  if (nt_out) {
-    HC(hipMemcpyAsync(out.T.data(), o_T, sizeof(int) * 3 * nt_out, hipMemcpyDeviceToHost, stream));
-    HC(hipMemcpyAsync(out.N.data(), o_N, sizeof(int) * 3 * nt_out, hipMemcpyDeviceToHost, stream));
-    HC(hipMemcpyAsync(out.tstat.data(), o_tstat, nt_out, hipMemcpyDeviceToHost, stream));
-  }
-  if (ns) HC(hipMemcpyAsync(out.S.data(), m.SV, sizeof(int2) * ns, hipMemcpyDeviceToHost, stream));
-  HC(hipMemcpyAsync(out.pstat.data(), o_pstat, np, hipMemcpyDeviceToHost, stream));
   out.tinst.clear();
-  if (m.batched && nt_out) {
-    out.tinst.resize((size_t)nt_out);
-    HC(hipMemcpyAsync(out.tinst.data(), o_inst, sizeof(int) * nt_out, hipMemcpyDeviceToHost, stream));
+  if (m.batched && nt_out) out.tinst.resize((size_t)nt_out);
+  HostDst d;
+  d.P = out.P.data();
+  d.T = out.T.data();
+  d.N = out.N.data();
+  d.S = out.S.data();
+  d.pstat = out.pstat.data();
+  d.tstat = out.tstat.data();
+  d.tinst = out.tinst.empty() ? nullptr : out.tinst.data();
+  download_into(d);
+}
+
+// Device-to-host in chunks through two pinned staging buffers: the copy
+// engine fills one while the host threads copy the other into the caller's
+// arrays (pageable, typically never touched: the page faults are spread
+// over the threads too).
+void Refiner::download_into(const HostDst &d) {
+  struct Piece {
+    void *dst;
+    const char *src;
+    size_t bytes;
+  };
+  std::vector<Piece> pieces;
+  if (d.pstat) {
+    if (np > o_pcap) {
+      dfree(o_pstat);
+      o_pcap = np;
+      dalloc(&o_pstat, (size_t)o_pcap);
+    }
+    k_pstatus<<<nblk(np), 256, 0, stream>>>(m.pflag, np, o_pstat);
+    HC(hipGetLastError());
+  }
+  const int skip = d.P_in_host ? std::min(np, d.np_in_host) : 0;
+  if (d.P && np > skip)
+    pieces.push_back({d.P + 2 * (size_t)skip, (const char *)(m.P + skip),
+                      sizeof(double2) * (size_t)(np - skip)});
+  if (d.T && nt_out) pieces.push_back({d.T, (const char *)o_T, sizeof(int) * 3 * (size_t)nt_out});
+  if (d.N && nt_out) pieces.push_back({d.N, (const char *)o_N, sizeof(int) * 3 * (size_t)nt_out});
+  if (d.S && ns) pieces.push_back({d.S, (const char *)m.SV, sizeof(int2) * (size_t)ns});
+  if (d.pstat && np) pieces.push_back({d.pstat, (const char *)o_pstat, (size_t)np});
+  if (d.tstat && nt_out) pieces.push_back({d.tstat, (const char *)o_tstat, (size_t)nt_out});
+  if (d.tinst && m.batched && nt_out)
+    pieces.push_back({d.tinst, (const char *)o_inst, sizeof(int) * (size_t)nt_out});
+  if (!copier) copier = new CopyPool(CopyPool::default_workers());
+  if (!h_stage[0]) {
+    stage_bytes = size_t(32) << 20;
+    for (int k = 0; k < 2; k++) HC(hipHostMalloc((void **)&h_stage[k], stage_bytes, hipHostMallocDefault));
+  }
+  struct Chunk {
+    char *dst;
+    const char *src;
+    size_t bytes;
+  };
+  std::vector<Chunk> ch;
+  for (const Piece &p : pieces)
+    for (size_t off = 0; off < p.bytes; off += stage_bytes)
+      ch.push_back({(char *)p.dst + off, p.src + off, std::min(stage_bytes, p.bytes - off)});
+  auto issue = [&](size_t k) {
+    HC(hipMemcpyAsync(h_stage[k & 1], ch[k].src, ch[k].bytes, hipMemcpyDeviceToHost, stream));
+    HC(hipEventRecord(ev_stage[k & 1], stream));
+  };
+  for (size_t k = 0; k < std::min<size_t>(2, ch.size()); k++) issue(k);
+  // the input points need no transfer: copied while the first chunks travel
+  if (d.P && skip) copier->copy(d.P, d.P_in_host, sizeof(double) * 2 * (size_t)skip);
+  for (size_t k = 0; k < ch.size(); k++) {
+    HC(hipEventSynchronize(ev_stage[k & 1]));
+    copier->copy(ch[k].dst, h_stage[k & 1], ch[k].bytes);
+    if (k + 2 < ch.size()) issue(k + 2);
   }
   HC(hipStreamSynchronize(stream));
 }

@@ -79,10 +79,16 @@ typedef struct TStatus { unsigned char _status; } TStatus;
  * reference.  `input` is a CDT in Triangle's convention (CCW triangles,
  * neighbour j opposite corner j, every convex-hull edge in segmentlist).
  * `result` receives pointlist/trianglelist/neighborlist/segmentlist
- * allocated with new[] (free with delete[]) and the four counts.
+ * allocated with new[] (free with delete[]) and the four counts; the first
+ * numberofpoints input points are the input points, unmoved.
  * `deletepolicy` and `debug_iter` are accepted and ignored, as in the
  * reference.  On a device error the function prints it and exits, as the
- * reference's checkCUDAError does (refine.cu:20-29). */
+ * reference's checkCUDAError does (refine.cu:20-29).
+ * It runs the chosen form (FP32-filtered exact predicates, three claim
+ * iterations with shared rings, the learned priority compiled into the
+ * library) unless gcdt_dropin_set_options chose other options; the device
+ * buffers of one cached refiner serve every call (gcdt_dropin_release frees
+ * them).  Calls are serialised by a process-wide lock. */
 void GPU_Refine_Quality(struct triangulateio *input,
                         struct triangulateio *result, double theta,
                         InsertPolicy insertpolicy, DeletePolicy deletepolicy,
@@ -210,6 +216,22 @@ typedef struct gcdt_kernel_time {
 } gcdt_kernel_time;
 
 void gcdt_default_options(gcdt_options *opt);
+
+/* Options and model of the drop-in GPU_Refine_Quality (no reference
+ * counterpart: the reference's entry point has one fixed method).
+ * gcdt_dropin_set_options: every field but theta_deg / offcenter / mode
+ * (those come from each call); NULL restores the chosen form.  Returns -1
+ * (gcdt_last_error) on invalid options.  gcdt_dropin_set_priority_model:
+ * the priority network's gcdt_mlp_nparam() FP32 weights, NULL = the
+ * trained weights compiled into the library.  gcdt_dropin_last_stats: the
+ * counters of the last call.  gcdt_dropin_release: frees the cached device
+ * buffers.  gcdt_dropin_free: delete[]s the arrays a call allocated, for
+ * callers (FFI) that cannot. */
+int gcdt_dropin_set_options(const gcdt_options *opt);
+int gcdt_dropin_set_priority_model(const float *params);
+int gcdt_dropin_last_stats(gcdt_stats *stats);
+void gcdt_dropin_release(void);
+void gcdt_dropin_free(struct triangulateio *result, PStatus *ps_debug, TStatus *ts_debug);
 
 /* Refine a CDT given in host memory; returns 0 on success.  On success
  * *out holds the refined mesh (free with gcdt_mesh_free). */
