@@ -6,18 +6,22 @@ than 2% of the domain width, convex hull enclosed by segments, constrained
 Delaunay triangulated on the host (untimed input preparation), then refined
 at theta = 25 deg with off-centers in Ruppert mode.
 
-A "step" is one full refinement of that CDT (GPU_Refine_Quality's work),
-starting from the CDT already resident in HBM and ending with the compacted
-output mesh in HBM.  Two forms are timed in the same run:
-  plain   FP64-filtered exact predicates, one claim iteration, host-driven
-          passes (the reference's structure)
-  chosen  FP32 -> FP64 -> double-double -> exact predicate cascade, claim
-          iterations with shared rings (winners' rings may overlap, their
-          cavities may not meet another winner's region), learned claim
-          priority (when the trained model is present), and the round driver
-          (host- or graph-driven) and launch shapes chosen by the autotuner
-          on this input (untimed)
-value = output triangles / second of wall time of the chosen form.
+A "step" is one call of the drop-in GPU_Refine_Quality (reference signature,
+refine.h:385-386) on that CDT: host triangulateio in, upload, refinement,
+compaction, download, host triangulateio out -- the boundary the reference
+times (main.cpp:1860-1868).  Two forms are timed in the same run:
+  plain   FP64-filtered exact predicates, one claim iteration, Triangle's
+          shortest-edge queue order, host-driven passes (the reference's
+          structure)
+  chosen  the drop-in's default: FP32 -> FP64 -> double-double -> exact
+          predicate cascade, claim iterations with shared rings (winners'
+          rings may overlap, their cavities may not meet another winner's
+          region), the learned claim priority compiled into the library,
+          fixed launch shapes
+value = output triangles / median wall time of the chosen form's calls.
+The same forms are also timed device-resident (a session whose CDT stays
+in HBM; the chosen form with autotuned shapes), and the plain form with the
+reference's lowest-index claim rule (priority 0, refine.cu:4159-4172).
 
 Prints ONE JSON line.  Also reports:
   roofline      the dominant kernel's algorithmic bytes per launch / its
@@ -260,6 +264,24 @@ def refine_workload_name(args):
             f"cdt-refine-{args.points}-uniform-{args.segments}-segs")
 
 
+def dropin_runs(g, P, T, N, SS, steps, warmup, tag, theta):
+    """W untimed and K timed calls of the drop-in GPU_Refine_Quality (host
+    arrays in, host arrays out); each call's own wall time, as the
+    reference times it (steady clock around the call, main.cpp:1860-1868)."""
+    for i in range(warmup):
+        r, dt = g.dropin_refine(P, T, N, SS, theta=theta)
+        log(f"# {tag} warmup {i}: {dt * 1e3:.1f} ms wall, {r.stats['device_ms']:.1f} ms device, "
+            f"{len(r.triangles)} tris, rounds {r.stats['rounds']}")
+    walls, devs = [], []
+    for i in range(steps):
+        r, dt = g.dropin_refine(P, T, N, SS, theta=theta)
+        walls.append(dt * 1e3)
+        devs.append(r.stats["device_ms"])
+        log(f"# {tag} step {i}: wall {dt * 1e3:.1f} ms, device {r.stats['device_ms']:.1f} ms")
+    MEDIANS[tag] = {"wall_ms": float(np.median(walls)), "device_ms": float(np.median(devs))}
+    return r, walls, devs
+
+
 def bench_refine(args):
     if args.calibration_copy:
         calibration_copy()
@@ -270,13 +292,34 @@ def bench_refine(args):
     log(f"# input: {len(P)} points, {len(T)} triangles, {len(SS)} segments "
         f"(prep {time.time() - t0:.1f}s)")
     kw = dict(theta=args.theta, insert=g.Offcenter, mode=g.RUPPERT)
-    # plain GPU path
-    st_p, ms_p, dev_p = None, None, None
+    # ---- headline: the drop-in, host arrays in and out (BASELINE boundary)
+    rp = None
     if not args.no_plain:
-        sp = g.Session(P, T, N, SS, **kw, **plain_options())
-        st_p, ms_p, dev_p = timed_runs(sp, args.steps, args.warmup, "plain")
-        sp.close()
-    # chosen form
+        g.dropin_set_options(plain=True, **plain_options())
+        rp, wp, dp = dropin_runs(g, P, T, N, SS, args.steps, args.warmup, "dropin plain",
+                                 args.theta)
+    g.dropin_reset()  # the chosen form with the compiled-in network
+    rc, wc, dc = dropin_runs(g, P, T, N, SS, args.steps, args.warmup, "dropin chosen", args.theta)
+    g.dropin_reset()
+    st = rc.stats
+    tris = int(st["n_triangles"])
+    ms = float(np.median(wc))
+    value = tris / (ms / 1e3)
+    # ---- device-resident sessions (extra fields): the CDT already in HBM
+    resident = {}
+    st_p = None
+    if not args.no_plain:
+        for name, o in (("plain", plain_options()),
+                        ("plain_lowest_index", dict(plain_options(), priority=0))):
+            sp = g.Session(P, T, N, SS, **kw, **o)
+            s_, m_, d_ = timed_runs(sp, args.steps, args.warmup, name)
+            sp.close()
+            resident[name] = {"ms_per_step": MEDIANS[name]["wall_ms"],
+                              "device_ms_per_step": MEDIANS[name]["device_ms"],
+                              "value": s_["n_triangles"] / (MEDIANS[name]["wall_ms"] / 1e3),
+                              "rounds": s_["rounds"], "n_points": s_["n_points"]}
+            if name == "plain":
+                st_p = s_
     co = chosen_options(args)
     sc = g.Session(P, T, N, SS, **kw, **co)
     if co["priority"] == 2:
@@ -286,36 +329,32 @@ def bench_refine(args):
         best, trials = sc.autotune(reps=1)
         tune = {"best": best, "trials": len(trials)}
         log(f"# autotune: {best}")
-    st, ms, dev = timed_runs(sc, args.steps, args.warmup, "chosen")
-    # exact invariants of the chosen form's mesh, checked in HBM (untimed)
+    st_s, ms_s, dev_s = timed_runs(sc, args.steps, args.warmup, "chosen")
+    resident["chosen"] = {"ms_per_step": MEDIANS["chosen"]["wall_ms"],
+                          "device_ms_per_step": MEDIANS["chosen"]["device_ms"],
+                          "value": st_s["n_triangles"] / (MEDIANS["chosen"]["wall_ms"] / 1e3),
+                          "rounds": st_s["rounds"], "n_points": st_s["n_points"],
+                          "options": co, "autotune": tune}
+    if "plain" in resident:
+        resident["speedup_vs_plain"] = (resident["plain"]["ms_per_step"] /
+                                        resident["chosen"]["ms_per_step"])
+    # exact invariants of the chosen form's mesh, checked in HBM (untimed):
+    # range, symmetry, orientation, boundary, Delaunay, subsegments, segment
+    # coverage, input points, quality with the MPW exemption, multiplicity
     t1 = time.perf_counter()
     gcheck = sc.check()
     gcheck["ms"] = (time.perf_counter() - t1) * 1e3
     sc.close()
-    tris = int(st["n_triangles"])
-    value = tris / (ms / 1e3)
     b_alg = whole_job_bytes(len(P), len(T), len(SS), st)
     whole = {"bytes": b_alg, "achieved_GBs": b_alg / (ms / 1e3) / 1e9,
              "frac": (b_alg / (ms / 1e3) / 1e9) / HBM_PEAK_GBS}
     roof, kt = None, None
     if not args.no_profile_step:
-        # per-launch events need the host-driven passes; the launch shapes
-        # are the tuned ones
-        po = dict(co)
-        if tune:
-            po.update({k: v for k, v in tune["best"].items()
-                       if k in ("grid_cavity", "grid_claim", "grid_apply", "cavity_lanes",
-                                "cav_lds_cap")})
+        # per-launch events need the host-driven passes; the drop-in's chosen
+        # form and launch shapes
+        po = dict(g.CHOSEN_FORM)
         kt, kst = profile_kernels(P, T, N, SS, args.theta, po)
-        roof = roofline(kt, kst, refine_workload_name(args), co["priority"] == 2)
-    dropin = None
-    if not args.no_dropin:
-        # PCIe-inclusive: host CDT in, host mesh out (the drop-in's contract)
-        t1 = time.perf_counter()
-        r = g.refine(P, T, N, SS, **kw, **{k: v for k, v in co.items() if k != "priority"})
-        dt = time.perf_counter() - t1
-        dropin = {"value": len(r.triangles) / dt, "ms": dt * 1e3,
-                  "note": "host arrays in and out (upload, refine, download), priority 1"}
+        roof = roofline(kt, kst, refine_workload_name(args), po["priority"] == 2)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, args.cpu_sample_points)
     out = {
         "metric": "output triangles/sec (refine, theta=25 deg, off-center, Ruppert): chosen form",
@@ -325,7 +364,8 @@ def bench_refine(args):
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms,
-        "device_ms_per_step": dev,
+        "mean_ms_per_step": float(np.mean(wc)),
+        "device_ms_per_step": float(np.median(dc)),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -335,17 +375,21 @@ def bench_refine(args):
                    "segments": args.segments, "domain": args.domain,
                    "max_segment_length": args.domain * 0.02, "theta_deg": args.theta,
                    "insert": "offcenter", "mode": "ruppert", "seed": args.seed,
-                   "chosen_options": co, "autotune": tune},
-        "plain": None if st_p is None else {
-            "value": st_p["n_triangles"] / (ms_p / 1e3), "ms_per_step": ms_p,
-            "device_ms_per_step": dev_p, "output": summary(st_p)},
-        "speedup_vs_plain": None if st_p is None else ms_p / ms,
-        "steiner_delta_vs_plain": None if st_p is None else
-            (st["n_points"] - st_p["n_points"]) / max(1, st_p["n_points"] - len(P)),
+                   "boundary": "drop-in GPU_Refine_Quality: host triangulateio in and out, "
+                               "median call wall time after warm-up (BASELINE.md section 2)",
+                   "chosen_options": g.CHOSEN_FORM},
+        "plain": None if rp is None else {
+            "value": rp.stats["n_triangles"] / (np.median(wp) / 1e3),
+            "ms_per_step": float(np.median(wp)), "device_ms_per_step": float(np.median(dp)),
+            "options": plain_options(), "output": summary(rp.stats)},
+        "speedup_vs_plain": None if rp is None else float(np.median(wp)) / ms,
+        "steiner_delta_vs_plain": None if rp is None else
+            (st["n_points"] - rp.stats["n_points"]) / max(1, rp.stats["n_points"] - len(P)),
+        "dropin_overhead": ms / float(np.median(dc)),
         "output": summary(st),
+        "device_resident": resident,
         "roofline": roof,
         "whole_job": whole,
-        "dropin_pcie_inclusive": dropin,
         "gpu_check": gcheck,
         "cpu_baseline": cpu,
         "kernel_times_ms": {k: round(v["total_ms"], 3) for k, v in kt.items()} if kt else None,

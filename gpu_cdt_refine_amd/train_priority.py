@@ -91,6 +91,10 @@ def main():
             "val_mse_constant_predictor": base_mse, "collect_s": t1 - t0, "train_s": t2 - t1}
     with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
         json.dump(meta, f, indent=1)
+    if os.path.abspath(args.out) == os.path.abspath(DEFAULT_OUT):
+        # the drop-in's compiled-in copy (rebuild libgcdt.so to pick it up)
+        from gpu_cdt_refine_amd import embed_model
+        embed_model.main()
     print(json.dumps(meta))
 
 
