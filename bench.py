@@ -612,6 +612,17 @@ def bench_batch(args):
     print(json.dumps(out))
 
 
+def bad_triangles(P, T, theta):
+    """Mask of the triangles whose smallest angle is below theta (atan2)."""
+    a, b, c = P[T[:, 0]], P[T[:, 1]], P[T[:, 2]]
+
+    def ang(p, q, r):
+        u, v = q - p, r - p
+        return np.degrees(np.abs(np.arctan2(u[:, 0] * v[:, 1] - u[:, 1] * v[:, 0],
+                                            (u * v).sum(1))))
+    return np.minimum(np.minimum(ang(a, b, c), ang(b, c, a)), ang(c, a, b)) < theta
+
+
 def bench_precision(args):
     """Precision stress: lattice + cocircular + uniform points, lattice-line
     segments; stage hit rates of the predicate cascade and Steiner-point error
@@ -630,20 +641,42 @@ def bench_precision(args):
     r = s.download()
     s.close()
     tot_pred = st["pred_fp32"] + st["pred_fp64"] + st["pred_dd"] + st["pred_exact"]
-    # Steiner error on a sample of the input's skinny triangles
+    # Steiner error on the triangles the refinement splits: the bad triangles
+    # of the input CDT and of the meshes after 2, 8 and 32 rounds (sessions
+    # capped with max_rounds), 300 of each, both insertion modes; the FP64
+    # construction (bit-identical to the kernels') against exact rationals
     rng = np.random.default_rng(0)
-    idx = rng.choice(len(T), min(2000, len(T)), replace=False)
-    errs = []
-    for t in idx:
-        a, o, d = P[T[t, 0]], P[T[t, 1]], P[T[t, 2]]
-        c = pyoracle.steiner_point(a, o, d, 25.0, 1)
-        if c is None:
-            continue
-        ex = pyoracle.steiner_point_exact(a, o, d, 25.0, 1)
-        err = max(abs(Fraction(c[0]) - ex[0]), abs(Fraction(c[1]) - ex[1]))
-        scale = max(abs(float(ex[0])), abs(float(ex[1])), 1e-300)
-        errs.append((float(err), float(err) / (np.spacing(scale))))
-    errs = np.array(errs)
+    errs = {1: [], 0: []}
+    for cap in (0, 2, 8, 32):
+        if cap == 0:
+            Pm, Tm = P, T
+        else:
+            sc = g.Session(P, T, N, SS, theta=25.0, max_rounds=cap, **co)
+            if co["priority"] == 2:
+                sc.set_priority_model(np.load(MODEL))
+            sc.run()
+            rm = sc.download()
+            sc.close()
+            Pm, Tm = rm.points, rm.triangles
+        idx = np.nonzero(bad_triangles(Pm, Tm, 25.0))[0]
+        idx = rng.choice(idx, min(300, len(idx)), replace=False)
+        for t in idx:
+            a, o, d = Pm[Tm[t, 0]], Pm[Tm[t, 1]], Pm[Tm[t, 2]]
+            for off in (1, 0):
+                c = pyoracle.steiner_point(a, o, d, 25.0, off)
+                if c is None:
+                    continue
+                ex = pyoracle.steiner_point_exact(a, o, d, 25.0, off)
+                err = max(abs(Fraction(c[0]) - ex[0]), abs(Fraction(c[1]) - ex[1]))
+                scale = max(abs(float(ex[0])), abs(float(ex[1])), 1e-300)
+                errs[off].append((float(err), float(err) / (np.spacing(scale))))
+    steiner_error = {}
+    for off, name in ((1, "offcenter"), (0, "circumcenter")):
+        e = np.array(errs[off])
+        steiner_error[name] = {"samples": int(len(e)), "max_abs": float(e[:, 0].max()),
+                               "median_ulps": float(np.median(e[:, 1])),
+                               "p99_ulps": float(np.percentile(e[:, 1], 99)),
+                               "max_ulps": float(e[:, 1].max())}
     out = {"metric": "precision stress: predicate stage hit rates and Steiner error",
            "value": st["n_triangles"] / (ms / 1e3), "unit": "triangles/s", "n_gpus": 1,
            "ms_per_step": ms, "higher_is_better": True,
@@ -651,9 +684,8 @@ def bench_precision(args):
            "stage_rates": {k: st[k] / max(1, tot_pred) for k in
                            ("pred_fp32", "pred_fp64", "pred_dd", "pred_exact")},
            "stage_counts": {k: st[k] for k in ("pred_fp32", "pred_fp64", "pred_dd", "pred_exact")},
-           "steiner_error": {"samples": int(len(errs)), "max_abs": float(errs[:, 0].max()),
-                             "median_ulps": float(np.median(errs[:, 1])),
-                             "max_ulps": float(errs[:, 1].max())},
+           "steiner_error": steiner_error,
+           "giveups": {k: st[k] for k in ("giveups", "giveup_tris_left", "giveup_segs_left")},
            "output": summary(st)}
     print(json.dumps(out))
 

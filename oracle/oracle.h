@@ -109,6 +109,19 @@ int orc_steiner_point(const double *apex, const double *org, const double *dest,
  * adaptive stage's that certifies it. */
 double orc_ccw_value(const double *pa, const double *pb, const double *pc);
 
+/* The refinement's per-element rules on plain coordinates (for closed-form
+ * tests): encroachment of segment a-b by p at theta degrees (ruppert = 1:
+ * diametral circle, 0: Chew's lens); the split point of segment A-B given
+ * which ends are shared with other subsegments (0 when it coincides with an
+ * endpoint); the quality test of triangle (apex, org, dest) without the
+ * Miller-Pav-Walkington exemption. */
+int orc_encroaches_xy(const double *a, const double *b, const double *p, double theta_deg,
+                      int ruppert);
+int orc_split_point_xy(const double *A, const double *B, int acuteorg, int acutedest,
+                       double *p);
+int orc_bad_angle_xy(const double *apex, const double *org, const double *dest,
+                     double theta_deg);
+
 /* Invariant checker.  Returns 0 when every check passes; otherwise the
  * number of violations, with per-check counts in viol[] (see check_oracle.c
  * for the index meaning). */

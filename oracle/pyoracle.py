@@ -95,6 +95,12 @@ def orc():
         L.orc_orient2d.argtypes = [vp, vp, vp]
         L.orc_incircle.argtypes = [vp, vp, vp, vp]
         L.orc_steiner_point.argtypes = [vp, vp, vp, C.c_double, C.c_int, vp]
+        L.orc_encroaches_xy.argtypes = [vp, vp, vp, C.c_double, C.c_int]
+        L.orc_encroaches_xy.restype = C.c_int
+        L.orc_split_point_xy.argtypes = [vp, vp, C.c_int, C.c_int, vp]
+        L.orc_split_point_xy.restype = C.c_int
+        L.orc_bad_angle_xy.argtypes = [vp, vp, vp, C.c_double]
+        L.orc_bad_angle_xy.restype = C.c_int
         L.orc_ccw_value.restype = C.c_double
         L.orc_ccw_value.argtypes = [vp, vp, vp]
         _orc = L
@@ -295,6 +301,27 @@ def steiner_point(apex, org, dest, theta, offcenter=1):
     out = np.empty(2)
     ok = orc().orc_steiner_point(_p(a), _p(o), _p(d), float(theta), int(offcenter), _p(out))
     return out if ok else None
+
+
+def encroaches(a, b, p, theta, ruppert=1):
+    """The refinement's encroachment rule for segment a-b and point p."""
+    a, b, p = (np.ascontiguousarray(x, np.float64) for x in (a, b, p))
+    return bool(orc().orc_encroaches_xy(_p(a), _p(b), _p(p), float(theta), int(ruppert)))
+
+
+def split_point(a, b, acuteorg=0, acutedest=0):
+    """The refinement's split point of segment a-b, or None when it would
+    coincide with an endpoint."""
+    a, b = (np.ascontiguousarray(x, np.float64) for x in (a, b))
+    out = np.zeros(2)
+    ok = orc().orc_split_point_xy(_p(a), _p(b), int(acuteorg), int(acutedest), _p(out))
+    return out if ok else None
+
+
+def bad_angle(apex, org, dest, theta):
+    """The refinement's quality test (no MPW exemption)."""
+    a, o, d = (np.ascontiguousarray(x, np.float64) for x in (apex, org, dest))
+    return bool(orc().orc_bad_angle_xy(_p(a), _p(o), _p(d), float(theta)))
 
 
 def steiner_point_exact(apex, org, dest, theta, offcenter=1):

@@ -137,23 +137,44 @@ static void use_inst(Mesh *m, int t) {
 
 /* checkseg4encroach: does apex p encroach subsegment (a,b)?
  * refine.cu:115-139, triangle.cpp:7138-7153 */
-static int encroaches(const Mesh *m, int a, int b, const double *p) {
-  const double *A = PT(m, a), *B = PT(m, b);
+static int encroaches_xy(const double *A, const double *B, const double *p, int ruppert,
+                         double lens) {
   double dot = (A[0] - p[0]) * (B[0] - p[0]) + (A[1] - p[1]) * (B[1] - p[1]);
   if (dot < 0.0) {
-    if (m->ruppert) return 1;
+    if (ruppert) return 1;
     double la = (A[0] - p[0]) * (A[0] - p[0]) + (A[1] - p[1]) * (A[1] - p[1]);
     double lb = (B[0] - p[0]) * (B[0] - p[0]) + (B[1] - p[1]) * (B[1] - p[1]);
-    if (dot * dot >= m->lens * la * lb) return 1;
+    if (dot * dot >= lens * la * lb) return 1;
   }
   return 0;
 }
 
+static int encroaches(const Mesh *m, int a, int b, const double *p) {
+  return encroaches_xy(PT(m, a), PT(m, b), p, m->ruppert, m->lens);
+}
+
+/* cos^2 theta and the lens constant (2 cos^2 theta - 1)^2 of an angle in
+ * degrees, as GPU_Refine_Quality derives them (refine.cu:6557-6567). */
+static void angle_constants(double theta_deg, double *goodangle, double *lens) {
+  double ga = cos(theta_deg * 3.141592653589793238462643383279502884197169399375105820974944592308 / 180.0);
+  *goodangle = ga * ga;
+  *lens = (2.0 * *goodangle - 1.0) * (2.0 * *goodangle - 1.0);
+}
+
+int orc_encroaches_xy(const double *a, const double *b, const double *p, double theta_deg,
+                      int ruppert) {
+  double goodangle, lens;
+  angle_constants(theta_deg, &goodangle, &lens);
+  return encroaches_xy(a, b, p, ruppert, lens);
+}
+
 /* testtriangle with Miller-Pav-Walkington exemption; triangle.cpp:7220-7384,
  * refine.cu:376-497.  org = v1, dest = v2, apex = v0 (orientation 0). */
-static int tri_is_bad(const Mesh *m, int t) {
-  const int *v = m->TV + 3 * (size_t)t;
-  const double *O = PT(m, v[1]), *D = PT(m, v[2]), *A = PT(m, v[0]);
+/* The angle test of testtriangle: cos^2 of the angle opposite the shortest
+ * edge against cos^2 theta; *e receives that edge (opposite corner e of
+ * (apex, org, dest)). */
+static int bad_angle_xy(const double *A, const double *O, const double *D, double goodangle,
+                        int *e_out) {
   double dxod = O[0] - D[0], dyod = O[1] - D[1];
   double dxda = D[0] - A[0], dyda = D[1] - A[1];
   double dxao = A[0] - O[0], dyao = A[1] - O[1];
@@ -161,21 +182,39 @@ static int tri_is_bad(const Mesh *m, int t) {
   double orglen = dxda * dxda + dyda * dyda;
   double destlen = dxao * dxao + dyao * dyao;
   double angle;
-  int e, b1, b2;
+  int e;
   if ((apexlen < orglen) && (apexlen < destlen)) {
     angle = dxda * dxao + dyda * dyao;
     angle = angle * angle / (orglen * destlen);
-    e = 0; b1 = v[1]; b2 = v[2];
+    e = 0;
   } else if (orglen < destlen) {
     angle = dxod * dxao + dyod * dyao;
     angle = angle * angle / (apexlen * destlen);
-    e = 1; b1 = v[2]; b2 = v[0];
+    e = 1;
   } else {
     angle = dxod * dxda + dyod * dyda;
     angle = angle * angle / (apexlen * orglen);
-    e = 2; b1 = v[0]; b2 = v[1];
+    e = 2;
   }
-  if (!(angle > m->goodangle)) return 0;
+  *e_out = e;
+  return angle > goodangle;
+}
+
+int orc_bad_angle_xy(const double *apex, const double *org, const double *dest,
+                     double theta_deg) {
+  double goodangle, lens;
+  int e;
+  angle_constants(theta_deg, &goodangle, &lens);
+  return bad_angle_xy(apex, org, dest, goodangle, &e);
+}
+
+static int tri_is_bad(const Mesh *m, int t) {
+  const int *v = m->TV + 3 * (size_t)t;
+  const double *O = PT(m, v[1]), *D = PT(m, v[2]), *A = PT(m, v[0]);
+  int e;
+  if (!bad_angle_xy(A, O, D, m->goodangle, &e)) return 0;
+  /* the shortest edge's endpoints (edge e joins corners e+1 and e+2) */
+  const int b1 = v[(e + 1) % 3], b2 = v[(e + 2) % 3];
   if ((m->pflag[b1] & PF_SEGV) && (m->pflag[b2] & PF_SEGV) &&
       m->TS[3 * (size_t)t + e] < 0) {
     int s1 = m->ppar[b1], s2 = m->ppar[b2];
@@ -398,6 +437,9 @@ static void seg_sides(const Mesh *m, int s, int *t1, int *j1, int *t2, int *j2) 
   else { *t1 = tb; *j1 = jb; *t2 = ta; *j2 = ja; }
 }
 
+static int split_point_xy(const double *A, const double *B, int acuteorg, int acutedest,
+                          double *p);
+
 /* splitencsegs split point: triangle.cpp:13290-13425, refine.cu:2686-2725.
  * Returns 0 if the point coincides with an endpoint (precision exhausted). */
 static int split_point(const Mesh *m, int s, double *p) {
@@ -415,7 +457,21 @@ static int split_point(const Mesh *m, int s, double *p) {
     if (m->TS[3 * t2 + (j2 + 1) % 3] >= 0) acuteorg = 1;
     if (m->TS[3 * t2 + (j2 + 2) % 3] >= 0) acutedest = 1;
   }
-  const double *A = PT(m, a), *B = PT(m, b);
+  return split_point_xy(PT(m, a), PT(m, b), acuteorg, acutedest, p);
+}
+
+/* The split point of segment A-B: the midpoint, or, when an end is shared
+ * with another subsegment (acuteorg / acutedest), a point at a power-of-two
+ * distance from that end (concentric shells; the destination wins when both
+ * are shared), then moved onto the segment's line by one rounding
+ * correction (triangle.cpp:13368-13415). */
+int orc_split_point_xy(const double *A, const double *B, int acuteorg, int acutedest,
+                       double *p) {
+  return split_point_xy(A, B, acuteorg, acutedest, p);
+}
+
+static int split_point_xy(const double *A, const double *B, int acuteorg, int acutedest,
+                          double *p) {
   double split;
   if (acuteorg || acutedest) {
     double len = sqrt((B[0] - A[0]) * (B[0] - A[0]) + (B[1] - A[1]) * (B[1] - A[1]));

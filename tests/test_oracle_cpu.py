@@ -345,3 +345,92 @@ def test_oracle_gives_up_triangles_stuck_on_given_up_subsegments(oracle):
     tot, viol, mina, ex = oracle.check_mesh(Po, To, No, So, P, SS, theta)
     assert 0 < viol["quality"] <= st["giveup_tris_left"], (viol, st)
     assert tot == viol["quality"], viol
+
+
+def _apex_angle_deg(a, b, p):
+    """Angle a-p-b in degrees from atan2 (independent of the dot formula)."""
+    ux, uy = a[0] - p[0], a[1] - p[1]
+    vx, vy = b[0] - p[0], b[1] - p[1]
+    return np.degrees(abs(np.arctan2(ux * vy - uy * vx, ux * vx + uy * vy)))
+
+
+def test_encroachment_is_the_apex_angle_rule(oracle):
+    """Closed form of the encroachment rule (triangle.cpp:7106-7203,
+    refine.cu:115-139): Ruppert's diametral circle holds p iff the angle
+    a-p-b exceeds 90 degrees; Chew's lens iff it is at least 180 - 2 theta.
+    Angles within 1e-7 degrees of a threshold are skipped."""
+    rng = np.random.default_rng(41)
+    seen = {(r, e): 0 for r in (0, 1) for e in (False, True)}
+    for k in range(20000):
+        scale = 10.0 ** rng.integers(-3, 6)
+        a = rng.uniform(-1, 1, 2) * scale
+        b = rng.uniform(-1, 1, 2) * scale
+        mid, half = 0.5 * (a + b), 0.5 * np.linalg.norm(b - a)
+        p = mid + rng.uniform(-1.3, 1.3, 2) * half
+        theta = float(rng.choice([20.7, 25.0, 28.0, 30.0, 33.0]))
+        ang = _apex_angle_deg(a, b, p)
+        for ruppert in (1, 0):
+            thr = 90.0 if ruppert else 180.0 - 2.0 * theta
+            if abs(ang - thr) < 1e-7:
+                continue
+            want = ang > thr
+            got = oracle.encroaches(a, b, p, theta, ruppert)
+            assert got == want, (a, b, p, theta, ruppert, ang)
+            seen[(ruppert, want)] += 1
+    assert min(seen.values()) > 500, seen
+
+
+def test_quality_test_is_the_smallest_angle_rule(oracle):
+    """Closed form of testtriangle's angle test (triangle.cpp:7244-7291): a
+    triangle is bad iff its smallest angle is below theta (angles within
+    1e-7 degrees of theta skipped)."""
+    rng = np.random.default_rng(43)
+    bad = good = 0
+    for k in range(20000):
+        scale = 10.0 ** rng.integers(-3, 6)
+        t = rng.uniform(-1, 1, (3, 2)) * scale
+        if k % 3 == 0:  # skinny ones
+            t[2] = t[0] + rng.uniform(0.05, 0.95) * (t[1] - t[0]) + rng.normal(0, 0.02, 2) * scale
+        angs = [_apex_angle_deg(t[(i + 1) % 3], t[(i + 2) % 3], t[i]) for i in range(3)]
+        theta = float(rng.choice([20.7, 25.0, 30.0]))
+        if abs(min(angs) - theta) < 1e-7 or min(angs) < 1e-6:
+            continue
+        got = oracle.bad_angle(t[0], t[1], t[2], theta)
+        assert got == (min(angs) < theta), (t, angs, theta)
+        bad += got
+        good += not got
+    assert bad > 2000 and good > 2000
+
+
+def test_split_point_rule(oracle):
+    """Closed form of the segment split (triangle.cpp:13368-13415,
+    refine.cu:2686-2725): the midpoint when no end is shared; otherwise a
+    point at a power-of-two distance d from the shared end (the destination
+    when both are shared) with len/3 <= d <= len/1.5 (concentric shells).
+    The point is on the segment's line within 2 ulps."""
+    from fractions import Fraction as F
+    rng = np.random.default_rng(47)
+    for k in range(6000):
+        scale = 10.0 ** rng.integers(-4, 7)
+        a = rng.uniform(-1, 1, 2) * scale
+        b = a + rng.uniform(-1, 1, 2) * scale * 10.0 ** rng.uniform(-6, 0)
+        ao, ad = int(k % 4 in (1, 3)), int(k % 4 in (2, 3))
+        p = oracle.split_point(a, b, ao, ad)
+        if p is None:
+            continue
+        L = float(np.linalg.norm(b - a))
+        if not (ao or ad):
+            assert np.allclose(p, 0.5 * (a + b), rtol=0, atol=8 * np.spacing(np.abs(a).max() + np.abs(b).max()))
+        else:
+            end = b if ad else a
+            d = float(np.linalg.norm(p - end))
+            k2 = np.log2(d)
+            assert abs(k2 - round(k2)) < 1e-6, (a, b, ao, ad, d)
+            assert L / 3 * (1 - 1e-9) <= d <= L / 1.5 * (1 + 1e-9), (L, d)
+        # on the segment's line within rounding: exact perpendicular distance
+        # at most 2 ulps of the largest coordinate
+        A, B, Q = ((F(x[0]), F(x[1])) for x in (a, b, p))
+        cross = abs((A[0] - Q[0]) * (B[1] - Q[1]) - (A[1] - Q[1]) * (B[0] - Q[0]))
+        ulp = F(float(np.spacing(max(np.abs(a).max(), np.abs(b).max(), np.abs(p).max()))))
+        assert cross * cross <= (2 * ulp) ** 2 * ((B[0] - A[0]) ** 2 + (B[1] - A[1]) ** 2), \
+            (a, b, p)
