@@ -584,14 +584,23 @@ extern "C" int gcdt_generate_pslg(int dist, int n, int nseg, double L,
   int G = std::max(1, std::min(4096, (int)std::ceil(L / cell)));
   cell = L / G;
   auto cidx = [&](double v) { return std::max(0, std::min(G - 1, (int)(v / cell))); };
-  std::vector<std::vector<int>> pcell((size_t)G * G), scell((size_t)G * G);
+  std::vector<std::vector<int>> pcell((size_t)G * G);
+  // accepted segments by cells of a finer grid (clustered inputs pack
+  // thousands of segments into one endpoint cell); only the crossing test's
+  // speed depends on it
+  const int GS = std::max(1, std::min(8192, 4 * G));
+  const double scell_w = L / GS;
+  auto sidx = [&](double v) { return std::max(0, std::min(GS - 1, (int)(v / scell_w))); };
+  std::vector<std::vector<int>> scell((size_t)GS * GS);
   for (int i = 0; i < n; i++) pcell[(size_t)cidx(points[2 * i + 1]) * G + cidx(points[2 * i])].push_back(i);
   std::vector<std::vector<int>> inc(n);  // accepted segments per vertex
   double cosmin = std::cos(min_angle_deg * 3.14159265358979323846 / 180.0);
   double good2 = cosmin * cosmin;
   int k = 0;
   long attempts = 0, max_attempts = 200L * nseg + 1000;
-  std::vector<int> cand;
+  std::vector<long> tested((size_t)nseg, 0);  // crossing-test stamp per segment
+  std::vector<double> sbox(4 * (size_t)nseg);   // accepted segments' x0 x1 y0 y1
+  long stamp_now = 0;
   int nlat_seg = dist == 2 ? nseg / 2 : 0;  // segments along lattice lines
   while (k < nseg && attempts++ < max_attempts) {
     int i = (int)rng.below((uint64_t)n), j = -1;
@@ -606,15 +615,26 @@ extern "C" int gcdt_generate_pslg(int dist, int n, int nseg, double L,
         if (lat[c + t * step] >= 0) { j = lat[c + t * step]; break; }
       if (j < 0) continue;
     } else if (maxlen > 0) {
+      // a uniformly drawn point of the 3 x 3 cells around point i (the
+      // cells' lists taken in row order, without copying them: dense
+      // clusters put 10^5 points in a block)
       int cx = cidx(points[2 * i]), cy = cidx(points[2 * i + 1]);
-      cand.clear();
+      size_t total = 0;
       for (int dy = -1; dy <= 1; dy++)
         for (int dx = -1; dx <= 1; dx++) {
           int x = cx + dx, y = cy + dy;
           if (x < 0 || y < 0 || x >= G || y >= G) continue;
-          for (int q : pcell[(size_t)y * G + x]) cand.push_back(q);
+          total += pcell[(size_t)y * G + x].size();
         }
-      j = cand[rng.below(cand.size())];
+      size_t r = rng.below(total);
+      for (int dy = -1; dy <= 1 && j < 0; dy++)
+        for (int dx = -1; dx <= 1 && j < 0; dx++) {
+          int x = cx + dx, y = cy + dy;
+          if (x < 0 || y < 0 || x >= G || y >= G) continue;
+          const std::vector<int> &pc = pcell[(size_t)y * G + x];
+          if (r < pc.size()) j = pc[r];
+          else r -= pc.size();
+        }
     } else {
       j = (int)rng.below((uint64_t)n);
     }
@@ -644,15 +664,22 @@ extern "C" int gcdt_generate_pslg(int dist, int n, int nseg, double L,
     }
     if (!ok) continue;
     // crossing test against segments in the cells the new segment overlaps
-    int x0 = cidx(std::min(points[2 * i], points[2 * j])), x1 = cidx(std::max(points[2 * i], points[2 * j]));
-    int y0 = cidx(std::min(points[2 * i + 1], points[2 * j + 1])), y1 = cidx(std::max(points[2 * i + 1], points[2 * j + 1]));
-    std::unordered_set<int> tested;
+    int x0 = sidx(std::min(points[2 * i], points[2 * j])), x1 = sidx(std::max(points[2 * i], points[2 * j]));
+    int y0 = sidx(std::min(points[2 * i + 1], points[2 * j + 1])), y1 = sidx(std::max(points[2 * i + 1], points[2 * j + 1]));
+    ++stamp_now;
+    const double bx0 = std::min(points[2 * i], points[2 * j]), bx1 = std::max(points[2 * i], points[2 * j]);
+    const double by0 = std::min(points[2 * i + 1], points[2 * j + 1]),
+                 by1 = std::max(points[2 * i + 1], points[2 * j + 1]);
     for (int y = y0; y <= y1 && ok; y++)
       for (int x = x0; x <= x1 && ok; x++)
-        for (int s : scell[(size_t)y * G + x]) {
-          if (!tested.insert(s).second) continue;
+        for (int s : scell[(size_t)y * GS + x]) {
+          if (tested[s] == stamp_now) continue;  // met in an earlier cell
+          tested[s] = stamp_now;
           int a2 = segs[2 * s], b2 = segs[2 * s + 1];
           if (a2 == i || a2 == j || b2 == i || b2 == j) continue;
+          // closed bounding boxes apart: no common point (exact)
+          const double *bb = &sbox[4 * (size_t)s];
+          if (bb[0] > bx1 || bb[1] < bx0 || bb[2] > by1 || bb[3] < by0) continue;
           int o1 = orient(points, i, j, a2) * orient(points, i, j, b2);
           int o2 = orient(points, a2, b2, i) * orient(points, a2, b2, j);
           if (o1 <= 0 && o2 <= 0) { ok = false; break; }
@@ -660,10 +687,14 @@ extern "C" int gcdt_generate_pslg(int dist, int n, int nseg, double L,
     if (!ok) continue;
     segs[2 * k] = i;
     segs[2 * k + 1] = j;
+    sbox[4 * (size_t)k] = bx0;
+    sbox[4 * (size_t)k + 1] = bx1;
+    sbox[4 * (size_t)k + 2] = by0;
+    sbox[4 * (size_t)k + 3] = by1;
     inc[i].push_back(k);
     inc[j].push_back(k);
     for (int y = y0; y <= y1; y++)
-      for (int x = x0; x <= x1; x++) scell[(size_t)y * G + x].push_back(k);
+      for (int x = x0; x <= x1; x++) scell[(size_t)y * GS + x].push_back(k);
     k++;
   }
   return k;
