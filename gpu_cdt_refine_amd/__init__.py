@@ -181,6 +181,8 @@ def lib():
                                                      vp, C.c_int, vp, C.c_int, C.c_double, vp,
                                                      vp]
         L.gcdt_check_mesh_device_noalloc.restype = C.c_int
+        L.gcdt_build_cdt_device.argtypes = [vp, C.c_int, vp, C.c_int, vp, vp]
+        L.gcdt_build_cdt_device.restype = C.c_int
         L.gcdt_dropin_set_options.argtypes = [vp]
         L.gcdt_dropin_set_options.restype = C.c_int
         L.gcdt_dropin_set_priority_model.argtypes = [vp]
@@ -768,15 +770,26 @@ def generate_pslg(n, nseg, L=1000.0, maxlen=0.0, dist=0, min_angle=5.0, seed=1):
     return P, S[:k].copy()
 
 
-def build_cdt(points, segments):
+def build_cdt(points, segments, device=False, stats=None):
     """CDT of points + segments with hull segments (Triangle's "pzQnc"
-    contract).  Returns (triangles, neighbors, segments)."""
+    contract).  Returns (triangles, neighbors, segments).  device=True builds
+    it on the GPU (gcdt_build_cdt_device); stats (a dict) then receives the
+    rounds, segment passes, pieces and device milliseconds."""
     P = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
     S = np.ascontiguousarray(segments, dtype=np.int32).reshape(-1, 2)
     c = C.c_void_p()
-    rc = lib().gcdt_build_cdt(_ptr(P), len(P), _ptr(S), len(S), C.byref(c))
-    if rc != 0:
-        _err(rc, "gcdt_build_cdt")
+    if device:
+        st = (C.c_double * 4)()
+        rc = lib().gcdt_build_cdt_device(_ptr(P), len(P), _ptr(S), len(S), C.byref(c), st)
+        if rc != 0:
+            _err(rc, "gcdt_build_cdt_device")
+        if stats is not None:
+            stats.update(rounds=int(st[0]), segment_passes=int(st[1]), segment_pieces=int(st[2]),
+                         device_ms=float(st[3]))
+    else:
+        rc = lib().gcdt_build_cdt(_ptr(P), len(P), _ptr(S), len(S), C.byref(c))
+        if rc != 0:
+            _err(rc, "gcdt_build_cdt")
     np_, nt, ns = C.c_int(), C.c_int(), C.c_int()
     lib().gcdt_cdt_counts(c, C.byref(np_), C.byref(nt), C.byref(ns))
     T = np.empty((nt.value, 3), np.int32)

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libgcdt.so")
 SOURCES = ["gcdt_refine.hip", "gcdt_capi.cpp", "gcdt_batch.cpp", "gcdt_tune.cpp", "cdt_host.cpp",
-           "gcdt_mlp.hip", "gcdt_check.hip", "cdt_io.cpp"]
+           "gcdt_mlp.hip", "gcdt_check.hip", "cdt_io.cpp", "gcdt_cdt.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

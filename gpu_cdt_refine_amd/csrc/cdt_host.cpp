@@ -27,6 +27,7 @@
 #include <unordered_set>
 #include <vector>
 
+#include "gcdt_cdt_result.h"
 #include "gpu_cdt_refine.h"
 #include "predicates.h"
 
@@ -367,10 +368,6 @@ struct Segs {
 
 }  // namespace
 
-struct gcdt_cdt {
-  int np = 0;
-  std::vector<int> T, N, S;
-};
 
 extern "C" int gcdt_build_cdt(const double *points, int np, const int *segs,
                               int ns, gcdt_cdt **out) {

@@ -211,6 +211,30 @@ int gcdt_session_check(gcdt_session *s, long long *viol, long long *exact_signs,
   }
 }
 
+int gcdt_build_cdt_device(const double *points, int np, const int *segs, int ns,
+                          gcdt_cdt **out, double *stats) {
+  g_err.clear();
+  try {
+    if (!out || (np > 0 && !points) || (ns > 0 && !segs) || np < 0 || ns < 0)
+      throw std::runtime_error("invalid arrays");
+    gcdt_cdt *c = new gcdt_cdt;
+    c->np = np;
+    try {
+      gcdt::build_cdt_device(points, np, segs, ns, c->T, c->N, c->S, c->stats);
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    if (stats)
+      for (int i = 0; i < 4; i++) stats[i] = c->stats[i];
+    *out = c;
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -2;
+  }
+}
+
 int gcdt_check_mesh_device_noalloc(const double *d_points, int np, const int *d_tris,
                                    const int *d_nbrs, int nt, const int *d_segs, int ns,
                                    const double *d_points_in, int np_in, const int *d_segs_in,

@@ -12,7 +12,15 @@
 #include "gpu_cdt_refine.h"
 #include "host_copy.h"
 
+#include "gcdt_cdt_result.h"
+
 namespace gcdt {
+
+// Device CDT construction (gcdt_cdt.hip); stats: insertion rounds, segment
+// passes, segment pieces, device milliseconds.
+void build_cdt_device(const double *points, int np, const int *segs, int ns,
+                      std::vector<int> &T_out, std::vector<int> &N_out, std::vector<int> &S_out,
+                      double *stats);
 
 struct MeshHost {
   int np = 0, nt = 0, ns = 0;

@@ -410,6 +410,18 @@ int gcdt_build_cdt(const double *points, int np, const int *segs, int ns,
 void gcdt_cdt_counts(const gcdt_cdt *c, int *np, int *nt, int *ns);
 void gcdt_cdt_copy(const gcdt_cdt *c, int *tris, int *nbrs, int *segs);
 void gcdt_cdt_free(gcdt_cdt *c);
+/* The same CDT built on the GPU (no reference counterpart: the reference
+ * builds its input with Triangle's triangulate("pzQnc") on the host,
+ * mesh.cpp:397, main.cpp:454): parallel Bowyer-Watson insertion with a ghost
+ * vertex, then segment recovery by retriangulating the crossed triangles'
+ * two sides.  Same output convention and the same triangles as Triangle
+ * where no four points are cocircular; the final segment list holds each
+ * input segment's pieces in order, then the hull edges that are not
+ * segments.  stats (may be NULL) receives insertion rounds, segment passes,
+ * segment pieces and device milliseconds.  Returns 0, or -2
+ * (gcdt_last_error) for invalid input or a failed build. */
+int gcdt_build_cdt_device(const double *points, int np, const int *segs, int ns,
+                          gcdt_cdt **out, double *stats);
 
 /* CDT files (reference GPU_Refine/mesh.cpp:402-532, saveCDT / readCDT).
  * gcdt_save_cdt_text writes the reference's text layout (counts of points,
