@@ -690,12 +690,78 @@ def bench_precision(args):
     print(json.dumps(out))
 
 
+def bench_cdt(args):
+    """Construction of the input CDT (points + segments -> triangles with hull
+    segments, Triangle's "pzQnc" contract) on the GPU (gcdt_build_cdt_device:
+    host arrays in and out, median wall time), next to this library's host
+    builder and the reference's Triangle (oracle/_ref, one core), on the
+    refine workload's 10^6 uniform points / 50,000 segments and the learned
+    workload's 2*10^6 clustered points / 100,000 segments.  The GPU result is
+    compared with the host builder's (the same triangle and segment sets)."""
+    import gpu_cdt_refine_amd as g
+    from oracle import pyoracle
+
+    def canon(T):
+        T = np.asarray(T).reshape(-1, 3)
+        r = np.argmin(T, axis=1)
+        i = np.arange(len(T))
+        return set(map(tuple, np.stack([T[i, r], T[i, (r + 1) % 3], T[i, (r + 2) % 3]],
+                                       1).tolist()))
+
+    cases = []
+    for n, ns, dist, seed, name in ((args.points, args.segments, 0, args.seed, "uniform"),
+                                    (2 * args.points, 2 * args.segments, 1, args.seed + 1000,
+                                     "clustered")):
+        t0 = time.perf_counter()
+        P, S0 = g.generate_pslg(n, ns, L=args.domain, maxlen=0.02 * args.domain, dist=dist,
+                                min_angle=5.0, seed=seed)
+        t_gen = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        Th, Nh, Sh = g.build_cdt(P, S0)
+        t_host = time.perf_counter() - t0
+        walls, devs = [], []
+        for i in range(args.warmup + args.steps):
+            st = {}
+            t0 = time.perf_counter()
+            T, N, S = g.build_cdt(P, S0, device=True, stats=st)
+            if i >= args.warmup:
+                walls.append((time.perf_counter() - t0) * 1e3)
+                devs.append(st["device_ms"])
+            log(f"# cdt {name} {n}: {(time.perf_counter() - t0) * 1e3:.1f} ms wall, {st}")
+        same = (canon(T) == canon(Th) and
+                {tuple(sorted(x)) for x in S.tolist()} == {tuple(sorted(x)) for x in Sh.tolist()})
+        case = {"workload": f"{name}-{n}-points-{ns}-segments", "points": n, "segments": ns,
+                "triangles": int(len(T)), "gpu_ms": float(np.median(walls)),
+                "gpu_device_ms": float(np.median(devs)), "rounds": st["rounds"],
+                "segment_passes": st["segment_passes"], "host_builder_ms": t_host * 1e3,
+                "generator_ms": t_gen * 1e3, "same_as_host_builder": bool(same)}
+        if not args.no_cpu_baseline and os.path.exists(pyoracle.TRIREF_SO):
+            t0 = time.perf_counter()
+            Pt, Tt, Nt, St = pyoracle.triangle_cdt(P, S0)
+            case["triangle_pzQnc_ms"] = (time.perf_counter() - t0) * 1e3
+        cases.append(case)
+    big = cases[-1]
+    out = {"metric": "input CDT construction on the GPU: points/sec (host arrays in and out)",
+           "value": big["points"] / (big["gpu_ms"] / 1e3), "unit": "points/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": big["gpu_ms"],
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "cdt-construction"}, "cases": cases,
+           "cpu_baseline": None if "triangle_pzQnc_ms" not in big else {
+               "value": big["points"] / (big["triangle_pzQnc_ms"] / 1e3), "unit": "points/s",
+               "cores": 1, "kind": "reference",
+               "sample": f"Triangle 1.6 'pzQnc' (reference GPU_Refine/triangle.cpp) on the same "
+                         f"{big['points']}-point input, one run"},
+           "host": host_info()}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="refine", choices=["refine", "learned", "batch", "precision", "grid"])
+    ap.add_argument("--workload", default="refine",
+                    choices=["refine", "learned", "batch", "precision", "grid", "cdt"])
     ap.add_argument("--points", type=int, default=1_000_000)
     ap.add_argument("--segments", type=int, default=50_000)
     ap.add_argument("--domain", type=float, default=1e6)
@@ -722,7 +788,7 @@ def main():
                           "(multi-GPU sharding is a design note only)"}))
         sys.exit(2)
     {"refine": bench_refine, "learned": bench_learned, "batch": bench_batch,
-     "precision": bench_precision, "grid": bench_grid}[args.workload](args)
+     "precision": bench_precision, "grid": bench_grid, "cdt": bench_cdt}[args.workload](args)
 
 
 if __name__ == "__main__":
