@@ -759,8 +759,11 @@ def device_constructions(tris, theta=25.0, offcenter=1):
 
 
 def generate_pslg(n, nseg, L=1000.0, maxlen=0.0, dist=0, min_angle=5.0, seed=1):
-    """Seeded synthetic PSLG: n points (dist 0 uniform in [0,L]^2, 1 Gaussian
-    clusters) and up to nseg non-crossing segments (host code in libgcdt)."""
+    """Seeded synthetic PSLG: n distinct points in [0,L]^2 (dist 0 uniform,
+    1 Gaussian clusters, 2 precision-stress mix, 3 disk, 4 circle ring,
+    5 grid lattice, 6 ellipse curve -- 3 to 6 are the reference generator's
+    families, mesh.cpp:131-193) and up to nseg non-crossing segments (host
+    code in libgcdt)."""
     P = np.empty((n, 2), np.float64)
     S = np.empty((nseg, 2), np.int32)
     k = lib().gcdt_generate_pslg(int(dist), int(n), int(nseg), float(L), float(maxlen),

@@ -366,6 +366,61 @@ struct Segs {
   int a, b;
 };
 
+// Points on the boundary of the convex hull, counterclockwise, collinear
+// boundary points included (each once).  Only points outside the open
+// octagon of the extreme points in eight directions can lie on the hull's
+// boundary; Andrew's monotone chain with exact orientation, popping only on
+// clockwise turns, orders them.
+std::vector<int> hull_boundary(const double *P, int np) {
+  int ext[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  auto key = [&](int i, int k) {
+    const double x = P[2 * i], y = P[2 * i + 1];
+    switch (k) {
+      case 0: return x;
+      case 1: return -x;
+      case 2: return y;
+      case 3: return -y;
+      case 4: return x + y;
+      case 5: return -(x + y);
+      case 6: return x - y;
+      default: return y - x;
+    }
+  };
+  for (int i = 1; i < np; i++)
+    for (int k = 0; k < 8; k++)
+      if (key(i, k) > key(ext[k], k)) ext[k] = i;
+  auto lex = [&](int a, int b) {
+    return P[2 * a] < P[2 * b] || (P[2 * a] == P[2 * b] && P[2 * a + 1] < P[2 * b + 1]);
+  };
+  auto chain = [&](std::vector<int> pts, bool keep_collinear) {
+    std::sort(pts.begin(), pts.end(), lex);
+    pts.erase(std::unique(pts.begin(), pts.end()), pts.end());
+    if (pts.size() < 3) return pts;
+    const int lim = keep_collinear ? -1 : 0;
+    std::vector<int> h(2 * pts.size());
+    size_t k = 0;
+    for (size_t i = 0; i < pts.size(); i++) {
+      while (k >= 2 && orient(P, h[k - 2], h[k - 1], pts[i]) <= lim) k--;
+      h[k++] = pts[i];
+    }
+    for (size_t i = pts.size() - 1, t = k + 1; i-- > 0;) {
+      while (k >= t && orient(P, h[k - 2], h[k - 1], pts[i]) <= lim) k--;
+      h[k++] = pts[i];
+    }
+    h.resize(k - 1);
+    return h;
+  };
+  std::vector<int> oct = chain(std::vector<int>(ext, ext + 8), false);
+  std::vector<int> cand;
+  for (int i = 0; i < np; i++) {
+    bool inside = oct.size() >= 3;
+    for (size_t e = 0; e < oct.size() && inside; e++)
+      inside = orient(P, oct[e], oct[(e + 1) % oct.size()], i) > 0;
+    if (!inside) cand.push_back(i);
+  }
+  return chain(cand, true);
+}
+
 }  // namespace
 
 
@@ -495,7 +550,7 @@ extern "C" int gcdt_generate_pslg(int dist, int n, int nseg, double L,
                                   double maxlen, double min_angle_deg,
                                   unsigned long long seed, double *points,
                                   int *segs) {
-  if (n < 3 || L <= 0) return -1;
+  if (n < 3 || L <= 0 || dist < 0 || dist > 6) return -1;
   Rng rng(seed);
   std::unordered_set<uint64_t> seen;
   seen.reserve((size_t)n * 2);
@@ -569,10 +624,36 @@ extern "C" int gcdt_generate_pslg(int dist, int n, int nseg, double L,
       x = centers[2 * k] + 0.05 * L * rng.normal();
       y = centers[2 * k + 1] + 0.05 * L * rng.normal();
       if (x < 0 || x > L || y < 0 || y > L) continue;
+    } else if (dist == 3) {
+      // disk: uniform in the disk of radius 0.45 L about the domain's centre
+      // (mesh.cpp:131-146)
+      double u = rng.uniform() - 0.5, v = rng.uniform() - 0.5;
+      if (u * u + v * v > 0.45 * 0.45) continue;
+      x = L * (u + 0.5);
+      y = L * (v + 0.5);
+    } else if (dist == 4) {
+      // circle: a ring of radii 0.45 L to 0.49 L, the radius drawn uniformly
+      // (so denser towards the inside) and the angle uniformly (mesh.cpp:148-161)
+      double d = 0.04 * rng.uniform(), a = 6.283185307179586 * rng.uniform();
+      x = L * (0.5 + (0.45 + d) * std::cos(a));
+      y = L * (0.5 + (0.45 + d) * std::sin(a));
+    } else if (dist == 5) {
+      // grid: a uniform draw rounded half up to the nearest node of a
+      // lattice with 10 n cells a side (mesh.cpp:163-176)
+      const double M = 10.0 * n, h = L / M;
+      x = h * std::floor(rng.uniform() * M + 0.5);
+      y = h * std::floor(rng.uniform() * M + 0.5);
+    } else if (dist == 6) {
+      // ellipse: on the curve with half-axes 0.25 L (x) and 0.5 L (y)
+      // about the domain's centre, the angle uniform (mesh.cpp:178-193)
+      double a = 6.283185307179586 * rng.uniform();
+      x = L * (0.5 + 0.25 * std::cos(a));
+      y = L * (0.5 + 0.5 * std::sin(a));
     } else {
       x = L * rng.uniform();
       y = L * rng.uniform();
     }
+    if (x < 0 || x > L || y < 0 || y > L) continue;
     if (accept(i, x, y)) i++;
   }
   if (nseg <= 0) return 0;
@@ -591,6 +672,13 @@ extern "C" int gcdt_generate_pslg(int dist, int n, int nseg, double L,
   std::vector<std::vector<int>> scell((size_t)GS * GS);
   for (int i = 0; i < n; i++) pcell[(size_t)cidx(points[2 * i + 1]) * G + cidx(points[2 * i])].push_back(i);
   std::vector<std::vector<int>> inc(n);  // accepted segments per vertex
+  // The hull's boundary edges join the segment list before the random
+  // segments are drawn (mesh.cpp:225-240), so a new segment keeps the
+  // minimum angle with them too and never duplicates one.  They are not
+  // written out: the CDT builder adds every hull edge itself.
+  const std::vector<int> ring = hull_boundary(points, n);
+  std::vector<int> ringpos(n, -1);
+  for (size_t r = 0; r < ring.size(); r++) ringpos[ring[r]] = (int)r;
   double cosmin = std::cos(min_angle_deg * 3.14159265358979323846 / 180.0);
   double good2 = cosmin * cosmin;
   int k = 0;
@@ -645,18 +733,24 @@ extern "C" int gcdt_generate_pslg(int dist, int n, int nseg, double L,
     double len2 = dx * dx + dy * dy;
     if (maxlen > 0 && len2 > maxlen * maxlen) continue;
     bool ok = true;
-    // incident segments: duplicate / angle check (mesh.cpp:287-352)
+    // incident segments and hull edges: duplicate / angle check
+    // (mesh.cpp:287-352)
+    auto too_close = [&](int e, int o, int f) {
+      if (f == o) return true;
+      double ux = points[2 * o] - points[2 * e], uy = points[2 * o + 1] - points[2 * e + 1];
+      double vx = points[2 * f] - points[2 * e], vy = points[2 * f + 1] - points[2 * e + 1];
+      double dot = ux * vx + uy * vy;
+      return dot > 0 && dot * dot > good2 * (ux * ux + uy * uy) * (vx * vx + vy * vy);
+    };
     for (int e : {i, j}) {
       int o = e == i ? j : i;
       for (int s : inc[e]) {
         int a2 = segs[2 * s], b2 = segs[2 * s + 1];
-        int f = a2 == e ? b2 : a2;
-        if (f == o) { ok = false; break; }
-        double ux = points[2 * o] - points[2 * e], uy = points[2 * o + 1] - points[2 * e + 1];
-        double vx = points[2 * f] - points[2 * e], vy = points[2 * f + 1] - points[2 * e + 1];
-        double dot = ux * vx + uy * vy;
-        if (dot > 0 && dot * dot > good2 * (ux * ux + uy * uy) * (vx * vx + vy * vy)) { ok = false; break; }
+        if (too_close(e, o, a2 == e ? b2 : a2)) { ok = false; break; }
       }
+      const int r = ringpos[e], R = (int)ring.size();
+      if (ok && r >= 0 && R >= 3)
+        ok = !too_close(e, o, ring[(r + 1) % R]) && !too_close(e, o, ring[(r + R - 1) % R]);
       if (!ok) break;
     }
     if (!ok) continue;

@@ -600,13 +600,19 @@ __global__ __launch_bounds__(SCAN_CHUNK) void k_scan_scatter(const int *f, int n
 // ---------------------------------------------------------------------------
 // Phase 2: constraint recovery
 // ---------------------------------------------------------------------------
-constexpr int KSEG = 512;  // crossed triangles of one segment region
-// per-segment workspace: region | left chain | right chain | boundary u, v,
-// outer | new triangles (3 ints) | Anglada stack (4 ints)
-constexpr int WS_SEG = KSEG + 2 * (KSEG + 2) + 3 * (KSEG + 2) + 3 * KSEG + 4 * (KSEG + 2);
+// Region capacity: crossed triangles of one segment region.  Passes start
+// at KSEG0; a region that does not fit reports SS_OVER, and once a pass
+// makes no progress the host grows the capacity (fewer segments a pass).
+constexpr int KSEG0 = 512;
+// per-segment workspace for capacity k: region | left chain | right chain |
+// boundary u, v, outer | new triangles (3 ints) | Anglada stack (4 ints)
+__host__ __device__ constexpr size_t ws_seg(int k) {
+  return (size_t)k + 2 * ((size_t)k + 2) + 3 * ((size_t)k + 2) + 3 * (size_t)k + 4 * ((size_t)k + 2);
+}
+__host__ __device__ constexpr size_t bo_off(int k) { return (size_t)k + 4 * ((size_t)k + 2); }
 
 enum { SS_PENDING = 0, SS_EXISTS = 1, SS_REGION = 2, SS_SPLIT_EDGE = 3, SS_ERROR = 4,
-       SS_DEFER = 5 };
+       SS_DEFER = 5, SS_OVER = 6 };
 constexpr int SLIT = -3;  // boundary entry whose far side is in the region
 
 // Triangles incident to each vertex: the smallest slot (deterministic).
@@ -637,11 +643,12 @@ struct SegOut {
 // triangles its interior crosses up to b or up to the first vertex z lying on
 // it.  Chains: left (vertices left of a->b) and right, in order from a.
 // Boundary edges of the region with their outer otri.
-__device__ SegOut seg_scan(const MeshG &m, int a, int b, const int *vtri, int *W) {
+__device__ SegOut seg_scan(const MeshG &m, int a, int b, const int *vtri, int *W, int kseg) {
   SegOut o{SS_ERROR, -1, 0, 0, 0, 0, -1, -1};
   int *R = W;
-  int *CL = R + KSEG, *CR = CL + (KSEG + 2);
-  int *Bu = CR + (KSEG + 2), *Bv = Bu + (KSEG + 2), *Bo = Bv + (KSEG + 2);
+  int *CL = R + kseg, *CR = CL + (kseg + 2);
+  int *Bu = CR + (kseg + 2), *Bv = Bu + (kseg + 2), *Bo = Bv + (kseg + 2);
+  const SegOut over{SS_OVER, -1, 0, 0, 0, 0, -1, -1};
   if (a < 0 || a >= m.np || b < 0 || b >= m.np || a == b) {
     flag(m, E_INDEX);
     return o;
@@ -714,7 +721,7 @@ __device__ SegOut seg_scan(const MeshG &m, int a, int b, const int *vtri, int *W
   // walk across the triangles the segment crosses; x right of a->b, y left
   int n = 0, nl = 0, nr = 0, nbnd = 0;
   auto add_bnd = [&](int tt, int j) -> bool {
-    if (nbnd == KSEG + 2) return false;
+    if (nbnd == kseg + 2) return false;
     const int ob = comp4(m.N[tt], j);
     if (ob >= 0 && !tri_ok(m, ob >> 2)) {
       flag(m, E_INDEX);
@@ -740,7 +747,7 @@ __device__ SegOut seg_scan(const MeshG &m, int a, int b, const int *vtri, int *W
   }
   int cur = t0, jx = (index_of(m.V[t0], a));  // crossed edge of cur: opposite a
   int L = y, Rr = x;
-  for (int guard = 0; guard < KSEG + 8; guard++) {
+  for (int guard = 0; guard < kseg + 8; guard++) {
     const int nb = comp4(m.N[cur], jx);
     if (nb < 0 || !tri_ok(m, nb >> 2)) {
       flag(m, E_TOPOLOGY);
@@ -753,10 +760,7 @@ __device__ SegOut seg_scan(const MeshG &m, int a, int b, const int *vtri, int *W
       flag(m, E_TOPOLOGY);
       return o;
     }
-    if (n == KSEG) {
-      flag(m, E_CAVITY);
-      return o;
-    }
+    if (n == kseg) return over;
     R[n++] = u;
     // u = (z, ., .): the edge (L, z) and the edge (z, Rr) are its other edges
     const int iL = index_of(vu, L), iR = index_of(vu, Rr);
@@ -766,10 +770,7 @@ __device__ SegOut seg_scan(const MeshG &m, int a, int b, const int *vtri, int *W
     }
     // edge (z, Rr) is opposite L; edge (L, z) is opposite Rr
     if (z == b) {
-      if (!add_bnd(u, iL) || !add_bnd(u, iR)) {
-        flag(m, E_CAVITY);
-        return o;
-      }
+      if (!add_bnd(u, iL) || !add_bnd(u, iR)) return over;
       o.status = SS_REGION;
       o.z = b;
       break;
@@ -777,35 +778,18 @@ __device__ SegOut seg_scan(const MeshG &m, int a, int b, const int *vtri, int *W
     const double2 Z = m.P[z];
     const int oz = orient2d_sign(pa.x, pa.y, pb.x, pb.y, Z.x, Z.y, false, nullptr);
     if (oz == 0) {  // the segment passes through z: the region ends here
-      if (!add_bnd(u, iL) || !add_bnd(u, iR)) {
-        flag(m, E_CAVITY);
-        return o;
-      }
+      if (!add_bnd(u, iL) || !add_bnd(u, iR)) return over;
       o.status = SS_REGION;
       o.z = z;
       break;
     }
     if (oz > 0) {  // z left: (L, z) is a boundary edge, the segment leaves through (z, Rr)
-      if (!add_bnd(u, iR)) {
-        flag(m, E_CAVITY);
-        return o;
-      }
-      if (nl == KSEG + 2) {
-        flag(m, E_CAVITY);
-        return o;
-      }
+      if (!add_bnd(u, iR) || nl == kseg + 2) return over;
       CL[nl++] = z;
       L = z;
       jx = iL;
     } else {  // z right: (z, Rr) is a boundary edge, the segment leaves through (L, z)
-      if (!add_bnd(u, iL)) {
-        flag(m, E_CAVITY);
-        return o;
-      }
-      if (nr == KSEG + 2) {
-        flag(m, E_CAVITY);
-        return o;
-      }
+      if (!add_bnd(u, iL) || nr == kseg + 2) return over;
       CR[nr++] = z;
       Rr = z;
       jx = iR;
@@ -831,24 +815,25 @@ __device__ SegOut seg_scan(const MeshG &m, int a, int b, const int *vtri, int *W
 
 // Per pending segment: scan (above) and record the outcome.
 __global__ void k_seg_scan(MeshG m, const int2 *seg, const int *pend, int npend, const int *vtri,
-                           int *W, int4 *info, int4 *info2) {
+                           int *W, int kseg, int4 *info, int4 *info2, int *nover) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < npend; k += gridDim.x * blockDim.x) {
     const int s = pend[k];
     const int2 e = seg[s];
-    const SegOut o = seg_scan(m, e.x, e.y, vtri, W + (size_t)k * WS_SEG);
+    const SegOut o = seg_scan(m, e.x, e.y, vtri, W + (size_t)k * ws_seg(kseg), kseg);
+    if (o.status == SS_OVER) atomicAdd(nover, 1);
     info[k] = make_int4(o.status, o.z, o.n, o.nbnd);
     info2[k] = make_int4(o.nl, o.nr, o.t_edge, o.j_edge);
   }
 }
 
 // Claims on the region and its ring; existing edges need none.
-__global__ void k_seg_claim(MeshG m, int npend, const int *W, const int4 *info,
+__global__ void k_seg_claim(MeshG m, int npend, const int *W, int kseg, const int4 *info,
                             unsigned long long ep) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < npend; k += gridDim.x * blockDim.x) {
     const int4 in = info[k];
     if (in.x != SS_REGION) continue;
-    const int *Wk = W + (size_t)k * WS_SEG;
-    const int *R = Wk, *Bo = Wk + KSEG + 2 * (KSEG + 2) + 2 * (KSEG + 2);
+    const int *Wk = W + (size_t)k * ws_seg(kseg);
+    const int *R = Wk, *Bo = Wk + bo_off(kseg);
     const unsigned long long key = claim_key(ep, k);
     for (int q = 0; q < in.z; q++) atomicMax(&m.claim[2 * (size_t)R[q]], key);
     for (int q = 0; q < in.w; q++)
@@ -856,14 +841,14 @@ __global__ void k_seg_claim(MeshG m, int npend, const int *W, const int4 *info,
   }
 }
 
-__global__ void k_seg_win(MeshG m, int npend, const int *W, const int4 *info,
+__global__ void k_seg_win(MeshG m, int npend, const int *W, int kseg, const int4 *info,
                           unsigned long long ep, int *win) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < npend; k += gridDim.x * blockDim.x) {
     const int4 in = info[k];
     int w = 0;
     if (in.x == SS_REGION) {
-      const int *Wk = W + (size_t)k * WS_SEG;
-      const int *R = Wk, *Bo = Wk + KSEG + 2 * (KSEG + 2) + 2 * (KSEG + 2);
+      const int *Wk = W + (size_t)k * ws_seg(kseg);
+      const int *R = Wk, *Bo = Wk + bo_off(kseg);
       const unsigned long long key = claim_key(ep, k);
       w = 1;
       for (int q = 0; q < in.z && w; q++) w = m.claim[2 * (size_t)R[q]] == key;
@@ -894,7 +879,7 @@ struct Piece {
 // matched edge by edge, inside the region or to the boundary's outer
 // triangles (whose back-links are repaired).
 __global__ void k_seg_apply(MeshG m, const int2 *seg_cur, const int *pend, int npend, int *W,
-                            const int4 *info, const int4 *info2, const int *win, double *arena,
+                            int kseg, const int4 *info, const int4 *info2, const int *win, double *arena,
                             Piece *pieces, int *npieces, int pcap, int2 *seg_next, int *done) {
   const int lane = blockIdx.x * blockDim.x + threadIdx.x;
   double *ws = arena + (size_t)lane * GCDT_INCIRCLE_WS;
@@ -904,12 +889,12 @@ __global__ void k_seg_apply(MeshG m, const int2 *seg_cur, const int *pend, int n
     const int a = seg_cur[s].x, b = seg_cur[s].y;
     const int4 in = info[k], in2 = info2[k];
     const int z = in.y, n = in.z, nbnd = in.w, nl = in2.x, nr = in2.y;
-    int *R = W + (size_t)k * WS_SEG;
-    int *CL = R + KSEG, *CR = CL + (KSEG + 2);
-    int *Bu = CR + (KSEG + 2), *Bv = Bu + (KSEG + 2), *Bo = Bv + (KSEG + 2);
-    int *NT = Bo + (KSEG + 2);
-    int *STK = NT + 3 * KSEG;
-    if (nl + nr != n || n > KSEG || nl < 1 || nr < 1) {
+    int *R = W + (size_t)k * ws_seg(kseg);
+    int *CL = R + kseg, *CR = CL + (kseg + 2);
+    int *Bu = CR + (kseg + 2), *Bv = Bu + (kseg + 2), *Bo = Bv + (kseg + 2);
+    int *NT = Bo + (kseg + 2);
+    int *STK = NT + 3 * (size_t)kseg;
+    if (nl + nr != n || n > kseg || nl < 1 || nr < 1) {
       flag(m, E_TOPOLOGY);
       continue;
     }
@@ -935,7 +920,7 @@ __global__ void k_seg_apply(MeshG m, const int2 *seg_cur, const int *pend, int n
           if (incircle_v(m, x, y, cb, ci, ws) > 0) best = i;
         }
         const int c = side == 0 ? CL[best] : CR[nr - 1 - best];
-        if (ne == n || sp + 2 > KSEG + 2) {
+        if (ne == n || sp + 2 > kseg + 2) {
           bad = true;
           break;
         }
@@ -1431,35 +1416,51 @@ void build_cdt_device(const double *points, int np, const int *segs, int ns,
     info = B.alloc<int4>(SBATCH);
     info2 = B.alloc<int4>(SBATCH);
     swin = B.alloc<int>(SBATCH);
-    W2 = B.alloc<int>((size_t)SBATCH * WS_SEG);
+    int kseg = KSEG0;
+    size_t w2_ints = (size_t)SBATCH * ws_seg(kseg);
+    W2 = B.alloc<int>(w2_ints);
+    int *nover = B.alloc<int>(1);
     int npend = ns;
     int last_pieces = -1;
     while (npend > 0) {
       if (++seg_passes > 1000000) throw std::runtime_error("segment recovery did not finish");
-      const int nb = std::min(npend, SBATCH);
+      const int batch = (int)std::min<size_t>(SBATCH, w2_ints / ws_seg(kseg));
+      const int nb = std::min(npend, batch);
+      CC(hipMemsetAsync(nover, 0, sizeof(int), B.st));
       CC(hipMemsetAsync(vtri, 0x7f, sizeof(int) * np, B.st));
       k_vtri<<<Builder::grid(cap), BLK, 0, B.st>>>(m, cap, vtri);
-      k_seg_scan<<<Builder::grid(nb, 64), 64, 0, B.st>>>(m, seg_cur, pend, nb, vtri, W2, info,
-                                                          info2);
+      k_seg_scan<<<Builder::grid(nb, 64), 64, 0, B.st>>>(m, seg_cur, pend, nb, vtri, W2, kseg,
+                                                          info, info2, nover);
       CC(hipMemcpyAsync(seg_next, seg_cur, sizeof(int2) * ns, hipMemcpyDeviceToDevice, B.st));
       k_seg_edges<<<Builder::grid(nb), BLK, 0, B.st>>>(m, seg_cur, pend, nb, info, pieces,
                                                        npieces, PCAP, seg_next, done);
       cep++;
-      k_seg_claim<<<Builder::grid(nb), BLK, 0, B.st>>>(m, nb, W2, info, cep);
-      k_seg_win<<<Builder::grid(nb), BLK, 0, B.st>>>(m, nb, W2, info, cep, swin);
-      k_seg_apply<<<EXACT_GRID, 64, 0, B.st>>>(m, seg_cur, pend, nb, W2, info, info2, swin, arena,
-                                               pieces, npieces, PCAP, seg_next, done);
+      k_seg_claim<<<Builder::grid(nb), BLK, 0, B.st>>>(m, nb, W2, kseg, info, cep);
+      k_seg_win<<<Builder::grid(nb), BLK, 0, B.st>>>(m, nb, W2, kseg, info, cep, swin);
+      k_seg_apply<<<EXACT_GRID, 64, 0, B.st>>>(m, seg_cur, pend, nb, W2, kseg, info, info2, swin,
+                                               arena, pieces, npieces, PCAP, seg_next, done);
       CC(hipGetLastError());
       CC(hipMemcpyAsync(seg_cur, seg_next, sizeof(int2) * ns, hipMemcpyDeviceToDevice, B.st));
       k_seg_pending_flags<<<Builder::grid(ns), BLK, 0, B.st>>>(done, ns, flags);
       CC(hipGetLastError());
       npend = B.compact(flags, ns, nullptr, pend2, nullptr);
       std::swap(pend, pend2);
-      int hpieces = 0;
+      int hpieces = 0, hover = 0;
       CC(hipMemcpyAsync(&hpieces, npieces, sizeof(int), hipMemcpyDeviceToHost, B.st));
+      CC(hipMemcpyAsync(&hover, nover, sizeof(int), hipMemcpyDeviceToHost, B.st));
       CC(hipStreamSynchronize(B.st));
       B.check_err("segment recovery");
-      if (hpieces == last_pieces) throw std::runtime_error("segment recovery made no progress");
+      if (hpieces == last_pieces) {
+        // no progress: only regions over capacity can be left
+        if (hover == 0) throw std::runtime_error("segment recovery made no progress");
+        if (kseg >= cap) throw std::runtime_error("segment region larger than the triangulation");
+        kseg = (int)std::min<long long>(4LL * kseg, cap);
+        if (ws_seg(kseg) > w2_ints) {
+          w2_ints = ws_seg(kseg);
+          W2 = B.alloc<int>(w2_ints);
+        }
+        continue;
+      }
       if (hpieces > PCAP) throw std::runtime_error("too many segment pieces");
       last_pieces = hpieces;
     }

@@ -392,7 +392,13 @@ int gcdt_device_count(void);
 
 /* Host-side input preparation (no device use).
  * gcdt_generate_pslg: seeded synthetic PSLG -- `dist` 0 uniform in
- * [0,L]^2, 1 Gaussian clusters (sigma = 0.05 L); nseg random non-crossing
+ * [0,L]^2, 1 Gaussian clusters (sigma = 0.05 L), 2 the precision-stress mix
+ * (jittered lattice, exactly cocircular groups, uniform background), and
+ * the reference generator's remaining families (mesh.cpp:131-193): 3 disk
+ * (radius 0.45 L), 4 circle (ring of radii 0.45 L..0.49 L), 5 grid (nodes of
+ * a lattice with 10 n cells a side), 6 ellipse (on the curve with half-axes
+ * 0.25 L and 0.5 L); every family is centred in [0,L]^2 and its points are
+ * distinct.  nseg random non-crossing
  * segments between point pairs, each no longer than maxlen (<= 0: no cap),
  * whose angle with any segment sharing an endpoint is at least
  * min_angle_deg.  Returns the number of segments produced.

@@ -17,6 +17,10 @@ def tri_set(T):
     (500, 25, 1000.0, 0.0, 0, 1),
     (5000, 250, 1000.0, 200.0, 0, 2),
     (8000, 400, 1e6, 2e4, 1, 3),
+    (3000, 150, 1000.0, 100.0, 3, 4),
+    (3000, 150, 1000.0, 100.0, 4, 5),
+    (3000, 150, 3e4, 3e3, 5, 6),
+    (3000, 150, 1000.0, 100.0, 6, 7),
 ])
 def test_cdt_matches_triangle(oracle, n, ns, L, maxlen, dist, seed):
     if not os.path.exists(oracle.TRIREF_SO):
@@ -49,6 +53,40 @@ def test_generator_contract(oracle):
     # deterministic for a seed
     P2, S2 = g.generate_pslg(3000, 200, L=1000.0, maxlen=100.0, dist=0, min_angle=10.0, seed=5)
     assert np.array_equal(P, P2) and np.array_equal(S, S2)
+
+
+@pytest.mark.parametrize("dist", [3, 4, 5, 6])
+def test_generator_families(dist):
+    """The reference generator's disk, circle, grid and ellipse families
+    (mesh.cpp:131-193), scaled to [0,L]^2: every point on its shape, all
+    distinct, deterministic for a seed."""
+    n, L = 4000, 1000.0
+    P, S = g.generate_pslg(n, 200, L=L, maxlen=100.0, dist=dist, min_angle=10.0, seed=9)
+    assert P.shape == (n, 2) and (P >= 0).all() and (P <= L).all()
+    assert len({tuple(p) for p in P.tolist()}) == n
+    # on the ellipse every point is a hull vertex, and a chord no longer than
+    # maxlen meets the hull edges at its ends at a small angle: few chords
+    # keep the minimum angle
+    assert len(S) == 200 if dist != 6 else 0 < len(S) < 200
+    u, v = P[:, 0] / L - 0.5, P[:, 1] / L - 0.5
+    r = np.hypot(u, v)
+    if dist == 3:
+        assert r.max() <= 0.45 + 1e-12 and r.min() < 0.05
+    elif dist == 4:
+        assert r.min() >= 0.45 - 1e-12 and r.max() <= 0.49 + 1e-12
+        # the radius is drawn uniformly, so the inner half of the ring holds
+        # about half of the points (not the 49% an area-uniform draw gives)
+        assert abs(np.mean(r < 0.47) - 0.5) < 0.03
+    elif dist == 5:
+        h = L / (10 * n)
+        k = P / h
+        assert np.abs(k - np.round(k)).max() < 1e-6
+    elif dist == 6:
+        assert np.abs((u / 0.25) ** 2 + (v / 0.5) ** 2 - 1).max() < 1e-12
+    P2, S2 = g.generate_pslg(n, 200, L=L, maxlen=100.0, dist=dist, min_angle=10.0, seed=9)
+    assert np.array_equal(P, P2) and np.array_equal(S, S2)
+    with pytest.raises(RuntimeError):
+        g.generate_pslg(n, 10, L=L, dist=7)
 
 
 def test_cdt_segment_through_collinear_points(oracle):

@@ -27,7 +27,9 @@ def _valid(oracle, P, T, N, S):
 
 
 @pytest.mark.parametrize("n,ns,dist,seed", [(2000, 100, 0, 1), (20000, 1000, 1, 2),
-                                            (60000, 3000, 0, 3)])
+                                            (60000, 3000, 0, 3), (20000, 1000, 3, 4),
+                                            (20000, 1000, 4, 5), (20000, 1000, 5, 6),
+                                            (20000, 1000, 6, 7)])
 def test_gpu_cdt_equals_triangle(gpu, oracle, n, ns, dist, seed):
     print(f"gpu cdt n={n} dist={dist}", flush=True)
     L = 1000.0 * (n / 10000) ** 0.5

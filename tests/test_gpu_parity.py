@@ -43,6 +43,38 @@ def test_gpu_equals_oracle(gpu, oracle, n, ns, seed, theta, off, mode):
     assert mina >= theta - 1e-6
 
 
+FAMILIES = [
+    # (dist, n points, theta, offcenter, mode): the reference generator's
+    # disk, circle (ring), grid and ellipse families (mesh.cpp:131-193),
+    # segments no longer than 5% of the domain
+    (3, 4000, 25.0, 1, 1),
+    (4, 4000, 28.0, 1, 0),
+    (5, 4000, 25.0, 0, 1),
+    (6, 800, 28.0, 1, 1),
+]
+
+
+@pytest.mark.parametrize("dist,n,theta,off,mode", FAMILIES)
+def test_gpu_equals_oracle_families(gpu, oracle, dist, n, theta, off, mode):
+    print(f"family dist={dist} n={n} theta={theta} off={off} mode={mode}", flush=True)
+    L = 1000.0 * (n / 10000) ** 0.5
+    P0, S0 = gpu.generate_pslg(n, n // 20, L=L, maxlen=0.05 * L, dist=dist, min_angle=5.0,
+                               seed=40 + dist)
+    P, T, N, S = oracle.triangle_cdt(P0, S0)
+    r = gpu.refine(P, T, N, S, theta=theta, insert=off, mode=mode)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, theta=theta, offcenter=off, mode=mode)
+    assert r.stats["status"] == 0 and st["status"] == 0
+    assert r.stats["rounds"] == st["rounds"]
+    assert np.array_equal(r.points.view(np.uint64), Po.view(np.uint64))
+    assert np.array_equal(r.triangles, To)
+    assert np.array_equal(r.neighbors, No)
+    assert np.array_equal(r.segments, So)
+    tot, viol, mina, ex = oracle.check_mesh(r.points, r.triangles, r.neighbors, r.segments,
+                                            P, S, theta)
+    assert tot == 0, viol
+    assert mina >= theta - 1e-6
+
+
 def test_gpu_deterministic(gpu, oracle):
     P, T, N, S = _input(oracle, 5000, 250, 11)
     a = gpu.refine(P, T, N, S, theta=28.0)
