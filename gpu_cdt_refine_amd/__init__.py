@@ -63,6 +63,7 @@ class Options(C.Structure):
         ("mixed", C.c_int),
         ("shared_ring", C.c_int),
         ("mix_below", C.c_int),
+        ("deletion", C.c_int),
     ]
 
 
@@ -120,6 +121,8 @@ class Stats(C.Structure):
         ("ws_ints_bad", C.c_longlong),
         ("giveup_tris_left", C.c_longlong),
         ("giveup_segs_left", C.c_longlong),
+        ("deleted", C.c_longlong),
+        ("flips", C.c_longlong),
     ]
 
     def as_dict(self):

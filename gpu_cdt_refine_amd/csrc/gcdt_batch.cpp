@@ -309,6 +309,8 @@ int gcdt_batch_run(gcdt_batch *b, gcdt_stats *stats) {
       tot.tri_examined_bad += s.tri_examined_bad;
       tot.ws_ints_seg += s.ws_ints_seg;
       tot.ws_ints_bad += s.ws_ints_bad;
+      tot.deleted += s.deleted;
+      tot.flips += s.flips;
       b->run_ms[w] = std::chrono::duration<double, std::milli>(
                          std::chrono::steady_clock::now() - t0).count();
       // the Refiner's buffers are reused by the next wave: fetch this one

@@ -1,4 +1,5 @@
 // C ABI of libgcdt.so and the reference-compatible GPU_Refine_Quality.
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -65,6 +66,7 @@ void gcdt_default_options(gcdt_options *o) {
   o->mixed = 0;
   o->shared_ring = 0;
   o->mix_below = 0;
+  o->deletion = 1;
 }
 
 }  // extern "C"
@@ -102,13 +104,15 @@ bool options_ok(const gcdt_options *o, std::string &g_err) {
     return false;
   }
   // The graph loop resets the 11-bit claim epoch only between graph
-  // launches; one launch uses claim_iters + 1 epochs per pass.
-  if (o->graph && (long long)o->graph_rounds * (o->graph_seg_passes + 1) * (o->claim_iters + 1) + 2 >
+  // launches; one launch uses claim_iters + 2 epochs per pass (claims, the
+  // winners' stamp, the vertex deletion step).
+  if (o->graph && (long long)o->graph_rounds * (o->graph_seg_passes + 1) * (o->claim_iters + 2) + 2 >
                       2047) {
-    g_err = "graph_rounds * (graph_seg_passes + 1) * (claim_iters + 1) + 2 must not exceed 2047 "
+    g_err = "graph_rounds * (graph_seg_passes + 1) * (claim_iters + 2) + 2 must not exceed 2047 "
             "(claim epochs of one graph launch)";
     return false;
   }
+  if (o->deletion < 0 || o->deletion > 2) { g_err = "deletion must be 0, 1 or 2"; return false; }
   if (o->graph && o->log_training) { g_err = "log_training needs the host-driven loop (graph = 0)"; return false; }
   if (o->mixed != 0 && o->mixed != 1) { g_err = "mixed must be 0 or 1"; return false; }
   if (o->shared_ring != 0 && o->shared_ring != 1) { g_err = "shared_ring must be 0 or 1"; return false; }
@@ -529,7 +533,6 @@ void GPU_Refine_Quality(struct triangulateio *input,
                         InsertPolicy insertpolicy, DeletePolicy deletepolicy,
                         int mode, int debug_iter, PStatus **ps_debug,
                         TStatus **ts_debug) {
-  (void)deletepolicy;
   (void)debug_iter;
   DropIn &d = dropin();
   std::lock_guard<std::mutex> lk(d.mu);
@@ -538,6 +541,8 @@ void GPU_Refine_Quality(struct triangulateio *input,
   opt.theta_deg = theta;
   opt.offcenter = insertpolicy == Offcenter ? 1 : 0;
   opt.mode = mode;
+  if (deletepolicy == noDeletion) opt.deletion = 0;
+  else if (deletepolicy == DeleteFromDT) opt.deletion = 2;
   g_err.clear();
   if (!input || !result || !valid_options(&opt)) {
     std::fprintf(stderr, "GPU_Refine_Quality: %s\n",
@@ -564,6 +569,7 @@ void GPU_Refine_Quality(struct triangulateio *input,
     result->numberoftriangles = nt;
     result->numberofcorners = 3;
     result->numberofsegments = ns;
+    result->numberofflips = (int)std::min<long long>(d.last.flips, INT_MAX);
     result->pointlist = new double[2 * (size_t)np];
     result->trianglelist = new int[3 * (size_t)nt];
     result->neighborlist = new int[3 * (size_t)nt];

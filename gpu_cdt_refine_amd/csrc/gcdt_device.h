@@ -11,7 +11,8 @@
 //                         or -1, S subsegment on edge j or -1, X[3] the
 //                         vertices' coordinates (copies of P)
 //   P     double2[capP]   point coordinates
-//   pflag uint8[capP]     PF_* bits;   ppar int[capP]: parent input segment
+//   pflag uint8[capP]     PF_* bits (PF_DEAD: deleted, dropped at export);
+//                         ppar int[capP]: parent input segment
 //   tflag uint8[capT]     TF_* bits
 //   SV    int2[capS]      subsegment endpoints (org, dest)
 //   SP    int[capS]       parent input segment;  ST int[capS]: an adjacent otri
@@ -33,7 +34,7 @@
 namespace gcdt {
 
 enum : uint8_t { TF_ALIVE = 1, TF_CHECK = 2, TF_BAD = 4, TF_GIVEUP = 8 };
-enum : uint8_t { PF_INPUT = 1, PF_SEGV = 2, PF_FREE = 4 };
+enum : uint8_t { PF_INPUT = 1, PF_SEGV = 2, PF_FREE = 4, PF_DEAD = 8 };
 enum : int {
   ST_VALID = 0,
   ST_REJ_ENC = 1,
@@ -65,6 +66,8 @@ enum : int {
   CNT_NWIN,      // winners of the pass
   CNT_LEFT_TRI,  // end of the run: alive triangles flagged TF_GIVEUP
   CNT_LEFT_SEG,  // end of the run: subsegments with senc == 2
+  CNT_DEL,       // vertices deleted
+  CNT_FLIPS,     // flips Lawson's algorithm would make for the pass's changes
   CNT_N = 24
 };
 
@@ -151,7 +154,7 @@ enum : int {
 enum : int {
   A_REJ_OVF = 0, A_REJ_ENC, A_REJ_BLK, A_GIVEUPS, A_LOST, A_FP32, A_FP64,
   A_EXACT, A_DD, A_BIG, A_SEGSPLITS, A_INSERTED, A_CANDEVALS, A_BUDGET,
-  A_EXAM_SEG, A_EXAM_BAD, A_WS_SEG, A_WS_BAD,
+  A_EXAM_SEG, A_EXAM_BAD, A_WS_SEG, A_WS_BAD, A_DELETED, A_FLIPS,
   A_NA = 20
 };
 struct Ctl {
@@ -186,6 +189,7 @@ struct PassD {
   Ctl *ctl;
   const int *ids_sel;  // selected candidates when ctl->i[C_USESEL]
   int *slow;           // candidates whose filters could not decide a sign
+  int eskip;  // graph loop: the pass's vertex deletion step took an epoch
 };
 
 __device__ __forceinline__ int pass_nc(const PassD &ps) {
@@ -199,7 +203,8 @@ __device__ __forceinline__ const int *pass_ids(const PassD &ps) {
   return (ps.ctl && ps.ctl->i[C_USESEL]) ? ps.ids_sel : ps.ids;
 }
 __device__ __forceinline__ unsigned long long pass_epoch0(const PassD &ps) {
-  return ps.ctl ? (unsigned long long)ps.ctl->i[C_EPOCH] + 1ull : ps.epoch0;
+  return ps.ctl ? (unsigned long long)ps.ctl->i[C_EPOCH] + 1ull + (unsigned long long)ps.eskip
+                : ps.epoch0;
 }
 __device__ __forceinline__ unsigned long long pass_epoch(const PassD &ps) {
   return ps.ctl ? pass_epoch0(ps) + (unsigned long long)ps.iter : ps.epoch;

@@ -31,6 +31,8 @@ struct MeshHost {
 };
 
 // One device-resident input CDT (a single mesh, or the union of a batch).
+struct DelD;  // vertex deletion step of a pass (gcdt_refine.hip)
+
 struct InputSet {
   int np = 0, nt = 0, ns = 0, n_inst = 0;
   int capP = 0, capT = 0, capS = 0;  // allocated sizes (reuse on upload)
@@ -38,6 +40,7 @@ struct InputSet {
   int *T = nullptr, *N = nullptr, *S = nullptr;
   int *tinst = nullptr;     // instance of every input triangle (batches)
   InstPar *ipar = nullptr;  // per-instance parameters (batches)
+  bool any_chew = false;    // a batch instance refines in Chew mode
 };
 
 enum KernelId {
@@ -131,6 +134,9 @@ class Refiner {
   void ensure_cb(int nb);
   int run_pass(const int *ids, int nc, int nseg, long long *progress, bool *overflowed,
                int *nwin_seg);
+  DelD del_base() const;
+  void enqueue_deletion(const DelD &dd);
+  void compact_points();
   PassD pass_base(int Kc) const;
   void enqueue_claims(PassD ps, int gc);
   void enqueue_seg_pass();
@@ -216,6 +222,16 @@ class Refiner {
   hipGraphExec_t gexec = nullptr;
   int g_K = 0;
   bool capturing = false;
+  // vertex deletion: on for this run; per-nomination workspace, vertex,
+  // star size, win flag
+  bool del_on = false, in_any_chew = false;
+  int *del_ws = nullptr, *del_dv = nullptr, *del_dk = nullptr, *del_win = nullptr;
+  // export with deleted vertices: new point ids and compacted copies
+  int *p_new = nullptr;
+  double2 *p_tmp = nullptr;
+  uint8_t *pf_tmp = nullptr;
+  int *pp_tmp = nullptr;
+  long long p_tmp_cap = 0;
   // batched refinement
   int *in_tinst = nullptr;
   InstPar *d_ipar = nullptr;

@@ -518,6 +518,13 @@ __device__ __forceinline__ bool hset_has(const int *h, int H, int x) {
 
 enum { SC_N = 0, SC_R, SC_OVF, SC_GIVEUP, SC_NBND, SC_ENC, SC_DSEG, SC_SLOW };
 constexpr int ST_NONE = -1;  // lane group without a candidate
+constexpr int ST_DEFER = 7;  // not evaluated this pass (vertex deletion)
+// vertex deletion (below): largest star, subsegment candidates per pass
+// that nominate vertices, workspace ints per nomination (star, link, outer
+// otri, subsegment: DEL_MAX each; new triangles: 3 (DEL_MAX - 2))
+constexpr int DEL_MAX = 32;
+constexpr int DEL_CANDS = 1 << 16;
+constexpr int DEL_WS = 4 * DEL_MAX + 3 * (DEL_MAX - 2);
 constexpr int ST_SLOW = 6;   // deferred: an undecided sign, or a cavity beyond
                              // the fast kernel's LDS capacity
 
@@ -589,7 +596,11 @@ __device__ __forceinline__ void cavity_body(const MeshD &m, const ParamsD &pr,
     double near2 = 0.0;  // a cavity vertex closer than this coincides with the point
     ParamsD lp = pr;
     if (act) {
-      if (IS_SEG) {
+      // skipped this pass: a bad triangle whose slot the deletion step
+      // rewrote
+      if (!IS_SEG && (m.tflag[ids[i]] & (TF_ALIVE | TF_BAD)) != (TF_ALIVE | TF_BAD)) {
+        status = ST_DEFER;
+      } else if (IS_SEG) {
         split_s = ids[i];
         int t1, j1, t2, j2;
         seg_sides(m, split_s, t1, j1, t2, j2);
@@ -1110,6 +1121,7 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_apply(MeshD m, ParamsD pr, PassD
     sbase = ps.ctl->i[C_NS];
   }
   const int gl = threadIdx.x % G, per = blockDim.x / G;
+  unsigned long long nflip = 0;
   for (int i = blockIdx.x * per + threadIdx.x / G; i < nc; i += gridDim.x * per) {
     if (!ps.cwin[i]) continue;
     // winners split subsegments first (candidates [0, nseg)), so a segment
@@ -1150,6 +1162,10 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_apply(MeshD m, ParamsD pr, PassD
       Pb = m.P[b];
     }
     const int nstar = open ? n + 1 : n + 2;
+    // flips of the same insertion by Lawson's algorithm: the first triangle
+    // (both triangles of a two-sided subsegment) split, then one flip per
+    // further cavity triangle
+    if (gl == 0) nflip += (unsigned long long)(n - ((is_seg && !open) ? 2 : 1));
     const int sA = split_s, sB = is_seg ? sbase + rank : -1;
     // first star triangle (in star order) at each end of the split, packed
     // as (q << 32 | otri) so that a minimum finds it
@@ -1241,6 +1257,301 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_apply(MeshD m, ParamsD pr, PassD
         m.T[st] = r;
         m.tflag[st] = 0;
       }
+    }
+  }
+  nflip = wave_sum(nflip);
+  if ((threadIdx.x & 63) == 0 && nflip) atomicAdd(&ps.cnt[CNT_FLIPS], nflip);
+}
+
+// ---------------------------------------------------------------------------
+// Vertex deletion (Chew's rule as Triangle applies it before splitting an
+// encroached subsegment, triangle.cpp:13319-13357; the reference marks the
+// same vertices and removes them by flips, refine.cu:604-677, 778-1070).
+// Before the cavities of a pass with subsegment candidates, each of the
+// first DEL_CANDS candidates nominates the apexes on both sides of its
+// subsegment (candidate d = 2 i + side) when the apex is a free vertex
+// strictly inside the subsegment's diametral circle.  A nominee whose star
+// is closed, has at most DEL_MAX triangles and no subsegment at the vertex,
+// and whose link polygon the ear rule retriangulates, is a deletion
+// candidate; it claims its star and the outer triangles across its link, the
+// smallest d wins each, and a candidate holding all of its triangles deletes
+// its vertex.  The pass's cavities are built afterwards: a subsegment splits
+// in the same pass as its apexes are deleted (Triangle deletes, then splits),
+// and bad-triangle candidates whose slot was rewritten skip the pass.  One
+// apex per side and pass is deleted, where Triangle repeats on the new apex.
+// The ear rule: repeatedly cut the first ear of the remaining link polygon
+// that turns left and whose circumcircle holds no other remaining vertex
+// strictly inside (exact signs); the cut triangles are the constrained
+// Delaunay triangles inside the star once the vertex is gone.  One lane per
+// candidate: stars are small and deletions rare.
+// ---------------------------------------------------------------------------
+
+struct DelD {
+  const int *ids;  // subsegment candidates of the pass
+  int nseg;        // host loop: their number (graph loop: from ctl)
+  Ctl *ctl;
+  unsigned long long epoch;  // host loop: the claim epoch
+  int policy;                // 1: Chew-mode instances only, 2: every instance
+  int *ws, *dv, *dk, *dwin;
+  double *pws;               // exact-predicate arena, one slot per lane
+  unsigned long long *cnt;
+  int use_fp32;
+};
+
+__device__ __forceinline__ int del_nseg(const DelD &d) {
+  const int n = d.ctl ? (d.ctl->i[C_GO] ? d.ctl->i[C_NC] : 0) : d.nseg;
+  return n < DEL_CANDS ? n : DEL_CANDS;
+}
+__device__ __forceinline__ unsigned long long del_epoch(const DelD &d) {
+  return d.ctl ? (unsigned long long)d.ctl->i[C_EPOCH] + 1ull : d.epoch;
+}
+
+// Apex of triangle t (corner j) on subsegment s: a deletion candidate?
+// Writes the star, link, outer otri, subsegments and the cut triangles to W
+// and the star size to *kout; returns the vertex or -1.
+__device__ int del_candidate(const MeshD &m, int s, int t, int j, int *W, int *kout,
+                             double *pws, bool fp32) {
+  int *star = W, *p = W + DEL_MAX, *o = W + 2 * DEL_MAX, *sg = W + 3 * DEL_MAX;
+  int *nt = W + 4 * DEL_MAX;
+  const int v = comp(m.T[t].V, j);
+  if ((m.pflag[v] & (PF_FREE | PF_DEAD)) != PF_FREE) return -1;
+  const int2 sv = m.SV[s];
+  const double2 O = m.P[sv.x], D = m.P[sv.y], V = m.P[v];
+  if (!((O.x - V.x) * (D.x - V.x) + (O.y - V.y) * (D.y - V.y) < 0.0)) return -1;
+  int cur = t, jc = j, k = 0;
+  for (;;) {
+    if (k == DEL_MAX) return -1;
+    const TriRec &r = m.T[cur];
+    const int x = comp(r.V, (jc + 1) % 3), y = comp(r.V, (jc + 2) % 3);
+    if (k == 0) p[0] = x;
+    else if (p[k] != x) return -1;
+    star[k] = cur;
+    o[k] = comp(r.N, jc);
+    sg[k] = comp(r.S, jc);
+    if (comp(r.S, (jc + 1) % 3) >= 0) return -1;  // edge (v, y) is a subsegment
+    const int nb = comp(r.N, (jc + 1) % 3);
+    if (nb < 0) return -1;
+    k++;
+    const int nx = nb >> 2;
+    if (nx == t) {
+      if (y != p[0]) return -1;
+      break;
+    }
+    if (k < DEL_MAX) p[k] = y;
+    const int4 vn = m.T[nx].V;
+    const int jn = vn.x == v ? 0 : (vn.y == v ? 1 : (vn.z == v ? 2 : -1));
+    if (jn < 0) return -1;
+    cur = nx;
+    jc = jn;
+  }
+  if (k < 3) return -1;
+  int rem[DEL_MAX];
+  int r = k, ne = 0;
+  for (int q = 0; q < k; q++) rem[q] = q;
+  while (r > 3) {
+    int found = -1;
+    for (int q = 0; q < r && found < 0; q++) {
+      const int qa = (q + r - 1) % r, qc = (q + 1) % r;
+      const double2 A = m.P[p[rem[qa]]], B = m.P[p[rem[q]]], C = m.P[p[rem[qc]]];
+      if (orient2d_sign(A.x, A.y, B.x, B.y, C.x, C.y, fp32, nullptr) <= 0) continue;
+      bool ok = true;
+      for (int w = 0; w < r && ok; w++) {
+        if (w == qa || w == q || w == qc) continue;
+        const double2 E = m.P[p[rem[w]]];
+        if (incircle_sign(A.x, A.y, B.x, B.y, C.x, C.y, E.x, E.y, pws, fp32, nullptr) > 0) ok = false;
+      }
+      if (ok) found = q;
+    }
+    if (found < 0) return -1;
+    nt[3 * ne] = p[rem[(found + r - 1) % r]];
+    nt[3 * ne + 1] = p[rem[found]];
+    nt[3 * ne + 2] = p[rem[(found + 1) % r]];
+    ne++;
+    for (int q = found; q + 1 < r; q++) rem[q] = rem[q + 1];
+    r--;
+  }
+  {
+    const double2 A = m.P[p[rem[0]]], B = m.P[p[rem[1]]], C = m.P[p[rem[2]]];
+    if (orient2d_sign(A.x, A.y, B.x, B.y, C.x, C.y, fp32, nullptr) <= 0) return -1;
+  }
+  nt[3 * ne] = p[rem[0]];
+  nt[3 * ne + 1] = p[rem[1]];
+  nt[3 * ne + 2] = p[rem[2]];
+  *kout = k;
+  return v;
+}
+
+// Nominations, stars and retriangulations.
+__global__ __launch_bounds__(64) void k_del_scan(MeshD m, ParamsD pr, DelD dd) {
+  const int n = del_nseg(dd);
+  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+  double *pws = dd.pws + (size_t)lane * GCDT_INCIRCLE_WS;
+  for (int i = lane; i < n; i += gridDim.x * blockDim.x) {
+    const int s = dd.ids[i];
+    int t1, j1, t2, j2;
+    seg_sides(m, s, t1, j1, t2, j2);
+    for (int side = 0; side < 2; side++) {
+      const int d = 2 * i + side, t = side ? t2 : t1, j = side ? j2 : j1;
+      int v = -1;
+      if (t >= 0) {
+        const ParamsD lp = params_at(pr, m, t);
+        if (dd.policy == 2 || (dd.policy == 1 && !lp.ruppert))
+          v = del_candidate(m, s, t, j, dd.ws + (size_t)d * DEL_WS, dd.dk + d, pws,
+                            dd.use_fp32 != 0);
+      }
+      dd.dv[d] = v;
+    }
+  }
+}
+
+__global__ void k_del_claim(MeshD m, DelD dd) {
+  const int nd = 2 * del_nseg(dd);
+  const unsigned long long ep = del_epoch(dd);
+  for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < nd; d += gridDim.x * blockDim.x) {
+    if (dd.dv[d] < 0) continue;
+    const int *W = dd.ws + (size_t)d * DEL_WS;
+    const int k = dd.dk[d];
+    const unsigned long long key = claim_key(ep, 0, d);
+    for (int q = 0; q < k; q++) {
+      atomicMax(&m.claim[2 * (size_t)W[q]], key);
+      const int o = W[2 * DEL_MAX + q];
+      if (o >= 0) atomicMax(&m.claim[2 * (size_t)(o >> 2)], key);
+    }
+  }
+}
+
+__global__ void k_del_win(MeshD m, DelD dd) {
+  const int nd = 2 * del_nseg(dd);
+  const unsigned long long ep = del_epoch(dd);
+  for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < nd; d += gridDim.x * blockDim.x) {
+    int w = dd.dv[d] >= 0;
+    if (w) {
+      const int *W = dd.ws + (size_t)d * DEL_WS;
+      const int k = dd.dk[d];
+      const unsigned long long key = claim_key(ep, 0, d);
+      for (int q = 0; q < k && w; q++) {
+        const int o = W[2 * DEL_MAX + q];
+        w = m.claim[2 * (size_t)W[q]] == key && (o < 0 || m.claim[2 * (size_t)(o >> 2)] == key);
+      }
+    }
+    dd.dwin[d] = w;
+  }
+}
+
+// Winners: the cut triangles take the star's slots in ascending order (the
+// two largest slots die); links inside the star are matched edge by edge,
+// link edges take the outer triangle (whose back-link is repaired) and the
+// subsegment (whose adjacent otri is repaired).
+__global__ void k_del_apply(MeshD m, DelD dd) {
+  const int nd = 2 * del_nseg(dd);
+  unsigned long long ndel = 0, nflip = 0;
+  for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < nd; d += gridDim.x * blockDim.x) {
+    if (!dd.dwin[d]) continue;
+    const int *W = dd.ws + (size_t)d * DEL_WS;
+    const int *star = W, *p = W + DEL_MAX, *o = W + 2 * DEL_MAX, *sg = W + 3 * DEL_MAX;
+    const int *nt = W + 4 * DEL_MAX;
+    const int k = dd.dk[d];
+    int sl[DEL_MAX];
+    for (int q = 0; q < k; q++) sl[q] = star[q];
+    for (int q = 1; q < k; q++) {
+      const int x = sl[q];
+      int w = q - 1;
+      while (w >= 0 && sl[w] > x) {
+        sl[w + 1] = sl[w];
+        w--;
+      }
+      sl[w + 1] = x;
+    }
+    const int inst = m.T[star[0]].V.w;
+    for (int q = 0; q < k - 2; q++) {
+      const int st = sl[q];
+      const int V0 = nt[3 * q], V1 = nt[3 * q + 1], V2 = nt[3 * q + 2];
+      int nn[3], ss[3];
+      for (int e = 0; e < 3; e++) {
+        const int u = e == 0 ? V1 : (e == 1 ? V2 : V0);
+        const int w = e == 0 ? V2 : (e == 1 ? V0 : V1);
+        int nb = -2, sgv = -1;
+        for (int i = 0; i < k && nb == -2; i++)
+          if (p[i] == u && p[(i + 1) % k] == w) {
+            nb = o[i];
+            sgv = sg[i];
+            if (nb >= 0) *nbr_slot(m.T, nb >> 2, nb & 3) = (st << 2) | e;
+            if (sgv >= 0) m.ST[sgv] = (st << 2) | e;
+          }
+        for (int q2 = 0; q2 < k - 2 && nb == -2; q2++) {
+          if (q2 == q) continue;
+          for (int e2 = 0; e2 < 3; e2++)
+            if (nt[3 * q2 + (e2 + 1) % 3] == w && nt[3 * q2 + (e2 + 2) % 3] == u) {
+              nb = (sl[q2] << 2) | e2;
+              break;
+            }
+        }
+        nn[e] = nb;
+        ss[e] = sgv;
+      }
+      TriRec r;
+      r.V = make_int4(V0, V1, V2, inst);
+      r.N = make_int4(nn[0], nn[1], nn[2], 0);
+      r.S = make_int4(ss[0], ss[1], ss[2], 0);
+      r.X[0] = m.P[V0];
+      r.X[1] = m.P[V1];
+      r.X[2] = m.P[V2];
+      r.pad[0] = r.pad[1] = make_int4(0, 0, 0, 0);
+      m.T[st] = r;
+      m.tflag[st] = TF_ALIVE | TF_CHECK;
+    }
+    for (int q = k - 2; q < k; q++) {
+      TriRec r;
+      r.V = make_int4(-1, -1, -1, 0);
+      r.N = make_int4(-1, -1, -1, 0);
+      r.S = make_int4(-1, -1, -1, 0);
+      r.X[0] = r.X[1] = r.X[2] = make_double2(0.0, 0.0);
+      r.pad[0] = r.pad[1] = make_int4(0, 0, 0, 0);
+      m.T[sl[q]] = r;
+      m.tflag[sl[q]] = 0;
+    }
+    m.pflag[dd.dv[d]] |= PF_DEAD;
+    ndel++;
+    nflip += (unsigned long long)(k - 3);
+  }
+  ndel = wave_sum(ndel);
+  nflip = wave_sum(nflip);
+  if ((threadIdx.x & 63) == 0) {
+    if (ndel) atomicAdd(&dd.cnt[CNT_DEL], ndel);
+    if (nflip) atomicAdd(&dd.cnt[CNT_FLIPS], nflip);
+  }
+}
+
+// Export with deleted vertices: live points keep their order (input points
+// are never deleted, so they keep their indices) and every reference is
+// renumbered.
+struct PredPAlive {
+  const uint8_t *pflag;
+  __device__ bool operator()(int i) const { return (pflag[i] & PF_DEAD) == 0; }
+};
+__global__ void k_pcompact(const double2 *P, const uint8_t *pflag, const int *ppar, int np,
+                           const int *pnew, double2 *P2, uint8_t *pflag2, int *ppar2) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
+    if (pflag[i] & PF_DEAD) continue;
+    const int q = pnew[i];
+    P2[q] = P[i];
+    pflag2[q] = pflag[i];
+    ppar2[q] = ppar[i];
+  }
+}
+__global__ void k_premap(MeshD m, int nt, int ns, const int *pnew) {
+  const int n = nt > ns ? nt : ns;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (i < nt && (m.tflag[i] & TF_ALIVE)) {
+      int4 v = m.T[i].V;
+      v.x = pnew[v.x];
+      v.y = pnew[v.y];
+      v.z = pnew[v.z];
+      m.T[i].V = v;
+    }
+    if (i < ns) {
+      const int2 sv = m.SV[i];
+      m.SV[i] = make_int2(pnew[sv.x], pnew[sv.y]);
     }
   }
 }
@@ -1605,9 +1916,11 @@ __device__ void ctl_accumulate(Ctl *c, int nwin) {
   c->acc[A_EXAM_BAD] += n[CNT_EXAM_BAD];
   c->acc[A_WS_SEG] += n[CNT_WS_SEG];
   c->acc[A_WS_BAD] += n[CNT_WS_BAD];
+  c->acc[A_DELETED] += n[CNT_DEL];
+  c->acc[A_FLIPS] += n[CNT_FLIPS];
 }
 
-__global__ void k_end_seg(Ctl *c, int iters) {
+__global__ void k_end_seg(Ctl *c, int iters, int eskip) {
   if (!c->i[C_GO]) return;
   int nwin = c->i[C_NWIN];
   ctl_accumulate(c, nwin);
@@ -1616,7 +1929,8 @@ __global__ void k_end_seg(Ctl *c, int iters) {
   c->i[C_NT] += 2 * nwin;
   c->i[C_NS] += nwin;
   c->i[C_SEG_PASSES]++;
-  c->i[C_EPOCH] += iters + 1;  // claim epochs + the winners' stamp
+  // the deletion step's epoch, claim epochs, the winners' stamp
+  c->i[C_EPOCH] += eskip + iters + 1;
   int K = c->i[C_K];
   if (c->cnt[CNT_OVERFLOW] > 0 && K < c->i[C_KMAX]) c->i[C_K] = min(2 * K, c->i[C_KMAX]);
 }
@@ -1805,6 +2119,8 @@ Refiner::~Refiner() {
   dfree(h_key); dfree(h_val);
   dfree(d_ctl); dfree(g_cnt); dfree(g_off);
   dfree(o_inst);
+  dfree(del_ws); dfree(del_dv); dfree(del_dk); dfree(del_win);
+  dfree(p_new); dfree(p_tmp); dfree(pf_tmp); dfree(pp_tmp);
   if (gexec) (void)hipGraphExecDestroy(gexec);
   if (graph) (void)hipGraphDestroy(graph);
   if (h_ctl) (void)hipHostFree(h_ctl);
@@ -1881,6 +2197,9 @@ void Refiner::copy_input(InputSet &in, const double *points, int np, const int *
                          const InstPar *ipar, int n_inst) {
   in.np = np; in.nt = nt; in.ns = ns;
   in.n_inst = tri_inst ? n_inst : 0;
+  in.any_chew = false;
+  if (tri_inst)
+    for (int i = 0; i < n_inst; i++) in.any_chew |= ipar[i].ruppert == 0;
   HC(hipMemcpyAsync(in.P, points, sizeof(double2) * np, hipMemcpyHostToDevice, stream));
   HC(hipMemcpyAsync(in.T, tris, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, stream));
   HC(hipMemcpyAsync(in.N, nbrs, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, stream));
@@ -1936,6 +2255,7 @@ void Refiner::select_input(int k) {
   in_tinst = in.tinst;
   d_ipar = in.ipar;
   n_inst = in.n_inst;
+  in_any_chew = in.any_chew;
 }
 
 // Largest mesh the 32-bit encodings hold: triangle ids travel as
@@ -2187,6 +2507,53 @@ PassD Refiner::pass_base(int Kc) const {
   return ps;
 }
 
+DelD Refiner::del_base() const {
+  DelD dd{};
+  dd.policy = opt.deletion;
+  dd.ws = del_ws;
+  dd.dv = del_dv;
+  dd.dk = del_dk;
+  dd.dwin = del_win;
+  dd.pws = c_pws;
+  dd.cnt = d_cnt;
+  dd.use_fp32 = pr.use_fp32;
+  return dd;
+}
+
+// The deletion step of a pass (host loop: dd.nseg and dd.epoch set; graph
+// loop: both come from the control block).  The scan kernel's grid is the
+// exact-predicate arena's (one arena slot per lane).
+void Refiner::enqueue_deletion(const DelD &dd) {
+  k_del_scan<<<pws_blocks * 4, 64, 0, stream>>>(m, pr, dd);
+  const int g = std::min(nblk(2LL * DEL_CANDS), 512);
+  k_del_claim<<<g, 256, 0, stream>>>(m, dd);
+  k_del_win<<<g, 256, 0, stream>>>(m, dd);
+  k_del_apply<<<g, 256, 0, stream>>>(m, dd);
+  HC(hipGetLastError());
+}
+
+// Drops deleted vertices: live points move to the front in index order and
+// triangles and subsegments are renumbered.
+void Refiner::compact_points() {
+  if (np > p_tmp_cap) {
+    dfree(p_new); dfree(p_tmp); dfree(pf_tmp); dfree(pp_tmp);
+    p_tmp_cap = np;
+    dalloc(&p_new, (size_t)p_tmp_cap);
+    dalloc(&p_tmp, (size_t)p_tmp_cap);
+    dalloc(&pf_tmp, (size_t)p_tmp_cap);
+    dalloc(&pp_tmp, (size_t)p_tmp_cap);
+  }
+  const int np2 = compact(PredPAlive{m.pflag}, np, nullptr, p_new, nullptr);
+  k_pcompact<<<std::min(nblk(np), 4096), 256, 0, stream>>>(m.P, m.pflag, m.ppar, np, p_new,
+                                                             p_tmp, pf_tmp, pp_tmp);
+  k_premap<<<std::min(nblk(std::max(nt, ns)), 4096), 256, 0, stream>>>(m, nt, ns, p_new);
+  HC(hipGetLastError());
+  HC(hipMemcpyAsync(m.P, p_tmp, sizeof(double2) * np2, hipMemcpyDeviceToDevice, stream));
+  HC(hipMemcpyAsync(m.pflag, pf_tmp, np2, hipMemcpyDeviceToDevice, stream));
+  HC(hipMemcpyAsync(m.ppar, pp_tmp, sizeof(int) * np2, hipMemcpyDeviceToDevice, stream));
+  np = np2;
+}
+
 // Claim iterations: claim -> win, repeated; from the second iteration on a
 // claim skips candidates blocked by an earlier iteration's winner stamps.  In
 // the host loop ps.epoch advances here; in the graph loop the kernels derive
@@ -2223,12 +2590,10 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
   ps.nc = nc;
   ps.nseg = nseg;
   int iters = std::max(1, opt.claim_iters);
-  if (epoch + iters + 2 >= (1ull << 11)) {  // epoch wrap: clear claims
+  if (epoch + iters + 3 >= (1ull << 11)) {  // epoch wrap: clear claims
     HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * 2 * capT, stream));
     epoch = 0;
   }
-  ps.epoch0 = epoch + 1;
-  ps.epoch = epoch + 1;
   ps.iter = 0;
   bool want_feat = nbad > 0 && (opt.priority == 2 || opt.log_training);
   if (want_feat && nc > feat_cap) {
@@ -2241,6 +2606,15 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
   ps.feat = want_feat ? c_feat : nullptr;
   ps.pass_index = pass_index;
   HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
+  if (del_on && nseg > 0) {  // vertex deletion first, with its own claim epoch
+    DelD dd = del_base();
+    dd.ids = ids;
+    dd.nseg = nseg;
+    dd.epoch = ++epoch;
+    enqueue_deletion(dd);
+  }
+  ps.epoch0 = epoch + 1;
+  ps.epoch = epoch + 1;
   // 64/lg candidates per wave64 block (the exact kernel's grid is capped by
   // its arena in launch_cavity)
   int lg = cavity_lanes(std::min(K, opt.cav_lds_cap));
@@ -2297,6 +2671,9 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
   read_counters();
   const int nwin = (int)h_cnt[CNT_NWIN];
   const int nws = (int)h_cnt[CNT_SEGWIN];
+  const long long ndel = (long long)h_cnt[CNT_DEL];
+  st.deleted += ndel;
+  st.flips += (long long)h_cnt[CNT_FLIPS];
   if (opt.profile && !capturing) ktimes[K_APPLY].items += nwin;  // winners applied
   np += nwin;
   nt += 2 * nwin;
@@ -2317,7 +2694,7 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
   st.ws_ints_bad += h_cnt[CNT_WS_BAD];
   st.seg_splits += nws;
   st.inserted += nwin - nws;
-  *progress = nwin + (long long)h_cnt[CNT_MARKS] + (long long)h_cnt[CNT_GIVEUPS];
+  *progress = nwin + ndel + (long long)h_cnt[CNT_MARKS] + (long long)h_cnt[CNT_GIVEUPS];
   *overflowed = h_cnt[CNT_OVERFLOW] > 0;
   *nwin_seg = nws;
   return nwin;
@@ -2396,11 +2773,20 @@ void Refiner::run() {
     pws_blocks = EXACT_GRID / 4;
     dalloc(&c_pws, (size_t)pws_blocks * 256 * GCDT_INCIRCLE_WS);
   }
+  // vertex deletion: Triangle's rule deletes in Chew-mode instances only
+  del_on = opt.deletion == 2 || (opt.deletion == 1 && (m.batched ? in_any_chew : !pr.ruppert));
+  if (del_on && !del_ws) {
+    dalloc(&del_ws, (size_t)2 * DEL_CANDS * DEL_WS);
+    dalloc(&del_dv, (size_t)2 * DEL_CANDS);
+    dalloc(&del_dk, (size_t)2 * DEL_CANDS);
+    dalloc(&del_win, (size_t)2 * DEL_CANDS);
+  }
   st.status = 1;
   if (opt.graph) loop_graph();
   else if (opt.mixed) loop_mixed();
   else loop_host();
   st.final_cav_cap = K;
+  if (st.deleted > 0) compact_points();
   // given-up elements still in the mesh: a run that converged otherwise
   // reports them as status 3, not as converged
   HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
@@ -2637,6 +3023,14 @@ void Refiner::enqueue_seg_pass() {
   ps.nseg = -1;
   ps.ctl = d_ctl;
   ps.cnt = ctl_cnt_ptr();
+  if (del_on) {  // vertex deletion first, with its own claim epoch
+    DelD dd = del_base();
+    dd.ids = tmp_list;
+    dd.ctl = d_ctl;
+    dd.cnt = ctl_cnt_ptr();
+    enqueue_deletion(dd);
+    ps.eskip = 1;
+  }
   launch_cavity(true, opt.grid_cavity, ps);
   enqueue_claims(ps, opt.grid_claim);
   k_dcount<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_cnt);
@@ -2644,7 +3038,7 @@ void Refiner::enqueue_seg_pass() {
   k_dscatter<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_off,
                                              nullptr, c_rank, nullptr);
   k_apply<APPLY_LANES><<<opt.grid_apply, LIST_BLOCK, 0, stream>>>(m, pr, ps, 0, 0, 0);
-  k_end_seg<<<1, 1, 0, stream>>>(d_ctl, std::max(1, opt.claim_iters));
+  k_end_seg<<<1, 1, 0, stream>>>(d_ctl, std::max(1, opt.claim_iters), del_on ? 1 : 0);
   HC(hipGetLastError());
 }
 
@@ -2746,7 +3140,7 @@ void Refiner::loop_graph() {
   const long long max_launches = 4LL * opt.max_rounds + 100000;
   for (long long launch = 0;; launch++) {
     if (launch > max_launches) throw std::runtime_error("graph loop made no progress");
-    if (h.i[C_EPOCH] + passes * (iters + 1) + 1 >= (1LL << 11)) {  // epoch wrap
+    if (h.i[C_EPOCH] + passes * (iters + 2) + 1 >= (1LL << 11)) {  // epoch wrap
       HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * 2 * capT, stream));
       h.i[C_EPOCH] = 0;
       HC(hipMemcpyAsync(d_ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, stream));
@@ -2805,6 +3199,8 @@ void Refiner::loop_graph() {
   st.seg_splits = (long long)h.acc[A_SEGSPLITS];
   st.inserted = (long long)h.acc[A_INSERTED];
   st.cand_evals = (long long)h.acc[A_CANDEVALS];
+  st.deleted = (long long)h.acc[A_DELETED];
+  st.flips = (long long)h.acc[A_FLIPS];
 }
 
 Refiner::DeviceMesh Refiner::device_mesh() const {

@@ -81,8 +81,11 @@ typedef struct TStatus { unsigned char _status; } TStatus;
  * `result` receives pointlist/trianglelist/neighborlist/segmentlist
  * allocated with new[] (free with delete[]) and the four counts; the first
  * numberofpoints input points are the input points, unmoved.
- * `deletepolicy` and `debug_iter` are accepted and ignored, as in the
- * reference.  On a device error the function prints it and exits, as the
+ * `debug_iter` is accepted and ignored, as in the reference.  `deletepolicy`
+ * (inert in the reference) selects vertex deletion: noDeletion turns it
+ * off, DeleteFromDT deletes in every mode (the reference's rule,
+ * refine.cu:604-677), any other value keeps the configured rule (default:
+ * in Chew mode, Triangle's).  numberofflips receives gcdt_stats.flips.  On a device error the function prints it and exits, as the
  * reference's checkCUDAError does (refine.cu:20-29).
  * It runs the chosen form (FP32-filtered exact predicates, three claim
  * iterations with shared rings, the learned priority compiled into the
@@ -175,6 +178,14 @@ typedef struct gcdt_options {
                             the round's bad pass, first in candidate order.
                             0 is the reference's round.  Host-driven loop
                             only (graph = 0).  Default 0 */
+  int deletion;          /* before an encroached subsegment splits, free
+                            vertices (inserted off segments) strictly inside
+                            its diametral circle that are apexes of its
+                            triangles are deleted and their star
+                            retriangulated: 0 never; 1 in Chew mode (mode !=
+                            1), Triangle's rule (triangle.cpp:13319-13357,
+                            default); 2 in every mode, the reference's
+                            (refine.cu:604-677) */
 } gcdt_options;
 
 typedef struct gcdt_stats {
@@ -202,6 +213,13 @@ typedef struct gcdt_stats {
                             (cavity + ring lists, ordered star boundary) */
   long long giveup_tris_left, giveup_segs_left; /* given-up elements still in
                             the refined mesh (see status 3) */
+  long long deleted;     /* free vertices deleted (options.deletion) */
+  long long flips;       /* edge flips Lawson's algorithm would make for the
+                            same insertions and deletions: per inserted
+                            vertex its cavity's triangles less one (less two
+                            on a subsegment with triangles on both sides), per
+                            deleted vertex its degree less three (the drop-in
+                            returns it as numberofflips, triangle.h:283) */
 } gcdt_stats;
 
 typedef struct gcdt_mesh gcdt_mesh;       /* host-side result */

@@ -65,6 +65,11 @@ typedef struct {
   int mix_below;      /* classic rounds: the segment phase ends once at most
                          this many subsegments are encroached; they join the
                          round's bad pass (0: the reference's round) */
+  int deletion;       /* free vertices inside an encroached subsegment's
+                         diametral circle are deleted before it splits:
+                         0 never, 1 in Chew mode (Triangle's rule,
+                         triangle.cpp:13319-13357), 2 in every mode (the
+                         reference's, refine.cu:604-677) */
 } orc_params;
 
 typedef struct {
@@ -77,6 +82,9 @@ typedef struct {
   long long cand_evals; /* bad-pass candidates evaluated */
   long long giveup_tris_left, giveup_segs_left; /* alive triangles still
                  flagged given-up; subsegments whose split was given up */
+  long long deleted;    /* free vertices deleted */
+  long long flips;      /* edge flips Lawson's algorithm would make for the
+                 same insertions and deletions (triangle.h:283 numberofflips) */
 } orc_stats;
 
 typedef struct orc_result orc_result;

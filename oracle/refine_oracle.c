@@ -45,12 +45,14 @@
 #define PF_INPUT 1
 #define PF_SEGV 2
 #define PF_FREE 4
+#define PF_DEAD 8
 
 #define ST_VALID 0
 #define ST_REJ_ENC 1
 #define ST_REJ_BLOCKED 2
 #define ST_OVERFLOW 3
 #define ST_GIVEUP 4
+#define ST_DEFER 6 /* not evaluated this pass (see the vertex deletion below) */
 
 #define WALK_LIMIT (1 << 22)
 
@@ -799,6 +801,218 @@ static int blocked_by(Mesh *m, int s, long long *marks) {
   return ST_REJ_BLOCKED;
 }
 
+/* ---- vertex deletion ----------------------------------------------------
+ * Chew's rule as Triangle applies it before splitting an encroached
+ * subsegment (triangle.cpp:13319-13357; the reference marks the same
+ * vertices, refine.cu:604-677): while the apex of the triangle on either side
+ * of the subsegment is a free vertex (inserted at a circumcenter or
+ * off-center, not on a segment) strictly inside the subsegment's diametral
+ * circle, that vertex is deleted.  Bulk-synchronous form: in a pass with
+ * subsegment candidates, every one of the first DEL_CANDS nominates its two
+ * side apexes; a nominee whose star is closed, has at most DEL_MAX triangles
+ * and no subsegment at the vertex, and whose link polygon the ear rule below
+ * retriangulates, is a deletion candidate (index 2 i + side).  Candidates
+ * claim their star and the outer triangles across the link; the smallest
+ * index wins each triangle and a candidate holding all of its triangles
+ * deletes its vertex.  The pass's cavities are built afterwards, so a
+ * subsegment splits in the same pass as its apexes are deleted (Triangle
+ * deletes, then splits); bad-triangle candidates whose triangle was
+ * retriangulated skip the pass.  One apex per side and pass is deleted,
+ * where Triangle repeats on the new apex until none qualifies.
+ *
+ * Retriangulation of the link polygon p[0..k) (counterclockwise): repeatedly
+ * cut the first ear (p[q-1], p[q], p[q+1]) of the remaining polygon that
+ * turns left and whose circumcircle holds no other remaining vertex strictly
+ * inside, until a triangle is left.  Each cut triangle is a Delaunay triangle
+ * of the link's vertices, which with the vertex gone are the constrained
+ * Delaunay triangles inside the star (the mesh outside the star does not
+ * change).  No ear found: the vertex is kept. */
+#define DEL_MAX 32
+#define DEL_CANDS (1 << 16)
+
+typedef struct {
+  int v, k;
+  int star[DEL_MAX];          /* star[i] = (v, p[i], p[i+1]) */
+  int p[DEL_MAX], o[DEL_MAX], s[DEL_MAX]; /* link, outer otri, subsegment of edge i */
+  int nt[3 * (DEL_MAX - 2)];  /* new triangles, counterclockwise, cut order */
+} DelCand;
+
+/* Apex of triangle t (corner j) on subsegment s: a deletion candidate? */
+static int del_candidate(const Mesh *m, int s, int t, int j, DelCand *d) {
+  const int v = m->TV[3 * t + j];
+  if ((m->pflag[v] & (PF_FREE | PF_DEAD)) != PF_FREE) return 0;
+  const double *O = PT(m, m->SV[2 * s]), *D = PT(m, m->SV[2 * s + 1]), *V = PT(m, v);
+  if (!((O[0] - V[0]) * (D[0] - V[0]) + (O[1] - V[1]) * (D[1] - V[1]) < 0.0)) return 0;
+  /* star, counterclockwise around v from t */
+  int cur = t, jc = j, k = 0;
+  for (;;) {
+    if (k == DEL_MAX) return 0;
+    const int x = m->TV[3 * cur + (jc + 1) % 3], y = m->TV[3 * cur + (jc + 2) % 3];
+    if (k == 0) d->p[0] = x;
+    else if (d->p[k] != x) return 0;
+    d->star[k] = cur;
+    d->o[k] = m->TN[3 * cur + jc];
+    d->s[k] = m->TS[3 * cur + jc];
+    /* edge (v, y) is opposite x */
+    if (m->TS[3 * cur + (jc + 1) % 3] >= 0) return 0;
+    const int nb = m->TN[3 * cur + (jc + 1) % 3];
+    if (nb < 0) return 0;
+    k++;
+    const int nx = nb >> 2;
+    if (nx == t) {
+      if (y != d->p[0]) return 0;
+      break;
+    }
+    if (k < DEL_MAX) d->p[k] = y;
+    int jn = -1;
+    for (int q = 0; q < 3; q++)
+      if (m->TV[3 * nx + q] == v) jn = q;
+    if (jn < 0) return 0;
+    cur = nx;
+    jc = jn;
+  }
+  if (k < 3) return 0;
+  /* ear cutting */
+  int rem[DEL_MAX], r = k, ne = 0;
+  for (int q = 0; q < k; q++) rem[q] = q;
+  while (r > 3) {
+    int found = -1;
+    for (int q = 0; q < r && found < 0; q++) {
+      const int qa = (q + r - 1) % r, qc = (q + 1) % r;
+      const double *A = PT(m, d->p[rem[qa]]), *B = PT(m, d->p[rem[q]]), *C = PT(m, d->p[rem[qc]]);
+      if (orc_orient2d(A, B, C) <= 0) continue;
+      int ok = 1;
+      for (int w = 0; w < r && ok; w++) {
+        if (w == qa || w == q || w == qc) continue;
+        if (orc_incircle(A, B, C, PT(m, d->p[rem[w]])) > 0) ok = 0;
+      }
+      if (ok) found = q;
+    }
+    if (found < 0) return 0;
+    d->nt[3 * ne] = d->p[rem[(found + r - 1) % r]];
+    d->nt[3 * ne + 1] = d->p[rem[found]];
+    d->nt[3 * ne + 2] = d->p[rem[(found + 1) % r]];
+    ne++;
+    for (int q = found; q + 1 < r; q++) rem[q] = rem[q + 1];
+    r--;
+  }
+  if (orc_orient2d(PT(m, d->p[rem[0]]), PT(m, d->p[rem[1]]), PT(m, d->p[rem[2]])) <= 0) return 0;
+  d->nt[3 * ne] = d->p[rem[0]];
+  d->nt[3 * ne + 1] = d->p[rem[1]];
+  d->nt[3 * ne + 2] = d->p[rem[2]];
+  d->v = v;
+  d->k = k;
+  return 1;
+}
+
+/* Replace a winner's star by the cut triangles: they take the star's
+ * slots in ascending order, the two largest slots die. */
+static void del_apply(Mesh *m, const DelCand *d) {
+  const int k = d->k;
+  int sl[DEL_MAX];
+  for (int q = 0; q < k; q++) sl[q] = d->star[q];
+  for (int q = 1; q < k; q++) {
+    int x = sl[q], w = q - 1;
+    while (w >= 0 && sl[w] > x) { sl[w + 1] = sl[w]; w--; }
+    sl[w + 1] = x;
+  }
+  const int inst = m->ipar ? m->tinst[d->star[0]] : 0;
+  for (int q = 0; q < k - 2; q++) {
+    const int st = sl[q];
+    const int *V = d->nt + 3 * q;
+    for (int e = 0; e < 3; e++) {
+      const int u = V[(e + 1) % 3], w = V[(e + 2) % 3];
+      int nb = -2, sg = -1;
+      for (int i = 0; i < k && nb == -2; i++)
+        if (d->p[i] == u && d->p[(i + 1) % k] == w) {
+          nb = d->o[i];
+          sg = d->s[i];
+          if (nb >= 0) m->TN[3 * (nb >> 2) + (nb & 3)] = (st << 2) | e;
+          if (sg >= 0) m->ST[sg] = (st << 2) | e;
+        }
+      for (int q2 = 0; q2 < k - 2 && nb == -2; q2++) {
+        if (q2 == q) continue;
+        for (int e2 = 0; e2 < 3; e2++)
+          if (d->nt[3 * q2 + (e2 + 1) % 3] == w && d->nt[3 * q2 + (e2 + 2) % 3] == u) {
+            nb = (sl[q2] << 2) | e2;
+            break;
+          }
+      }
+      m->TN[3 * st + e] = nb;
+      m->TS[3 * st + e] = sg;
+    }
+    m->TV[3 * st] = V[0];
+    m->TV[3 * st + 1] = V[1];
+    m->TV[3 * st + 2] = V[2];
+    m->tflag[st] = TF_ALIVE | TF_CHECK;
+    if (m->ipar) m->tinst[st] = inst;
+  }
+  for (int q = k - 2; q < k; q++) {
+    const int st = sl[q];
+    m->TV[3 * st] = m->TV[3 * st + 1] = m->TV[3 * st + 2] = -1;
+    m->TN[3 * st] = m->TN[3 * st + 1] = m->TN[3 * st + 2] = -1;
+    m->TS[3 * st] = m->TS[3 * st + 1] = m->TS[3 * st + 2] = -1;
+    m->tflag[st] = 0;
+  }
+  m->pflag[d->v] |= PF_DEAD;
+}
+
+/* The deletion step of a pass over subsegment candidates ids[0..nseg);
+ * returns the number of vertices deleted. */
+static long long delete_step(Mesh *m, const int *ids, int nseg, int policy, orc_stats *st) {
+  const int ns2 = nseg < DEL_CANDS ? nseg : DEL_CANDS;
+  const int nd = 2 * ns2;
+  if (nd == 0) return 0;
+  DelCand *dc = malloc(sizeof(DelCand) * (size_t)nd);
+  uint8_t *ok = calloc((size_t)nd, 1);
+  for (int i = 0; i < ns2; i++) {
+    const int s = ids[i];
+    int t1, j1, t2, j2;
+    seg_sides(m, s, &t1, &j1, &t2, &j2);
+    for (int side = 0; side < 2; side++) {
+      const int t = side ? t2 : t1, j = side ? j2 : j1;
+      if (t < 0) continue;
+      use_inst(m, t);
+      if (!(policy == 2 || (policy == 1 && !m->ruppert))) continue;
+      if (del_candidate(m, s, t, j, &dc[2 * i + side])) ok[2 * i + side] = 1;
+    }
+  }
+  int *claim = malloc(sizeof(int) * (size_t)m->nt);
+  for (int t = 0; t < m->nt; t++) claim[t] = INT32_MAX;
+  for (int d = 0; d < nd; d++) {
+    if (!ok[d]) continue;
+    for (int q = 0; q < dc[d].k; q++) {
+      int t = dc[d].star[q];
+      if (claim[t] > d) claim[t] = d;
+      if (dc[d].o[q] >= 0) {
+        t = dc[d].o[q] >> 2;
+        if (claim[t] > d) claim[t] = d;
+      }
+    }
+  }
+  long long ndel = 0;
+  uint8_t *win = calloc((size_t)nd, 1);
+  for (int d = 0; d < nd; d++) {
+    if (!ok[d]) continue;
+    int w = 1;
+    for (int q = 0; q < dc[d].k && w; q++)
+      w = claim[dc[d].star[q]] == d && (dc[d].o[q] < 0 || claim[dc[d].o[q] >> 2] == d);
+    win[d] = (uint8_t)w;
+  }
+  for (int d = 0; d < nd; d++)
+    if (win[d]) {
+      del_apply(m, &dc[d]);
+      ndel++;
+      st->flips += dc[d].k - 3;
+    }
+  st->deleted += ndel;
+  free(win);
+  free(claim);
+  free(ok);
+  free(dc);
+  return ndel;
+}
+
 static int cmp_ll(const void *a, const void *b) {
   long long x = *(const long long *)a, y = *(const long long *)b;
   return x < y ? -1 : (x > y);
@@ -821,19 +1035,25 @@ static int cands_per_pass(long long budget, int K) {
  * winners (*nwin_seg of them segment splits); sets *overflowed if any cavity
  * exceeded K. */
 static int run_pass(Mesh *m, const int *ids, int nc, int nseg, int K,
-                    int Kmax, int priority, int claim_iters, int shared_ring, orc_stats *st,
-                    int *overflowed, long long *progress, int *nwin_seg) {
+                    int Kmax, int priority, int claim_iters, int shared_ring, int deletion,
+                    orc_stats *st, int *overflowed, long long *progress, int *nwin_seg) {
   long long marks = 0, gave = st->giveups;
   Cand *cd = calloc((size_t)nc, sizeof(Cand));
   int *pool = malloc(sizeof(int) * (size_t)(2 * K + 2));
+  const long long ndel = deletion ? delete_step(m, ids, nseg, deletion, st) : 0;
   *overflowed = 0;
   for (int i = 0; i < nc; i++) {
     Cand *c = &cd[i];
     const int is_seg = i < nseg;
     c->id = ids[i];
-    c->prio = (priority == 1 && !is_seg) ? shortest_edge_bucket(m, ids[i]) : 0;
     c->L = pool; /* temporary; copied out below if valid */
     int seed0 = -1, seed1 = -1, split_s = -1;
+    /* skipped: a bad triangle whose slot the deletion step rewrote */
+    if (!is_seg && (m->tflag[ids[i]] & (TF_ALIVE | TF_BAD)) != (TF_ALIVE | TF_BAD)) {
+      c->status = ST_DEFER;
+      goto done;
+    }
+    c->prio = (priority == 1 && !is_seg) ? shortest_edge_bucket(m, ids[i]) : 0;
     if (is_seg) {
       int s = ids[i];
       split_s = s;
@@ -993,6 +1213,12 @@ static int run_pass(Mesh *m, const int *ids, int nc, int nseg, int K,
   for (int i = 0; i < nc; i++) {
     if (!win[i]) continue;
     const int is_seg = i < nseg;
+    /* flips of the same insertion done by Lawson's algorithm: the first
+     * triangle (two on a segment with both sides) split, then one flip per
+     * further cavity triangle */
+    int t1 = 0, j1, t2 = 0, j2;
+    if (is_seg) seg_sides(m, cd[i].id, &t1, &j1, &t2, &j2);
+    st->flips += cd[i].n - ((is_seg && t1 >= 0 && t2 >= 0) ? 2 : 1);
     int p = pbase + rank;
     m->P[2 * p] = cd[i].c[0];
     m->P[2 * p + 1] = cd[i].c[1];
@@ -1009,7 +1235,7 @@ static int run_pass(Mesh *m, const int *ids, int nc, int nseg, int K,
   st->seg_splits += nws;
   st->inserted += nwin - nws;
   *nwin_seg = nws;
-  *progress = nwin + marks + (st->giveups - gave);
+  *progress = nwin + ndel + marks + (st->giveups - gave);
   if (getenv("ORC_TRACE")) {
     long long nval = 0, csum = 0, cmax = 0;
     for (int i = 0; i < nc; i++)
@@ -1195,7 +1421,7 @@ static int mixed_round(Mesh *m, const orc_params *prm, orc_stats *st, int *K, in
   int of = 0, nws = 0;
   long long prog = 0;
   int nw = run_pass(m, L, ne + nb, ne, *K, prm->cav_cap_max, prm->priority, prm->claim_iters,
-                    prm->shared_ring, st, &of, &prog, &nws);
+                    prm->shared_ring, prm->deletion, st, &of, &prog, &nws);
   if (ne > 0) st->seg_passes++;
   if (nb > 0) {
     st->bad_passes++;
@@ -1243,7 +1469,7 @@ static orc_result *refine_impl(int np, const double *points, int nt,
       int of = 0, nws = 0;
       long long prog = 0;
       run_pass(m, ids, ne, ne, K, prm->cav_cap_max, prm->priority, prm->claim_iters, prm->shared_ring,
-               st, &of, &prog, &nws);
+               prm->deletion, st, &of, &prog, &nws);
       st->seg_passes++;
       if (of && K < prm->cav_cap_max) K = K * 2 > prm->cav_cap_max ? prm->cav_cap_max : K * 2;
     }
@@ -1259,18 +1485,26 @@ static orc_result *refine_impl(int np, const double *points, int nt,
   for (int s = 0; s < m->ns; s++) st->giveup_segs_left += m->senc[s] == 2;
   if (st->status == 0 && (st->giveup_tris_left > 0 || st->giveup_segs_left > 0)) st->status = 3;
 
-  /* export: live triangles in slot order */
+  /* export: live points in index order (deleted vertices dropped, the
+   * others renumbered), live triangles in slot order */
   orc_result *r = calloc(1, sizeof(orc_result));
+  int *pnew = malloc(sizeof(int) * (size_t)(m->np > 0 ? m->np : 1));
+  int npl = 0;
+  for (int p = 0; p < m->np; p++) pnew[p] = (m->pflag[p] & PF_DEAD) ? -1 : npl++;
   int *newid = malloc(sizeof(int) * (size_t)(m->nt > 0 ? m->nt : 1));
   int nalive = 0;
   for (int t = 0; t < m->nt; t++) newid[t] = (m->tflag[t] & TF_ALIVE) ? nalive++ : -1;
-  r->np = m->np;
+  r->np = npl;
   r->nt = nalive;
   r->ns = m->ns;
-  r->P = malloc(sizeof(double) * 2 * (size_t)(m->np > 0 ? m->np : 1));
-  memcpy(r->P, m->P, sizeof(double) * 2 * (size_t)m->np);
-  r->pflag = malloc((size_t)(m->np > 0 ? m->np : 1));
-  memcpy(r->pflag, m->pflag, (size_t)m->np);
+  r->P = malloc(sizeof(double) * 2 * (size_t)(npl > 0 ? npl : 1));
+  r->pflag = malloc((size_t)(npl > 0 ? npl : 1));
+  for (int p = 0; p < m->np; p++) {
+    if (pnew[p] < 0) continue;
+    r->P[2 * (size_t)pnew[p]] = m->P[2 * (size_t)p];
+    r->P[2 * (size_t)pnew[p] + 1] = m->P[2 * (size_t)p + 1];
+    r->pflag[pnew[p]] = m->pflag[p];
+  }
   r->T = malloc(sizeof(int) * 3 * (size_t)(nalive > 0 ? nalive : 1));
   r->I = m->ipar ? malloc(sizeof(int) * (size_t)(nalive > 0 ? nalive : 1)) : NULL;
   r->N = malloc(sizeof(int) * 3 * (size_t)(nalive > 0 ? nalive : 1));
@@ -1278,15 +1512,16 @@ static orc_result *refine_impl(int np, const double *points, int nt,
     int q = newid[t];
     if (q < 0) continue;
     for (int j = 0; j < 3; j++) {
-      r->T[3 * q + j] = m->TV[3 * t + j];
+      r->T[3 * q + j] = pnew[m->TV[3 * t + j]];
       int nb = m->TN[3 * t + j];
       r->N[3 * q + j] = nb < 0 ? -1 : newid[nb >> 2];
     }
     if (r->I) r->I[q] = m->tinst[t];
   }
   r->S = malloc(sizeof(int) * 2 * (size_t)(m->ns > 0 ? m->ns : 1));
-  memcpy(r->S, m->SV, sizeof(int) * 2 * (size_t)m->ns);
+  for (int q = 0; q < 2 * m->ns; q++) r->S[q] = pnew[m->SV[q]];
   free(newid);
+  free(pnew);
   free_mesh(m);
   return r;
 }

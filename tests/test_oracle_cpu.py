@@ -289,7 +289,7 @@ def test_mixed_rounds_invariants(oracle, theta, off, mode, ci):
     assert st["rounds"] < stc["seg_passes"] + stc["bad_passes"]
     assert abs((len(Po) - len(Pc)) / (len(Pc) - len(P))) < 0.05
     assert st["seg_splits"] + len(S) == len(So)
-    assert st["inserted"] + st["seg_splits"] + len(P) == len(Po)
+    assert st["inserted"] + st["seg_splits"] - st["deleted"] + len(P) == len(Po)
 
 
 @pytest.mark.parametrize("theta,off,mode,ci", [(25, 1, 1, 1), (25, 1, 1, 3), (30, 0, 0, 2)])
@@ -434,3 +434,50 @@ def test_split_point_rule(oracle):
         ulp = F(float(np.spacing(max(np.abs(a).max(), np.abs(b).max(), np.abs(p).max()))))
         assert cross * cross <= (2 * ulp) ** 2 * ((B[0] - A[0]) ** 2 + (B[1] - A[1]) ** 2), \
             (a, b, p)
+
+
+@pytest.mark.parametrize("seed,theta", [(1, 25.0), (2, 30.0), (3, 20.7)])
+def test_vertex_deletion_pins(oracle, seed, theta):
+    """Vertex deletion (Chew mode, Triangle's rule triangle.cpp:13319-13357):
+    the mesh stays a constrained Delaunay triangulation meeting every
+    invariant (exact checker, so a wrong retriangulation of a star fails);
+    the point count accounts for every insertion and deletion, input points
+    keep their indices; and the Steiner count moves towards the reference's
+    own Triangle -q (which deletes) compared with no deletion."""
+    pts, segs = oracle.random_pslg(5000, 250, L=1000.0, maxlen=200.0, seed=seed)
+    P, T, N, S = oracle.triangle_cdt(pts, segs)
+    P0, T0, N0, S0, pf0, st0 = oracle.oracle_refine(P, T, N, S, theta=theta, mode=0, deletion=0)
+    P1, T1, N1, S1, pf1, st1 = oracle.oracle_refine(P, T, N, S, theta=theta, mode=0, deletion=1)
+    assert st0["deleted"] == 0 and st1["deleted"] > 0
+    assert st1["status"] == 0
+    tot, viol, mina, ex = oracle.check_mesh(P1, T1, N1, S1, P, S, theta)
+    assert tot == 0, viol
+    assert mina >= theta - 1e-7
+    assert np.array_equal(P1[:len(P)].view(np.uint64), P.view(np.uint64))
+    assert st1["inserted"] + st1["seg_splits"] - st1["deleted"] + len(P) == len(P1)
+    assert st1["seg_splits"] + len(S) == len(S1)
+    # every output point is used by a triangle (deleted vertices are gone)
+    assert len(np.unique(T1)) == len(P1)
+    # flips: per insertion cavity size less one or two, per deletion degree
+    # less three -- at least the insertions' count
+    assert st1["flips"] > st1["inserted"]
+    if os.path.exists(oracle.TRIREF_SO):
+        tq = len(oracle.triangle_refine(P, T, S, theta, 0)[0])
+        assert abs(len(P1) - tq) < abs(len(P0) - tq)
+
+
+def test_vertex_deletion_modes(oracle):
+    """deletion = 1 acts in Chew mode only; in Ruppert mode no Steiner
+    point is ever inserted inside a diametral circle, so deletion = 2 finds
+    none there and the mesh equals the run without deletion."""
+    P, T, N, S = _cdt(oracle, 3000, 150, 12)
+    a = oracle.oracle_refine(P, T, N, S, theta=25.0, mode=1, deletion=0)
+    b = oracle.oracle_refine(P, T, N, S, theta=25.0, mode=1, deletion=2)
+    assert b[5]["deleted"] == 0
+    for x, y in zip(a[:4], b[:4]):
+        assert np.array_equal(x, y)
+    c = oracle.oracle_refine(P, T, N, S, theta=25.0, mode=0, deletion=2)
+    d = oracle.oracle_refine(P, T, N, S, theta=25.0, mode=0, deletion=1)
+    assert c[5]["deleted"] == d[5]["deleted"] > 0
+    for x, y in zip(c[:4], d[:4]):
+        assert np.array_equal(x, y)
