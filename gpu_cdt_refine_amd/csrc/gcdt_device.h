@@ -34,7 +34,8 @@
 namespace gcdt {
 
 enum : uint8_t { TF_ALIVE = 1, TF_CHECK = 2, TF_BAD = 4, TF_GIVEUP = 8 };
-enum : uint8_t { PF_INPUT = 1, PF_SEGV = 2, PF_FREE = 4, PF_DEAD = 8 };
+// PF_NEW: inserted in the run's last round (set at export)
+enum : uint8_t { PF_INPUT = 1, PF_SEGV = 2, PF_FREE = 4, PF_DEAD = 8, PF_NEW = 16 };
 enum : int {
   ST_VALID = 0,
   ST_REJ_ENC = 1,
@@ -149,6 +150,7 @@ enum : int {
   C_CAPP, C_CAPT, C_CAPS,
   C_MAXROUNDS, C_KMAX, C_CANDMIN, C_CANDFACTOR, C_ITERS, C_PRIORITY,
   C_NTOUT,
+  C_NP_ROUND,    // live points when the current round opened
   C_NI = 64
 };
 enum : int {

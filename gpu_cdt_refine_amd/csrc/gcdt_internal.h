@@ -173,6 +173,7 @@ class Refiner {
   int2 *seg_in = nullptr;
   long long capP = 0, capT = 0, capS = 0;
   int K = 32;
+  int np_round = 0;  // live points when the last round opened
   unsigned long long epoch = 0;
   // candidates
   int c_cap = 0;

@@ -1600,8 +1600,13 @@ __global__ void k_pstatus(const uint8_t *pflag, int np, uint8_t *out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   uint8_t f = pflag[i];
-  // PStatus bits: 1 Steiner, 2 segment split
-  out[i] = (uint8_t)(((f & PF_INPUT) ? 0 : 2) | ((f & PF_SEGV) ? 4 : 0));
+  // PStatus bits: 1 Steiner, 2 segment split, 3 new (refine.h:36-106)
+  out[i] = (uint8_t)(((f & PF_INPUT) ? 0 : 2) | ((f & PF_SEGV) ? 4 : 0) | ((f & PF_NEW) ? 8 : 0));
+}
+
+__global__ void k_mark_new(uint8_t *pflag, int from, int np) {
+  for (int i = from + blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x)
+    pflag[i] |= PF_NEW;
 }
 
 // Device predicate evaluation for tests.
@@ -1884,6 +1889,7 @@ __global__ void k_gate_seg(Ctl *c) {
     } else {
       c->i[C_ROUND_OPEN] = 1;
       c->i[C_ROUNDS]++;
+      c->i[C_NP_ROUND] = c->i[C_NP];
     }
   }
   c->i[C_GO] = go;
@@ -2706,6 +2712,7 @@ void Refiner::run() {
   HC(hipEventRecord(ev0, stream));
   // --- initialise working mesh from the uploaded input --------------------
   np = np0; nt = nt0; ns = ns0;
+  np_round = np0;
   long long wantP = std::min(std::max<long long>(16, 6LL * np0), MAX_POINTS);
   long long wantT = std::min(std::max<long long>(16, 6LL * nt0), MAX_TRIANGLES);
   long long wantS = std::min(std::max<long long>(16, 8LL * ns0 + 16), MAX_POINTS);
@@ -2786,6 +2793,12 @@ void Refiner::run() {
   else if (opt.mixed) loop_mixed();
   else loop_host();
   st.final_cav_cap = K;
+  // points inserted in the last round (the reference's PStatus "new" bit,
+  // refine.h:36-106, cleared at every round's start)
+  if (np > np_round) {
+    k_mark_new<<<std::min(nblk(np - np_round), 4096), 256, 0, stream>>>(m.pflag, np_round, np);
+    HC(hipGetLastError());
+  }
   if (st.deleted > 0) compact_points();
   // given-up elements still in the mesh: a run that converged otherwise
   // reports them as status 3, not as converged
@@ -2834,6 +2847,7 @@ void Refiner::loop_host() {
   ensure_cand(std::min(cands_per_pass(K), std::max(4 * nt0, 1024)));
   for (int round = 0; round < opt.max_rounds; round++) {
     st.rounds = round + 1;
+    np_round = np;
     // segment phase: until at most mix_below subsegments are encroached
     int ne;
     for (;;) {
@@ -2860,6 +2874,7 @@ void Refiner::loop_mixed() {
   ensure_cand(std::min(cands_per_pass(K), std::max(4 * nt0, 1024)));
   for (int round = 0; round < opt.max_rounds; round++) {
     st.rounds = round + 1;
+    np_round = np;
     ensure_tmp(ns);
     int ne = compact(PredSenc{m.senc}, ns, tmp_list, nullptr, nullptr);
     if (bad_pass(ne, &prev_win)) break;
@@ -3117,6 +3132,7 @@ void Refiner::loop_graph() {
   memset(&h, 0, sizeof(Ctl));
   h.i[C_STATUS] = 1;
   h.i[C_NP] = np;
+  h.i[C_NP_ROUND] = np;
   h.i[C_NT] = nt;
   h.i[C_NS] = ns;
   h.i[C_K] = K;
@@ -3201,6 +3217,7 @@ void Refiner::loop_graph() {
   st.cand_evals = (long long)h.acc[A_CANDEVALS];
   st.deleted = (long long)h.acc[A_DELETED];
   st.flips = (long long)h.acc[A_FLIPS];
+  np_round = h.i[C_NP_ROUND];
 }
 
 Refiner::DeviceMesh Refiner::device_mesh() const {

@@ -46,6 +46,7 @@
 #define PF_SEGV 2
 #define PF_FREE 4
 #define PF_DEAD 8
+#define PF_NEW 16 /* inserted in the run's last round (set at export) */
 
 #define ST_VALID 0
 #define ST_REJ_ENC 1
@@ -1450,8 +1451,10 @@ static orc_result *refine_impl(int np, const double *points, int nt,
   int *ids = NULL;
   int idcap = 0;
   st->status = 1;
+  int np_round = m->np;
   for (int round = 0; round < prm->max_rounds; round++) {
     st->rounds = round + 1;
+    np_round = m->np;
     if (prm->mixed) {
       if (mixed_round(m, prm, st, &K, &prev_win, &ids, &idcap)) break;
       continue;
@@ -1485,6 +1488,7 @@ static orc_result *refine_impl(int np, const double *points, int nt,
   for (int s = 0; s < m->ns; s++) st->giveup_segs_left += m->senc[s] == 2;
   if (st->status == 0 && (st->giveup_tris_left > 0 || st->giveup_segs_left > 0)) st->status = 3;
 
+  for (int p = np_round; p < m->np; p++) m->pflag[p] |= PF_NEW;
   /* export: live points in index order (deleted vertices dropped, the
    * others renumbered), live triangles in slot order */
   orc_result *r = calloc(1, sizeof(orc_result));

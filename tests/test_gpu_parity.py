@@ -207,3 +207,24 @@ def test_gpu_equals_oracle_given_up_subsegments(gpu, oracle):
                        segments_in=S, theta=theta)
     assert c["quality"] <= st["giveup_tris_left"], (c, st)
     assert gpu.violations(c) == c["quality"], c
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_gpu_pstatus_matches_oracle(gpu, oracle, graph):
+    """Debug point status (PStatus bits 1 Steiner, 2 segment split, 3 new:
+    inserted in the run's last round, refine.h:36-106) of a run stopped by
+    the round cap, and of a converged one, against the oracle's point flags."""
+    P, T, N, S = _input(oracle, 5000, 250, 31)
+    for rounds in (6, 100000):
+        print(f"pstatus graph={graph} rounds={rounds}", flush=True)
+        kw = dict(theta=27.0, mode=0, max_rounds=rounds)
+        r = gpu.refine(P, T, N, S, graph=graph, **kw)
+        Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, **kw)
+        assert r.stats["status"] == st["status"] == (1 if rounds == 6 else 0)
+        assert np.array_equal(r.points.view(np.uint64), Po.view(np.uint64))
+        want = (np.where(pf & 1, 0, 2) | np.where(pf & 2, 4, 0) | np.where(pf & 16, 8, 0))
+        assert np.array_equal(r.pstatus, want.astype(np.uint8))
+        new = (r.pstatus & 8) != 0
+        assert not new[:len(P)].any()
+        if rounds == 6:
+            assert new.sum() > 0
