@@ -101,8 +101,10 @@ def chosen_options(args):
     # 2-half-octave buckets (learned_shift 1 keeps the Steiner count within a
     # few percent of the plain path); the autotuner may switch to the
     # graph-driven loop (graph=1)
+    # launch shapes start from the drop-in's (the autotuner may move them)
     o = dict(pred_mode=1, claim_iters=args.claim_iters, priority=1, graph=0, shared_ring=1,
-             cand_factor=args.cand_factor)
+             cand_factor=args.cand_factor, grid_cavity=4096, grid_claim=2048, grid_apply=1024,
+             cavity_lanes=16, cav_lds_cap=32)
     if args.learned and os.path.exists(MODEL):
         o["priority"] = 2
         o["learned_shift"] = args.learned_shift
@@ -326,9 +328,27 @@ def bench_refine(args):
         sc.set_priority_model(np.load(MODEL))
     tune = None
     if not args.no_autotune:
-        best, trials = sc.autotune(reps=1)
-        tune = {"best": best, "trials": len(trials)}
+        # each setting timed three times (median): boxes vary by ~10%, a
+        # single run per setting picks noise
+        best, trials = sc.autotune(reps=3)
+        tune = {"best": best, "trials": trials}
         log(f"# autotune: {best}")
+        # the graph-driven loop at its own best structure, for the record
+        gbest = None
+        for gr in (1, 2, 4, 8):
+            for gs in (1, 2, 3):
+                o = dict(co, **{k: v for k, v in best.items() if k.startswith("grid_") or
+                                k in ("cavity_lanes", "cav_lds_cap")})
+                o.update(graph=1, graph_rounds=gr, graph_seg_passes=gs)
+                sg = g.Session(P, T, N, SS, **kw, **o)
+                if co["priority"] == 2:
+                    sg.set_priority_model(np.load(MODEL))
+                ts = sorted(sg.run()["device_ms"] for _ in range(3))
+                sg.close()
+                if gbest is None or ts[1] < gbest["device_ms"]:
+                    gbest = {"device_ms": ts[1], "graph_rounds": gr, "graph_seg_passes": gs}
+        tune["graph_loop_best"] = gbest
+        log(f"# graph loop best: {gbest}")
     st_s, ms_s, dev_s = timed_runs(sc, args.steps, args.warmup, "chosen")
     resident["chosen"] = {"ms_per_step": MEDIANS["chosen"]["wall_ms"],
                           "device_ms_per_step": MEDIANS["chosen"]["device_ms"],

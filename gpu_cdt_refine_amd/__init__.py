@@ -30,8 +30,8 @@ RUPPERT, CHEW = 1, 0
 # 3x the previous winners, the learned priority coarsened by one half-octave,
 # and the launch shapes the autotuner picked on the 10^6-point workload.
 CHOSEN_FORM = dict(pred_mode=1, claim_iters=3, shared_ring=1, cand_factor=3, priority=2,
-                   learned_shift=1, grid_cavity=4096, grid_claim=1024, grid_apply=8192,
-                   cavity_lanes=8, cav_lds_cap=16)
+                   learned_shift=1, grid_cavity=4096, grid_claim=2048, grid_apply=1024,
+                   cavity_lanes=16, cav_lds_cap=32)
 
 
 class Options(C.Structure):
@@ -64,6 +64,7 @@ class Options(C.Structure):
         ("shared_ring", C.c_int),
         ("mix_below", C.c_int),
         ("deletion", C.c_int),
+        ("vmm", C.c_int),
     ]
 
 
@@ -123,6 +124,8 @@ class Stats(C.Structure):
         ("giveup_segs_left", C.c_longlong),
         ("deleted", C.c_longlong),
         ("flips", C.c_longlong),
+        ("growths", C.c_longlong),
+        ("grow_in_place", C.c_longlong),
     ]
 
     def as_dict(self):

@@ -311,6 +311,8 @@ int gcdt_batch_run(gcdt_batch *b, gcdt_stats *stats) {
       tot.ws_ints_bad += s.ws_ints_bad;
       tot.deleted += s.deleted;
       tot.flips += s.flips;
+      tot.growths += s.growths;
+      tot.grow_in_place = s.grow_in_place;
       b->run_ms[w] = std::chrono::duration<double, std::milli>(
                          std::chrono::steady_clock::now() - t0).count();
       // the Refiner's buffers are reused by the next wave: fetch this one

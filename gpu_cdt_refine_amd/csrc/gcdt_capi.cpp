@@ -67,6 +67,7 @@ void gcdt_default_options(gcdt_options *o) {
   o->shared_ring = 0;
   o->mix_below = 0;
   o->deletion = 1;
+  o->vmm = 1;
 }
 
 }  // extern "C"
@@ -113,6 +114,7 @@ bool options_ok(const gcdt_options *o, std::string &g_err) {
     return false;
   }
   if (o->deletion < 0 || o->deletion > 2) { g_err = "deletion must be 0, 1 or 2"; return false; }
+  if (o->vmm != 0 && o->vmm != 1) { g_err = "vmm must be 0 or 1"; return false; }
   if (o->graph && o->log_training) { g_err = "log_training needs the host-driven loop (graph = 0)"; return false; }
   if (o->mixed != 0 && o->mixed != 1) { g_err = "mixed must be 0 or 1"; return false; }
   if (o->shared_ring != 0 && o->shared_ring != 1) { g_err = "shared_ring must be 0 or 1"; return false; }
@@ -448,10 +450,10 @@ void chosen_defaults(gcdt_options *o) {
   o->priority = 2;
   o->learned_shift = 1;
   o->grid_cavity = 4096;
-  o->grid_claim = 1024;
-  o->grid_apply = 8192;
-  o->cavity_lanes = 8;
-  o->cav_lds_cap = 16;
+  o->grid_claim = 2048;
+  o->grid_apply = 1024;
+  o->cavity_lanes = 16;
+  o->cav_lds_cap = 32;
 }
 void dropin_configure(DropIn &d) {
   if (d.configured) return;

@@ -11,6 +11,7 @@
 #include "gcdt_device.h"
 #include "gpu_cdt_refine.h"
 #include "host_copy.h"
+#include "vm_range.h"
 
 #include "gcdt_cdt_result.h"
 
@@ -153,6 +154,9 @@ class Refiner {
   void launch_cavity(bool is_seg, int grid, const PassD &ps);
   int select_by_priority(int *ids, int nb, int cap, int **out_ids);
   void ensure_capacity(long long needP, long long needT, long long needS);
+  bool vm_setup();
+  void vm_release_all();
+  void vm_map_mesh(long long cp, long long ct, long long cs);
   void ensure_cand(int nc);
   void ensure_ws(long long ints);
   void ensure_tmp(int n);
@@ -172,6 +176,11 @@ class Refiner {
   int seg_in_cap = 0;
   int2 *seg_in = nullptr;
   long long capP = 0, capT = 0, capS = 0;
+  // growth in place (options.vmm on a device with virtual memory
+  // management): every array a pass can outgrow lives in a reserved range
+  bool use_vm = false, vm_checked = false;
+  VmRange vm_P, vm_pflag, vm_ppar, vm_T, vm_tflag, vm_tbucket, vm_claim, vm_SV, vm_SP, vm_ST,
+      vm_senc, vm_tmp, vm_sel_prio, vm_sel_tmp, vm_sel_list;
   int K = 32;
   int np_round = 0;  // live points when the last round opened
   unsigned long long epoch = 0;

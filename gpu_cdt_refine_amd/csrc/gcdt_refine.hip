@@ -2056,6 +2056,11 @@ static void dgrow(T **p, size_t oldn, size_t newn, hipStream_t s) {
 
 static int nblk(long long n, int b = 256) { return (int)((n + b - 1) / b); }
 
+// Largest mesh the 32-bit encodings hold: triangle ids travel as
+// otri = tri << 2 | edge in an int, point and subsegment ids as ints.
+constexpr long long MAX_TRIANGLES = (1LL << 29) - 1;
+constexpr long long MAX_POINTS = (1LL << 31) - 1;
+
 Refiner::Refiner(const gcdt_options &o) : opt(o) {
   HC(hipSetDevice(opt.device));
   HC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -2110,6 +2115,11 @@ void Refiner::set_angle(double theta_deg, int offcenter, int mode) {
 }
 
 Refiner::~Refiner() {
+  (void)hipStreamSynchronize(stream);
+  if (use_vm) {  // owned by the reserved ranges (released by their destructors)
+    tmp_list = nullptr;
+    sel_prio = sel_tmp = sel_list = nullptr;
+  }
   free_mesh();
   for (auto &in : inputs) {
     dfree(in.P); dfree(in.T); dfree(in.N); dfree(in.S); dfree(in.tinst); dfree(in.ipar);
@@ -2149,10 +2159,82 @@ Refiner::~Refiner() {
 }
 
 void Refiner::free_mesh() {
+  if (use_vm) return;  // the reserved ranges live as long as the refiner
   dfree(m.P); dfree(m.pflag); dfree(m.ppar);
   dfree(m.T); dfree(m.tflag); dfree(m.tbucket); dfree(m.claim);
   dfree(m.SV); dfree(m.SP); dfree(m.ST); dfree(m.senc);
   capP = capT = capS = 0;
+}
+
+// Reserves the ranges of every growable array at its largest size (the
+// 32-bit encodings' limits); false, with nothing held, when the device or the
+// runtime cannot map memory into reserved ranges.
+bool Refiner::vm_setup() {
+  const int dev = opt.device;
+  if (!VmRange::supported(dev)) return false;
+  const size_t mp = (size_t)MAX_POINTS, mt = (size_t)MAX_TRIANGLES;
+  const bool ok =
+      vm_P.reserve(dev, mp * sizeof(double2)) && vm_pflag.reserve(dev, mp) &&
+      vm_ppar.reserve(dev, mp * sizeof(int)) && vm_T.reserve(dev, mt * sizeof(TriRec)) &&
+      vm_tflag.reserve(dev, mt) && vm_tbucket.reserve(dev, mt * sizeof(uint16_t)) &&
+      vm_claim.reserve(dev, 2 * mt * sizeof(unsigned long long)) &&
+      vm_SV.reserve(dev, mp * sizeof(int2)) && vm_SP.reserve(dev, mp * sizeof(int)) &&
+      vm_ST.reserve(dev, mp * sizeof(int)) && vm_senc.reserve(dev, mp * sizeof(int)) &&
+      vm_tmp.reserve(dev, mp * sizeof(int)) && vm_sel_prio.reserve(dev, mt * sizeof(int)) &&
+      vm_sel_tmp.reserve(dev, mt * sizeof(int)) && vm_sel_list.reserve(dev, mt * sizeof(int));
+  if (!ok) {
+    vm_release_all();
+    return false;
+  }
+  m.P = (double2 *)vm_P.ptr();
+  m.pflag = (uint8_t *)vm_pflag.ptr();
+  m.ppar = (int *)vm_ppar.ptr();
+  m.T = (TriRec *)vm_T.ptr();
+  m.tflag = (uint8_t *)vm_tflag.ptr();
+  m.tbucket = (uint16_t *)vm_tbucket.ptr();
+  m.claim = (unsigned long long *)vm_claim.ptr();
+  m.SV = (int2 *)vm_SV.ptr();
+  m.SP = (int *)vm_SP.ptr();
+  m.ST = (int *)vm_ST.ptr();
+  m.senc = (int *)vm_senc.ptr();
+  return true;
+}
+
+void Refiner::vm_release_all() {
+  for (VmRange *r : {&vm_P, &vm_pflag, &vm_ppar, &vm_T, &vm_tflag, &vm_tbucket, &vm_claim, &vm_SV,
+                     &vm_SP, &vm_ST, &vm_senc, &vm_tmp, &vm_sel_prio, &vm_sel_tmp, &vm_sel_list})
+    r->release();
+  m.P = nullptr; m.pflag = nullptr; m.ppar = nullptr; m.T = nullptr; m.tflag = nullptr;
+  m.tbucket = nullptr; m.claim = nullptr; m.SV = nullptr; m.SP = nullptr; m.ST = nullptr;
+  m.senc = nullptr;
+  capP = capT = capS = 0;
+}
+
+// Maps the mesh arrays up to the given capacities (in place; new claim
+// words start at zero).
+void Refiner::vm_map_mesh(long long cp, long long ct, long long cs) {
+  if (cp > capP) {
+    vm_P.map_to((size_t)cp * sizeof(double2));
+    vm_pflag.map_to((size_t)cp);
+    vm_ppar.map_to((size_t)cp * sizeof(int));
+    capP = cp;
+  }
+  if (ct > capT) {
+    vm_T.map_to((size_t)ct * sizeof(TriRec));
+    vm_tflag.map_to((size_t)ct);
+    vm_tbucket.map_to((size_t)ct * sizeof(uint16_t));
+    vm_claim.map_to((size_t)ct * 2 * sizeof(unsigned long long));
+    HC(hipMemsetAsync(m.claim + 2 * (size_t)capT, 0,
+                      sizeof(unsigned long long) * 2 * (size_t)(ct - capT), stream));
+    capT = ct;
+  }
+  if (cs > capS) {
+    vm_SV.map_to((size_t)cs * sizeof(int2));
+    vm_SP.map_to((size_t)cs * sizeof(int));
+    vm_ST.map_to((size_t)cs * sizeof(int));
+    vm_senc.map_to((size_t)cs * sizeof(int));
+    capS = cs;
+  }
 }
 
 void Refiner::upload(const double *points, int np, const int *tris,
@@ -2264,15 +2346,18 @@ void Refiner::select_input(int k) {
   in_any_chew = in.any_chew;
 }
 
-// Largest mesh the 32-bit encodings hold: triangle ids travel as
-// otri = tri << 2 | edge in an int, point and subsegment ids as ints.
-constexpr long long MAX_TRIANGLES = (1LL << 29) - 1;
-constexpr long long MAX_POINTS = (1LL << 31) - 1;
 
 void Refiner::ensure_capacity(long long needP, long long needT, long long needS) {
   if (needT > MAX_TRIANGLES || needP > MAX_POINTS || needS > MAX_POINTS)
     throw std::runtime_error("the refined mesh would exceed 2^29 triangles / 2^31 points "
                              "(32-bit encodings); the refinement does not converge on this input");
+  if (needP > capP || needT > capT || needS > capS) st.growths++;
+  if (use_vm) {  // in place: map more, nothing moves
+    vm_map_mesh(needP > capP ? std::min(std::max<long long>(needP, capP + capP / 2), MAX_POINTS) : capP,
+                needT > capT ? std::min(std::max<long long>(needT, capT + capT / 2), MAX_TRIANGLES) : capT,
+                needS > capS ? std::min(std::max<long long>(needS, capS + capS / 2), MAX_POINTS) : capS);
+    return;
+  }
   if (needP > capP) {
     long long c = std::min(std::max<long long>(needP, capP + capP / 2), MAX_POINTS);
     dgrow(&m.P, (size_t)np, (size_t)c, stream);
@@ -2717,7 +2802,26 @@ void Refiner::run() {
   long long wantT = std::min(std::max<long long>(16, 6LL * nt0), MAX_TRIANGLES);
   long long wantS = std::min(std::max<long long>(16, 8LL * ns0 + 16), MAX_POINTS);
   const bool want_inst = in_tinst != nullptr;
-  if (capP < wantP || capT < wantT || capS < wantS) {
+  if (!vm_checked) {
+    // in-place growth when the device maps memory into reserved ranges; a
+    // runtime that reserves but refuses the first mapping (some tools
+    // intercepting the calls do) leaves the reallocating arrays in charge
+    vm_checked = true;
+    use_vm = false;
+    if (opt.vmm && vm_setup()) {
+      try {
+        vm_map_mesh(wantP, wantT, wantS);
+        use_vm = true;
+      } catch (const std::exception &) {
+        HC(hipStreamSynchronize(stream));
+        vm_release_all();
+      }
+    }
+  }
+  st.grow_in_place = use_vm ? 1 : 0;
+  if (use_vm) {
+    vm_map_mesh(std::max(capP, wantP), std::max(capT, wantT), std::max(capS, wantS));
+  } else if (capP < wantP || capT < wantT || capS < wantS) {
     free_mesh();
     capP = wantP; capT = wantT; capS = wantS;
     dalloc(&m.P, capP); dalloc(&m.pflag, capP); dalloc(&m.ppar, capP);
@@ -2955,11 +3059,20 @@ void Refiner::size_pass_buffers() {
   ensure_cand(cands_per_pass(K));
   ensure_tmp((int)std::max(capT, capS));
   if (capT > sel_cap) {
-    dfree(sel_prio); dfree(sel_tmp); dfree(sel_list);
     sel_cap = (int)capT;
-    dalloc(&sel_prio, (size_t)sel_cap);
-    dalloc(&sel_tmp, (size_t)sel_cap);
-    dalloc(&sel_list, (size_t)sel_cap);
+    if (use_vm) {
+      vm_sel_prio.map_to((size_t)sel_cap * sizeof(int));
+      vm_sel_tmp.map_to((size_t)sel_cap * sizeof(int));
+      vm_sel_list.map_to((size_t)sel_cap * sizeof(int));
+      sel_prio = (int *)vm_sel_prio.ptr();
+      sel_tmp = (int *)vm_sel_tmp.ptr();
+      sel_list = (int *)vm_sel_list.ptr();
+    } else {
+      dfree(sel_prio); dfree(sel_tmp); dfree(sel_list);
+      dalloc(&sel_prio, (size_t)sel_cap);
+      dalloc(&sel_tmp, (size_t)sel_cap);
+      dalloc(&sel_list, (size_t)sel_cap);
+    }
   }
   if (!d_hist) dalloc(&d_hist, NBUCKET);
   if (opt.priority == 2 && feat_cap < c_cap) {
@@ -3176,7 +3289,12 @@ void Refiner::loop_graph() {
       h.i[C_CAPT] = (int)std::min<long long>(capT, INT32_MAX);
       h.i[C_CAPS] = (int)std::min<long long>(capS, INT32_MAX);
       h.i[C_SUSPEND] = 0;
-      recapture = true;
+      if (use_vm) {  // grown in place: the captured graph's arguments still hold
+        size_pass_buffers();
+        HC(hipMemcpyAsync(d_ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, stream));
+      } else {
+        recapture = true;
+      }
     }
     if (h.i[C_K] != g_K) {  // cavity capacity doubled
       K = h.i[C_K];
@@ -3238,9 +3356,15 @@ Refiner::DeviceMesh Refiner::device_mesh() const {
 
 void Refiner::ensure_tmp(int n) {
   if (n <= tmp_list_cap) return;
-  dfree(tmp_list);
-  tmp_list_cap = std::max(n, 2 * tmp_list_cap);
-  dalloc(&tmp_list, (size_t)tmp_list_cap);
+  const int c = (int)std::min<long long>(std::max<long long>(n, 2LL * tmp_list_cap), MAX_POINTS);
+  if (use_vm) {  // in place: the graph loop's captured lists keep their address
+    vm_tmp.map_to((size_t)c * sizeof(int));
+    tmp_list = (int *)vm_tmp.ptr();
+  } else {
+    dfree(tmp_list);
+    dalloc(&tmp_list, (size_t)c);
+  }
+  tmp_list_cap = c;
 }
 
 void Refiner::download(MeshHost &out) {

@@ -186,6 +186,11 @@ typedef struct gcdt_options {
                             1), Triangle's rule (triangle.cpp:13319-13357,
                             default); 2 in every mode, the reference's
                             (refine.cu:604-677) */
+  int vmm;               /* 1: arrays a pass can outgrow live in reserved
+                            address ranges and grow in place by mapping more
+                            memory (no copy; the graph loop keeps its captured
+                            graph) when the device supports it (default);
+                            0: they are reallocated and copied */
 } gcdt_options;
 
 typedef struct gcdt_stats {
@@ -220,6 +225,8 @@ typedef struct gcdt_stats {
                             on a subsegment with triangles on both sides), per
                             deleted vertex its degree less three (the drop-in
                             returns it as numberofflips, triangle.h:283) */
+  long long growths;     /* times the mesh arrays grew during the run */
+  long long grow_in_place; /* 1: they grew in place (options.vmm) */
 } gcdt_stats;
 
 typedef struct gcdt_mesh gcdt_mesh;       /* host-side result */

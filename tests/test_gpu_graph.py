@@ -114,3 +114,23 @@ def test_autotune_keeps_the_mesh(gpu, oracle):
     finally:
         s.close()
     _same(a, b)
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_mesh_growth_in_place_equals_copy(gpu, oracle, graph):
+    """A small input whose refined mesh outgrows the initial capacity (6x
+    the input) several times: with options.vmm the arrays grow in place
+    (reserved ranges, more memory mapped; the graph loop keeps its graph),
+    without it they are reallocated and copied -- the same mesh either way,
+    equal to the oracle's."""
+    print(f"growth graph={graph}", flush=True)
+    P, T, N, S = _input(oracle, 300, 60, 58, maxlen=1000.0)
+    v = gpu.refine(P, T, N, S, theta=30.0, graph=graph, vmm=1)
+    c = gpu.refine(P, T, N, S, theta=30.0, graph=graph, vmm=0)
+    assert v.stats["growths"] > 1 and c.stats["growths"] == v.stats["growths"]
+    assert v.stats["grow_in_place"] == 1 and c.stats["grow_in_place"] == 0
+    _same(v, c)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, theta=30.0)
+    assert np.array_equal(v.points.view(np.uint64), Po.view(np.uint64))
+    assert np.array_equal(v.triangles, To)
+    assert np.array_equal(v.neighbors, No)
