@@ -217,6 +217,8 @@ def lib():
         L.gcdt_mlp_train.restype = C.c_int
         L.gcdt_mfma_probe.argtypes = [vp, vp, vp]
         L.gcdt_mfma_probe.restype = C.c_int
+        L.gcdt_mfma_probe32.argtypes = [vp, vp, vp]
+        L.gcdt_mfma_probe32.restype = C.c_int
         L.gcdt_mlp_last_error.restype = C.c_char_p
         L.gcdt_batch_create.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                         C.POINTER(Options), C.c_longlong, C.POINTER(vp)]
@@ -875,4 +877,15 @@ def mfma_probe(A, B):
     rc = lib().gcdt_mfma_probe(_ptr(A), _ptr(B), _ptr(D))
     if rc:
         _mlp_err(rc, "gcdt_mfma_probe")
+    return D
+
+
+def mfma_probe32(A, B):
+    """D = A (16x32) @ B (32x16) through the 16x16x32 BF16 MFMA."""
+    A = np.ascontiguousarray(A, np.float32).reshape(16, 32)
+    B = np.ascontiguousarray(B, np.float32).reshape(32, 16)
+    D = np.empty((16, 16), np.float32)
+    rc = lib().gcdt_mfma_probe32(_ptr(A), _ptr(B), _ptr(D))
+    if rc:
+        _mlp_err(rc, "gcdt_mfma_probe32")
     return D

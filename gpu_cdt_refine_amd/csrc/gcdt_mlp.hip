@@ -50,6 +50,18 @@ __device__ __forceinline__ floatx4 mfma(bf16x4 a, bf16x4 b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// gfx950's double-K BF16 MFMA (v_mfma_f32_16x16x32_bf16), used by the
+// inference kernel's 64-wide hidden layer: two instructions per 16x16 output
+// block instead of four.  Operand layout (checked by k_mfma_probe32 against a
+// host product): A[16][32]: lane l holds row l%16, k = 8*(l/16) + 0..7;
+// B[32][16]: lane l holds col l%16, k = 8*(l/16) + 0..7; D as for 16x16x16.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short shortx8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ floatx4 mfma32(shortx8 a, shortx8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
 // One wave computes the forward pass of 16 samples starting at row0.
 // Outputs D-layout activations (pre-ReLU kept implicitly by value > 0).
 struct Fwd {
@@ -125,6 +137,87 @@ __global__ __launch_bounds__(64) void k_mlp_forward(const float *X, int n,
         if (row < n) y[row] = f.y[j];
       }
     __syncthreads();
+  }
+}
+
+// Inference: the forward pass above with the weights converted to BF16 once
+// per block and kept in LDS, four waves per block, each wave taking 16 rows
+// at a time; layer 2 on the 16x16x32 MFMA.  Same roundings as the training
+// forward (inputs, weights and hidden activations in BF16, FP32 accumulate);
+// only layer 2's accumulation order differs.
+constexpr int INF_WAVES = 4;
+// LDS written by some lanes of a wave and read by others: release, the
+// wave's convergence point, acquire (wavefront scope).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+constexpr int W2T_STRIDE = MLP_H + 8;  // transposed W2 rows, padded against bank conflicts
+constexpr int H1_STRIDE = MLP_H + 8;
+__global__ __launch_bounds__(64 * INF_WAVES) void k_mlp_infer(const float *X, int n,
+                                                              const float *P, float *y,
+                                                              const int *dn, const int *dgo) {
+  __shared__ short w1[MLP_IN * MLP_H];                 // [k][c]
+  __shared__ __align__(16) short w2t[MLP_H * W2T_STRIDE];  // [c][k]
+  __shared__ float b1[MLP_H], b2[MLP_H], w3[MLP_H];
+  __shared__ __align__(16) short h1[INF_WAVES][16 * H1_STRIDE];  // per wave: [row][k]
+  if (dn) n = *dgo ? *dn : 0;
+  if (n <= 0) return;
+  for (int i = threadIdx.x; i < MLP_IN * MLP_H; i += blockDim.x) w1[i] = f2bf(P[OFF_W1 + i]);
+  for (int i = threadIdx.x; i < MLP_H * MLP_H; i += blockDim.x) {
+    const int k = i / MLP_H, c = i % MLP_H;
+    w2t[c * W2T_STRIDE + k] = f2bf(P[OFF_W2 + i]);
+  }
+  for (int i = threadIdx.x; i < MLP_H; i += blockDim.x) {
+    b1[i] = P[OFF_B1 + i];
+    b2[i] = P[OFF_B2 + i];
+    w3[i] = bf2f(f2bf(P[OFF_W3 + i]));
+  }
+  __syncthreads();
+  const float b3 = P[OFF_B3];
+  const int wv = threadIdx.x / 64, l = threadIdx.x & 63, r = l % 16, kb = l / 16;
+  short *H = h1[wv];
+  for (int row0 = (blockIdx.x * INF_WAVES + wv) * 16; row0 < n;
+       row0 += gridDim.x * INF_WAVES * 16) {
+    // layer 1 (K = 16): A = X[row0 + r][4kb..4kb+3]
+    bf16x4 a;
+    const int row = row0 + r;
+    for (int j = 0; j < 4; j++)
+      a[j] = row < n ? f2bf(X[(size_t)row * MLP_IN + 4 * kb + j]) : (short)0;
+    for (int cb = 0; cb < 4; cb++) {
+      bf16x4 b;
+      for (int j = 0; j < 4; j++) b[j] = w1[(4 * kb + j) * MLP_H + cb * 16 + r];
+      floatx4 z = {0.f, 0.f, 0.f, 0.f};
+      z = mfma(a, b, z);
+      const float bias = b1[cb * 16 + r];
+      // D layout: column cb*16 + r, rows 4kb + j
+      for (int j = 0; j < 4; j++) H[(4 * kb + j) * H1_STRIDE + cb * 16 + r] = f2bf(fmaxf(z[j] + bias, 0.f));
+    }
+    wave_sync();  // H written by the wave's lanes, read by others
+    // layer 2 (K = 64 = 2 x 32): A row r, k = kk*32 + 8kb + 0..7 (contiguous in H)
+    floatx4 h2[4];
+    for (int cb = 0; cb < 4; cb++) {
+      floatx4 z = {0.f, 0.f, 0.f, 0.f};
+      for (int kk = 0; kk < 2; kk++) {
+        const shortx8 a2 = *reinterpret_cast<const shortx8 *>(&H[r * H1_STRIDE + kk * 32 + 8 * kb]);
+        const shortx8 b2v =
+            *reinterpret_cast<const shortx8 *>(&w2t[(cb * 16 + r) * W2T_STRIDE + kk * 32 + 8 * kb]);
+        z = mfma32(a2, b2v, z);
+      }
+      const float bias = b2[cb * 16 + r];
+      for (int j = 0; j < 4; j++) z[j] = fmaxf(z[j] + bias, 0.f);
+      h2[cb] = z;
+    }
+    // layer 3: y[row] = sum_c H2[row][c] W3[c] + b3 over the 16 lanes of a group
+    for (int j = 0; j < 4; j++) {
+      float s = 0.f;
+      for (int cb = 0; cb < 4; cb++) s += h2[cb][j] * w3[cb * 16 + r];
+      for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 16);
+      const int ro = row0 + 4 * kb + j;
+      if (r == 0 && ro < n) y[ro] = s + b3;
+    }
+    wave_sync();  // H is rewritten by the next rows
   }
 }
 
@@ -254,22 +347,40 @@ __global__ void k_mfma_probe(const float *A, const float *B, float *D) {
   for (int j = 0; j < 4; j++) D[(4 * kb + j) * 16 + r] = z[j];
 }
 
+// Device layout probe of the 16x16x32 MFMA: D = A(16x32) B(32x16).
+__global__ void k_mfma_probe32(const float *A, const float *B, float *D) {
+  int l = threadIdx.x, r = l % 16, kb = l / 16;
+  shortx8 a, b;
+  for (int j = 0; j < 8; j++) {
+    a[j] = f2bf(A[r * 32 + 8 * kb + j]);
+    b[j] = f2bf(B[(8 * kb + j) * 16 + r]);
+  }
+  floatx4 z = {0.f, 0.f, 0.f, 0.f};
+  z = mfma32(a, b, z);
+  for (int j = 0; j < 4; j++) D[(4 * kb + j) * 16 + r] = z[j];
+}
+
 static void hc(hipError_t e, const char *w) {
   if (e != hipSuccess) throw std::runtime_error(std::string(w) + ": " + hipGetErrorString(e));
 }
 
 // Inference inside the refinement: scores for n candidates on `stream`.
+// Blocks of the inference kernel for n rows: enough to fill the chip, each
+// block converting the weights once.
+static int infer_blocks(int n) {
+  return std::max(1, std::min((n + 64 * 8 - 1) / (64 * 8), 1024));
+}
+
 void mlp_forward_device(const float *dX, int n, const float *dP, float *dy,
                         hipStream_t stream) {
-  int blocks = std::min((n + 15) / 16, 8192);
-  if (blocks > 0) k_mlp_forward<<<blocks, 64, 0, stream>>>(dX, n, dP, dy, nullptr, nullptr);
+  if (n > 0) k_mlp_infer<<<infer_blocks(n), 64 * INF_WAVES, 0, stream>>>(dX, n, dP, dy, nullptr, nullptr);
   hc(hipGetLastError(), "mlp forward launch");
 }
 
 void mlp_forward_device_n(const float *dX, const int *dn, const int *dgo,
                           const float *dP, float *dy, int blocks,
                           hipStream_t stream) {
-  k_mlp_forward<<<blocks, 64, 0, stream>>>(dX, 0, dP, dy, dn, dgo);
+  k_mlp_infer<<<std::min(blocks, 1024), 64 * INF_WAVES, 0, stream>>>(dX, 0, dP, dy, dn, dgo);
   hc(hipGetLastError(), "mlp forward launch");
 }
 
@@ -292,8 +403,7 @@ int gcdt_mlp_forward(const float *X, int n, const float *params, float *y) {
     hc(hipMalloc(&dy, sizeof(float) * (size_t)std::max(n, 1)), "malloc");
     hc(hipMemcpy(dX, X, sizeof(float) * (size_t)n * MLP_IN, hipMemcpyHostToDevice), "copy");
     hc(hipMemcpy(dP, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice), "copy");
-    int blocks = std::min((n + 15) / 16, 4096);
-    if (blocks > 0) k_mlp_forward<<<blocks, 64>>>(dX, n, dP, dy, nullptr, nullptr);
+    if (n > 0) k_mlp_infer<<<infer_blocks(n), 64 * INF_WAVES>>>(dX, n, dP, dy, nullptr, nullptr);
     hc(hipGetLastError(), "launch");
     hc(hipDeviceSynchronize(), "sync");
     hc(hipMemcpy(y, dy, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost), "copy");
@@ -402,6 +512,27 @@ int gcdt_mlp_train(const float *X, const float *t, int n, int epochs, int batch,
 }
 
 // D = A B for 16x16 FP32 matrices rounded to BF16 (MFMA layout probe).
+// D(16x16) = A(16x32) B(32x16) through the 16x16x32 MFMA (layout check).
+int gcdt_mfma_probe32(const float *A, const float *B, float *D) {
+  try {
+    float *dA, *dB, *dD;
+    hc(hipMalloc(&dA, 2048), "malloc");
+    hc(hipMalloc(&dB, 2048), "malloc");
+    hc(hipMalloc(&dD, 1024), "malloc");
+    hc(hipMemcpy(dA, A, 2048, hipMemcpyHostToDevice), "copy");
+    hc(hipMemcpy(dB, B, 2048, hipMemcpyHostToDevice), "copy");
+    k_mfma_probe32<<<1, 64>>>(dA, dB, dD);
+    hc(hipGetLastError(), "launch");
+    hc(hipDeviceSynchronize(), "sync");
+    hc(hipMemcpy(D, dD, 1024, hipMemcpyDeviceToHost), "copy");
+    (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dD);
+    return 0;
+  } catch (const std::exception &e) {
+    g_mlp_err = e.what();
+    return -2;
+  }
+}
+
 int gcdt_mfma_probe(const float *A, const float *B, float *D) {
   try {
     float *dA, *dB, *dD;

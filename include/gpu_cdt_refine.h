@@ -407,6 +407,9 @@ int gcdt_mlp_train(const float *X, const float *t, int n, int epochs, int batch,
                    float lr, unsigned long long seed, float *params,
                    float *losses);
 int gcdt_mfma_probe(const float *A, const float *B, float *D);
+/* D(16x16) = A(16x32) B(32x16) through gfx950's 16x16x32 BF16 MFMA (the
+ * inference kernel's operand layout, checked against a host product). */
+int gcdt_mfma_probe32(const float *A, const float *B, float *D);
 const char *gcdt_mlp_last_error(void);
 
 /* Last error message of this thread ("" if none). */

@@ -18,6 +18,18 @@ def test_mfma_layout(gpu):
     assert np.allclose(D, ref, rtol=1e-5, atol=1e-5)
 
 
+def test_mfma32_layout(gpu):
+    """gfx950's 16x16x32 BF16 MFMA with the operand layout the inference
+    kernel uses: equal to the product of the BF16-rounded operands."""
+    from oracle.mlp_oracle import bf16
+    rng = np.random.default_rng(2)
+    A = rng.normal(size=(16, 32)).astype(np.float32)
+    B = rng.normal(size=(32, 16)).astype(np.float32)
+    D = gpu.mfma_probe32(A, B)
+    ref = bf16(A).astype(np.float64) @ bf16(B).astype(np.float64)
+    assert np.allclose(D, ref, rtol=1e-5, atol=1e-5)
+
+
 def test_mlp_forward_and_grad_vs_oracle(gpu):
     from oracle import mlp_oracle as mo
     rng = np.random.default_rng(1)
