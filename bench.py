@@ -102,7 +102,8 @@ def chosen_options(args):
     # few percent of the plain path); the autotuner may switch to the
     # graph-driven loop (graph=1)
     # launch shapes start from the drop-in's (the autotuner may move them)
-    o = dict(pred_mode=1, claim_iters=args.claim_iters, priority=1, graph=0, shared_ring=1,
+    o = dict(pred_mode=1, claim_iters=args.claim_iters, seg_claim_iters=2, priority=1, graph=0,
+             shared_ring=1,
              cand_factor=args.cand_factor, grid_cavity=4096, grid_claim=2048, grid_apply=1024,
              cavity_lanes=16, cav_lds_cap=32)
     if args.learned and os.path.exists(MODEL):

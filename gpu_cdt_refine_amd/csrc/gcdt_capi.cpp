@@ -67,6 +67,7 @@ void gcdt_default_options(gcdt_options *o) {
   o->shared_ring = 0;
   o->mix_below = 0;
   o->deletion = 1;
+  o->seg_claim_iters = 0;
   o->vmm = 1;
 }
 
@@ -83,6 +84,7 @@ bool options_ok(const gcdt_options *o, std::string &g_err) {
   if (o->cand_budget < 6LL * o->cav_cap_max + 16) { g_err = "cand_budget smaller than one candidate at cav_cap_max"; return false; }
   if (o->priority < 0 || o->priority > 2) { g_err = "priority must be 0, 1 or 2"; return false; }
   if (o->claim_iters < 1 || o->claim_iters > 64) { g_err = "claim_iters must be in [1, 64]"; return false; }
+  if (o->seg_claim_iters < 0 || o->seg_claim_iters > 64) { g_err = "seg_claim_iters must be in [0, 64]"; return false; }
   if (o->graph_seg_passes < 1 || o->graph_seg_passes > 16 || o->graph_rounds < 1 || o->graph_rounds > 64) {
     g_err = "graph_seg_passes must be in [1, 16] and graph_rounds in [1, 64]";
     return false;
@@ -107,7 +109,8 @@ bool options_ok(const gcdt_options *o, std::string &g_err) {
   // The graph loop resets the 11-bit claim epoch only between graph
   // launches; one launch uses claim_iters + 2 epochs per pass (claims, the
   // winners' stamp, the vertex deletion step).
-  if (o->graph && (long long)o->graph_rounds * (o->graph_seg_passes + 1) * (o->claim_iters + 2) + 2 >
+  if (o->graph && (long long)o->graph_rounds * (o->graph_seg_passes + 1) *
+                          (std::max(o->claim_iters, o->seg_claim_iters) + 2) + 2 >
                       2047) {
     g_err = "graph_rounds * (graph_seg_passes + 1) * (claim_iters + 2) + 2 must not exceed 2047 "
             "(claim epochs of one graph launch)";
@@ -445,6 +448,7 @@ void chosen_defaults(gcdt_options *o) {
   gcdt_default_options(o);
   o->pred_mode = 1;
   o->claim_iters = 3;
+  o->seg_claim_iters = 2;
   o->shared_ring = 1;
   o->cand_factor = 3;
   o->priority = 2;

@@ -55,6 +55,8 @@ enum KernelId {
   K_APPLY,
   K_EXPORT,
   K_MLP,
+  K_DEL_SCAN,
+  K_DEL_APPLY,
   K_NKERN
 };
 
@@ -139,7 +141,8 @@ class Refiner {
   void enqueue_deletion(const DelD &dd);
   void compact_points();
   PassD pass_base(int Kc) const;
-  void enqueue_claims(PassD ps, int gc);
+  void enqueue_claims(PassD ps, int gc, int iters);
+  int iters_for(bool seg_only) const;
   void enqueue_seg_pass();
   void enqueue_bad_pass();
   void capture_graph();
@@ -233,9 +236,9 @@ class Refiner {
   int g_K = 0;
   bool capturing = false;
   // vertex deletion: on for this run; per-nomination workspace, vertex,
-  // star size, win flag
+  // star size
   bool del_on = false, in_any_chew = false;
-  int *del_ws = nullptr, *del_dv = nullptr, *del_dk = nullptr, *del_win = nullptr;
+  int *del_ws = nullptr, *del_dv = nullptr, *del_dk = nullptr;
   // export with deleted vertices: new point ids and compacted copies
   int *p_new = nullptr;
   double2 *p_tmp = nullptr;

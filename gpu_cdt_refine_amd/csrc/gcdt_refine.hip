@@ -1292,7 +1292,7 @@ struct DelD {
   Ctl *ctl;
   unsigned long long epoch;  // host loop: the claim epoch
   int policy;                // 1: Chew-mode instances only, 2: every instance
-  int *ws, *dv, *dk, *dwin;
+  int *ws, *dv, *dk;
   double *pws;               // exact-predicate arena, one slot per lane
   unsigned long long *cnt;
   int use_fp32;
@@ -1308,36 +1308,51 @@ __device__ __forceinline__ unsigned long long del_epoch(const DelD &d) {
 
 // Apex of triangle t (corner j) on subsegment s: a deletion candidate?
 // Writes the star, link, outer otri, subsegments and the cut triangles to W
-// and the star size to *kout; returns the vertex or -1.
+// and the star size to *kout; returns the vertex or -1.  The link's indices,
+// coordinates (taken from the star's records) and the ear bookkeeping live
+// in this lane's column of the block's LDS arrays (lane-interleaved: entry i
+// of lane l at i * 64 + l).
+constexpr int DEL_LANES = 64;
 __device__ int del_candidate(const MeshD &m, int s, int t, int j, int *W, int *kout,
-                             double *pws, bool fp32) {
+                             double *pws, bool fp32, int *lp, double2 *lc,
+                             unsigned char *lrem) {
+  const int ln = threadIdx.x;
   int *star = W, *p = W + DEL_MAX, *o = W + 2 * DEL_MAX, *sg = W + 3 * DEL_MAX;
   int *nt = W + 4 * DEL_MAX;
-  const int v = comp(m.T[t].V, j);
+  const TriRec &r0 = m.T[t];
+  const int v = comp(r0.V, j);
   if ((m.pflag[v] & (PF_FREE | PF_DEAD)) != PF_FREE) return -1;
   const int2 sv = m.SV[s];
-  const double2 O = m.P[sv.x], D = m.P[sv.y], V = m.P[v];
+  const double2 O = m.P[sv.x], D = m.P[sv.y], V = r0.X[j];
   if (!((O.x - V.x) * (D.x - V.x) + (O.y - V.y) * (D.y - V.y) < 0.0)) return -1;
   int cur = t, jc = j, k = 0;
   for (;;) {
     if (k == DEL_MAX) return -1;
     const TriRec &r = m.T[cur];
-    const int x = comp(r.V, (jc + 1) % 3), y = comp(r.V, (jc + 2) % 3);
-    if (k == 0) p[0] = x;
-    else if (p[k] != x) return -1;
+    const int4 rv = r.V, rn = r.N, rs = r.S;
+    const int x = comp(rv, (jc + 1) % 3), y = comp(rv, (jc + 2) % 3);
+    if (k == 0) {
+      lp[ln] = x;
+      lc[ln] = r.X[(jc + 1) % 3];
+    } else if (lp[k * DEL_LANES + ln] != x) {
+      return -1;
+    }
     star[k] = cur;
-    o[k] = comp(r.N, jc);
-    sg[k] = comp(r.S, jc);
-    if (comp(r.S, (jc + 1) % 3) >= 0) return -1;  // edge (v, y) is a subsegment
-    const int nb = comp(r.N, (jc + 1) % 3);
+    o[k] = comp(rn, jc);
+    sg[k] = comp(rs, jc);
+    if (comp(rs, (jc + 1) % 3) >= 0) return -1;  // edge (v, y) is a subsegment
+    const int nb = comp(rn, (jc + 1) % 3);
     if (nb < 0) return -1;
     k++;
     const int nx = nb >> 2;
     if (nx == t) {
-      if (y != p[0]) return -1;
+      if (y != lp[ln]) return -1;
       break;
     }
-    if (k < DEL_MAX) p[k] = y;
+    if (k < DEL_MAX) {
+      lp[k * DEL_LANES + ln] = y;
+      lc[k * DEL_LANES + ln] = r.X[(jc + 2) % 3];
+    }
     const int4 vn = m.T[nx].V;
     const int jn = vn.x == v ? 0 : (vn.y == v ? 1 : (vn.z == v ? 2 : -1));
     if (jn < 0) return -1;
@@ -1345,144 +1360,164 @@ __device__ int del_candidate(const MeshD &m, int s, int t, int j, int *W, int *k
     jc = jn;
   }
   if (k < 3) return -1;
-  int rem[DEL_MAX];
+  for (int q = 0; q < k; q++) p[q] = lp[q * DEL_LANES + ln];
+  auto REM = [&](int q) -> int { return lrem[q * DEL_LANES + ln]; };
+  auto PT = [&](int q) -> double2 { return lc[REM(q) * DEL_LANES + ln]; };
+  auto ID = [&](int q) -> int { return lp[REM(q) * DEL_LANES + ln]; };
   int r = k, ne = 0;
-  for (int q = 0; q < k; q++) rem[q] = q;
+  for (int q = 0; q < k; q++) lrem[q * DEL_LANES + ln] = (unsigned char)q;
   while (r > 3) {
     int found = -1;
     for (int q = 0; q < r && found < 0; q++) {
       const int qa = (q + r - 1) % r, qc = (q + 1) % r;
-      const double2 A = m.P[p[rem[qa]]], B = m.P[p[rem[q]]], C = m.P[p[rem[qc]]];
+      const double2 A = PT(qa), B = PT(q), C = PT(qc);
       if (orient2d_sign(A.x, A.y, B.x, B.y, C.x, C.y, fp32, nullptr) <= 0) continue;
       bool ok = true;
       for (int w = 0; w < r && ok; w++) {
         if (w == qa || w == q || w == qc) continue;
-        const double2 E = m.P[p[rem[w]]];
+        const double2 E = PT(w);
         if (incircle_sign(A.x, A.y, B.x, B.y, C.x, C.y, E.x, E.y, pws, fp32, nullptr) > 0) ok = false;
       }
       if (ok) found = q;
     }
     if (found < 0) return -1;
-    nt[3 * ne] = p[rem[(found + r - 1) % r]];
-    nt[3 * ne + 1] = p[rem[found]];
-    nt[3 * ne + 2] = p[rem[(found + 1) % r]];
+    nt[3 * ne] = ID((found + r - 1) % r);
+    nt[3 * ne + 1] = ID(found);
+    nt[3 * ne + 2] = ID((found + 1) % r);
     ne++;
-    for (int q = found; q + 1 < r; q++) rem[q] = rem[q + 1];
+    for (int q = found; q + 1 < r; q++) lrem[q * DEL_LANES + ln] = lrem[(q + 1) * DEL_LANES + ln];
     r--;
   }
   {
-    const double2 A = m.P[p[rem[0]]], B = m.P[p[rem[1]]], C = m.P[p[rem[2]]];
+    const double2 A = PT(0), B = PT(1), C = PT(2);
     if (orient2d_sign(A.x, A.y, B.x, B.y, C.x, C.y, fp32, nullptr) <= 0) return -1;
   }
-  nt[3 * ne] = p[rem[0]];
-  nt[3 * ne + 1] = p[rem[1]];
-  nt[3 * ne + 2] = p[rem[2]];
+  nt[3 * ne] = ID(0);
+  nt[3 * ne + 1] = ID(1);
+  nt[3 * ne + 2] = ID(2);
   *kout = k;
   return v;
 }
 
-// Nominations, stars and retriangulations.
-__global__ __launch_bounds__(64) void k_del_scan(MeshD m, ParamsD pr, DelD dd) {
-  const int n = del_nseg(dd);
+// Nominations, stars and retriangulations, one lane per nomination (both
+// sides of a subsegment in parallel); a candidate then claims its star and
+// the outer triangles across its link (claims commute, so no other lane's
+// star needs to be known).
+__global__ __launch_bounds__(DEL_LANES) void k_del_scan(MeshD m, ParamsD pr, DelD dd) {
+  __shared__ int s_lp[DEL_MAX * DEL_LANES];
+  __shared__ double2 s_lc[DEL_MAX * DEL_LANES];
+  __shared__ unsigned char s_rem[DEL_MAX * DEL_LANES];
+  const int nd = 2 * del_nseg(dd);
+  const unsigned long long ep = del_epoch(dd);
   const int lane = blockIdx.x * blockDim.x + threadIdx.x;
   double *pws = dd.pws + (size_t)lane * GCDT_INCIRCLE_WS;
-  for (int i = lane; i < n; i += gridDim.x * blockDim.x) {
-    const int s = dd.ids[i];
+  for (int d = lane; d < nd; d += gridDim.x * blockDim.x) {
+    const int s = dd.ids[d >> 1], side = d & 1;
     int t1, j1, t2, j2;
     seg_sides(m, s, t1, j1, t2, j2);
-    for (int side = 0; side < 2; side++) {
-      const int d = 2 * i + side, t = side ? t2 : t1, j = side ? j2 : j1;
-      int v = -1;
-      if (t >= 0) {
-        const ParamsD lp = params_at(pr, m, t);
-        if (dd.policy == 2 || (dd.policy == 1 && !lp.ruppert))
-          v = del_candidate(m, s, t, j, dd.ws + (size_t)d * DEL_WS, dd.dk + d, pws,
-                            dd.use_fp32 != 0);
+    const int t = side ? t2 : t1, j = side ? j2 : j1;
+    int v = -1;
+    int *W = dd.ws + (size_t)d * DEL_WS;
+    if (t >= 0) {
+      const ParamsD lp = params_at(pr, m, t);
+      if (dd.policy == 2 || (dd.policy == 1 && !lp.ruppert))
+        v = del_candidate(m, s, t, j, W, dd.dk + d, pws, dd.use_fp32 != 0, s_lp, s_lc, s_rem);
+    }
+    dd.dv[d] = v;
+    if (v >= 0) {
+      const int k = dd.dk[d];
+      const unsigned long long key = claim_key(ep, 0, d);
+      for (int q = 0; q < k; q++) {
+        atomicMax(&m.claim[2 * (size_t)W[q]], key);
+        const int o = W[2 * DEL_MAX + q];
+        if (o >= 0) atomicMax(&m.claim[2 * (size_t)(o >> 2)], key);
       }
-      dd.dv[d] = v;
     }
   }
 }
 
-__global__ void k_del_claim(MeshD m, DelD dd) {
+// A candidate holding every triangle of its star and ring deletes its
+// vertex right away: winners' regions are disjoint and the other lanes' win
+// tests read only claim words, which no apply writes.  The cut triangles
+// take the star's slots in ascending order (the two largest slots die);
+// links inside the star are matched edge by edge, link edges take the outer
+// triangle (whose back-link is repaired) and the subsegment (whose adjacent
+// otri is repaired).
+__global__ __launch_bounds__(DEL_LANES) void k_del_apply(MeshD m, DelD dd) {
+  // this lane's link, outer otri, subsegments, sorted slots and cut
+  // triangles, lane-interleaved in LDS for the edge matching below
+  __shared__ int a_p[DEL_MAX * DEL_LANES], a_o[DEL_MAX * DEL_LANES], a_s[DEL_MAX * DEL_LANES],
+      a_sl[DEL_MAX * DEL_LANES], a_nt[3 * (DEL_MAX - 2) * DEL_LANES];
+  const int ln = threadIdx.x;
   const int nd = 2 * del_nseg(dd);
   const unsigned long long ep = del_epoch(dd);
+  unsigned long long ndel = 0, nflip = 0;
   for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < nd; d += gridDim.x * blockDim.x) {
     if (dd.dv[d] < 0) continue;
     const int *W = dd.ws + (size_t)d * DEL_WS;
     const int k = dd.dk[d];
-    const unsigned long long key = claim_key(ep, 0, d);
+    auto P_ = [&](int i) -> int & { return a_p[i * DEL_LANES + ln]; };
+    auto O_ = [&](int i) -> int & { return a_o[i * DEL_LANES + ln]; };
+    auto S_ = [&](int i) -> int & { return a_s[i * DEL_LANES + ln]; };
+    auto SL = [&](int i) -> int & { return a_sl[i * DEL_LANES + ln]; };
+    auto NT = [&](int i) -> int & { return a_nt[i * DEL_LANES + ln]; };
+    // the workspace first (independent loads), then every claim word of the
+    // star and ring (independent again): two round trips, not two per entry
+    // (unrolled by 8: eight loads in flight before their first use)
+#pragma unroll 8
     for (int q = 0; q < k; q++) {
-      atomicMax(&m.claim[2 * (size_t)W[q]], key);
-      const int o = W[2 * DEL_MAX + q];
-      if (o >= 0) atomicMax(&m.claim[2 * (size_t)(o >> 2)], key);
+      SL(q) = W[q];
+      O_(q) = W[2 * DEL_MAX + q];
     }
-  }
-}
-
-__global__ void k_del_win(MeshD m, DelD dd) {
-  const int nd = 2 * del_nseg(dd);
-  const unsigned long long ep = del_epoch(dd);
-  for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < nd; d += gridDim.x * blockDim.x) {
-    int w = dd.dv[d] >= 0;
-    if (w) {
-      const int *W = dd.ws + (size_t)d * DEL_WS;
-      const int k = dd.dk[d];
+    {
       const unsigned long long key = claim_key(ep, 0, d);
-      for (int q = 0; q < k && w; q++) {
-        const int o = W[2 * DEL_MAX + q];
-        w = m.claim[2 * (size_t)W[q]] == key && (o < 0 || m.claim[2 * (size_t)(o >> 2)] == key);
+      bool w = true;
+#pragma unroll 8
+      for (int q = 0; q < k; q++) {
+        const int o = O_(q);
+        const bool a = m.claim[2 * (size_t)SL(q)] == key;
+        const bool b = o < 0 || m.claim[2 * (size_t)(o >> 2)] == key;
+        w = w & a & b;
       }
+      if (!w) continue;
     }
-    dd.dwin[d] = w;
-  }
-}
-
-// Winners: the cut triangles take the star's slots in ascending order (the
-// two largest slots die); links inside the star are matched edge by edge,
-// link edges take the outer triangle (whose back-link is repaired) and the
-// subsegment (whose adjacent otri is repaired).
-__global__ void k_del_apply(MeshD m, DelD dd) {
-  const int nd = 2 * del_nseg(dd);
-  unsigned long long ndel = 0, nflip = 0;
-  for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < nd; d += gridDim.x * blockDim.x) {
-    if (!dd.dwin[d]) continue;
-    const int *W = dd.ws + (size_t)d * DEL_WS;
-    const int *star = W, *p = W + DEL_MAX, *o = W + 2 * DEL_MAX, *sg = W + 3 * DEL_MAX;
-    const int *nt = W + 4 * DEL_MAX;
-    const int k = dd.dk[d];
-    int sl[DEL_MAX];
-    for (int q = 0; q < k; q++) sl[q] = star[q];
+#pragma unroll 8
+    for (int q = 0; q < k; q++) {
+      P_(q) = W[DEL_MAX + q];
+      S_(q) = W[3 * DEL_MAX + q];
+    }
+#pragma unroll 8
+    for (int q = 0; q < 3 * (k - 2); q++) NT(q) = W[4 * DEL_MAX + q];
+    const int inst = m.T[SL(0)].V.w;
     for (int q = 1; q < k; q++) {
-      const int x = sl[q];
+      const int x = SL(q);
       int w = q - 1;
-      while (w >= 0 && sl[w] > x) {
-        sl[w + 1] = sl[w];
+      while (w >= 0 && SL(w) > x) {
+        SL(w + 1) = SL(w);
         w--;
       }
-      sl[w + 1] = x;
+      SL(w + 1) = x;
     }
-    const int inst = m.T[star[0]].V.w;
     for (int q = 0; q < k - 2; q++) {
-      const int st = sl[q];
-      const int V0 = nt[3 * q], V1 = nt[3 * q + 1], V2 = nt[3 * q + 2];
+      const int st = SL(q);
+      const int V0 = NT(3 * q), V1 = NT(3 * q + 1), V2 = NT(3 * q + 2);
       int nn[3], ss[3];
       for (int e = 0; e < 3; e++) {
         const int u = e == 0 ? V1 : (e == 1 ? V2 : V0);
         const int w = e == 0 ? V2 : (e == 1 ? V0 : V1);
         int nb = -2, sgv = -1;
         for (int i = 0; i < k && nb == -2; i++)
-          if (p[i] == u && p[(i + 1) % k] == w) {
-            nb = o[i];
-            sgv = sg[i];
+          if (P_(i) == u && P_((i + 1) % k) == w) {
+            nb = O_(i);
+            sgv = S_(i);
             if (nb >= 0) *nbr_slot(m.T, nb >> 2, nb & 3) = (st << 2) | e;
             if (sgv >= 0) m.ST[sgv] = (st << 2) | e;
           }
         for (int q2 = 0; q2 < k - 2 && nb == -2; q2++) {
           if (q2 == q) continue;
           for (int e2 = 0; e2 < 3; e2++)
-            if (nt[3 * q2 + (e2 + 1) % 3] == w && nt[3 * q2 + (e2 + 2) % 3] == u) {
-              nb = (sl[q2] << 2) | e2;
+            if (NT(3 * q2 + (e2 + 1) % 3) == w && NT(3 * q2 + (e2 + 2) % 3) == u) {
+              nb = (SL(q2) << 2) | e2;
               break;
             }
         }
@@ -1507,8 +1542,8 @@ __global__ void k_del_apply(MeshD m, DelD dd) {
       r.S = make_int4(-1, -1, -1, 0);
       r.X[0] = r.X[1] = r.X[2] = make_double2(0.0, 0.0);
       r.pad[0] = r.pad[1] = make_int4(0, 0, 0, 0);
-      m.T[sl[q]] = r;
-      m.tflag[sl[q]] = 0;
+      m.T[SL(q)] = r;
+      m.tflag[SL(q)] = 0;
     }
     m.pflag[dd.dv[d]] |= PF_DEAD;
     ndel++;
@@ -2135,7 +2170,7 @@ Refiner::~Refiner() {
   dfree(h_key); dfree(h_val);
   dfree(d_ctl); dfree(g_cnt); dfree(g_off);
   dfree(o_inst);
-  dfree(del_ws); dfree(del_dv); dfree(del_dk); dfree(del_win);
+  dfree(del_ws); dfree(del_dv); dfree(del_dk);
   dfree(p_new); dfree(p_tmp); dfree(pf_tmp); dfree(pp_tmp);
   if (gexec) (void)hipGraphExecDestroy(gexec);
   if (graph) (void)hipGraphDestroy(graph);
@@ -2422,7 +2457,7 @@ const char *Refiner::kname(int id) {
   static const char *names[] = {"init",       "mark_bad", "compact",
                                 "seg_cavity", "bad_cavity", "claim",
                                 "win",        "apply",      "export",
-                                "mlp"};
+                                "mlp",        "del_scan",   "del_apply"};
   return names[id];
 }
 
@@ -2604,7 +2639,6 @@ DelD Refiner::del_base() const {
   dd.ws = del_ws;
   dd.dv = del_dv;
   dd.dk = del_dk;
-  dd.dwin = del_win;
   dd.pws = c_pws;
   dd.cnt = d_cnt;
   dd.use_fp32 = pr.use_fp32;
@@ -2615,11 +2649,17 @@ DelD Refiner::del_base() const {
 // loop: both come from the control block).  The scan kernel's grid is the
 // exact-predicate arena's (one arena slot per lane).
 void Refiner::enqueue_deletion(const DelD &dd) {
-  k_del_scan<<<pws_blocks * 4, 64, 0, stream>>>(m, pr, dd);
-  const int g = std::min(nblk(2LL * DEL_CANDS), 512);
-  k_del_claim<<<g, 256, 0, stream>>>(m, dd);
-  k_del_win<<<g, 256, 0, stream>>>(m, dd);
-  k_del_apply<<<g, 256, 0, stream>>>(m, dd);
+  // host loop: grids sized by the pass's subsegment candidates (most
+  // segment passes hold few); graph loop: sized for the largest pass
+  const long long n = dd.ctl ? DEL_CANDS : std::min(dd.nseg, DEL_CANDS);
+  const int gs = (int)std::max(1LL, std::min<long long>(pws_blocks * 4, (2 * n + 63) / 64));
+  const int g = (int)std::max(1LL, std::min<long long>(2048, (2 * n + 63) / 64));
+  tbegin(K_DEL_SCAN);
+  k_del_scan<<<gs, DEL_LANES, 0, stream>>>(m, pr, dd);
+  tend(K_DEL_SCAN, dd.ctl ? 0 : n);
+  tbegin(K_DEL_APPLY);
+  k_del_apply<<<g, DEL_LANES, 0, stream>>>(m, dd);
+  tend(K_DEL_APPLY, 0);
   HC(hipGetLastError());
 }
 
@@ -2649,8 +2689,13 @@ void Refiner::compact_points() {
 // claim skips candidates blocked by an earlier iteration's winner stamps.  In
 // the host loop ps.epoch advances here; in the graph loop the kernels derive
 // it from the control block.
-void Refiner::enqueue_claims(PassD ps, int gc) {
-  int iters = std::max(1, opt.claim_iters);
+// Claim iterations of a pass: segment-only passes may take fewer
+// (options.seg_claim_iters) than passes with bad triangles.
+int Refiner::iters_for(bool seg_only) const {
+  return std::max(1, seg_only && opt.seg_claim_iters > 0 ? opt.seg_claim_iters : opt.claim_iters);
+}
+
+void Refiner::enqueue_claims(PassD ps, int gc, int iters) {
   ps.iters = iters;
   for (int it = 0; it < iters; it++) {
     if (!ps.ctl) ps.epoch = ++epoch;
@@ -2680,7 +2725,8 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
   ps.ids = ids;
   ps.nc = nc;
   ps.nseg = nseg;
-  int iters = std::max(1, opt.claim_iters);
+  const int iters = iters_for(nseg == nc);
+  ps.iters = iters;
   if (epoch + iters + 3 >= (1ull << 11)) {  // epoch wrap: clear claims
     HC(hipMemsetAsync(m.claim, 0, sizeof(unsigned long long) * 2 * capT, stream));
     epoch = 0;
@@ -2752,7 +2798,7 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
       log_t.push_back(std::log2(1.0f + (float)hd[i]));
     }
   }
-  enqueue_claims(ps, gc);
+  enqueue_claims(ps, gc, iters);
   compact(PredWin{c_win}, nc, nullptr, c_rank, nullptr, d_cnt + CNT_NWIN);
   tbegin(K_APPLY);
   k_apply<APPLY_LANES><<<std::min(nblk((long long)nc * APPLY_LANES, LIST_BLOCK), opt.grid_apply),
@@ -2890,7 +2936,6 @@ void Refiner::run() {
     dalloc(&del_ws, (size_t)2 * DEL_CANDS * DEL_WS);
     dalloc(&del_dv, (size_t)2 * DEL_CANDS);
     dalloc(&del_dk, (size_t)2 * DEL_CANDS);
-    dalloc(&del_win, (size_t)2 * DEL_CANDS);
   }
   st.status = 1;
   if (opt.graph) loop_graph();
@@ -3160,13 +3205,15 @@ void Refiner::enqueue_seg_pass() {
     ps.eskip = 1;
   }
   launch_cavity(true, opt.grid_cavity, ps);
-  enqueue_claims(ps, opt.grid_claim);
+  const int seg_iters = iters_for(true);
+  ps.iters = seg_iters;
+  enqueue_claims(ps, opt.grid_claim, seg_iters);
   k_dcount<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_cnt);
   k_dscan<FinWin><<<1, GC, 0, stream>>>(g_cnt, g_off, d_ctl, -1, FinWin{});
   k_dscatter<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_off,
                                              nullptr, c_rank, nullptr);
   k_apply<APPLY_LANES><<<opt.grid_apply, LIST_BLOCK, 0, stream>>>(m, pr, ps, 0, 0, 0);
-  k_end_seg<<<1, 1, 0, stream>>>(d_ctl, std::max(1, opt.claim_iters), del_on ? 1 : 0);
+  k_end_seg<<<1, 1, 0, stream>>>(d_ctl, seg_iters, del_on ? 1 : 0);
   HC(hipGetLastError());
 }
 
@@ -3205,13 +3252,15 @@ void Refiner::enqueue_bad_pass() {
     mlp_forward_device_n(c_feat, ci + C_NC, ci + C_GO, mlp_params, c_score, 4096, stream);
     k_learned_prio<<<4096, 256, 0, stream>>>(ps, c_score, opt.learned_shift);
   }
-  enqueue_claims(ps, opt.grid_claim);
+  const int bad_iters = iters_for(false);
+  ps.iters = bad_iters;
+  enqueue_claims(ps, opt.grid_claim, bad_iters);
   k_dcount<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_cnt);
   k_dscan<FinWin><<<1, GC, 0, stream>>>(g_cnt, g_off, d_ctl, -1, FinWin{});
   k_dscatter<PredWin><<<GC, CB, 0, stream>>>(PredWin{c_win}, d_ctl, C_NC, -1, g_off,
                                              nullptr, c_rank, nullptr);
   k_apply<APPLY_LANES><<<opt.grid_apply, LIST_BLOCK, 0, stream>>>(m, pr, ps, 0, 0, 0);
-  k_end_bad<<<1, 1, 0, stream>>>(d_ctl, std::max(1, opt.claim_iters));
+  k_end_bad<<<1, 1, 0, stream>>>(d_ctl, bad_iters);
   HC(hipGetLastError());
 }
 
@@ -3264,7 +3313,7 @@ void Refiner::loop_graph() {
   size_pass_buffers();
   HC(hipMemcpyAsync(d_ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, stream));
   capture_graph();
-  const int iters = std::max(1, opt.claim_iters);
+  const int iters = std::max(iters_for(false), iters_for(true));
   const long long passes = (long long)opt.graph_rounds * (opt.graph_seg_passes + 1);
   const long long max_launches = 4LL * opt.max_rounds + 100000;
   for (long long launch = 0;; launch++) {

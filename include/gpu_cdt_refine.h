@@ -186,6 +186,8 @@ typedef struct gcdt_options {
                             1), Triangle's rule (triangle.cpp:13319-13357,
                             default); 2 in every mode, the reference's
                             (refine.cu:604-677) */
+  int seg_claim_iters;   /* claim iterations of passes that split only
+                            subsegments (0: claim_iters; default 0) */
   int vmm;               /* 1: arrays a pass can outgrow live in reserved
                             address ranges and grow in place by mapping more
                             memory (no copy; the graph loop keeps its captured

@@ -65,6 +65,8 @@ typedef struct {
   int mix_below;      /* classic rounds: the segment phase ends once at most
                          this many subsegments are encroached; they join the
                          round's bad pass (0: the reference's round) */
+  int seg_claim_iters; /* claim iterations of passes that split only
+                          subsegments (0: claim_iters) */
   int deletion;       /* free vertices inside an encroached subsegment's
                          diametral circle are deleted before it splits:
                          0 never, 1 in Chew mode (Triangle's rule,

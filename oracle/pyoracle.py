@@ -48,7 +48,7 @@ class OrcParams(C.Structure):
                 ("cand_budget", C.c_longlong), ("max_rounds", C.c_int),
                 ("priority", C.c_int), ("cand_factor", C.c_int), ("cand_min", C.c_int),
                 ("claim_iters", C.c_int), ("mixed", C.c_int), ("shared_ring", C.c_int),
-                ("mix_below", C.c_int), ("deletion", C.c_int)]
+                ("mix_below", C.c_int), ("seg_claim_iters", C.c_int), ("deletion", C.c_int)]
 
 
 class OrcStats(C.Structure):
@@ -189,7 +189,8 @@ def triangle_refine(points, triangles, segments, theta, mode=1):
 def oracle_refine(points, triangles, neighbors, segments, theta=25.0, offcenter=1,
                   mode=1, cav_cap0=32, cav_cap_max=2048, cand_budget=1 << 30,
                   max_rounds=100000, priority=1, cand_factor=4, cand_min=16384,
-                  claim_iters=1, mixed=0, shared_ring=0, mix_below=0, deletion=1):
+                  claim_iters=1, mixed=0, shared_ring=0, mix_below=0, deletion=1,
+                  seg_claim_iters=0):
     """Run the CPU restatement; returns (P, T, N, S, pflag, stats).
     deletion: 0 never delete vertices, 1 in Chew mode (Triangle's rule),
     2 in every mode (the reference's)."""
@@ -199,7 +200,7 @@ def oracle_refine(points, triangles, neighbors, segments, theta=25.0, offcenter=
     S = np.ascontiguousarray(segments, np.int32).reshape(-1, 2)
     prm = OrcParams(theta, offcenter, mode, cav_cap0, cav_cap_max, cand_budget,
                     max_rounds, priority, cand_factor, cand_min, claim_iters, mixed, shared_ring,
-                    mix_below, deletion)
+                    mix_below, seg_claim_iters, deletion)
     st = OrcStats()
     L = orc()
     r = L.orc_refine(len(P), _p(P), len(T), _p(T), _p(N), len(S), _p(S),
@@ -274,12 +275,12 @@ def oracle_refine_batch(instances, theta, offcenter, mode, **params):
     md = np.ascontiguousarray(np.broadcast_to(np.asarray(mode, np.int32), (k,)))
     d = dict(cav_cap0=32, cav_cap_max=2048, cand_budget=1 << 30, max_rounds=100000,
              priority=1, cand_factor=4, cand_min=16384, claim_iters=1, mixed=0, shared_ring=0,
-             mix_below=0, deletion=1)
+             mix_below=0, deletion=1, seg_claim_iters=0)
     d.update(params)
     prm = OrcParams(25.0, 1, 1, d["cav_cap0"], d["cav_cap_max"], d["cand_budget"],
                     d["max_rounds"], d["priority"], d["cand_factor"], d["cand_min"],
                     d["claim_iters"], d["mixed"], d["shared_ring"], d["mix_below"],
-                    d["deletion"])
+                    d["seg_claim_iters"], d["deletion"])
     st = OrcStats()
     L = orc()
     r = L.orc_refine_batch(len(P), _p(P), len(T), _p(T), _p(N), len(S), _p(S), _p(I), k,

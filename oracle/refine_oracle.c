@@ -1421,7 +1421,9 @@ static int mixed_round(Mesh *m, const orc_params *prm, orc_stats *st, int *K, in
   nb = select_bad(m, B, nb, capb, prm->priority);
   int of = 0, nws = 0;
   long long prog = 0;
-  int nw = run_pass(m, L, ne + nb, ne, *K, prm->cav_cap_max, prm->priority, prm->claim_iters,
+  /* a pass that splits only subsegments takes seg_claim_iters iterations */
+  const int ci = (nb == 0 && prm->seg_claim_iters > 0) ? prm->seg_claim_iters : prm->claim_iters;
+  int nw = run_pass(m, L, ne + nb, ne, *K, prm->cav_cap_max, prm->priority, ci,
                     prm->shared_ring, prm->deletion, st, &of, &prog, &nws);
   if (ne > 0) st->seg_passes++;
   if (nb > 0) {
@@ -1471,7 +1473,8 @@ static orc_result *refine_impl(int np, const double *points, int nt,
       if (ne > cap) ne = cap;
       int of = 0, nws = 0;
       long long prog = 0;
-      run_pass(m, ids, ne, ne, K, prm->cav_cap_max, prm->priority, prm->claim_iters, prm->shared_ring,
+      run_pass(m, ids, ne, ne, K, prm->cav_cap_max, prm->priority,
+               prm->seg_claim_iters > 0 ? prm->seg_claim_iters : prm->claim_iters, prm->shared_ring,
                prm->deletion, st, &of, &prog, &nws);
       st->seg_passes++;
       if (of && K < prm->cav_cap_max) K = K * 2 > prm->cav_cap_max ? prm->cav_cap_max : K * 2;

@@ -85,11 +85,19 @@ def test_gpu_deterministic(gpu, oracle):
     assert np.array_equal(a.segments, b.segments)
 
 
-@pytest.mark.parametrize("iters", [2, 3])
-def test_gpu_equals_oracle_claim_iters(gpu, oracle, iters):
+@pytest.mark.parametrize("iters,seg_iters", [(2, 0), (3, 0), (3, 2)])
+def test_gpu_equals_oracle_claim_iters(gpu, oracle, iters, seg_iters):
+    """Claim iterations per pass; seg_iters > 0 gives passes that split only
+    subsegments their own count (host loop and graph loop)."""
     P, T, N, S = _input(oracle, 5000, 250, 21)
-    r = gpu.refine(P, T, N, S, theta=28.0, claim_iters=iters)
-    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, theta=28.0, claim_iters=iters)
+    r = gpu.refine(P, T, N, S, theta=28.0, claim_iters=iters, seg_claim_iters=seg_iters)
+    if seg_iters:
+        rg = gpu.refine(P, T, N, S, theta=28.0, claim_iters=iters, seg_claim_iters=seg_iters,
+                        graph=1)
+        assert np.array_equal(rg.points.view(np.uint64), r.points.view(np.uint64))
+        assert np.array_equal(rg.triangles, r.triangles)
+    Po, To, No, So, pf, st = oracle.oracle_refine(P, T, N, S, theta=28.0, claim_iters=iters,
+                                                  seg_claim_iters=seg_iters)
     assert np.array_equal(r.points.view(np.uint64), Po.view(np.uint64))
     assert np.array_equal(r.triangles, To)
     assert np.array_equal(r.neighbors, No)
