@@ -295,7 +295,8 @@ def bench_refine(args):
     log(f"# input: {len(P)} points, {len(T)} triangles, {len(SS)} segments "
         f"(prep {time.time() - t0:.1f}s)")
     kw = dict(theta=args.theta, insert=g.Offcenter, mode=g.RUPPERT)
-    # ---- headline: the drop-in, host arrays in and out (BASELINE boundary)
+    # ---- headline: the drop-in, host arrays in and out (the reference's
+    # timing boundary: steady clock around GPU_Refine_Quality, main.cpp:1860-1868)
     rp = None
     if not args.no_plain:
         g.dropin_set_options(plain=True, **plain_options())
@@ -397,7 +398,7 @@ def bench_refine(args):
                    "max_segment_length": args.domain * 0.02, "theta_deg": args.theta,
                    "insert": "offcenter", "mode": "ruppert", "seed": args.seed,
                    "boundary": "drop-in GPU_Refine_Quality: host triangulateio in and out, "
-                               "median call wall time after warm-up (BASELINE.md section 2)",
+                               "median call wall time after warm-up (main.cpp:1860-1868)",
                    "chosen_options": g.CHOSEN_FORM},
         "plain": None if rp is None else {
             "value": rp.stats["n_triangles"] / (np.median(wp) / 1e3),
