@@ -69,7 +69,8 @@ enum : int {
   CNT_LEFT_SEG,  // end of the run: subsegments with senc == 2
   CNT_DEL,       // vertices deleted
   CNT_FLIPS,     // flips Lawson's algorithm would make for the pass's changes
-  CNT_N = 24
+  CNT_NENC,      // encroached subsegments after the pass (listed behind its apply)
+  CNT_N = 25
 };
 
 #define GCDT_WALK_LIMIT (1 << 22)

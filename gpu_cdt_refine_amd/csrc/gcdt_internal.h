@@ -136,7 +136,7 @@ class Refiner {
               unsigned long long *total64 = nullptr, bool counted = false);
   void ensure_cb(int nb);
   int run_pass(const int *ids, int nc, int nseg, long long *progress, bool *overflowed,
-               int *nwin_seg);
+               int *nwin_seg, bool list_enc = false);
   DelD del_base() const;
   void enqueue_deletion(const DelD &dd);
   void compact_points();
@@ -186,6 +186,9 @@ class Refiner {
       vm_senc, vm_tmp, vm_sel_prio, vm_sel_tmp, vm_sel_list;
   int K = 32;
   int np_round = 0;  // live points when the last round opened
+  // encroached subsegments listed in tmp_list behind the last pass's apply
+  // (-1: not listed; the next segment phase compacts them itself)
+  int listed_enc = -1;
   unsigned long long epoch = 0;
   // candidates
   int c_cap = 0;

@@ -47,6 +47,15 @@ struct PredSenc {
   const int *senc;
   __device__ bool operator()(int i) const { return senc[i] == 1; }
 };
+// PredSenc over the subsegments that exist after a pass whose new-subsegment
+// count is still on the device: [0, ns0 + *nadd) of a range sized for every
+// candidate splitting
+struct PredSencN {
+  const int *senc;
+  int ns0;
+  const unsigned long long *nadd;
+  __device__ bool operator()(int i) const { return i < ns0 + (int)*nadd && senc[i] == 1; }
+};
 struct PredBad {
   const uint8_t *tflag;
   __device__ bool operator()(int i) const {
@@ -2714,8 +2723,12 @@ void Refiner::enqueue_claims(PassD ps, int gc, int iters) {
 // One conflict pass over ids[0..nc): candidates below nseg split
 // subsegments, the rest are bad triangles.  Returns the number of winners,
 // *nwin_seg of them subsegment splits.
+//
+// list_enc: also list the subsegments encroached after the pass in tmp_list
+// (listed_enc), queued behind the apply kernel so that one synchronisation
+// serves both; the next segment phase then starts from that list.
 int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, bool *overflowed,
-                      int *nwin_seg) {
+                      int *nwin_seg, bool list_enc) {
   const int nbad = nc - nseg;
   // room for every candidate winning: the winners' count is only read back
   // with the pass counters, after the apply kernel
@@ -2805,7 +2818,14 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
                          LIST_BLOCK, 0, stream>>>(m, pr, ps, np, nt, ns);
   tend(K_APPLY, 0);
   HC(hipGetLastError());
+  // every segment candidate may add one subsegment; the list buffer is not
+  // grown here (the pass may still read it)
+  const bool listed = list_enc && (long long)ns + nseg <= tmp_list_cap;
+  if (listed)
+    compact(PredSencN{m.senc, ns, d_cnt + CNT_SEGWIN}, ns + nseg, tmp_list, nullptr, nullptr,
+            d_cnt + CNT_NENC);
   read_counters();
+  listed_enc = listed ? (int)h_cnt[CNT_NENC] : -1;
   const int nwin = (int)h_cnt[CNT_NWIN];
   const int nws = (int)h_cnt[CNT_SEGWIN];
   const long long ndel = (long long)h_cnt[CNT_DEL];
@@ -2993,6 +3013,7 @@ void Refiner::run() {
 // Host-driven rounds: the host reads every count and decides every step.
 void Refiner::loop_host() {
   int prev_win = -1;
+  listed_enc = -1;
   ensure_cand(std::min(cands_per_pass(K), std::max(4 * nt0, 1024)));
   for (int round = 0; round < opt.max_rounds; round++) {
     st.rounds = round + 1;
@@ -3000,15 +3021,20 @@ void Refiner::loop_host() {
     // segment phase: until at most mix_below subsegments are encroached
     int ne;
     for (;;) {
-      ensure_tmp(ns);
-      ne = compact(PredSenc{m.senc}, ns, tmp_list, nullptr, nullptr);
+      if (listed_enc >= 0) {  // listed behind the last pass
+        ne = listed_enc;
+        listed_enc = -1;
+      } else {
+        ensure_tmp(ns);
+        ne = compact(PredSenc{m.senc}, ns, tmp_list, nullptr, nullptr);
+      }
       if (ne <= opt.mix_below) break;
       int nc = std::min(ne, cands_per_pass(K));
       ensure_cand(nc);
       long long prog = 0;
       bool of = false;
       int nws = 0;
-      run_pass(tmp_list, nc, nc, &prog, &of, &nws);
+      run_pass(tmp_list, nc, nc, &prog, &of, &nws, true);
       st.seg_passes++;
       if (of && K < opt.cav_cap_max) K = std::min(2 * K, opt.cav_cap_max);
     }
@@ -3070,7 +3096,7 @@ bool Refiner::bad_pass(int ne, int *prev_win) {
   long long prog = 0;
   bool of = false;
   int nws = 0;
-  int nwin = run_pass(ids, nc, ne, &prog, &of, &nws);
+  int nwin = run_pass(ids, nc, ne, &prog, &of, &nws, !opt.mixed);
   if (ne > 0) st.seg_passes++;
   if (nbc > 0) {
     pass_index++;
