@@ -193,7 +193,21 @@ struct PassD {
   const int *ids_sel;  // selected candidates when ctl->i[C_USESEL]
   int *slow;           // candidates whose filters could not decide a sign
   int eskip;  // graph loop: the pass's vertex deletion step took an epoch
+  // learned priority folded in by the first claim iteration (host loop):
+  // the network's scores, and the bucket coarsening shift; nullptr: cprio
+  // is final
+  const float *score;
+  int lshift;
 };
+
+// Learned claim priority of a bad-triangle candidate: its shortest-edge
+// bucket coarsened by 2^shift half-octaves, then higher predicted conflict
+// first (the score quantised to 8 bits).
+__device__ __forceinline__ int learned_prio(int bucket, float score, int shift) {
+  const float q = score * 32.f;
+  const int qi = q != q ? 0 : (int)fminf(fmaxf(q, 0.f), 255.f);
+  return ((bucket >> shift) << 8) | (255 - qi);
+}
 
 __device__ __forceinline__ int pass_nc(const PassD &ps) {
   if (!ps.ctl) return ps.nc;

@@ -135,6 +135,7 @@ class Refiner {
   int compact(Pred pred, int n, int *out_list, int *out_rank, const int *src,
               unsigned long long *total64 = nullptr, bool counted = false);
   void ensure_cb(int nb);
+  void flush_counters();
   int run_pass(const int *ids, int nc, int nseg, long long *progress, bool *overflowed,
                int *nwin_seg, bool list_enc = false);
   DelD del_base() const;
@@ -220,6 +221,8 @@ class Refiner {
   unsigned long long *d_cnt = nullptr;
   int *h_scalar = nullptr;
   unsigned long long *h_cnt = nullptr;
+  unsigned long long *d_host_cnt = nullptr;  // h_cnt as the device sees it (mapped)
+  bool cnt_zero = false;  // d_cnt is known to be all zero (flushed by the last pass)
   // segment hash
   unsigned long long *h_key = nullptr;
   int *h_val = nullptr;

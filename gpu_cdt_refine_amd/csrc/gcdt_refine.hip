@@ -1030,6 +1030,7 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_claim(MeshD m, PassD ps, int *ca
   unsigned ws_seg = 0, ws_bad = 0;  // workspace ints the cavity kernels wrote
   for (int i = blockIdx.x * per + threadIdx.x / G; i < nc; i += gridDim.x * per) {
     const int *L = ps.ws + (size_t)i * ps.R;
+    int prio;
     if (ps.iter == 0) {
       const bool on = ps.cstatus[i] == ST_VALID;
       if (gl == 0) {
@@ -1037,6 +1038,11 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_claim(MeshD m, PassD ps, int *ca
         ps.cwin[i] = 0;
       }
       if (!on) continue;
+      prio = ps.cprio[i];
+      if (ps.score && i >= nseg) {  // the learned priority, kept for the later kernels
+        prio = learned_prio(prio, ps.score[i], ps.lshift);
+        if (gl == 0) ps.cprio[i] = prio;
+      }
       if (gl == 0) {  // cavity + ring list, then the star boundary (4 ints per edge)
         const unsigned w = (unsigned)(ps.cn[i] + ps.cr[i] + 4 * (ps.cn[i] + 2));
         if (i < nseg) ws_seg += w;
@@ -1058,8 +1064,9 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_claim(MeshD m, PassD ps, int *ca
         if (gl == 0) cact[i] = 0;
         continue;
       }
+      prio = ps.cprio[i];
     }
-    const unsigned long long key = claim_key(ep, ps.cprio[i], i);
+    const unsigned long long key = claim_key(ep, prio, i);
     const int n = ps.cn[i], k = n + ps.cr[i];
     // shared rings: cavity entries claim word 0, ring entries word 1;
     // otherwise every entry claims word 0
@@ -1697,9 +1704,7 @@ __global__ void k_learned_prio(PassD ps, const float *score, int shift) {
   for (int i = pass_nseg(ps) + blockIdx.x * blockDim.x + threadIdx.x; i < nc;
        i += gridDim.x * blockDim.x) {
     if (ps.cstatus[i] != ST_VALID) continue;
-    float q = score[i] * 32.f;
-    int qi = q != q ? 0 : (int)fminf(fmaxf(q, 0.f), 255.f);
-    ps.cprio[i] = ((ps.cprio[i] >> shift) << 8) | (255 - qi);
+    ps.cprio[i] = learned_prio(ps.cprio[i], score[i], shift);
   }
 }
 
@@ -2112,7 +2117,10 @@ Refiner::Refiner(const gcdt_options &o) : opt(o) {
   // compaction totals are written here by the scan kernel
   HC(hipHostMalloc((void **)&h_total, 64 * sizeof(int), hipHostMallocCoherent | hipHostMallocMapped));
   HC(hipHostGetDevicePointer((void **)&d_host_total, h_total, 0));
-  HC(hipHostMalloc((void **)&h_cnt, CNT_N * sizeof(unsigned long long), hipHostMallocDefault));
+  // pass counters: written here by k_flush_counters
+  HC(hipHostMalloc((void **)&h_cnt, CNT_N * sizeof(unsigned long long),
+                   hipHostMallocCoherent | hipHostMallocMapped));
+  HC(hipHostGetDevicePointer((void **)&d_host_cnt, h_cnt, 0));
   HC(hipEventCreate(&ev0));
   HC(hipEventCreate(&ev1));
   // cavity and apply kernels take up to BIG_LDS_BYTES of dynamic LDS
@@ -2523,6 +2531,26 @@ void Refiner::ensure_cb(int nb) {
 void Refiner::read_counters() {
   HC(hipMemcpyAsync(h_cnt, d_cnt, CNT_N * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
   HC(hipStreamSynchronize(stream));
+  cnt_zero = false;
+}
+
+// End of a pass: the counters to the host's mapped copy (read after the
+// stream synchronisation) and back to zero for the next pass -- one launch
+// in place of a copy and a fill.
+__global__ void k_flush_counters(unsigned long long *cnt, unsigned long long *host) {
+  const int k = threadIdx.x;
+  if (k < CNT_N) {
+    __hip_atomic_store(host + k, cnt[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    cnt[k] = 0ull;
+  }
+  __threadfence_system();
+}
+
+void Refiner::flush_counters() {
+  k_flush_counters<<<1, 64, 0, stream>>>(d_cnt, d_host_cnt);
+  HC(hipGetLastError());
+  HC(hipStreamSynchronize(stream));
+  cnt_zero = true;
 }
 
 // Candidate workspace of at least `ints` ints (grown by half again at least,
@@ -2755,7 +2783,8 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
   }
   ps.feat = want_feat ? c_feat : nullptr;
   ps.pass_index = pass_index;
-  HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
+  if (!cnt_zero) HC(hipMemsetAsync(d_cnt, 0, CNT_N * sizeof(unsigned long long), stream));
+  cnt_zero = false;
   if (del_on && nseg > 0) {  // vertex deletion first, with its own claim epoch
     DelD dd = del_base();
     dd.ids = ids;
@@ -2787,8 +2816,10 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
     if (!mlp_params) throw std::runtime_error("priority 2 needs a model (set_priority_model)");
     tbegin(K_MLP);
     mlp_forward_device(c_feat + (size_t)nseg * FEAT, nbad, mlp_params, c_score + nseg, stream);
-    k_learned_prio<<<std::min(nblk(nbad), 4096), 256, 0, stream>>>(ps, c_score, opt.learned_shift);
     tend(K_MLP, nbad);
+    // the first claim iteration turns the scores into claim priorities
+    ps.score = c_score;
+    ps.lshift = opt.learned_shift;
     HC(hipGetLastError());
   }
   if (nbad > 0 && opt.log_training) {  // conflict logs for training (pure bad passes)
@@ -2824,7 +2855,7 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
   if (listed)
     compact(PredSencN{m.senc, ns, d_cnt + CNT_SEGWIN}, ns + nseg, tmp_list, nullptr, nullptr,
             d_cnt + CNT_NENC);
-  read_counters();
+  flush_counters();
   listed_enc = listed ? (int)h_cnt[CNT_NENC] : -1;
   const int nwin = (int)h_cnt[CNT_NWIN];
   const int nws = (int)h_cnt[CNT_SEGWIN];

@@ -134,3 +134,28 @@ def test_mesh_growth_in_place_equals_copy(gpu, oracle, graph):
     assert np.array_equal(v.points.view(np.uint64), Po.view(np.uint64))
     assert np.array_equal(v.triangles, To)
     assert np.array_equal(v.neighbors, No)
+
+
+def test_graph_equals_host_loop_learned_priority(gpu, oracle):
+    """The learned claim priority: the host loop folds the network's scores
+    into the priorities inside its first claim iteration, the graph loop in a
+    kernel of its own; both give the same mesh."""
+    import os
+    model = np.load(os.path.join(os.path.dirname(gpu.__file__), "priority_mlp.npy"))
+    P, T, N, S = _input(oracle, n=6000, ns=300, seed=60)
+    kw = dict(theta=25.0, priority=2, claim_iters=3, shared_ring=1, cand_factor=3,
+              pred_mode=1, learned_shift=1)
+    out = []
+    for graph in (0, 1):
+        s = gpu.Session(P, T, N, S, graph=graph, **kw)
+        try:
+            s.set_priority_model(model)
+            st = s.run()
+            out.append((st, s.download()))
+        finally:
+            s.close()
+    (sh, h), (sg, g) = out
+    assert sh["status"] == 0 and sh["bad_passes"] > 1
+    for k in STAT_KEYS:
+        assert sg[k] == sh[k], k
+    _same(g, h)
