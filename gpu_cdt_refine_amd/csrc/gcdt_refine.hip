@@ -3219,7 +3219,10 @@ static void launch_cavity_t(int lg, int grid, size_t lds, hipStream_t s, const M
 // over the candidates it deferred (their count stays on the device).
 void Refiner::launch_cavity(bool is_seg, int grid, const PassD &ps) {
   int kf = std::min(ps.K, ps.Kf);
-  int lg = cavity_lanes(kf), lgs = cavity_lanes(ps.K);
+  // the exact kernel takes a whole wave per candidate: its candidates are
+  // the cavities beyond the fast kernel's LDS (33..K triangles, whose sort
+  // and boundary cycle cost O(n^2 / lanes)) and the undecided signs
+  int lg = cavity_lanes(kf), lgs = 64;
   size_t lds = sizeof(int) * (size_t)(64 / lg) * group_lds_ints(kf, lg);
   size_t lds_slow = sizeof(int) * (size_t)(64 / lgs) * group_lds_ints(ps.K, lgs);
   if (lds_slow > (size_t)BIG_LDS_BYTES)
