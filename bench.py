@@ -99,10 +99,12 @@ def chosen_options(args):
     # FP32 filter, claim iterations, shared rings, a bad pass evaluating up to
     # 3x the previous pass's winners, learned priority inside
     # 2-half-octave buckets (learned_shift 1 keeps the Steiner count within a
-    # few percent of the plain path); the autotuner may switch to the
-    # graph-driven loop (graph=1)
+    # few percent of the plain path), segment phases that hand their last
+    # few (at most 4) encroached subsegments to the bad pass (host-driven
+    # loop; the graph-driven loop is timed separately, without it)
     # launch shapes start from the drop-in's (the autotuner may move them)
-    o = dict(pred_mode=1, claim_iters=args.claim_iters, seg_claim_iters=2, priority=1, graph=0,
+    o = dict(pred_mode=1, claim_iters=args.claim_iters, seg_claim_iters=2, mix_below=4,
+             priority=1, graph=0,
              shared_ring=1,
              cand_factor=args.cand_factor, grid_cavity=4096, grid_claim=2048, grid_apply=1024,
              cavity_lanes=16, cav_lds_cap=32)
@@ -341,7 +343,7 @@ def bench_refine(args):
             for gs in (1, 2, 3):
                 o = dict(co, **{k: v for k, v in best.items() if k.startswith("grid_") or
                                 k in ("cavity_lanes", "cav_lds_cap")})
-                o.update(graph=1, graph_rounds=gr, graph_seg_passes=gs)
+                o.update(graph=1, graph_rounds=gr, graph_seg_passes=gs, mix_below=0)
                 sg = g.Session(P, T, N, SS, **kw, **o)
                 if co["priority"] == 2:
                     sg.set_priority_model(np.load(MODEL))

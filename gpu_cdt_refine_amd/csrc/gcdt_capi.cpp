@@ -442,13 +442,16 @@ DropIn &dropin() {
 }
 // The chosen form: FP32 filter, three claim iterations with shared rings,
 // a bad pass taking up to 3x the previous pass's winners, the learned
-// priority (coarsened by one half-octave); launch shapes the autotuner picked
-// on the 10^6-point workload (they change timing only, never the mesh).
+// priority (coarsened by one half-octave), segment phases that hand their
+// last few (at most 4) encroached subsegments to the bad pass; launch shapes
+// the autotuner picked on the 10^6-point workload (they change timing only,
+// never the mesh).
 void chosen_defaults(gcdt_options *o) {
   gcdt_default_options(o);
   o->pred_mode = 1;
   o->claim_iters = 3;
   o->seg_claim_iters = 2;
+  o->mix_below = 4;
   o->shared_ring = 1;
   o->cand_factor = 3;
   o->priority = 2;

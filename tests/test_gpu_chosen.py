@@ -11,8 +11,8 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-CHOSEN = dict(pred_mode=1, claim_iters=3, shared_ring=1, cand_factor=3, priority=2,
-              learned_shift=1, graph=0)
+CHOSEN = dict(pred_mode=1, claim_iters=3, seg_claim_iters=2, mix_below=4, shared_ring=1,
+              cand_factor=3, priority=2, learned_shift=1, graph=0)
 PLAIN = dict(pred_mode=0, claim_iters=1, priority=1, graph=0)
 
 

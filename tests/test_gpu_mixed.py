@@ -118,6 +118,13 @@ MIX_BELOW_CASES = [
      dict(theta=25.0, mix_below=200, priority=0, cand_min=64, cand_factor=2)),
     ("mb1000-large", dict(n=20000, ns=1000, seed=94),
      dict(theta=25.0, mix_below=1000, shared_ring=1, claim_iters=3)),
+    # the chosen form's setting: only the last few encroached subsegments
+    # of a segment phase join the bad pass
+    ("mb4-chosen-claims", dict(n=12000, ns=600, seed=95),
+     dict(theta=25.0, mix_below=4, shared_ring=1, claim_iters=3, seg_claim_iters=2,
+          cand_factor=3, pred_mode=1)),
+    ("mb4-chew", dict(n=10000, ns=500, seed=96),
+     dict(theta=25.0, mode=0, mix_below=4, shared_ring=1, claim_iters=3, seg_claim_iters=2)),
 ]
 
 
