@@ -89,7 +89,7 @@ typedef struct TStatus { unsigned char _status; } TStatus;
  * reference's checkCUDAError does (refine.cu:20-29).
  * It runs the chosen form (FP32-filtered exact predicates, three claim
  * iterations with shared rings, the learned priority compiled into the
- * library) unless gcdt_dropin_set_options chose other options; the device
+ * library, mix_below 4) unless gcdt_dropin_set_options chose other options; the device
  * buffers of one cached refiner serve every call (gcdt_dropin_release frees
  * them).  Calls are serialised by a process-wide lock. */
 void GPU_Refine_Quality(struct triangulateio *input,
