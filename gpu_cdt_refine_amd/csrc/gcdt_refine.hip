@@ -1698,7 +1698,9 @@ __global__ void k_constructions(const double *q, int n, ParamsD pr, double *ccw,
 // predicted conflict degree).  Inside a (coarsened) shortest-edge bucket the
 // candidate whose cavity and ring overlap the most other candidates' regions
 // wins: it sits inside a cluster of bad triangles and its insertion clears
-// the most of it, so clusters resolve in fewer rounds.
+// the most of it, so clusters resolve in fewer rounds.  The graph loop
+// applies it with this kernel; the host loop inside its first claim
+// iteration (PassD::score).
 __global__ void k_learned_prio(PassD ps, const float *score, int shift) {
   const int nc = pass_nc(ps);
   for (int i = pass_nseg(ps) + blockIdx.x * blockDim.x + threadIdx.x; i < nc;
