@@ -128,6 +128,8 @@ class Refiner {
   std::vector<float> log_X, log_t;
 
  private:
+  void ensure_stage();
+  void upload_staged(void *dst, const void *src, size_t bytes, size_t &k);
   void copy_input(InputSet &in, const double *points, int np, const int *tris, const int *nbrs,
                   int nt, const int *segs, int ns, const int *tri_inst, const InstPar *ipar,
                   int n_inst);
@@ -261,9 +263,10 @@ class Refiner {
   unsigned long long *d_val = nullptr, *h_val4 = nullptr;
   // download staging: two pinned buffers, filled by the copy engine while
   // host threads drain the other one into the caller's arrays
-  char *h_stage[2] = {nullptr, nullptr};
+  static constexpr int NSTAGE = 4;  // pinned staging buffers of the download
+  char *h_stage[NSTAGE] = {};
   size_t stage_bytes = 0;
-  hipEvent_t ev_stage[2] = {nullptr, nullptr};
+  hipEvent_t ev_stage[NSTAGE] = {};
   CopyPool *copier = nullptr;
   struct KT {
     long long launches = 0;

@@ -15,6 +15,7 @@
 // apply never races with another apply and no flip or link-repair phase
 // (refine.cu:2328-2490) is needed.
 #include <hip/hip_runtime.h>
+#include <sys/mman.h>
 
 #include <algorithm>
 #include <climits>
@@ -2146,7 +2147,8 @@ Refiner::Refiner(const gcdt_options &o) : opt(o) {
   HC(hipHostMalloc((void **)&h_ctl, sizeof(Ctl), hipHostMallocDefault));
   dalloc(&d_val, 4);
   HC(hipHostMalloc((void **)&h_val4, 4 * sizeof(unsigned long long), hipHostMallocDefault));
-  for (int k = 0; k < 2; k++) HC(hipEventCreateWithFlags(&ev_stage[k], hipEventDisableTiming));
+  for (int k = 0; k < NSTAGE; k++)
+    HC(hipEventCreateWithFlags(&ev_stage[k], hipEventDisableTiming));
   set_angle(opt.theta_deg, opt.offcenter, opt.mode);
   pr.use_fp32 = opt.pred_mode == 1;
   pr.priority = opt.priority;
@@ -2200,7 +2202,7 @@ Refiner::~Refiner() {
   dfree(d_val);
   if (h_val4) (void)hipHostFree(h_val4);
   delete copier;
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < NSTAGE; k++) {
     if (h_stage[k]) (void)hipHostFree(h_stage[k]);
     (void)hipEventDestroy(ev_stage[k]);
   }
@@ -2334,6 +2336,32 @@ __global__ void k_validate_input(const int *T, const int *N, int nt, const int *
 
 // Copies one input CDT into `in` (buffers large enough) and validates its
 // indices on the device before any kernel gathers through them.
+// Pinned staging buffers (shared by upload and download) and the host copy
+// threads.
+void Refiner::ensure_stage() {
+  if (!copier) copier = new CopyPool(CopyPool::default_workers());
+  if (!h_stage[0]) {
+    stage_bytes = size_t(32) << 20;
+    for (int k = 0; k < NSTAGE; k++)
+      HC(hipHostMalloc((void **)&h_stage[k], stage_bytes, hipHostMallocDefault));
+  }
+}
+
+// Host-to-device through the staging buffers: the host threads fill one
+// while the copy engine drains the others (a pageable copy is staged by
+// the runtime at about half the pinned rate).  k counts the chunks issued
+// so far; a buffer is refilled only after its previous transfer ended.
+void Refiner::upload_staged(void *dst, const void *src, size_t bytes, size_t &k) {
+  for (size_t off = 0; off < bytes; off += stage_bytes, k++) {
+    const size_t b = std::min(stage_bytes, bytes - off);
+    const int s = (int)(k % NSTAGE);
+    if (k >= (size_t)NSTAGE) HC(hipEventSynchronize(ev_stage[s]));
+    copier->copy(h_stage[s], (const char *)src + off, b);
+    HC(hipMemcpyAsync((char *)dst + off, h_stage[s], b, hipMemcpyHostToDevice, stream));
+    HC(hipEventRecord(ev_stage[s], stream));
+  }
+}
+
 void Refiner::copy_input(InputSet &in, const double *points, int np, const int *tris,
                          const int *nbrs, int nt, const int *segs, int ns, const int *tri_inst,
                          const InstPar *ipar, int n_inst) {
@@ -2342,13 +2370,16 @@ void Refiner::copy_input(InputSet &in, const double *points, int np, const int *
   in.any_chew = false;
   if (tri_inst)
     for (int i = 0; i < n_inst; i++) in.any_chew |= ipar[i].ruppert == 0;
-  HC(hipMemcpyAsync(in.P, points, sizeof(double2) * np, hipMemcpyHostToDevice, stream));
-  HC(hipMemcpyAsync(in.T, tris, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, stream));
-  HC(hipMemcpyAsync(in.N, nbrs, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, stream));
-  if (ns) HC(hipMemcpyAsync(in.S, segs, sizeof(int) * 2 * ns, hipMemcpyHostToDevice, stream));
+  ensure_stage();
+  HC(hipStreamSynchronize(stream));  // the staging buffers are free
+  size_t k = 0;
+  upload_staged(in.P, points, sizeof(double2) * np, k);
+  upload_staged(in.T, tris, sizeof(int) * 3 * (size_t)nt, k);
+  upload_staged(in.N, nbrs, sizeof(int) * 3 * (size_t)nt, k);
+  if (ns) upload_staged(in.S, segs, sizeof(int) * 2 * (size_t)ns, k);
   if (tri_inst) {
-    HC(hipMemcpyAsync(in.tinst, tri_inst, sizeof(int) * nt, hipMemcpyHostToDevice, stream));
-    HC(hipMemcpyAsync(in.ipar, ipar, sizeof(InstPar) * n_inst, hipMemcpyHostToDevice, stream));
+    upload_staged(in.tinst, tri_inst, sizeof(int) * (size_t)nt, k);
+    upload_staged(in.ipar, ipar, sizeof(InstPar) * (size_t)n_inst, k);
   }
   HC(hipMemsetAsync(d_val, 0, 4 * sizeof(unsigned long long), stream));
   k_validate_input<<<std::min(nblk(std::max(3LL * nt, 2LL * ns)), 4096), 256, 0, stream>>>(
@@ -3501,10 +3532,10 @@ void Refiner::download(MeshHost &out) {
   download_into(d);
 }
 
-// Device-to-host in chunks through two pinned staging buffers: the copy
-// engine fills one while the host threads copy the other into the caller's
-// arrays (pageable, typically never touched: the page faults are spread
-// over the threads too).
+// Device-to-host in chunks through NSTAGE pinned staging buffers: the copy
+// engine fills the next ones while the host threads copy a filled one into
+// the caller's arrays (pageable, typically never touched: the page faults
+// are spread over the threads too).
 void Refiner::download_into(const HostDst &d) {
   struct Piece {
     void *dst;
@@ -3532,10 +3563,15 @@ void Refiner::download_into(const HostDst &d) {
   if (d.tstat && nt_out) pieces.push_back({d.tstat, (const char *)o_tstat, (size_t)nt_out});
   if (d.tinst && m.batched && nt_out)
     pieces.push_back({d.tinst, (const char *)o_inst, sizeof(int) * (size_t)nt_out});
-  if (!copier) copier = new CopyPool(CopyPool::default_workers());
-  if (!h_stage[0]) {
-    stage_bytes = size_t(32) << 20;
-    for (int k = 0; k < 2; k++) HC(hipHostMalloc((void **)&h_stage[k], stage_bytes, hipHostMallocDefault));
+  ensure_stage();
+  // the caller's arrays are untouched: ask for 2 MiB pages, so their first
+  // touch (in the copies below) faults once per 2 MiB instead of per 4 KiB
+  // (a hint; ignored where transparent huge pages are off)
+  for (const Piece &p : pieces) {
+    const uintptr_t huge = uintptr_t(2) << 20;
+    const uintptr_t lo = ((uintptr_t)p.dst + huge - 1) & ~(huge - 1);
+    const uintptr_t hi = ((uintptr_t)p.dst + p.bytes) & ~(huge - 1);
+    if (hi > lo) (void)madvise((void *)lo, hi - lo, MADV_HUGEPAGE);
   }
   struct Chunk {
     char *dst;
@@ -3547,16 +3583,17 @@ void Refiner::download_into(const HostDst &d) {
     for (size_t off = 0; off < p.bytes; off += stage_bytes)
       ch.push_back({(char *)p.dst + off, p.src + off, std::min(stage_bytes, p.bytes - off)});
   auto issue = [&](size_t k) {
-    HC(hipMemcpyAsync(h_stage[k & 1], ch[k].src, ch[k].bytes, hipMemcpyDeviceToHost, stream));
-    HC(hipEventRecord(ev_stage[k & 1], stream));
+    HC(hipMemcpyAsync(h_stage[k % NSTAGE], ch[k].src, ch[k].bytes, hipMemcpyDeviceToHost,
+                      stream));
+    HC(hipEventRecord(ev_stage[k % NSTAGE], stream));
   };
-  for (size_t k = 0; k < std::min<size_t>(2, ch.size()); k++) issue(k);
+  for (size_t k = 0; k < std::min<size_t>(NSTAGE, ch.size()); k++) issue(k);
   // the input points need no transfer: copied while the first chunks travel
   if (d.P && skip) copier->copy(d.P, d.P_in_host, sizeof(double) * 2 * (size_t)skip);
   for (size_t k = 0; k < ch.size(); k++) {
-    HC(hipEventSynchronize(ev_stage[k & 1]));
-    copier->copy(ch[k].dst, h_stage[k & 1], ch[k].bytes);
-    if (k + 2 < ch.size()) issue(k + 2);
+    HC(hipEventSynchronize(ev_stage[k % NSTAGE]));
+    copier->copy(ch[k].dst, h_stage[k % NSTAGE], ch[k].bytes);
+    if (k + NSTAGE < ch.size()) issue(k + NSTAGE);
   }
   HC(hipStreamSynchronize(stream));
 }
