@@ -106,7 +106,7 @@ def chosen_options(args):
     o = dict(pred_mode=1, claim_iters=args.claim_iters, seg_claim_iters=2, mix_below=4,
              priority=1, graph=0,
              shared_ring=1,
-             cand_factor=args.cand_factor, grid_cavity=4096, grid_claim=2048, grid_apply=1024,
+             cand_factor=args.cand_factor, grid_cavity=4096, grid_claim=2048, grid_apply=2048,
              cavity_lanes=16, cav_lds_cap=32)
     if args.learned and os.path.exists(MODEL):
         o["priority"] = 2

@@ -32,7 +32,7 @@ RUPPERT, CHEW = 1, 0
 CHOSEN_FORM = dict(pred_mode=1, claim_iters=3, seg_claim_iters=2, mix_below=4, shared_ring=1,
                    cand_factor=3,
                    priority=2,
-                   learned_shift=1, grid_cavity=4096, grid_claim=2048, grid_apply=1024,
+                   learned_shift=1, grid_cavity=4096, grid_claim=2048, grid_apply=2048,
                    cavity_lanes=16, cav_lds_cap=32)
 
 

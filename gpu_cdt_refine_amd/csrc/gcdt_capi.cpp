@@ -458,7 +458,7 @@ void chosen_defaults(gcdt_options *o) {
   o->learned_shift = 1;
   o->grid_cavity = 4096;
   o->grid_claim = 2048;
-  o->grid_apply = 1024;
+  o->grid_apply = 2048;
   o->cavity_lanes = 16;
   o->cav_lds_cap = 32;
 }
