@@ -1162,7 +1162,7 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_apply(MeshD m, ParamsD pr, PassD
     const int *Bv = Bu + (ps.K + 2);
     const int *Bo = Bv + (ps.K + 2);
     const int *Bs = Bo + (ps.K + 2);
-    const int inst = m.T[L[0]].V.w;
+    const int inst = m.batched ? m.T[L[0]].V.w : 0;  // one instance: no gather
     const ParamsD lp = params_at(pr, m, L[0]);
     bool open = false;
     int a = -1, b = -1, par = -1;
@@ -1505,7 +1505,7 @@ __global__ __launch_bounds__(DEL_LANES) void k_del_apply(MeshD m, DelD dd) {
     }
 #pragma unroll 8
     for (int q = 0; q < 3 * (k - 2); q++) NT(q) = W[4 * DEL_MAX + q];
-    const int inst = m.T[SL(0)].V.w;
+    const int inst = m.batched ? m.T[SL(0)].V.w : 0;
     for (int q = 1; q < k; q++) {
       const int x = SL(q);
       int w = q - 1;
