@@ -2334,8 +2334,6 @@ __global__ void k_validate_input(const int *T, const int *N, int nt, const int *
   if (b3) atomicAdd(&bad[3], b3);
 }
 
-// Copies one input CDT into `in` (buffers large enough) and validates its
-// indices on the device before any kernel gathers through them.
 // Pinned staging buffers (shared by upload and download) and the host copy
 // threads.
 void Refiner::ensure_stage() {
@@ -2362,6 +2360,8 @@ void Refiner::upload_staged(void *dst, const void *src, size_t bytes, size_t &k)
   }
 }
 
+// Copies one input CDT into `in` (buffers large enough) and validates its
+// indices on the device before any kernel gathers through them.
 void Refiner::copy_input(InputSet &in, const double *points, int np, const int *tris,
                          const int *nbrs, int nt, const int *segs, int ns, const int *tri_inst,
                          const InstPar *ipar, int n_inst) {
