@@ -1006,6 +1006,7 @@ __device__ __forceinline__ unsigned long long claim_key(unsigned long long epoch
 // within the wave and every lane of a group follows the same candidate, so a
 // group reaches each ballot / shuffle with all of its lanes.
 constexpr int LIST_BLOCK = 256;
+static_assert(LIST_BLOCK == CB, "k_rank_apply ranks the winners with the compaction's tile");
 constexpr int CLAIM_LANES = 16;
 constexpr int APPLY_LANES = 4;
 
@@ -1127,20 +1128,11 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_win(MeshD m, PassD ps, const int
 // its lanes writes, and no lane reads a record another lane writes: the new
 // point's coordinates come from the candidate, not from P.
 template <int G>
-__global__ __launch_bounds__(LIST_BLOCK) void k_apply(MeshD m, ParamsD pr, PassD ps, int pbase,
-                                                      int tbase, int sbase) {
-  const int nc = pass_nc(ps);
-  const int nseg = pass_nseg(ps);
-  const int *ids = pass_ids(ps);
-  if (ps.ctl) {
-    pbase = ps.ctl->i[C_NP];
-    tbase = ps.ctl->i[C_NT];
-    sbase = ps.ctl->i[C_NS];
-  }
-  const int gl = threadIdx.x % G, per = blockDim.x / G;
-  unsigned long long nflip = 0;
-  for (int i = blockIdx.x * per + threadIdx.x / G; i < nc; i += gridDim.x * per) {
-    if (!ps.cwin[i]) continue;
+__device__ __forceinline__ void apply_one(const MeshD &m, const ParamsD &pr, const PassD &ps,
+                                          int i, int gl, int nseg, const int *ids, int pbase,
+                                          int tbase, int sbase, unsigned long long &nflip) {
+  {
+    if (!ps.cwin[i]) return;
     // winners split subsegments first (candidates [0, nseg)), so a segment
     // winner's rank is also its rank among the new subsegments
     const bool is_seg = i < nseg;
@@ -1276,6 +1268,43 @@ __global__ __launch_bounds__(LIST_BLOCK) void k_apply(MeshD m, ParamsD pr, PassD
       }
     }
   }
+}
+
+template <int G>
+__global__ __launch_bounds__(LIST_BLOCK) void k_apply(MeshD m, ParamsD pr, PassD ps, int pbase,
+                                                      int tbase, int sbase) {
+  const int nc = pass_nc(ps);
+  const int nseg = pass_nseg(ps);
+  const int *ids = pass_ids(ps);
+  if (ps.ctl) {
+    pbase = ps.ctl->i[C_NP];
+    tbase = ps.ctl->i[C_NT];
+    sbase = ps.ctl->i[C_NS];
+  }
+  const int gl = threadIdx.x % G, per = blockDim.x / G;
+  unsigned long long nflip = 0;
+  for (int i = blockIdx.x * per + threadIdx.x / G; i < nc; i += gridDim.x * per)
+    apply_one<G>(m, pr, ps, i, gl, nseg, ids, pbase, tbase, sbase, nflip);
+  nflip = wave_sum(nflip);
+  if ((threadIdx.x & 63) == 0 && nflip) atomicAdd(&ps.cnt[CNT_FLIPS], nflip);
+}
+
+// Small host-loop passes (nc <= SMALL_PASS): one block ranks the winners
+// (the compaction's one-tile scatter) and then applies them -- one launch
+// instead of two; the ranks reach the block's other threads through the
+// barrier.
+constexpr int SMALL_PASS = 256;
+template <int G>
+__global__ __launch_bounds__(LIST_BLOCK) void k_rank_apply(MeshD m, ParamsD pr, PassD ps,
+                                                           int pbase, int tbase, int sbase) {
+  const int nc = ps.nc;
+  const int run = scatter_tile(PredWin{ps.cwin}, nc, 0, 0, nullptr, ps.crank, nullptr);
+  if (threadIdx.x == 0) ps.cnt[CNT_NWIN] = (unsigned long long)run;
+  __syncthreads();
+  const int gl = threadIdx.x % G, per = blockDim.x / G;
+  unsigned long long nflip = 0;
+  for (int i = threadIdx.x / G; i < nc; i += per)
+    apply_one<G>(m, pr, ps, i, gl, ps.nseg, ps.ids, pbase, tbase, sbase, nflip);
   nflip = wave_sum(nflip);
   if ((threadIdx.x & 63) == 0 && nflip) atomicAdd(&ps.cnt[CNT_FLIPS], nflip);
 }
@@ -2876,11 +2905,18 @@ int Refiner::run_pass(const int *ids, int nc, int nseg, long long *progress, boo
     }
   }
   enqueue_claims(ps, gc, iters);
-  compact(PredWin{c_win}, nc, nullptr, c_rank, nullptr, d_cnt + CNT_NWIN);
-  tbegin(K_APPLY);
-  k_apply<APPLY_LANES><<<std::min(nblk((long long)nc * APPLY_LANES, LIST_BLOCK), opt.grid_apply),
-                         LIST_BLOCK, 0, stream>>>(m, pr, ps, np, nt, ns);
-  tend(K_APPLY, 0);
+  if (nc <= SMALL_PASS) {  // ranks and apply in one launch
+    tbegin(K_APPLY);
+    k_rank_apply<APPLY_LANES><<<1, LIST_BLOCK, 0, stream>>>(m, pr, ps, np, nt, ns);
+    tend(K_APPLY, 0);
+  } else {
+    compact(PredWin{c_win}, nc, nullptr, c_rank, nullptr, d_cnt + CNT_NWIN);
+    tbegin(K_APPLY);
+    k_apply<APPLY_LANES><<<std::min(nblk((long long)nc * APPLY_LANES, LIST_BLOCK),
+                                    opt.grid_apply),
+                           LIST_BLOCK, 0, stream>>>(m, pr, ps, np, nt, ns);
+    tend(K_APPLY, 0);
+  }
   HC(hipGetLastError());
   // every segment candidate may add one subsegment; the list buffer is not
   // grown here (the pass may still read it)
