@@ -309,7 +309,7 @@ def test_shared_ring_invariants(oracle, theta, off, mode, ci):
     assert abs((len(Po) - len(Pc)) / (len(Pc) - len(P))) < 0.05
 
 
-@pytest.mark.parametrize("mix_below", [20, 200])
+@pytest.mark.parametrize("mix_below", [4, 20, 200])
 def test_mix_below_invariants(oracle, mix_below):
     """Rounds whose segment phase hands its last few encroached subsegments
     to the bad pass: same invariants, fewer passes, Steiner count within 5%."""
